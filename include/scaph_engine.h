@@ -1,0 +1,104 @@
+/*
+ * scaph_engine.h — C-ABI of the host BFS engine (libscaph_host.so).
+ *
+ * This is the binding surface for callers that are not C++ (ctypes, cgo,
+ * JNI ...).  It exposes the reference's App lifecycle (src/app.cuh:120-135):
+ * construct from a command line (App(argc, argv)), load_graph(), run() one
+ * BFS, read the levels (h_value, src/app.cuh:262-264) and check().
+ * Every call returns 0 on success or a nonzero status; when `err` is
+ * non-NULL a message is written there (truncated to errlen).  Nothing
+ * exits the process.
+ */
+#ifndef SCAPH_ENGINE_H
+#define SCAPH_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct scaph_engine scaph_engine;
+
+typedef struct scaph_bfs_stats {
+  uint32_t src;
+  uint32_t depth;             /* levels run */
+  double ms;                  /* t_BFS: init .. levels final on device */
+  uint64_t n_reached;         /* vertices reached (n_c) */
+  uint64_t m_reached;         /* sum of their out-degrees (m_c) */
+  uint64_t page_execs;        /* paged mode: page kernel launches */
+  uint64_t page_execs_cached; /* ... of pages already resident */
+  uint64_t pages_full;        /* full-page copies */
+  uint64_t pages_compacted;   /* compacted (cold) page copies */
+  uint64_t bytes_h2d;         /* host-to-device page bytes */
+  double h2d_ms;              /* copy-stream busy time */
+} scaph_bfs_stats;
+
+typedef struct scaph_level_record {
+  uint32_t level;
+  int32_t dir; /* 0 push, 1 pull */
+  uint64_t n_f, m_f, m_u;
+  uint64_t n_next, m_next, scanned;
+  float ms;
+  float page_hist[12];
+} scaph_level_record;
+
+/* App(argc, argv): flags in gflags syntax (argv[0] is the program name). */
+int scaph_engine_create(int argc, const char* const* argv, scaph_engine** out, char* err,
+                        size_t errlen);
+void scaph_engine_destroy(scaph_engine* e);
+/* load_graph(): -graphfile, else -gen. */
+int scaph_engine_load(scaph_engine* e, char* err, size_t errlen);
+/* Load a flat CSR from host memory (offsets u64[n+1], ngh u32[offsets[n]]). */
+int scaph_engine_load_csr(scaph_engine* e, uint32_t n, const uint64_t* offsets,
+                          const uint32_t* ngh, int keep_host, char* err, size_t errlen);
+/* Build an R-MAT graph directly in device memory. */
+int scaph_engine_gen_rmat_device(scaph_engine* e, uint32_t scale, uint32_t edge_factor, double a,
+                                 double b, double c, uint64_t seed, int keep_host, char* err,
+                                 size_t errlen);
+int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_pages,
+                      int* resident, int* has_pull);
+/* run(): one BFS from src.  schedule: NULL keeps the -schedule flag, else
+ * "fixed" | "push" | "pull" | "learned". */
+int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* err,
+                     size_t errlen);
+int scaph_engine_set_schedule(scaph_engine* e, const char* name, double alpha, double beta,
+                              char* err, size_t errlen);
+/* Learned schedule weights: SCAPH_MLP_NPARAM bf16 values + feature scale. */
+int scaph_engine_set_model(scaph_engine* e, const uint16_t* params_bf16, const float* feat_mean,
+                           const float* feat_scale, float prefetch_gain, char* err,
+                           size_t errlen);
+int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_t max,
+                               uint32_t* count);
+int scaph_engine_levels(scaph_engine* e, uint32_t* host_out);
+int scaph_engine_validate(scaph_engine* e, uint32_t src, uint64_t* out2);
+int scaph_engine_out_degrees(scaph_engine* e, uint32_t* host_out);
+/* Copy of the host CSR kept at load (offsets n+1, ngh m); 1 if none kept. */
+int scaph_engine_host_csr(scaph_engine* e, uint64_t* offsets, uint32_t* ngh);
+/* check(): CPU BFS from src compared with the device levels; *errors gets
+ * the number of mismatching vertices (reference-capped and exact forms). */
+int scaph_engine_check(scaph_engine* e, uint32_t src, uint64_t* errors, char* err, size_t errlen);
+/* Device-side graph handles (for tests that drive kernels directly). */
+int scaph_engine_graph_desc(scaph_engine* e, int transposed, void* desc_out /* scaph_graph_desc* */);
+int scaph_engine_device_degree(scaph_engine* e, const uint32_t** degree);
+
+/* Host-side tools, no device needed. */
+int scaph_host_page_cuts(const uint64_t* offsets, uint32_t n, uint64_t pagesize_words,
+                         int weighted, void* pages_out /* scaph_page*, may be NULL */,
+                         uint32_t max_pages, uint32_t* num_pages);
+int scaph_host_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                        uint64_t seed, uint64_t* offsets /* n+1 */, uint32_t* ngh /* cap */,
+                        uint64_t ngh_cap, uint64_t* num_edges);
+int scaph_host_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint64_t* offsets, uint32_t* ngh,
+                      uint64_t ngh_cap, uint64_t* num_edges);
+int scaph_host_write_graph(const char* base, uint32_t n, const uint64_t* offsets,
+                           const uint32_t* ngh, uint64_t pagesize_words, int idx64,
+                           int write_wgh, char* err, size_t errlen);
+int scaph_host_cpu_bfs(uint32_t n, const uint64_t* offsets, const uint32_t* ngh, uint32_t src,
+                       uint32_t inf, uint32_t* levels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

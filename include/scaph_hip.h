@@ -1,0 +1,226 @@
+/*
+ * scaph_hip.h — C-ABI of the gfx950 BFS kernel library (libscaph_hip.so).
+ *
+ * Everything here is plain C: device buffers are passed as raw pointers,
+ * sizes as integers, streams/events as opaque handles.  Every entry point
+ * returns an int status (0 = success, otherwise the hipError_t value) and
+ * never exits the process; scaph_error_string() turns a status into text.
+ * The caller owns every buffer; nothing is allocated implicitly except where
+ * an entry point says so (the graph builders allocate their own temporary
+ * sort storage and free it before returning).
+ *
+ * Reference interfaces replaced (paths relative to the Scaph sources):
+ *   - cudaMalloc / cudaMallocHost / cudaMemcpyAsync / cudaStreamQuery call
+ *     sites: src/app.cuh:288-339, 191-197, 371-393, 421, 447; src/graph.h:100
+ *   - bfs_init / setsrc:           app/bfs.cu:20-37, 58-65
+ *   - gpu_push_csr:                src/gpu_engine.cuh:41-169
+ *   - cond / update (writeMin):    app/bfs_kernel.cuh:6-16, src/gpu_tools.cuh:53-64
+ *   - get_page_info/get_total_info src/gpu_engine.cuh:397-456
+ *   - compress_stat (cold pages):  src/compress.h:18-56
+ * Entry points without a reference counterpart (pull expand, bitmap
+ * conversions, generators, MLP, batched BFS) say so in their comment.
+ */
+#ifndef SCAPH_HIP_H
+#define SCAPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCAPH_UNREACHED 0xFFFFFFFFu /* level of a vertex no path reaches */
+#define SCAPH_STATUS_NOT_READY 600  /* == hipErrorNotReady (stream/event busy) */
+#define SCAPH_STATUS_INVALID 1      /* == hipErrorInvalidValue */
+
+/* On-disk / in-memory page record: src/page.h:22-29 (Page<uint32>). */
+typedef struct scaph_page {
+  uint32_t left;    /* first vertex of the page */
+  uint32_t right;   /* last vertex of the page (inclusive) */
+  uint32_t nodenum; /* right - left + 1 */
+  uint32_t edgenum; /* out-edges stored in the page */
+} scaph_page;
+
+/* A paged CSR resident (wholly or partly) in device memory.
+ * Page p's buffer starts at page_ptr[p] and holds, as in the files written
+ * by tools/bin2pagecsr.cpp:221-242, Node{vtx,idx,len}[nodenum] (3 u32 words
+ * each, idx relative to the page's edge array) followed by the u32 neighbour
+ * IDs.  page_ptr and pages are DEVICE arrays of num_pages entries. */
+typedef struct scaph_graph_desc {
+  const uint64_t* page_ptr;  /* device: u32* address of each page buffer */
+  const scaph_page* pages;   /* device: page list */
+  uint32_t num_pages;
+  uint32_t num_vertices;
+} scaph_graph_desc;
+
+typedef struct scaph_device_props {
+  char name[256];
+  char arch[64];
+  int compute_units;
+  int xcds;                 /* 0 when the runtime does not report it */
+  int wave_size;
+  size_t lds_per_block;
+  size_t total_mem;
+  size_t l2_bytes;
+  int clock_khz;
+  int mem_clock_khz;
+  int mem_bus_width;
+} scaph_device_props;
+
+/* ---------------------------------------------------------------- runtime */
+const char* scaph_error_string(int status);
+int scaph_init(int device);
+int scaph_device_count(int* count);
+int scaph_get_device_props(int device, scaph_device_props* out);
+int scaph_stream_create(void** stream);
+int scaph_stream_destroy(void* stream);
+int scaph_stream_query(void* stream); /* 0 = idle, SCAPH_STATUS_NOT_READY = busy */
+int scaph_stream_sync(void* stream);
+int scaph_stream_wait_event(void* stream, void* event);
+int scaph_event_create(void** event);
+int scaph_event_destroy(void* event);
+int scaph_event_record(void* event, void* stream);
+int scaph_event_query(void* event);
+int scaph_event_sync(void* event);
+int scaph_event_elapsed_ms(void* start, void* stop, float* ms);
+int scaph_device_sync(void);
+int scaph_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ----------------------------------------------------------------- memory */
+int scaph_dev_alloc(void** ptr, size_t bytes);
+int scaph_dev_free(void* ptr);
+int scaph_host_pinned_alloc(void** ptr, size_t bytes);
+int scaph_host_pinned_free(void* ptr);
+int scaph_copy_h2d_async(void* dst, const void* src, size_t bytes, void* stream);
+int scaph_copy_d2h_async(void* dst, const void* src, size_t bytes, void* stream);
+int scaph_copy_d2d_async(void* dst, const void* src, size_t bytes, void* stream);
+/* Fill n u32 words with value (a kernel, safe to capture in a hipGraph). */
+int scaph_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream);
+
+/* ------------------------------------------------------------- BFS state */
+/* Counters written by the expand kernels, one record per level. */
+typedef struct scaph_level_counters {
+  unsigned long long n_next;   /* vertices first reached in this level */
+  unsigned long long m_next;   /* sum of their out-degrees */
+  unsigned long long q_len;    /* entries appended to the next queue */
+  unsigned long long scanned;  /* edges examined (push: all; pull: until hit) */
+} scaph_level_counters;
+
+/* Level init (app/bfs.cu:20-37): level[v] = SCAPH_UNREACHED for v < n,
+ * visited = static_visited (bitmap, may be NULL for all-zero) then the
+ * source bit set, level[src] = 0, queue[0] = src.  Bitmaps hold
+ * ceil(n/64)*2 u32 words.  counters (one record) is zeroed. */
+int scaph_bfs_init(uint32_t* level, uint32_t* visited, const uint32_t* static_visited,
+                   uint32_t n, uint32_t src, uint32_t* queue,
+                   scaph_level_counters* counters, void* stream);
+
+/* Bitmap of vertices with in-degree 0 (they are never reached except as
+ * the source, so the pull expand skips them).  in_degree: device u32[n].
+ * Also sets the padding bits past n.  No reference counterpart. */
+int scaph_bitmap_from_zero_degree(uint32_t* bitmap, const uint32_t* in_degree, uint32_t n,
+                                  void* stream);
+
+/* Workspace bytes the push expand needs for a frontier of up to
+ * max_entries vertices (or a page of max_entries Node records) whose
+ * out-edges total at most max_edges. */
+size_t scaph_push_workspace_bytes(uint32_t max_entries, unsigned long long max_edges);
+
+/* Top-down (push) expand of one level from a frontier queue — the resident
+ * counterpart of gpu_push_csr (src/gpu_engine.cuh:41-169) with the BFS
+ * relax of app/bfs_kernel.cuh:6-16 turned into a level-synchronous claim:
+ * a vertex v reached by an edge for the first time gets
+ * level[v] = cur_level + 1, its visited bit, and a slot in q_out.
+ * n_in is the frontier length, m_in the sum of its out-degrees (known from
+ * the previous level's counters; used only to size the launch).
+ * degree: device out-degree u32[n] (src/app.cuh:302, d_degree). */
+int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree,
+                   uint32_t* level, uint32_t* visited, const uint32_t* q_in, uint32_t n_in,
+                   unsigned long long m_in, uint32_t* q_out, scaph_level_counters* counters,
+                   uint32_t cur_level, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Bottom-up (pull) expand of one level: every unvisited vertex scans its
+ * in-neighbours (transposed paged CSR) for one in the frontier bitmap.
+ * Writes level/visited for the vertices found and the full next-frontier
+ * bitmap (all words, so it needs no clearing).  No reference counterpart
+ * (the reference engine is push-only). */
+int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* degree, uint32_t* level,
+                   uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
+                   scaph_level_counters* counters, uint32_t cur_level, void* stream);
+
+/* Frontier format conversions (no reference counterpart). */
+int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint32_t* bitmap,
+                                   uint32_t n, void* stream);
+int scaph_frontier_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* queue,
+                                   scaph_level_counters* counters, void* stream);
+
+/* Per-page active counts — get_page_info + get_total_info
+ * (src/gpu_engine.cuh:397-456): for each page p,
+ *   nodes[p] = #active v in [left,right],
+ *   datas[p] = sum over active v of (deg(v) + 3)   (2*deg+3 if weighted),
+ * totals[0] = sum nodes, totals[1] = sum datas.  "Active" is a bit in the
+ * frontier bitmap (the reference's bool stat[] array). pages: device. */
+int scaph_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_t* front,
+                      const uint32_t* degree, int weighted, uint32_t* nodes, uint32_t* datas,
+                      unsigned long long* totals, void* stream);
+
+/* Push expand of ONE page buffer (full page, or a compacted cold page as
+ * written by compress_stat, src/compress.h:18-40) — gpu_push_csr's unit of
+ * work (src/app.cuh:438).  A Node is active when the frontier bitmap has
+ * its vtx bit.  Newly reached vertices get level/visited and a bit in
+ * front_out (atomic OR; front_out must be cleared before the level).
+ * edgenum bounds the page's edges (sizes the launch), n = vertex count.
+ * Counters accumulate across calls: clear them at the start of a level. */
+int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned long long edgenum,
+                        uint32_t n, const uint32_t* degree, uint32_t* level, uint32_t* visited,
+                        const uint32_t* front_in, uint32_t* front_out,
+                        scaph_level_counters* counters, uint32_t cur_level, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+/* Level-array check kernel: counts edges (u,v) violating
+ * level[v] <= level[u] + 1 and reached vertices without a parent at
+ * level-1 (BFS invariants).  out[0] = violations of the edge rule,
+ * out[1] = reached non-source vertices lacking a parent. */
+int scaph_bfs_validate(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
+                       const uint32_t* level, uint32_t src, unsigned long long* out,
+                       void* stream);
+
+/* ------------------------------------------------------------ generators */
+/* R-MAT edge generator (no reference counterpart; the reference ships no
+ * generator).  Edge e draws `scale` quadrant choices from SplitMix64 words
+ * keyed by seed; quadrant thresholds are u32 (thr_a, thr_ab, thr_abc).
+ * Vertex IDs are permuted by the keyed bijection scaph_perm_key.
+ * Writes packed keys (u << 32 | v) for e in [e_begin, e_end); self-loops
+ * become the key (n-1, n-1) so they collapse to one key after dedup. */
+int scaph_gen_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t scale,
+                        uint32_t thr_a, uint32_t thr_ab, uint32_t thr_abc, uint64_t seed,
+                        void* stream);
+
+/* Sorts keys, removes duplicates and the self-loop key, and writes a flat
+ * CSR: offsets u64[n+1], neighbours u32[*num_edges], out-degree u32[n].
+ * transpose != 0 swaps the roles of the key halves (builds in-edges).
+ * keys is clobbered; keys_tmp must hold num_keys u64; ngh must hold
+ * num_keys u32.  Allocates and frees its own sort scratch. */
+int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, uint32_t n,
+                        int transpose, uint64_t* offsets, uint32_t* ngh, uint32_t* degree,
+                        uint64_t* num_edges, void* stream);
+
+/* Writes the pages of a flat device CSR into page buffers, in the byte
+ * layout of tools/bin2pagecsr.cpp:221-242 (Node records then neighbours).
+ * pages/page_ptr: device arrays (page cuts computed by the host). Words of
+ * each buffer past the used part are NOT touched. */
+int scaph_pages_from_csr(const uint64_t* offsets, const uint32_t* ngh, const scaph_page* pages,
+                         const uint64_t* page_ptr, uint32_t num_pages, uint32_t n, void* stream);
+
+/* ------------------------------------------------------------ hipGraph */
+/* Capture everything launched on `stream` between begin and end into an
+ * executable graph (opaque handle); launch replays it on `stream`. */
+int scaph_graph_capture_begin(void* stream);
+int scaph_graph_capture_end(void* stream, void** exec);
+int scaph_graph_launch(void* exec, void* stream);
+int scaph_graph_destroy(void* exec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCAPH_HIP_H */

@@ -1,0 +1,220 @@
+"""scaph_amd — MI355X-native BFS hot path of the Scaph out-of-core graph engine.
+
+Python mirror of the reference's App interface (src/app.cuh:120-135,
+app/bfs.cu:39-117) over the C-ABI engine in ``libscaph_host.so``:
+
+    eng = BFS("-max_memory=65536", "-schedule=fixed")
+    eng.load_csr(offsets, ngh)          # or eng.load() for -graphfile / -gen
+    stats = eng.run(src)                # one traversal (t_BFS, n_c, m_c)
+    levels = eng.levels()               # uint32, 0xFFFFFFFF = unreached
+    eng.check(src)                      # CPU BFS comparison (bfs::check)
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import UNREACHED, LibraryMissing, ScaphError  # noqa: F401
+
+__all__ = ["BFS", "generate_rmat", "generate_er", "page_cuts", "write_graph", "UNREACHED",
+           "LibraryMissing", "ScaphError", "device_props"]
+
+
+def _err():
+    return ctypes.create_string_buffer(1024)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class BFS:
+    """The BFS app: construct from gflags-style flags, load a graph, run."""
+
+    def __init__(self, *flags):
+        lib = _lib.host()
+        argv = [b"bfs"] + [f.encode() for f in flags]
+        arr = (ctypes.c_char_p * len(argv))(*argv)
+        h = ctypes.c_void_p()
+        e = _err()
+        _lib.check(lib.scaph_engine_create(len(argv), arr, ctypes.byref(h), e, 1024), e.value.decode())
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.scaph_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- load_graph ------------------------------------------------------
+    def load(self):
+        e = _err()
+        _lib.check(self._lib.scaph_engine_load(self._h, e, 1024), e.value.decode())
+
+    def load_csr(self, offsets, ngh, keep_host=True):
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        ngh = np.ascontiguousarray(ngh, dtype=np.uint32)
+        n = len(offsets) - 1
+        if ngh.size == 0:
+            ngh = np.zeros(1, np.uint32)
+        e = _err()
+        _lib.check(self._lib.scaph_engine_load_csr(self._h, n, _ptr(offsets), _ptr(ngh),
+                                                  int(keep_host), e, 1024), e.value.decode())
+
+    def gen_rmat_device(self, scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1, keep_host=False):
+        e = _err()
+        _lib.check(self._lib.scaph_engine_gen_rmat_device(self._h, scale, edge_factor, a, b, c,
+                                                         seed, int(keep_host), e, 1024),
+                   e.value.decode())
+
+    def info(self):
+        n = ctypes.c_uint32()
+        m = ctypes.c_uint64()
+        np_ = ctypes.c_uint32()
+        res = ctypes.c_int()
+        pull = ctypes.c_int()
+        _lib.check(self._lib.scaph_engine_info(self._h, ctypes.byref(n), ctypes.byref(m),
+                                               ctypes.byref(np_), ctypes.byref(res),
+                                               ctypes.byref(pull)))
+        return {"n": n.value, "m": m.value, "pages": np_.value, "resident": bool(res.value),
+                "pull": bool(pull.value)}
+
+    # -- run -------------------------------------------------------------
+    def run(self, src):
+        st = _lib.BfsStats()
+        e = _err()
+        _lib.check(self._lib.scaph_engine_bfs(self._h, src, ctypes.byref(st), e, 1024),
+                   e.value.decode())
+        return {k: getattr(st, k) for k, _ in _lib.BfsStats._fields_}
+
+    def set_schedule(self, name, alpha=0.0, beta=0.0):
+        e = _err()
+        _lib.check(self._lib.scaph_engine_set_schedule(self._h, name.encode(), alpha, beta, e,
+                                                       1024), e.value.decode())
+
+    def set_model(self, params_bf16, mean, scale, prefetch_gain=0.0):
+        p = np.ascontiguousarray(params_bf16, dtype=np.uint16)
+        m = np.ascontiguousarray(mean, dtype=np.float32)
+        s = np.ascontiguousarray(scale, dtype=np.float32)
+        e = _err()
+        _lib.check(self._lib.scaph_engine_set_model(self._h, _ptr(p), _ptr(m), _ptr(s),
+                                                    prefetch_gain, e, 1024), e.value.decode())
+
+    def records(self):
+        cnt = ctypes.c_uint32()
+        self._lib.scaph_engine_level_records(self._h, None, 0, ctypes.byref(cnt))
+        arr = (_lib.LevelRecord * max(cnt.value, 1))()
+        self._lib.scaph_engine_level_records(self._h, arr, cnt.value, ctypes.byref(cnt))
+        out = []
+        for r in arr[:cnt.value]:
+            d = {k: getattr(r, k) for k, _ in _lib.LevelRecord._fields_ if k != "page_hist"}
+            d["page_hist"] = list(r.page_hist)
+            out.append(d)
+        return out
+
+    def levels(self):
+        n = self.info()["n"]
+        out = np.empty(n, np.uint32)
+        _lib.check(self._lib.scaph_engine_levels(self._h, _ptr(out)))
+        return out
+
+    def out_degrees(self):
+        n = self.info()["n"]
+        out = np.empty(n, np.uint32)
+        _lib.check(self._lib.scaph_engine_out_degrees(self._h, _ptr(out)))
+        return out
+
+    def host_csr(self):
+        inf = self.info()
+        off = np.empty(inf["n"] + 1, np.uint64)
+        ngh = np.empty(max(inf["m"], 1), np.uint32)
+        _lib.check(self._lib.scaph_engine_host_csr(self._h, _ptr(off), _ptr(ngh)),
+                   "no host CSR kept")
+        return off, ngh[:inf["m"]]
+
+    def validate(self, src):
+        out = np.zeros(2, np.uint64)
+        _lib.check(self._lib.scaph_engine_validate(self._h, src, _ptr(out)))
+        return int(out[0]), int(out[1])
+
+    def check(self, src):
+        errors = ctypes.c_uint64()
+        e = _err()
+        _lib.check(self._lib.scaph_engine_check(self._h, src, ctypes.byref(errors), e, 1024),
+                   e.value.decode())
+        return errors.value
+
+    def graph_desc(self, transposed=False):
+        d = _lib.GraphDesc()
+        _lib.check(self._lib.scaph_engine_graph_desc(self._h, int(transposed), ctypes.byref(d)))
+        return d
+
+    def device_degree(self):
+        p = ctypes.c_void_p()
+        _lib.check(self._lib.scaph_engine_device_degree(self._h, ctypes.byref(p)))
+        return p.value
+
+
+def generate_rmat(scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1):
+    """Host R-MAT generator (same edges as the device builder)."""
+    lib = _lib.host()
+    n = 1 << scale
+    off = np.empty(n + 1, np.uint64)
+    cap = edge_factor * n
+    ngh = np.empty(max(cap, 1), np.uint32)
+    m = ctypes.c_uint64()
+    _lib.check(lib.scaph_host_gen_rmat(scale, edge_factor, a, b, c, seed, _ptr(off), _ptr(ngh),
+                                       cap, ctypes.byref(m)))
+    return off, ngh[:m.value].copy()
+
+
+def generate_er(n, m, seed=1):
+    lib = _lib.host()
+    off = np.empty(n + 1, np.uint64)
+    ngh = np.empty(max(m, 1), np.uint32)
+    E = ctypes.c_uint64()
+    _lib.check(lib.scaph_host_gen_er(n, m, seed, _ptr(off), _ptr(ngh), m, ctypes.byref(E)))
+    return off, ngh[:E.value].copy()
+
+
+def page_cuts(offsets, pagesize_words, weighted=False):
+    lib = _lib.host()
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    n = len(offsets) - 1
+    cnt = ctypes.c_uint32()
+    _lib.check(lib.scaph_host_page_cuts(_ptr(offsets), n, pagesize_words, int(weighted), None, 0,
+                                        ctypes.byref(cnt)))
+    pages = np.zeros((cnt.value, 4), np.uint32)
+    _lib.check(lib.scaph_host_page_cuts(_ptr(offsets), n, pagesize_words, int(weighted),
+                                        _ptr(pages), cnt.value, ctypes.byref(cnt)))
+    return pages
+
+
+def write_graph(base, offsets, ngh, pagesize_words=0, idx64=True, write_wgh=False):
+    lib = _lib.host()
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    ngh = np.ascontiguousarray(ngh, dtype=np.uint32)
+    if ngh.size == 0:
+        ngh = np.zeros(1, np.uint32)
+    e = _err()
+    _lib.check(lib.scaph_host_write_graph(str(base).encode(), len(offsets) - 1, _ptr(offsets),
+                                          _ptr(ngh), pagesize_words, int(idx64), int(write_wgh),
+                                          e, 1024), e.value.decode())
+
+
+def device_props(device=0):
+    lib = _lib.hip()
+    p = _lib.DeviceProps()
+    _lib.check(lib.scaph_get_device_props(device, ctypes.byref(p)))
+    return {"name": p.name.decode(), "arch": p.arch.decode(), "cu": p.compute_units,
+            "total_mem": p.total_mem, "l2": p.l2_bytes, "mem_clock_khz": p.mem_clock_khz,
+            "bus_width": p.mem_bus_width}

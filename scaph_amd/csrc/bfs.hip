@@ -1,0 +1,776 @@
+// bfs.hip — gfx950 BFS hot path: level init, push (top-down) expand,
+// pull (bottom-up) expand, frontier conversions, per-page active counts and
+// the level-array validator.
+//
+// The reference engine (src/gpu_engine.cuh:41-169) runs an asynchronous,
+// label-correcting push: per edge a CAS-loop writeMin on value[] and a byte
+// store into stat[].  Here every level is synchronous: the first edge that
+// reaches an unvisited vertex claims it with one atomicOr on a visited
+// bitmap (V/8 bytes, small enough to stay in the last-level cache), after
+// which the level is written with a plain store.  BFS distances are unique,
+// so the final levels equal the reference fixpoint (app/bfs_kernel.cuh:6-16
+// relax d > s + 1 -> d = s + 1) for every schedule.
+//
+// Push load balance: the frontier's adjacency lists are laid end to end
+// (block-local compaction + scan of the out-degrees), cut into tiles of
+// kTile edges, and every 256-thread block expands whole tiles — a hub with a
+// million out-edges is spread over ~1000 blocks instead of one CTA.
+#include "common.h"
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kTile = 1024;                   // edges per expand tile
+constexpr uint32_t kPerThread = kTile / kBlock;    // 4 consecutive edges
+constexpr uint32_t kPullSerial = 16;               // per-lane scan before wave help
+constexpr uint32_t kScanThreads = 1024;
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct PushWs {
+  uint64_t* ent_ep;     // [nb*256] edge-list address of each kept entry (block-compacted)
+  uint32_t* ent_len;    // [nb*256]
+  uint32_t* blk_cnt;    // [nb]
+  uint64_t* blk_edges;  // [nb]
+  uint32_t* blk_coff;   // [nb] exclusive entry offset
+  uint64_t* blk_eoff;   // [nb] exclusive edge offset
+  uint64_t* gx;         // [M] global exclusive edge prefix (dense)
+  uint64_t* gep;        // [M] edge-list address (dense)
+  uint32_t* tstart;     // [tiles+1] entry containing the tile's first edge
+  unsigned long long* totals;  // [4] entries, edges, tiles
+  size_t bytes;
+};
+
+PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_edges) {
+  PushWs w;
+  const size_t nb = (max_entries + kBlock - 1) / kBlock + 1;
+  const size_t m = (size_t)max_entries + 1;
+  const size_t tiles = (size_t)((max_edges + kTile - 1) / kTile) + 2;
+  size_t off = 0;
+  char* p = static_cast<char*>(base);
+  auto take = [&](size_t bytes) {
+    char* r = p ? p + off : nullptr;
+    off += align256(bytes);
+    return r;
+  };
+  w.ent_ep = reinterpret_cast<uint64_t*>(take(nb * kBlock * 8));
+  w.ent_len = reinterpret_cast<uint32_t*>(take(nb * kBlock * 4));
+  w.blk_cnt = reinterpret_cast<uint32_t*>(take(nb * 4));
+  w.blk_edges = reinterpret_cast<uint64_t*>(take(nb * 8));
+  w.blk_coff = reinterpret_cast<uint32_t*>(take(nb * 4));
+  w.blk_eoff = reinterpret_cast<uint64_t*>(take(nb * 8));
+  w.gx = reinterpret_cast<uint64_t*>(take(m * 8));
+  w.gep = reinterpret_cast<uint64_t*>(take(m * 8));
+  w.tstart = reinterpret_cast<uint32_t*>(take(tiles * 4));
+  w.totals = reinterpret_cast<unsigned long long*>(take(64));
+  w.bytes = off;
+  return w;
+}
+
+// ------------------------------------------------------------------ init
+
+__global__ void __launch_bounds__(kBlock)
+k_bfs_init(uint32_t* level, uint32_t* visited, const uint32_t* static_visited, uint32_t n,
+           size_t words) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t v = tid; v < n; v += stride) level[v] = SCAPH_UNREACHED;
+  for (size_t w = tid; w < words; w += stride) {
+    uint32_t x = static_visited ? static_visited[w] : 0u;
+    // bits of vertices >= n never become active
+    const size_t first = w * 32;
+    if (first + 32 > n) {
+      const uint32_t valid = first >= n ? 0u : (uint32_t)(n - first);
+      x |= valid >= 32 ? 0u : (~0u << valid);
+    }
+    visited[w] = x;
+  }
+}
+
+__global__ void k_bfs_set_source(uint32_t* level, uint32_t* visited, uint32_t src,
+                                 uint32_t* queue, scaph_level_counters* counters) {
+  level[src] = 0;
+  visited[src >> 5] |= 1u << (src & 31);
+  if (queue) queue[0] = src;
+  if (counters) {
+    counters->n_next = 0;
+    counters->m_next = 0;
+    counters->q_len = 0;
+    counters->scanned = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_zero_degree_bitmap(uint32_t* bitmap, const uint32_t* in_degree, uint32_t n, size_t words) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+    uint32_t x = 0;
+    const size_t first = w * 32;
+    for (uint32_t b = 0; b < 32; ++b) {
+      const size_t v = first + b;
+      if (v >= n || in_degree[v] == 0) x |= 1u << b;
+    }
+    bitmap[w] = x;
+  }
+}
+
+// ------------------------------------------------------------ push: prep
+
+// Block-local compaction of the entries with len > 0.
+__device__ __forceinline__ void prep_store(bool keep, uint64_t ep, uint32_t len, uint64_t* ent_ep,
+                                           uint32_t* ent_len, uint32_t* blk_cnt,
+                                           uint64_t* blk_edges) {
+  __shared__ uint32_t wcnt[kBlock / SCAPH_WAVE];
+  __shared__ unsigned long long wedge[kBlock / SCAPH_WAVE];
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x / SCAPH_WAVE;
+  const uint64_t m = __ballot(keep);
+  const uint32_t r = lanes_below(m);
+  const unsigned long long es = wave_sum((unsigned long long)(keep ? len : 0u));
+  if (lane == 0) {
+    wcnt[wid] = (uint32_t)__popcll(m);
+    wedge[wid] = es;
+  }
+  __syncthreads();
+  uint32_t off = 0, total = 0;
+  unsigned long long etot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kBlock / SCAPH_WAVE; ++w) {
+    if (w < wid) off += wcnt[w];
+    total += wcnt[w];
+    etot += wedge[w];
+  }
+  const size_t base = (size_t)blockIdx.x * kBlock;
+  if (keep) {
+    ent_ep[base + off + r] = ep;
+    ent_len[base + off + r] = len;
+  }
+  if (threadIdx.x == 0) {
+    blk_cnt[blockIdx.x] = total;
+    blk_edges[blockIdx.x] = etot;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_prep_queue(scaph_graph_desc g, const uint32_t* q, uint32_t n_in, uint64_t* ent_ep,
+             uint32_t* ent_len, uint32_t* blk_cnt, uint64_t* blk_edges) {
+  __shared__ PageLds s;
+  stage_pages(s, g);
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t len = 0;
+  uint64_t ep = 0;
+  if (i < n_in) {
+    const uint32_t v = q[i];
+    if (v < g.num_vertices) {
+      NodeRef nr = load_node(s, g, v);
+      len = nr.len;
+      ep = reinterpret_cast<uint64_t>(nr.edges);
+    }
+  }
+  prep_store(len > 0, ep, len, ent_ep, ent_len, blk_cnt, blk_edges);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_prep_page(const uint32_t* buf, uint32_t nodenum, const uint32_t* front, uint64_t* ent_ep,
+            uint32_t* ent_len, uint32_t* blk_cnt, uint64_t* blk_edges) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t len = 0;
+  uint64_t ep = 0;
+  bool keep = false;
+  if (i < nodenum) {
+    const uint32_t vtx = buf[3ull * i];
+    const uint32_t idx = buf[3ull * i + 1];
+    len = buf[3ull * i + 2];
+    keep = len > 0 && bit_test(front, vtx);
+    ep = reinterpret_cast<uint64_t>(buf + 3ull * nodenum + idx);
+  }
+  prep_store(keep, ep, keep ? len : 0u, ent_ep, ent_len, blk_cnt, blk_edges);
+}
+
+// Exclusive scan of the per-block counts (one 1024-thread block).
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, uint32_t* blk_coff,
+              uint64_t* blk_eoff, unsigned long long* totals, unsigned long long tile_cap) {
+  __shared__ uint32_t wc[kScanThreads / SCAPH_WAVE];
+  __shared__ unsigned long long we[kScanThreads / SCAPH_WAVE];
+  const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
+  const uint32_t b0 = threadIdx.x * per;
+  const uint32_t b1 = min(b0 + per, nb);
+  uint32_t c = 0;
+  unsigned long long e = 0;
+  for (uint32_t b = b0; b < b1; ++b) {
+    c += blk_cnt[b];
+    e += blk_edges[b];
+  }
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x / SCAPH_WAVE;
+  const uint32_t ci = wave_incl_scan(c);
+  const unsigned long long ei = wave_incl_scan_u64(e);
+  if (lane == SCAPH_WAVE - 1) {
+    wc[wid] = ci;
+    we[wid] = ei;
+  }
+  __syncthreads();
+  uint32_t coff = 0, ctot = 0;
+  unsigned long long eoff = 0, etot = 0;
+  for (uint32_t w = 0; w < kScanThreads / SCAPH_WAVE; ++w) {
+    if (w < wid) {
+      coff += wc[w];
+      eoff += we[w];
+    }
+    ctot += wc[w];
+    etot += we[w];
+  }
+  uint32_t rc = coff + ci - c;
+  unsigned long long re = eoff + ei - e;
+  for (uint32_t b = b0; b < b1; ++b) {
+    blk_coff[b] = rc;
+    blk_eoff[b] = re;
+    rc += blk_cnt[b];
+    re += blk_edges[b];
+  }
+  if (threadIdx.x == 0) {
+    // A caller that under-reports the frontier's edges gets a truncated
+    // (wrong, detectable) level instead of writes past the tile table.
+    if (etot > tile_cap * kTile) etot = tile_cap * kTile;
+    totals[0] = ctot;
+    totals[1] = etot;
+    totals[2] = (etot + kTile - 1) / kTile;
+    totals[3] = tile_cap;
+  }
+}
+
+// Dense entry arrays + the first entry of every tile.
+__global__ void __launch_bounds__(kBlock)
+k_mark_tiles(const uint64_t* ent_ep, const uint32_t* ent_len, const uint32_t* blk_cnt,
+             const uint32_t* blk_coff, const uint64_t* blk_eoff, uint64_t* gx, uint64_t* gep,
+             uint32_t* tstart, const unsigned long long* totals) {
+  __shared__ unsigned long long wtot[kBlock / SCAPH_WAVE];
+  const uint32_t b = blockIdx.x;
+  const uint32_t j = threadIdx.x;
+  const uint32_t cnt = blk_cnt[b];
+  const size_t src = (size_t)b * kBlock + j;
+  const unsigned long long len = j < cnt ? ent_len[src] : 0ull;
+  unsigned long long total;
+  const unsigned long long incl = block256_incl_scan(len, wtot, &total);
+  if (j < cnt) {
+    const uint32_t gi = blk_coff[b] + j;
+    const unsigned long long g0 = blk_eoff[b] + incl - len;
+    gx[gi] = g0;
+    gep[gi] = ent_ep[src];
+    const unsigned long long t0 = (g0 + kTile - 1) / kTile;
+    unsigned long long t1 = (g0 + len + kTile - 1) / kTile;
+    if (t1 > totals[2]) t1 = totals[2];
+    for (unsigned long long t = t0; t < t1; ++t) tstart[t] = gi;
+  }
+}
+
+// --------------------------------------------------------- push: expand
+
+enum { OUT_QUEUE = 0, OUT_BITMAP = 1 };
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
+         const unsigned long long* totals, const uint32_t* degree, uint32_t* level,
+         uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
+         scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
+  __shared__ unsigned long long s_gx[kTile + 1];
+  __shared__ unsigned long long s_ep[kTile + 1];
+  __shared__ unsigned long long red[3][kBlock / SCAPH_WAVE];
+  const unsigned long long n_ent = totals[0];
+  const unsigned long long n_edges = totals[1];
+  const unsigned long long n_tiles = totals[2];
+  unsigned long long acc_n = 0, acc_m = 0;
+  for (unsigned long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const unsigned long long s0 = t * kTile;
+    const unsigned long long e0 = min(s0 + kTile, n_edges);
+    const uint32_t i0 = tstart[t];
+    const uint32_t i1 = (t + 1 < n_tiles) ? tstart[t + 1] : (uint32_t)(n_ent - 1);
+    uint32_t cnt = i1 - i0 + 1;
+    if (cnt > kTile + 1) cnt = kTile + 1;
+    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+      s_gx[k] = gx[i0 + k];
+      s_ep[k] = gep[i0 + k];
+    }
+    __syncthreads();
+    const unsigned long long r0 = s0 + (unsigned long long)threadIdx.x * kPerThread;
+    uint32_t lo = 0, hi = cnt;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_gx[mid] <= r0) lo = mid; else hi = mid;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPerThread; ++j) {
+      const unsigned long long r = r0 + j;
+      bool claimed = false;
+      uint32_t v = 0;
+      if (r < e0) {
+        while (lo + 1 < cnt && s_gx[lo + 1] <= r) ++lo;
+        const uint32_t* ep = reinterpret_cast<const uint32_t*>(s_ep[lo]);
+        v = ep[r - s_gx[lo]];
+        if (v < n) {
+          const uint32_t w = v >> 5, bit = 1u << (v & 31);
+          if (!(visited[w] & bit)) {
+            const uint32_t old = atomicOr(&visited[w], bit);
+            if (!(old & bit)) {
+              claimed = true;
+              level[v] = next_level;
+              acc_n += 1;
+              acc_m += degree[v];
+              if (MODE == OUT_BITMAP) atomicOr(&front_out[w], bit);
+            }
+          }
+        }
+      }
+      if (MODE == OUT_QUEUE) {
+        const uint64_t m = __ballot(claimed);
+        if (m) {
+          const uint32_t lane = lane_id();
+          const int leader = __ffsll((unsigned long long)m) - 1;
+          unsigned long long base = 0;
+          if ((int)lane == leader) base = atomicAdd(&counters->q_len, (unsigned long long)__popcll(m));
+          base = wave_bcast64(base, leader);
+          if (claimed) q_out[base + lanes_below(m)] = v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const unsigned long long sn = wave_sum(acc_n);
+  const unsigned long long sm = wave_sum(acc_m);
+  const uint32_t wid = threadIdx.x / SCAPH_WAVE;
+  if (lane_id() == 0) {
+    red[0][wid] = sn;
+    red[1][wid] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b = 0;
+    for (uint32_t w = 0; w < kBlock / SCAPH_WAVE; ++w) {
+      a += red[0][w];
+      b += red[1][w];
+    }
+    if (a) atomicAdd(&counters->n_next, a);
+    if (b) atomicAdd(&counters->m_next, b);
+    if (blockIdx.x == 0) atomicAdd(&counters->scanned, n_edges);
+  }
+}
+
+// ------------------------------------------------------------------ pull
+
+__global__ void __launch_bounds__(kBlock)
+k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* visited,
+       const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
+       uint32_t next_level) {
+  __shared__ PageLds s;
+  __shared__ unsigned long long red[3][kBlock / SCAPH_WAVE];
+  stage_pages(s, g);
+  const uint32_t n = g.num_vertices;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x / SCAPH_WAVE;
+  const uint64_t nchunks = ((uint64_t)n + 63) / 64;
+  const uint64_t gw = (uint64_t)blockIdx.x * (kBlock / SCAPH_WAVE) + wid;
+  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / SCAPH_WAVE);
+  unsigned long long acc_n = 0, acc_m = 0, acc_s = 0;
+  for (uint64_t chunk = gw; chunk < nchunks; chunk += nw) {
+    const uint32_t v = (uint32_t)(chunk * 64 + lane);
+    const size_t word = chunk * 2 + (lane >> 5);
+    const uint32_t vw = visited[word];
+    const bool unv = v < n && !((vw >> (lane & 31)) & 1u);
+    const uint64_t mu = __ballot(unv);
+    if (mu == 0) {
+      if ((lane & 31) == 0) front_out[word] = 0u;
+      continue;
+    }
+    bool found = false;
+    bool longl = false;
+    NodeRef nr;
+    nr.edges = nullptr;
+    nr.len = 0;
+    if (unv) {
+      nr = load_node(s, g, v);
+      const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
+      uint32_t k = 0;
+      for (; k < lim; ++k) {
+        const uint32_t u = nr.edges[k];
+        if (bit_test(front_in, u)) {
+          found = true;
+          ++k;
+          break;
+        }
+      }
+      acc_s += k;
+      longl = !found && nr.len > kPullSerial;
+    }
+    uint64_t ml = __ballot(longl);
+    while (ml) {
+      const int j = __ffsll((unsigned long long)ml) - 1;
+      ml &= ml - 1;
+      const uint64_t ep = wave_bcast64(reinterpret_cast<uint64_t>(nr.edges), j);
+      const uint32_t len = wave_bcast(nr.len, j);
+      const uint32_t* e = reinterpret_cast<const uint32_t*>(ep);
+      for (uint32_t k = kPullSerial; k < len; k += SCAPH_WAVE) {
+        const uint32_t kk = k + lane;
+        const bool hit = kk < len && bit_test(front_in, e[kk]);
+        const uint64_t mh = __ballot(hit);
+        if (mh) {
+          if ((int)lane == j) found = true;
+          if (lane == 0) acc_s += (uint32_t)(__ffsll((unsigned long long)mh)) + (k - kPullSerial);
+          break;
+        }
+        if (lane == 0) acc_s += min(len - k, (uint32_t)SCAPH_WAVE);
+      }
+    }
+    const uint64_t mf = __ballot(found);
+    if (found) {
+      level[v] = next_level;
+      acc_n += 1;
+      acc_m += degree[v];
+    }
+    if ((lane & 31) == 0) {
+      const uint32_t fw = (uint32_t)(mf >> (lane & 32));
+      visited[word] = vw | fw;
+      front_out[word] = fw;
+    }
+  }
+  const unsigned long long sn = wave_sum(acc_n);
+  const unsigned long long sm = wave_sum(acc_m);
+  const unsigned long long ss = wave_sum(acc_s);
+  if (lane == 0) {
+    red[0][wid] = sn;
+    red[1][wid] = sm;
+    red[2][wid] = ss;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b = 0, c = 0;
+    for (uint32_t w = 0; w < kBlock / SCAPH_WAVE; ++w) {
+      a += red[0][w];
+      b += red[1][w];
+      c += red[2][w];
+    }
+    if (a) atomicAdd(&counters->n_next, a);
+    if (b) atomicAdd(&counters->m_next, b);
+    if (c) atomicAdd(&counters->scanned, c);
+  }
+}
+
+// ------------------------------------------------------- conversions
+
+__global__ void __launch_bounds__(kBlock)
+k_queue_to_bitmap(const uint32_t* q, uint32_t nq, uint32_t* bitmap, uint32_t n) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += stride) {
+    const uint32_t v = q[i];
+    if (v < n) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* q,
+                  scaph_level_counters* counters) {
+  const size_t words = ((size_t)n + 31) / 32;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const uint32_t lane = lane_id();
+  // uniform trip count across the wave so the ballot-free scan is safe
+  const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x - lane;
+  for (size_t w0 = first; w0 < words; w0 += stride) {
+    const size_t w = w0 + lane;
+    uint32_t bits = 0;
+    if (w < words) {
+      bits = bitmap[w];
+      const size_t base = w * 32;
+      if (base + 32 > n) {
+        const uint32_t valid = (uint32_t)(n - base);
+        bits &= valid >= 32 ? ~0u : ((1u << valid) - 1u);
+      }
+    }
+    const uint32_t c = __popc(bits);
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t tot = wave_bcast(incl, SCAPH_WAVE - 1);
+    unsigned long long base = 0;
+    if (tot) {
+      if (lane == 0) base = atomicAdd(&counters->q_len, (unsigned long long)tot);
+      base = wave_bcast64(base, 0);
+    }
+    unsigned long long o = base + incl - c;
+    while (bits) {
+      const uint32_t b = __ffs(bits) - 1;
+      bits &= bits - 1;
+      q[o++] = (uint32_t)(w * 32 + b);
+    }
+  }
+}
+
+// --------------------------------------------------------- page counts
+
+__global__ void __launch_bounds__(kBlock)
+k_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_t* front,
+              const uint32_t* degree, int weighted, uint32_t* nodes, uint32_t* datas,
+              unsigned long long* totals) {
+  __shared__ unsigned long long red[2][kBlock / SCAPH_WAVE];
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x / SCAPH_WAVE;
+  for (uint32_t p = blockIdx.x; p < num_pages; p += gridDim.x) {
+    const uint32_t left = pages[p].left, right = pages[p].right;
+    const uint32_t w0 = left >> 5, w1 = right >> 5;
+    unsigned long long cn = 0, cd = 0;
+    for (uint32_t w = w0 + threadIdx.x; w <= w1; w += kBlock) {
+      uint32_t bits = front[w];
+      if (w == w0) bits &= ~0u << (left & 31);
+      if (w == w1) bits &= (right & 31) == 31 ? ~0u : ((1u << ((right & 31) + 1)) - 1u);
+      cn += __popc(bits);
+      while (bits) {
+        const uint32_t b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const uint32_t d = degree[w * 32 + b];
+        cd += weighted ? (2ull * d + 3ull) : (d + 3ull);
+      }
+    }
+    cn = wave_sum(cn);
+    cd = wave_sum(cd);
+    if (lane == 0) {
+      red[0][wid] = cn;
+      red[1][wid] = cd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long a = 0, b = 0;
+      for (uint32_t w = 0; w < kBlock / SCAPH_WAVE; ++w) {
+        a += red[0][w];
+        b += red[1][w];
+      }
+      nodes[p] = (uint32_t)a;
+      datas[p] = (uint32_t)b;
+      if (a) atomicAdd(&totals[0], a);
+      if (b) atomicAdd(&totals[1], b);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// ------------------------------------------------------------ validation
+
+__global__ void __launch_bounds__(kBlock)
+k_validate_edges(scaph_graph_desc g, const uint32_t* level, unsigned long long* out) {
+  __shared__ PageLds s;
+  stage_pages(s, g);
+  const uint32_t n = g.num_vertices;
+  unsigned long long bad = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += stride) {
+    const uint32_t lu = level[u];
+    if (lu == SCAPH_UNREACHED) continue;
+    NodeRef nr = load_node(s, g, u);
+    for (uint32_t k = 0; k < nr.len; ++k) {
+      const uint32_t lv = level[nr.edges[k]];
+      if (lv == SCAPH_UNREACHED || lv > lu + 1) ++bad;
+    }
+  }
+  bad = wave_sum(bad);
+  if (lane_id() == 0 && bad) atomicAdd(&out[0], bad);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_validate_parents(scaph_graph_desc gin, const uint32_t* level, uint32_t src,
+                   unsigned long long* out) {
+  __shared__ PageLds s;
+  stage_pages(s, gin);
+  const uint32_t n = gin.num_vertices;
+  unsigned long long bad = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const uint32_t lv = level[v];
+    if (lv == SCAPH_UNREACHED) continue;
+    if (v == src) {
+      if (lv != 0) ++bad;
+      continue;
+    }
+    if (lv == 0) { ++bad; continue; }
+    NodeRef nr = load_node(s, gin, v);
+    bool ok = false;
+    for (uint32_t k = 0; k < nr.len && !ok; ++k) ok = level[nr.edges[k]] == lv - 1;
+    if (!ok) ++bad;
+  }
+  bad = wave_sum(bad);
+  if (lane_id() == 0 && bad) atomicAdd(&out[1], bad);
+}
+
+inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
+  size_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+int launch_push_tail(const PushWs& w, uint32_t nb, unsigned long long max_edges,
+                     const uint32_t* degree, uint32_t* level, uint32_t* visited, uint32_t* q_out,
+                     uint32_t* front_out, scaph_level_counters* counters, uint32_t next_level,
+                     uint32_t n, int mode, hipStream_t st) {
+  const unsigned long long tile_cap = (max_edges + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
+                     nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap);
+  SCAPH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mark_tiles, dim3(nb), dim3(kBlock), 0, st, w.ent_ep, w.ent_len, w.blk_cnt,
+                     w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
+  SCAPH_LAUNCH_CHECK();
+  const unsigned grid = grid_for((size_t)((max_edges + kTile - 1) / kTile), 1, 8192);
+  if (mode == OUT_QUEUE) {
+    hipLaunchKernelGGL(k_expand<OUT_QUEUE>, dim3(grid), dim3(kBlock), 0, st, w.gx, w.gep,
+                       w.tstart, w.totals, degree, level, visited, q_out, front_out, counters,
+                       next_level, n);
+  } else {
+    hipLaunchKernelGGL(k_expand<OUT_BITMAP>, dim3(grid), dim3(kBlock), 0, st, w.gx, w.gep,
+                       w.tstart, w.totals, degree, level, visited, q_out, front_out, counters,
+                       next_level, n);
+  }
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scaph_bfs_init(uint32_t* level, uint32_t* visited, const uint32_t* static_visited, uint32_t n,
+                   uint32_t src, uint32_t* queue, scaph_level_counters* counters, void* stream) {
+  if (!level || !visited || n == 0 || src >= n) return SCAPH_STATUS_INVALID;
+  const size_t words = (((size_t)n + 63) / 64) * 2;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(n, kBlock * 4, 16384)), dim3(kBlock), 0, st, level,
+                     visited, static_visited, n, words);
+  SCAPH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_bfs_set_source, dim3(1), dim3(1), 0, st, level, visited, src, queue,
+                     counters);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_bitmap_from_zero_degree(uint32_t* bitmap, const uint32_t* in_degree, uint32_t n,
+                                  void* stream) {
+  if (!bitmap || !in_degree) return SCAPH_STATUS_INVALID;
+  const size_t words = (((size_t)n + 63) / 64) * 2;
+  hipLaunchKernelGGL(k_zero_degree_bitmap, dim3(grid_for(words, kBlock, 16384)), dim3(kBlock), 0,
+                     as_stream(stream), bitmap, in_degree, n, words);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+size_t scaph_push_workspace_bytes(uint32_t max_entries, unsigned long long max_edges) {
+  return carve_push_ws(nullptr, max_entries, max_edges).bytes;
+}
+
+int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree, uint32_t* level,
+                   uint32_t* visited, const uint32_t* q_in, uint32_t n_in,
+                   unsigned long long m_in, uint32_t* q_out, scaph_level_counters* counters,
+                   uint32_t cur_level, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!out_graph || !degree || !level || !visited || !q_out || !counters || !workspace)
+    return SCAPH_STATUS_INVALID;
+  if (n_in == 0) return 0;
+  if (!q_in) return SCAPH_STATUS_INVALID;
+  PushWs w = carve_push_ws(workspace, n_in, m_in);
+  if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  const uint32_t nb = (n_in + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_prep_queue, dim3(nb), dim3(kBlock), 0, st, *out_graph, q_in, n_in,
+                     w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
+  SCAPH_LAUNCH_CHECK();
+  return launch_push_tail(w, nb, m_in, degree, level, visited, q_out, nullptr, counters,
+                          cur_level + 1, out_graph->num_vertices, OUT_QUEUE, st);
+}
+
+int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned long long edgenum,
+                          uint32_t n, const uint32_t* degree, uint32_t* level, uint32_t* visited,
+                          const uint32_t* front_in, uint32_t* front_out,
+                          scaph_level_counters* counters, uint32_t cur_level, void* workspace,
+                          size_t workspace_bytes, void* stream) {
+  if (!page_buf || !degree || !level || !visited || !front_in || !front_out || !counters ||
+      !workspace)
+    return SCAPH_STATUS_INVALID;
+  if (nodenum == 0) return 0;
+  PushWs w = carve_push_ws(workspace, nodenum, edgenum);
+  if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  const uint32_t nb = (nodenum + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_prep_page, dim3(nb), dim3(kBlock), 0, st, page_buf, nodenum, front_in,
+                     w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
+  SCAPH_LAUNCH_CHECK();
+  return launch_push_tail(w, nb, edgenum, degree, level, visited, nullptr, front_out, counters,
+                          cur_level + 1, n, OUT_BITMAP, st);
+}
+
+int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* degree, uint32_t* level,
+                   uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
+                   scaph_level_counters* counters, uint32_t cur_level, void* stream) {
+  if (!in_graph || !degree || !level || !visited || !front_in || !front_out || !counters)
+    return SCAPH_STATUS_INVALID;
+  const uint64_t nchunks = ((uint64_t)in_graph->num_vertices + 63) / 64;
+  const unsigned grid = grid_for(nchunks, kBlock / SCAPH_WAVE, 256 * 16);
+  hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), 0, as_stream(stream), *in_graph, degree,
+                     level, visited, front_in, front_out, counters, cur_level + 1);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint32_t* bitmap,
+                                   uint32_t n, void* stream) {
+  if (!bitmap) return SCAPH_STATUS_INVALID;
+  const size_t words = (((size_t)n + 63) / 64) * 2;
+  int rc = scaph_fill_u32(bitmap, 0u, words, stream);
+  if (rc) return rc;
+  if (n_queue == 0) return 0;
+  if (!queue) return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_queue_to_bitmap, dim3(grid_for(n_queue, kBlock, 16384)), dim3(kBlock), 0,
+                     as_stream(stream), queue, n_queue, bitmap, n);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_frontier_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* queue,
+                                   scaph_level_counters* counters, void* stream) {
+  if (!bitmap || !queue || !counters) return SCAPH_STATUS_INVALID;
+  const size_t words = ((size_t)n + 31) / 32;
+  hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid_for(words, kBlock, 8192)), dim3(kBlock), 0,
+                     as_stream(stream), bitmap, n, queue, counters);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_t* front,
+                      const uint32_t* degree, int weighted, uint32_t* nodes, uint32_t* datas,
+                      unsigned long long* totals, void* stream) {
+  if (!pages || !front || !degree || !nodes || !datas || !totals) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, totals, 2u);
+  SCAPH_LAUNCH_CHECK();
+  if (num_pages == 0) return 0;
+  hipLaunchKernelGGL(k_page_counts, dim3(grid_for(num_pages, 1, 4096)), dim3(kBlock), 0, st, pages,
+                     num_pages, front, degree, weighted, nodes, datas, totals);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_bfs_validate(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
+                       const uint32_t* level, uint32_t src, unsigned long long* out,
+                       void* stream) {
+  if (!out_graph || !level || !out) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, out, 2u);
+  SCAPH_LAUNCH_CHECK();
+  const unsigned grid = grid_for(out_graph->num_vertices, kBlock, 16384);
+  hipLaunchKernelGGL(k_validate_edges, dim3(grid), dim3(kBlock), 0, st, *out_graph, level, out);
+  SCAPH_LAUNCH_CHECK();
+  if (in_graph) {
+    hipLaunchKernelGGL(k_validate_parents, dim3(grid), dim3(kBlock), 0, st, *in_graph, level, src,
+                       out);
+    SCAPH_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // extern "C"

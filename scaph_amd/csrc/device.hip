@@ -1,0 +1,169 @@
+// device.hip — runtime part of the C-ABI: device selection, streams,
+// events, memory and copies.  Replaces the CUDA runtime call sites of the
+// reference engine (src/app.cuh:288-339 allocations and streams,
+// src/app.cuh:191-197/385/391 copies, src/app.cuh:421/447 stream polling,
+// src/graph.h:100 pinned page buffers).  Streams are created at the default
+// priority.
+#include <string.h>
+
+#include "common.h"
+
+extern "C" {
+
+const char* scaph_error_string(int status) {
+  if (status == 0) return "success";
+  return hipGetErrorString(static_cast<hipError_t>(status));
+}
+
+int scaph_init(int device) {
+  SCAPH_TRY(hipSetDevice(device));
+  SCAPH_TRY(hipFree(nullptr));  // create the context now, not on first use
+  return 0;
+}
+
+int scaph_device_count(int* count) {
+  if (!count) return SCAPH_STATUS_INVALID;
+  SCAPH_TRY(hipGetDeviceCount(count));
+  return 0;
+}
+
+int scaph_get_device_props(int device, scaph_device_props* out) {
+  if (!out) return SCAPH_STATUS_INVALID;
+  hipDeviceProp_t p;
+  SCAPH_TRY(hipGetDeviceProperties(&p, device));
+  memset(out, 0, sizeof(*out));
+  strncpy(out->name, p.name, sizeof(out->name) - 1);
+  strncpy(out->arch, p.gcnArchName, sizeof(out->arch) - 1);
+  out->compute_units = p.multiProcessorCount;
+  out->wave_size = p.warpSize;
+  out->lds_per_block = p.sharedMemPerBlock;
+  out->total_mem = p.totalGlobalMem;
+  out->l2_bytes = p.l2CacheSize;
+  out->clock_khz = p.clockRate;
+  out->mem_clock_khz = p.memoryClockRate;
+  out->mem_bus_width = p.memoryBusWidth;
+  out->xcds = 0;
+  return 0;
+}
+
+int scaph_stream_create(void** stream) {
+  if (!stream) return SCAPH_STATUS_INVALID;
+  hipStream_t s;
+  SCAPH_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return 0;
+}
+
+int scaph_stream_destroy(void* stream) { return (int)hipStreamDestroy(as_stream(stream)); }
+int scaph_stream_query(void* stream) { return (int)hipStreamQuery(as_stream(stream)); }
+int scaph_stream_sync(void* stream) { return (int)hipStreamSynchronize(as_stream(stream)); }
+int scaph_stream_wait_event(void* stream, void* event) {
+  return (int)hipStreamWaitEvent(as_stream(stream), reinterpret_cast<hipEvent_t>(event), 0);
+}
+
+int scaph_event_create(void** event) {
+  if (!event) return SCAPH_STATUS_INVALID;
+  hipEvent_t e;
+  SCAPH_TRY(hipEventCreate(&e));
+  *event = e;
+  return 0;
+}
+int scaph_event_destroy(void* event) { return (int)hipEventDestroy(reinterpret_cast<hipEvent_t>(event)); }
+int scaph_event_record(void* event, void* stream) {
+  return (int)hipEventRecord(reinterpret_cast<hipEvent_t>(event), as_stream(stream));
+}
+int scaph_event_query(void* event) { return (int)hipEventQuery(reinterpret_cast<hipEvent_t>(event)); }
+int scaph_event_sync(void* event) { return (int)hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)); }
+int scaph_event_elapsed_ms(void* start, void* stop, float* ms) {
+  if (!ms) return SCAPH_STATUS_INVALID;
+  return (int)hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),
+                                  reinterpret_cast<hipEvent_t>(stop));
+}
+int scaph_device_sync(void) { return (int)hipDeviceSynchronize(); }
+int scaph_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return SCAPH_STATUS_INVALID;
+  return (int)hipMemGetInfo(free_bytes, total_bytes);
+}
+
+int scaph_dev_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return SCAPH_STATUS_INVALID;
+  *ptr = nullptr;
+  if (bytes == 0) return 0;
+  return (int)hipMalloc(ptr, bytes);
+}
+int scaph_dev_free(void* ptr) { return ptr ? (int)hipFree(ptr) : 0; }
+
+int scaph_host_pinned_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return SCAPH_STATUS_INVALID;
+  *ptr = nullptr;
+  if (bytes == 0) return 0;
+  return (int)hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+}
+int scaph_host_pinned_free(void* ptr) { return ptr ? (int)hipHostFree(ptr) : 0; }
+
+int scaph_copy_h2d_async(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return 0;
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream));
+}
+int scaph_copy_d2h_async(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return 0;
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream));
+}
+int scaph_copy_d2d_async(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return 0;
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream));
+}
+
+}  // extern "C"
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* dst, uint32_t value, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // 16-byte stores for the aligned bulk
+  const size_t n4 = ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) ? n / 4 : 0;
+  uint4* d4 = reinterpret_cast<uint4*>(dst);
+  const uint4 v4 = make_uint4(value, value, value, value);
+  for (size_t j = i; j < n4; j += stride) d4[j] = v4;
+  for (size_t j = n4 * 4 + i; j < n; j += stride) dst[j] = value;
+}
+
+extern "C" int scaph_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream) {
+  if (n == 0) return 0;
+  if (!dst) return SCAPH_STATUS_INVALID;
+  size_t blocks = (n / 4 + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), dst,
+                     value, n);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+// ------------------------------------------------------------- hipGraph
+extern "C" {
+
+int scaph_graph_capture_begin(void* stream) {
+  return (int)hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal);
+}
+
+int scaph_graph_capture_end(void* stream, void** exec) {
+  if (!exec) return SCAPH_STATUS_INVALID;
+  hipGraph_t g = nullptr;
+  SCAPH_TRY(hipStreamEndCapture(as_stream(stream), &g));
+  hipGraphExec_t ge = nullptr;
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return (int)e;
+  *exec = ge;
+  return 0;
+}
+
+int scaph_graph_launch(void* exec, void* stream) {
+  return (int)hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(exec), as_stream(stream));
+}
+
+int scaph_graph_destroy(void* exec) {
+  return exec ? (int)hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(exec)) : 0;
+}
+
+}  // extern "C"

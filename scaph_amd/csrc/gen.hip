@@ -1,0 +1,239 @@
+// gen.hip — synthetic graph construction on the device: R-MAT edge keys,
+// sort + dedup into a flat CSR (out- or in-edges), and page buffers in the
+// tools/bin2pagecsr.cpp:221-242 layout.  None of this is timed as part of a
+// BFS; it exists so a 2-billion-edge graph is built in seconds on the GPU
+// instead of minutes on the host.  The edge stream is defined in
+// rmat_spec.h and reproduced bit for bit by the host generator.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "common.h"
+#include "rmat_spec.h"
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
+  size_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t count, uint32_t scale, uint32_t thr_a,
+            uint32_t thr_ab, uint32_t thr_abc, uint64_t seed, scaph_spec::PermKey pk) {
+  const uint64_t n = 1ull << scale;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    uint32_t u, v;
+    scaph_spec::rmat_edge(seed, e_begin + i, scale, thr_a, thr_ab, thr_abc, &u, &v);
+    u = (uint32_t)scaph_spec::perm_range(pk, u, n);
+    v = (uint32_t)scaph_spec::perm_range(pk, v, n);
+    if (u == v) u = v = (uint32_t)(n - 1);
+    keys[i] = ((uint64_t)u << 32) | v;
+  }
+}
+
+// (u<<32|v) -> packed (a<<bits|b) with (a,b) = (u,v) or (v,u).  Keys that
+// are self-loops (u == v) all become the largest key (n-1, n-1).
+__global__ void __launch_bounds__(kBlock)
+k_pack(uint64_t* keys, uint64_t count, uint32_t bits, int transpose, uint32_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint64_t k = keys[i];
+    uint32_t u = (uint32_t)(k >> 32), v = (uint32_t)k;
+    if (u == v) u = v = n - 1;
+    const uint32_t a = transpose ? v : u;
+    const uint32_t b = transpose ? u : v;
+    keys[i] = ((uint64_t)a << bits) | b;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_run_starts(const uint64_t* uniq, uint64_t count, uint32_t bits, uint32_t* ngh, uint64_t* start) {
+  const uint64_t mask = (1ull << bits) - 1;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint64_t k = uniq[i];
+    const uint32_t a = (uint32_t)(k >> bits);
+    ngh[i] = (uint32_t)(k & mask);
+    if (i == 0 || (uint32_t)(uniq[i - 1] >> bits) != a) start[a] = i;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_run_ends(const uint64_t* uniq, uint64_t count, uint32_t bits, const uint64_t* start,
+           uint32_t* degree) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint32_t a = (uint32_t)(uniq[i] >> bits);
+    if (i + 1 == count || (uint32_t)(uniq[i + 1] >> bits) != a)
+      degree[a] = (uint32_t)(i + 1 - start[a]);
+  }
+}
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+// Nodes of every page: Node{vtx, idx (page-relative), len}.
+__global__ void __launch_bounds__(kBlock)
+k_page_nodes(const uint64_t* offsets, const scaph_page* pages, const uint64_t* page_ptr,
+             uint32_t num_pages, uint32_t n) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    uint32_t lo = 0, hi = num_pages;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pages[mid].left <= v) lo = mid; else hi = mid;
+    }
+    const uint32_t left = pages[lo].left;
+    if (v > pages[lo].right) continue;
+    uint32_t* buf = reinterpret_cast<uint32_t*>(page_ptr[lo]);
+    const uint32_t i = v - left;
+    buf[3ull * i] = v;
+    buf[3ull * i + 1] = (uint32_t)(offsets[v] - offsets[left]);
+    buf[3ull * i + 2] = (uint32_t)(offsets[v + 1] - offsets[v]);
+  }
+}
+
+// Neighbour arrays: blockIdx.y = page.
+__global__ void __launch_bounds__(kBlock)
+k_page_edges(const uint64_t* offsets, const uint32_t* ngh, const scaph_page* pages,
+             const uint64_t* page_ptr) {
+  const scaph_page pg = pages[blockIdx.y];
+  const uint64_t e0 = offsets[pg.left], e1 = offsets[(uint64_t)pg.right + 1];
+  uint32_t* dst = reinterpret_cast<uint32_t*>(page_ptr[blockIdx.y]) + 3ull * pg.nodenum;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e1 - e0; i += stride)
+    dst[i] = ngh[e0 + i];
+}
+
+inline uint32_t bits_for(uint32_t n) {
+  uint32_t b = 1;
+  while (b < 32 && (1ull << b) < n) ++b;
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scaph_gen_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t scale,
+                        uint32_t thr_a, uint32_t thr_ab, uint32_t thr_abc, uint64_t seed,
+                        void* stream) {
+  if (!keys || scale == 0 || scale > 31 || e_end < e_begin) return SCAPH_STATUS_INVALID;
+  if (e_end == e_begin) return 0;
+  const scaph_spec::PermKey pk = scaph_spec::make_perm_key(seed, scale);
+  const uint64_t count = e_end - e_begin;
+  hipLaunchKernelGGL(k_rmat_keys, dim3(grid_for(count, kBlock * 8, 65536)), dim3(kBlock), 0,
+                     as_stream(stream), keys, e_begin, count, scale, thr_a, thr_ab, thr_abc, seed,
+                     pk);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, uint32_t n,
+                        int transpose, uint64_t* offsets, uint32_t* ngh, uint32_t* degree,
+                        uint64_t* num_edges, void* stream) {
+  if (!keys || !keys_tmp || !offsets || !ngh || !degree || !num_edges || n == 0)
+    return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  const uint32_t bits = bits_for(n);
+  SCAPH_TRY(hipMemsetAsync(degree, 0, (size_t)n * 4, st));
+  if (num_keys == 0) {
+    SCAPH_TRY(hipMemsetAsync(offsets, 0, ((size_t)n + 1) * 8, st));
+    SCAPH_TRY(hipStreamSynchronize(st));
+    *num_edges = 0;
+    return 0;
+  }
+  hipLaunchKernelGGL(k_pack, dim3(grid_for(num_keys, kBlock * 8, 65536)), dim3(kBlock), 0, st,
+                     keys, num_keys, bits, transpose, n);
+  SCAPH_LAUNCH_CHECK();
+
+  // sort (ping-pong between keys and keys_tmp)
+  rocprim::double_buffer<uint64_t> db(keys, keys_tmp);
+  size_t sort_bytes = 0;
+  SCAPH_TRY(rocprim::radix_sort_keys(nullptr, sort_bytes, db, (size_t)num_keys, 0u, 2u * bits, st));
+  uint64_t* sorted_count = nullptr;
+  void* tmp = nullptr;
+  size_t uniq_bytes = 0;
+  SCAPH_TRY(rocprim::unique(nullptr, uniq_bytes, db.current(), db.alternate(), sorted_count,
+                            (size_t)num_keys, rocprim::equal_to<uint64_t>(), st));
+  size_t scan_bytes = 0;
+  SCAPH_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, degree, offsets, (uint64_t)0, (size_t)n,
+                                    rocprim::plus<uint64_t>(), st));
+  size_t tmp_bytes = sort_bytes;
+  if (uniq_bytes > tmp_bytes) tmp_bytes = uniq_bytes;
+  if (scan_bytes > tmp_bytes) tmp_bytes = scan_bytes;
+  tmp_bytes = (tmp_bytes + 255) & ~size_t(255);
+  SCAPH_TRY(hipMalloc(&tmp, tmp_bytes + 256));
+  uint64_t* d_count = reinterpret_cast<uint64_t*>(static_cast<char*>(tmp) + tmp_bytes);
+  int rc = 0;
+  do {
+    size_t b = sort_bytes;
+    if ((rc = (int)rocprim::radix_sort_keys(tmp, b, db, (size_t)num_keys, 0u, 2u * bits, st)))
+      break;
+    uint64_t* in = db.current();
+    uint64_t* out = db.alternate();
+    b = uniq_bytes;
+    if ((rc = (int)rocprim::unique(tmp, b, in, out, d_count, (size_t)num_keys,
+                                   rocprim::equal_to<uint64_t>(), st)))
+      break;
+    uint64_t h_count = 0, last = 0;
+    if ((rc = (int)hipMemcpyAsync(&h_count, d_count, 8, hipMemcpyDeviceToHost, st))) break;
+    if ((rc = (int)hipStreamSynchronize(st))) break;
+    if (h_count > 0) {
+      if ((rc = (int)hipMemcpyAsync(&last, out + (h_count - 1), 8, hipMemcpyDeviceToHost, st)))
+        break;
+      if ((rc = (int)hipStreamSynchronize(st))) break;
+      const uint64_t loop_key = ((uint64_t)(n - 1) << bits) | (uint64_t)(n - 1);
+      if (last == loop_key) --h_count;
+    }
+    // `in` is free now: reuse it for the run starts (n u64 <= num_keys u64?)
+    uint64_t* start = in;
+    if ((size_t)n > num_keys) {
+      // tiny graph with more vertices than raw edges: need a separate buffer
+      if ((rc = (int)hipMalloc(reinterpret_cast<void**>(&start), (size_t)n * 8))) break;
+    }
+    if (h_count > 0) {
+      hipLaunchKernelGGL(k_run_starts, dim3(grid_for(h_count, kBlock * 8, 65536)), dim3(kBlock),
+                         0, st, out, h_count, bits, ngh, start);
+      if ((rc = (int)hipGetLastError())) break;
+      hipLaunchKernelGGL(k_run_ends, dim3(grid_for(h_count, kBlock * 8, 65536)), dim3(kBlock), 0,
+                         st, out, h_count, bits, start, degree);
+      if ((rc = (int)hipGetLastError())) break;
+    }
+    b = scan_bytes;
+    if ((rc = (int)rocprim::exclusive_scan(tmp, b, degree, offsets, (uint64_t)0, (size_t)n,
+                                           rocprim::plus<uint64_t>(), st)))
+      break;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, offsets + n, h_count);
+    if ((rc = (int)hipGetLastError())) break;
+    if ((rc = (int)hipStreamSynchronize(st))) break;
+    if (start != in) (void)hipFree(start);
+    *num_edges = h_count;
+  } while (0);
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int scaph_pages_from_csr(const uint64_t* offsets, const uint32_t* ngh, const scaph_page* pages,
+                         const uint64_t* page_ptr, uint32_t num_pages, uint32_t n, void* stream) {
+  if (!offsets || !pages || !page_ptr) return SCAPH_STATUS_INVALID;
+  if (num_pages == 0 || n == 0) return 0;
+  if (num_pages > 65535) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_page_nodes, dim3(grid_for(n, kBlock * 4, 16384)), dim3(kBlock), 0, st,
+                     offsets, pages, page_ptr, num_pages, n);
+  SCAPH_LAUNCH_CHECK();
+  if (ngh) {
+    hipLaunchKernelGGL(k_page_edges, dim3(64, num_pages), dim3(kBlock), 0, st, offsets, ngh, pages,
+                       page_ptr);
+    SCAPH_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// app.h — the per-app interface of the reference (src/app.cuh:72-135,
+// app/bfs.cu:39-117) over the MI355X engine.
+//
+// App<vertex_t,index_t,value_t> keeps the reference's public surface:
+// App(argc, argv) parses the gflags-style command line, load_graph() loads
+// -graphfile (or a -gen spec), run() performs one BFS from -src and prints
+// the reference's per-run banner, gpu_init() is the app's pure-virtual
+// device initialisation and check() its CPU verification.  The work is
+// done by scaph::Engine; only 32-bit vertex/index/value types exist, as in
+// the reference's only instantiation (app/bfs.cu:121-123).
+#ifndef SCAPH_HOST_APP_H
+#define SCAPH_HOST_APP_H
+
+#include <cstdio>
+#include <iostream>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "engine.h"
+
+namespace scaph {
+
+extern double Total_throughput;  // src/app.cuh:49
+
+template <typename vertex_t, typename index_t, typename value_t>
+class App {
+  static_assert(sizeof(vertex_t) == 4 && sizeof(index_t) == 4 && sizeof(value_t) == 4,
+                "the paged layout is 32-bit (src/page.h Node<u32,u32>)");
+
+ public:
+  App(int argc, char** argv) {
+    std::vector<std::string> args;
+    for (int i = 1; i < argc; ++i) args.push_back(argv[i]);
+    std::string err;
+    if (!parse_config(args, &cfg, &positional, &err)) throw Error(SCAPH_STATUS_INVALID, err);
+    engine = new Engine(cfg);
+    num_stream = cfg.num_stream;
+    num_blks = cfg.num_blks;
+    num_thds = cfg.num_thds;
+    pagesize = (vertex_t)((uint64_t)cfg.pagesize << 20);
+    chunksize = pagesize / 32;
+    threshold = cfg.threshold * chunksize;
+    gpu_memsize = (int64_t)cfg.max_memory << 20;
+  }
+  explicit App(const Config& c) : cfg(c) {
+    engine = new Engine(cfg);
+    pagesize = (vertex_t)((uint64_t)cfg.pagesize << 20);
+    chunksize = pagesize / 32;
+    threshold = cfg.threshold * chunksize;
+  }
+  virtual ~App() { delete engine; }
+  App(const App&) = delete;
+  App& operator=(const App&) = delete;
+
+  // src/app.cuh:267-345
+  void load_graph() {
+    engine->load_from_config();
+    vert_count = engine->num_vertices();
+    edge_count = (index_t)engine->num_edges();
+    list_size = (int)engine->pages().size();
+    pool_size = engine->pool_size();
+    pagesize = (vertex_t)engine->page_words();
+    std::printf("|========================Dataset Information========================|\n");
+    std::printf("|                       graph name      : %-26s|\n",
+                cfg.graphfile.empty() ? cfg.gen.c_str() : cfg.graphfile.c_str());
+    std::printf("|                       graph nodes     : %-26u|\n", (unsigned)vert_count);
+    std::printf("|                       graph edges     : %-26llu|\n",
+                (unsigned long long)engine->num_edges());
+  }
+
+  // src/app.cuh:220-265: one traversal, timed, with the reference banner.
+  void run() {
+    gpu_init();
+    last = engine->bfs(source);
+    const double secs = last.ms / 1e3;
+    const double gteps_ref = secs > 0 ? (double)engine->num_edges() / secs / 1e9 : 0.0;
+    const double gteps = secs > 0 ? (double)last.m_reached / secs / 1e9 : 0.0;
+    std::printf("|                         BFS costs     : %.8f seconds        |\n", secs);
+    std::printf("|                         Throughput    : %.8f GTEPS          |\n", gteps_ref);
+    std::printf("|                 Traversed-edge rate   : %.8f GTEPS          |\n", gteps);
+    Total_throughput += gteps_ref;
+    if (cfg.check) {
+      h_value.resize(vert_count);
+      engine->levels_to_host(h_value.data());
+    }
+  }
+
+  void iter_init() {}  // per-level bookkeeping lives inside Engine::bfs
+  virtual void gpu_init() = 0;
+  virtual bool check() { return true; }
+
+  Config cfg;
+  std::vector<std::string> positional;
+  Engine* engine = nullptr;
+  RunStats last;
+  uint32_t source = 0;
+  int64_t gpu_memsize = 0;
+  vertex_t vert_count = 0;
+  index_t edge_count = 0;
+  vertex_t pagesize = 0, chunksize = 0, threshold = 0;
+  int list_size = 0, pool_size = 0;
+  int num_blks = 256, num_thds = 256, num_stream = 5;
+  std::vector<value_t> h_value;
+};
+
+// The BFS plugin (app/bfs.cu:39-117).
+template <typename vertex_t, typename index_t, typename value_t>
+class bfs : public App<vertex_t, index_t, value_t> {
+ public:
+  using Base = App<vertex_t, index_t, value_t>;
+  bfs(int argc, char** argv) : Base(argc, argv) {}
+  explicit bfs(const Config& c) : Base(c) {}
+
+  // bfs_init + setsrc (app/bfs.cu:20-37, 58-65) run inside Engine::bfs;
+  // here the source is chosen and checked.
+  void gpu_init() override {
+    if (this->cfg.src < 0 || (uint64_t)this->cfg.src >= this->engine->num_vertices())
+      throw Error(SCAPH_STATUS_INVALID, "-src is out of range");
+    this->source = (uint32_t)this->cfg.src;
+  }
+
+  // app/bfs.cu:67-117: CPU BFS and an element-wise comparison.  Compared in
+  // the reference's capped form (63 = unreached) and in exact form.
+  bool check() override {
+    const HostCsr* g = this->engine->host_csr();
+    if (!g) {
+      std::cout << "check skipped: no host copy of the graph" << std::endl;
+      return false;
+    }
+    if (this->h_value.size() != this->vert_count) {
+      this->h_value.resize(this->vert_count);
+      this->engine->levels_to_host(this->h_value.data());
+    }
+    std::vector<uint32_t> cpu;
+    cpu_bfs(*g, this->source, 63u, &cpu);
+    uint64_t errors = 0;
+    for (vertex_t vid = 0; vid < this->vert_count; ++vid) {
+      const uint32_t gpu = compat63(this->h_value[vid]);
+      if (gpu != cpu[vid]) {
+        if (errors < 10)
+          std::cout << "vid: " << vid << " gpu: " << gpu << " cpu: " << cpu[vid] << std::endl;
+        ++errors;
+      }
+    }
+    std::vector<uint32_t> exact;
+    cpu_bfs(*g, this->source, SCAPH_UNREACHED, &exact);
+    for (vertex_t vid = 0; vid < this->vert_count; ++vid)
+      if (exact[vid] != this->h_value[vid]) ++errors;
+    std::cout << "total errors: " << errors << std::endl;
+    return errors == 0;
+  }
+};
+
+}  // namespace scaph
+
+#endif

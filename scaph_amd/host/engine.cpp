@@ -1,0 +1,586 @@
+// engine.cpp — resident BFS engine: graph upload, the level loop with the
+// push/pull direction switch, validation and the CPU check.
+#include "engine.h"
+
+#include <algorithm>
+#include <cstring>
+#include <deque>
+
+#include "flags.h"
+#include "paged.h"
+
+namespace scaph {
+
+double Total_throughput = 0;
+
+namespace {
+
+constexpr uint64_t kAlignWords = 64;  // page buffers start on 256-byte boundaries
+constexpr size_t kMaxTimedLevels = 1024;
+
+uint64_t align_words(uint64_t w) { return (w + kAlignWords - 1) / kAlignWords * kAlignWords; }
+
+size_t bitmap_words(uint32_t n) { return (((size_t)n + 63) / 64) * 2; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ flags
+
+Config& global_flags() {
+  static Config c;
+  static bool registered = false;
+  if (!registered) {
+    registered = true;
+    FlagRegistry& r = FlagRegistry::global();
+    r.add("pagesize", FlagType::kInt32, &c.pagesize, "subgraph pagesize");
+    r.add("max_memory", FlagType::kInt32, &c.max_memory, "gpu maximum memory size");
+    r.add("multi", FlagType::kInt32, &c.multi, "max entry times");
+    r.add("num_stream", FlagType::kInt32, &c.num_stream, "gpu stream number");
+    r.add("num_blks", FlagType::kInt32, &c.num_blks, "kernel mapping blocks");
+    r.add("num_thds", FlagType::kInt32, &c.num_thds, "kernel mapping threads");
+    r.add("threshold", FlagType::kInt32, &c.threshold, "threshold of compress point");
+    r.add("check", FlagType::kBool, &c.check, "check result");
+    r.add("tricknum", FlagType::kInt32, &c.tricknum, "trick number");
+    r.add("src", FlagType::kInt64, &c.src, "traversal start point");
+    r.add("graphfile", FlagType::kString, &c.graphfile, "graphfile path");
+    r.add("gen", FlagType::kString, &c.gen, "generator spec used when graphfile is empty");
+    r.add("alpha", FlagType::kDouble, &c.alpha, "push->pull when m_f > m_u/alpha");
+    r.add("beta", FlagType::kDouble, &c.beta, "pull->push when n_f shrinks below n/beta");
+    r.add("schedule", FlagType::kString, &c.schedule, "fixed|push|pull|learned");
+    r.add("model", FlagType::kString, &c.model, "learned schedule weights");
+    r.add("rounds", FlagType::kInt32, &c.rounds, "timed rounds");
+    r.add("device", FlagType::kInt32, &c.device, "HIP device ordinal");
+    r.add("levels_out", FlagType::kString, &c.levels_out, "write u32 levels to this file");
+    r.add("compat63", FlagType::kBool, &c.compat63, "report levels capped at 63 like the reference");
+    r.add("trace", FlagType::kString, &c.trace, "per-iteration JSONL trace file");
+    r.add("build_pull", FlagType::kBool, &c.build_pull, "keep transposed pages for pull");
+    r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");
+  }
+  return c;
+}
+
+bool parse_config(const std::vector<std::string>& args, Config* out,
+                  std::vector<std::string>* positional, std::string* err) {
+  Config& g = global_flags();
+  g = Config();
+  std::vector<std::string> pos;
+  bool help = false;
+  if (!parse_flag_vector(args, &pos, err, &help)) return false;
+  if (help) {
+    if (err) *err = FlagRegistry::global().usage();
+    return false;
+  }
+  if (g.pagesize <= 0 || g.pagesize > 1023) {
+    if (err) *err = "ERROR: -pagesize must be in 1..1023 (Mi words)";
+    return false;
+  }
+  if (g.threshold < 0 || g.threshold > 16) {
+    // the compacted copy must fit the half-page tail of each pinned page
+    // buffer (src/compress.h:47 with src/graph.h:100)
+    if (err) *err = "ERROR: -threshold must be in 0..16 (1/32 pages)";
+    return false;
+  }
+  if (g.num_stream < 1 || g.num_stream > 64 || g.multi < 1 || g.multi > 64) {
+    if (err) *err = "ERROR: -num_stream and -multi must be in 1..64";
+    return false;
+  }
+  if (g.page_words < 0 || (g.page_words > 0 && g.page_words < 64)) {
+    if (err) *err = "ERROR: -page_words must be 0 or at least 64";
+    return false;
+  }
+  if (g.alpha <= 0 || g.beta <= 0) {
+    if (err) *err = "ERROR: -alpha and -beta must be positive";
+    return false;
+  }
+  if (g.schedule != "fixed" && g.schedule != "push" && g.schedule != "pull" &&
+      g.schedule != "learned") {
+    if (err) *err = "ERROR: -schedule must be fixed, push, pull or learned";
+    return false;
+  }
+  *out = g;
+  if (positional) *positional = pos;
+  return true;
+}
+
+// ----------------------------------------------------------------- engine
+
+Engine::Engine(const Config& cfg)
+    : cfg_(cfg), P_(cfg.page_words > 0 ? (uint64_t)cfg.page_words : (uint64_t)cfg.pagesize << 20) {
+  H_ERR(scaph_init(cfg.device));
+  main_.reset(new Stream());
+  hcount_.alloc(4096);
+}
+
+Engine::~Engine() = default;
+
+void Engine::reset_graph() {
+  n_ = 0;
+  m_ = 0;
+  host_ = HostCsr();
+  pages_out_.clear();
+  pages_in_.clear();
+  pool_out_.reset();
+  pool_in_.reset();
+  dpages_out_.reset();
+  dptr_out_.reset();
+  dpages_in_.reset();
+  dptr_in_.reset();
+  deg_out_.reset();
+  deg_in_.reset();
+  static_visited_.reset();
+  h_deg_out_.clear();
+  g_out_ = scaph_graph_desc{};
+  g_in_ = scaph_graph_desc{};
+  has_in_ = false;
+  resident_ = true;
+  paged_.reset();
+}
+
+uint64_t Engine::device_bytes() const {
+  return pool_out_.bytes() + pool_in_.bytes() + deg_out_.bytes() + deg_in_.bytes() +
+         static_visited_.bytes() + level_.bytes() + visited_.bytes() + front_[0].bytes() +
+         front_[1].bytes() + queue_[0].bytes() + queue_[1].bytes() + ws_.bytes();
+}
+
+// Budget of src/app.cuh:281-286, 323-324: max_memory MiB minus the vertex
+// arrays (value, stat, degree) and the per-page lists, divided into P-word
+// page slots.  The pull expand's transposed pages are budgeted too.
+void Engine::plan_residency() {
+  int64_t budget = (int64_t)cfg_.max_memory << 20;
+  const int64_t list = (int64_t)pages_out_.size();
+  budget -= (int64_t)n_ * (4 + 1 + 4);
+  budget -= list * (4 + 4 + 16);
+  const int64_t slot = (int64_t)P_ * 4;
+  if (budget <= slot) throw Error(SCAPH_STATUS_INVALID, "vertex data out of gpu memory. exit!");
+  int64_t pool = budget / slot;
+  pool_size_ = (int)std::min<int64_t>(pool, list);
+  resident_ = pool_size_ >= list;
+  const int64_t need_in = (int64_t)pages_in_.size();
+  has_in_ = cfg_.build_pull && resident_ && need_in > 0 && pool - list >= need_in;
+}
+
+void Engine::upload_pages(const HostCsr& g, bool weighted, std::vector<scaph_page>* pages,
+                          DevBuf* pool, DevBuf* d_pages, DevBuf* d_ptr, scaph_graph_desc* desc) {
+  std::vector<uint64_t> offs(pages->size());
+  uint64_t total = 0;
+  for (size_t p = 0; p < pages->size(); ++p) {
+    offs[p] = total;
+    total += align_words(page_used_words((*pages)[p], weighted));
+  }
+  pool->alloc(std::max<uint64_t>(total, 1) * 4);
+  // stage through a pinned buffer, a few pages at a time
+  const uint64_t stage_words = std::max<uint64_t>(P_, 1ull << 22);
+  PinnedBuf stage;
+  stage.alloc(stage_words * 4);
+  uint32_t* host = stage.get<uint32_t>();
+  void* st = main_->get();
+  for (size_t p = 0; p < pages->size(); ++p) {
+    const uint64_t used = page_used_words((*pages)[p], weighted);
+    fill_page(g, (*pages)[p], weighted, host);
+    H_ERR(scaph_copy_h2d_async(pool->get<uint32_t>() + offs[p], host, used * 4, st));
+    main_->sync();
+  }
+  std::vector<uint64_t> ptr(pages->size());
+  for (size_t p = 0; p < pages->size(); ++p)
+    ptr[p] = reinterpret_cast<uint64_t>(pool->get<uint32_t>() + offs[p]);
+  d_pages->alloc(std::max<size_t>(pages->size(), 1) * sizeof(scaph_page));
+  d_ptr->alloc(std::max<size_t>(pages->size(), 1) * 8);
+  H_ERR(scaph_copy_h2d_async(d_pages->get(), pages->data(), pages->size() * sizeof(scaph_page), st));
+  H_ERR(scaph_copy_h2d_async(d_ptr->get(), ptr.data(), ptr.size() * 8, st));
+  main_->sync();
+  desc->page_ptr = d_ptr->get<const uint64_t>();
+  desc->pages = d_pages->get<const scaph_page>();
+  desc->num_pages = (uint32_t)pages->size();
+  desc->num_vertices = g.n;
+}
+
+void Engine::finish_setup(const std::vector<uint32_t>* in_deg_host) {
+  void* st = main_->get();
+  if (has_in_) {
+    static_visited_.alloc(bitmap_words(n_) * 4);
+    if (!deg_in_.get()) {
+      deg_in_.alloc((size_t)n_ * 4);
+      H_ERR(scaph_copy_h2d_async(deg_in_.get(), in_deg_host->data(), (size_t)n_ * 4, st));
+    }
+    H_ERR(scaph_bitmap_from_zero_degree(static_visited_.get<uint32_t>(), deg_in_.get<uint32_t>(),
+                                        n_, st));
+  }
+  alloc_state();
+  main_->sync();
+}
+
+void Engine::alloc_state() {
+  const size_t words = bitmap_words(n_);
+  level_.alloc((size_t)n_ * 4);
+  visited_.alloc(words * 4);
+  front_[0].alloc(words * 4);
+  front_[1].alloc(words * 4);
+  queue_[0].alloc(std::max<size_t>(n_, 1) * 4);
+  queue_[1].alloc(std::max<size_t>(n_, 1) * 4);
+  counters_.alloc(4 * sizeof(scaph_level_counters));
+  uint64_t max_edges = m_;
+  if (!resident_) max_edges = std::max<uint64_t>(max_edges, P_);
+  ws_bytes_ = scaph_push_workspace_bytes(std::max<uint32_t>(n_, 1), std::max<uint64_t>(max_edges, 1));
+  ws_.alloc(ws_bytes_);
+  const size_t np = std::max<size_t>(pages_out_.size(), 1);
+  page_nodes_.alloc(np * 4);
+  page_datas_.alloc(np * 4);
+  totals_.alloc(64);
+}
+
+void Engine::load_csr(HostCsr g, bool keep_host) {
+  reset_graph();
+  if (g.offsets.size() != (size_t)g.n + 1) throw Error(SCAPH_STATUS_INVALID, "malformed CSR");
+  n_ = g.n;
+  m_ = g.num_edges();
+  std::string err;
+  if (!page_cuts(g, P_, false, &pages_out_, &err)) throw Error(SCAPH_STATUS_INVALID, err);
+  HostCsr t;
+  std::vector<uint32_t> in_deg;
+  if (cfg_.build_pull) {
+    t = transpose(g);
+    if (!page_cuts(t, P_, false, &pages_in_, &err)) throw Error(SCAPH_STATUS_INVALID, err);
+  }
+  plan_residency();
+  h_deg_out_.resize(n_);
+  for (uint32_t v = 0; v < n_; ++v) h_deg_out_[v] = g.degree(v);
+  deg_out_.alloc(std::max<size_t>(n_, 1) * 4);
+  H_ERR(scaph_copy_h2d_async(deg_out_.get(), h_deg_out_.data(), (size_t)n_ * 4, main_->get()));
+  if (resident_) {
+    upload_pages(g, false, &pages_out_, &pool_out_, &dpages_out_, &dptr_out_, &g_out_);
+    if (has_in_) {
+      upload_pages(t, false, &pages_in_, &pool_in_, &dpages_in_, &dptr_in_, &g_in_);
+      in_deg.resize(n_);
+      for (uint32_t v = 0; v < n_; ++v) in_deg[v] = t.degree(v);
+    }
+    finish_setup(&in_deg);
+    if (keep_host) host_ = std::move(g);
+  } else {
+    host_ = std::move(g);
+    finish_setup(nullptr);
+    setup_paged();
+  }
+}
+
+void Engine::load_files(const std::string& base) {
+  // fetch_graph (src/graph.h:24-108): the flat CSR files plus graph.info and
+  // page.info; page buffers are re-derived from the flat CSR and checked
+  // against page.info so that a stale page set is refused.
+  HostCsr g;
+  std::string err;
+  if (!read_csr_files(base, &g, &err)) throw Error(SCAPH_STATUS_INVALID, err);
+  GraphInfo info;
+  std::vector<scaph_page> listed;
+  if (read_graph_info(base, &info, &err)) {
+    if (info.nodenum != g.n || (uint32_t)info.edgenum != (uint32_t)g.num_edges())
+      throw Error(SCAPH_STATUS_INVALID, "graph.info does not match the CSR files");
+    P_ = info.pagesize;  // src/graph.h:86 takes the page size from graph.info
+    if (!read_page_info(base, &listed, &err)) throw Error(SCAPH_STATUS_INVALID, err);
+    std::vector<scaph_page> cuts;
+    if (!page_cuts(g, P_, false, &cuts, &err)) throw Error(SCAPH_STATUS_INVALID, err);
+    if (cuts.size() != listed.size() ||
+        std::memcmp(cuts.data(), listed.data(), cuts.size() * sizeof(scaph_page)) != 0)
+      throw Error(SCAPH_STATUS_INVALID, "page.info does not match the page cut of the CSR");
+  }
+  load_csr(std::move(g), true);
+}
+
+void Engine::load_from_config() {
+  if (!cfg_.graphfile.empty()) {
+    load_files(cfg_.graphfile);
+    return;
+  }
+  if (cfg_.gen.empty()) throw Error(SCAPH_STATUS_INVALID, "no -graphfile and no -gen given");
+  if (cfg_.gen.rfind("rmat-device:", 0) == 0) {
+    HostCsr dummy;
+    std::string err;
+    // parse the same keys as the host generator
+    RmatParams p;
+    std::string rest = cfg_.gen.substr(12);
+    size_t pos = 0;
+    while (pos < rest.size()) {
+      size_t comma = rest.find(',', pos);
+      std::string item = rest.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos);
+      size_t eq = item.find('=');
+      if (eq != std::string::npos) {
+        std::string k = item.substr(0, eq), v = item.substr(eq + 1);
+        if (k == "scale") p.scale = (uint32_t)std::stoul(v);
+        else if (k == "ef") p.edge_factor = (uint32_t)std::stoul(v);
+        else if (k == "a") p.a = std::stod(v);
+        else if (k == "b") p.b = std::stod(v);
+        else if (k == "c") p.c = std::stod(v);
+        else if (k == "seed") p.seed = std::stoull(v);
+      }
+      if (comma == std::string::npos) break;
+      pos = comma + 1;
+    }
+    gen_rmat_device(p, cfg_.check);
+    return;
+  }
+  HostCsr g;
+  std::string err;
+  if (!generate_from_spec(cfg_.gen, &g, &err)) throw Error(SCAPH_STATUS_INVALID, err);
+  load_csr(std::move(g), true);
+}
+
+void Engine::gen_rmat_device(const RmatParams& p, bool keep_host_csr) {
+  reset_graph();
+  if (p.scale == 0 || p.scale > 31) throw Error(SCAPH_STATUS_INVALID, "rmat scale must be 1..31");
+  void* st = main_->get();
+  const uint32_t n = (uint32_t)(1ull << p.scale);
+  const uint64_t raw = (uint64_t)p.edge_factor << p.scale;
+  uint32_t ta, tab, tabc;
+  rmat_thresholds(p, &ta, &tab, &tabc);
+  n_ = n;
+  DevBuf keys(raw * 8), tmp(raw * 8), offs(((size_t)n + 1) * 8), ngh(std::max<uint64_t>(raw, 1) * 4);
+  std::vector<uint64_t> h_off((size_t)n + 1);
+  std::string err;
+  for (int dir = 0; dir < 2; ++dir) {
+    if (dir == 1 && !cfg_.build_pull) break;
+    DevBuf deg((size_t)n * 4);
+    H_ERR(scaph_gen_rmat_keys(keys.get<uint64_t>(), 0, raw, p.scale, ta, tab, tabc, p.seed, st));
+    uint64_t E = 0;
+    H_ERR(scaph_csr_from_keys(keys.get<uint64_t>(), tmp.get<uint64_t>(), raw, n, dir,
+                              offs.get<uint64_t>(), ngh.get<uint32_t>(), deg.get<uint32_t>(), &E,
+                              st));
+    H_ERR(scaph_copy_d2h_async(h_off.data(), offs.get(), h_off.size() * 8, st));
+    main_->sync();
+    if (dir == 0) m_ = E;
+    else if (E != m_) throw Error(SCAPH_STATUS_INVALID, "transposed edge count differs");
+    std::vector<scaph_page>& pages = dir == 0 ? pages_out_ : pages_in_;
+    if (!page_cuts_raw(h_off.data(), n, P_, false, &pages, &err))
+      throw Error(SCAPH_STATUS_INVALID, err);
+    if (dir == 0) {
+      if (keep_host_csr) {
+        host_.n = n;
+        host_.offsets = h_off;
+        host_.ngh.resize(E);
+        H_ERR(scaph_copy_d2h_async(host_.ngh.data(), ngh.get(), E * 4, st));
+        main_->sync();
+      }
+      // page budget is decided once the transposed cut is known (below)
+    }
+    if (dir == 1 || !cfg_.build_pull) {
+      plan_residency();
+      if (!resident_)
+        throw Error(SCAPH_STATUS_INVALID,
+                    "device-generated graphs must fit the -max_memory budget (raise it or load "
+                    "from files for out-of-core runs)");
+    }
+    // page buffers packed in one allocation
+    DevBuf& pool = dir == 0 ? pool_out_ : pool_in_;
+    DevBuf& dpages = dir == 0 ? dpages_out_ : dpages_in_;
+    DevBuf& dptr = dir == 0 ? dptr_out_ : dptr_in_;
+    scaph_graph_desc& desc = dir == 0 ? g_out_ : g_in_;
+    uint64_t total = 0;
+    std::vector<uint64_t> ptr(pages.size());
+    for (size_t i = 0; i < pages.size(); ++i) {
+      ptr[i] = total;
+      total += align_words(page_used_words(pages[i], false));
+    }
+    pool.alloc(std::max<uint64_t>(total, 1) * 4);
+    for (auto& x : ptr) x = reinterpret_cast<uint64_t>(pool.get<uint32_t>() + x);
+    dpages.alloc(std::max<size_t>(pages.size(), 1) * sizeof(scaph_page));
+    dptr.alloc(std::max<size_t>(pages.size(), 1) * 8);
+    H_ERR(scaph_copy_h2d_async(dpages.get(), pages.data(), pages.size() * sizeof(scaph_page), st));
+    H_ERR(scaph_copy_h2d_async(dptr.get(), ptr.data(), ptr.size() * 8, st));
+    H_ERR(scaph_pages_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(),
+                               dpages.get<scaph_page>(), dptr.get<uint64_t>(),
+                               (uint32_t)pages.size(), n, st));
+    main_->sync();
+    desc.page_ptr = dptr.get<const uint64_t>();
+    desc.pages = dpages.get<const scaph_page>();
+    desc.num_pages = (uint32_t)pages.size();
+    desc.num_vertices = n;
+    if (dir == 0) {
+      deg_out_ = std::move(deg);
+      h_deg_out_.resize(n);
+      H_ERR(scaph_copy_d2h_async(h_deg_out_.data(), deg_out_.get(), (size_t)n * 4, st));
+      main_->sync();
+    } else {
+      deg_in_ = std::move(deg);
+    }
+  }
+  if (!has_in_) {
+    pool_in_.reset();
+    dpages_in_.reset();
+    dptr_in_.reset();
+    deg_in_.reset();
+    g_in_ = scaph_graph_desc{};
+  }
+  keys.reset();
+  tmp.reset();
+  ngh.reset();
+  offs.reset();
+  finish_setup(nullptr);
+}
+
+uint32_t Engine::out_degree(uint32_t v) {
+  if (v >= n_) throw Error(SCAPH_STATUS_INVALID, "vertex out of range");
+  if (!h_deg_out_.empty()) return h_deg_out_[v];
+  uint32_t d = 0;
+  H_ERR(scaph_copy_d2h_async(&d, deg_out_.get<uint32_t>() + v, 4, main_->get()));
+  main_->sync();
+  return d;
+}
+
+uint32_t Engine::in_degree(uint32_t v) {
+  if (v >= n_ || !deg_in_.get()) return 0;
+  uint32_t d = 0;
+  H_ERR(scaph_copy_d2h_async(&d, deg_in_.get<uint32_t>() + v, 4, main_->get()));
+  main_->sync();
+  return d;
+}
+
+// Beamer's direction rule (GAP's form): push -> pull when the frontier's
+// edges exceed the unexplored edges / alpha; pull -> push once the
+// frontier shrinks and holds at most n / beta vertices.
+Dir Engine::beamer(const SchedState& s) const {
+  if (!has_in_) return Dir::kPush;
+  if (cfg_.schedule == "push") return Dir::kPush;
+  if (cfg_.schedule == "pull") return Dir::kPull;
+  if (s.cur == Dir::kPush)
+    return (double)s.m_f > (double)s.m_u / cfg_.alpha ? Dir::kPull : Dir::kPush;
+  const bool shrinking = s.n_f < s.n_f_prev;
+  return (shrinking && (double)s.n_f <= (double)s.n / cfg_.beta) ? Dir::kPush : Dir::kPull;
+}
+
+RunStats Engine::bfs(uint32_t src) {
+  if (n_ == 0) throw Error(SCAPH_STATUS_INVALID, "no graph loaded");
+  if (src >= n_) throw Error(SCAPH_STATUS_INVALID, "source vertex out of range");
+  return resident_ ? bfs_resident(src) : bfs_paged(src);
+}
+
+RunStats Engine::bfs_resident(uint32_t src) {
+  void* st = main_->get();
+  RunStats rs;
+  rs.src = src;
+  while (ev_.size() < 2) ev_.emplace_back(new Event());
+  const uint64_t deg_src = out_degree(src);
+  auto* cnt = counters_.get<scaph_level_counters>();
+  auto* hc = hcount_.get<scaph_level_counters>();
+  uint32_t* level = level_.get<uint32_t>();
+  uint32_t* visited = visited_.get<uint32_t>();
+  const uint32_t* deg = deg_out_.get<uint32_t>();
+
+  ev_[0]->record(st);
+  H_ERR(scaph_bfs_init(level, visited, has_in_ ? static_visited_.get<uint32_t>() : nullptr, n_,
+                       src, queue_[0].get<uint32_t>(), cnt, st));
+  Dir dir = Dir::kPush;
+  uint64_t n_f = 1, m_f = deg_src, m_u = m_ - deg_src, n_f_prev = 0;
+  int qa = 0, fa = 0;
+  bool queue_ok = true, front_ok = false;
+  uint32_t cur = 0;
+  rs.n_reached = 1;
+  rs.m_reached = deg_src;
+  std::vector<size_t> lev_ev;
+  while (n_f > 0) {
+    SchedState ss{dir, cur, n_, m_, n_f, m_f, m_u, n_f_prev,
+                  rs.levels.empty() ? nullptr : &rs.levels.back()};
+    Dir d = sched_ ? sched_(ss) : beamer(ss);
+    if (d == Dir::kPull && !has_in_) d = Dir::kPush;
+    if (d == Dir::kPull && !front_ok) {
+      H_ERR(scaph_frontier_queue_to_bitmap(queue_[qa].get<uint32_t>(), (uint32_t)n_f,
+                                           front_[fa].get<uint32_t>(), n_, st));
+      front_ok = true;
+    }
+    if (d == Dir::kPush && !queue_ok) {
+      H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt + 1), 0u, 8, st));
+      H_ERR(scaph_frontier_bitmap_to_queue(front_[fa].get<uint32_t>(), n_,
+                                           queue_[qa].get<uint32_t>(), cnt + 1, st));
+      queue_ok = true;
+    }
+    H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, 8, st));
+    if (d == Dir::kPush) {
+      H_ERR(scaph_bfs_push(&g_out_, deg, level, visited, queue_[qa].get<uint32_t>(), (uint32_t)n_f,
+                           m_f, queue_[qa ^ 1].get<uint32_t>(), cnt, cur, ws_.get(), ws_bytes_,
+                           st));
+      qa ^= 1;
+      queue_ok = true;
+      front_ok = false;
+    } else {
+      H_ERR(scaph_bfs_pull(&g_in_, deg, level, visited, front_[fa].get<uint32_t>(),
+                           front_[fa ^ 1].get<uint32_t>(), cnt, cur, st));
+      fa ^= 1;
+      front_ok = true;
+      queue_ok = false;
+    }
+    if (rs.levels.size() < kMaxTimedLevels) {
+      const size_t e = 2 + rs.levels.size();
+      while (ev_.size() <= e) ev_.emplace_back(new Event());
+      ev_[e]->record(st);
+      lev_ev.push_back(e);
+    }
+    H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
+    main_->sync();
+    LevelRecord rec;
+    rec.level = cur;
+    rec.dir = (int)d;
+    rec.n_f = n_f;
+    rec.m_f = m_f;
+    rec.m_u = m_u;
+    rec.n_next = hc->n_next;
+    rec.m_next = hc->m_next;
+    rec.scanned = hc->scanned;
+    rs.levels.push_back(rec);
+    n_f_prev = n_f;
+    n_f = hc->n_next;
+    m_f = hc->m_next;
+    m_u = m_u > m_f ? m_u - m_f : 0;
+    rs.n_reached += n_f;
+    rs.m_reached += m_f;
+    dir = d;
+    ++cur;
+  }
+  ev_[1]->record(st);
+  ev_[1]->sync();
+  rs.ms = Event::elapsed_ms(*ev_[0], *ev_[1]);
+  size_t prev = 0;
+  for (size_t i = 0; i < lev_ev.size(); ++i) {
+    const Event& a = i == 0 ? *ev_[0] : *ev_[lev_ev[i - 1]];
+    rs.levels[i].ms = Event::elapsed_ms(a, *ev_[lev_ev[i]]);
+    (void)prev;
+  }
+  rs.depth = cur;
+  return rs;
+}
+
+void Engine::levels_to_host(uint32_t* out) const {
+  H_ERR(scaph_copy_d2h_async(out, level_.get(), (size_t)n_ * 4, main_->get()));
+  main_->sync();
+}
+
+std::array<uint64_t, 2> Engine::validate(uint32_t src) {
+  if (!resident_) throw Error(SCAPH_STATUS_INVALID, "validation needs resident pages");
+  DevBuf out(64);
+  H_ERR(scaph_bfs_validate(&g_out_, has_in_ ? &g_in_ : nullptr, level_.get<uint32_t>(), src,
+                           out.get<unsigned long long>(), main_->get()));
+  std::array<uint64_t, 2> r{};
+  H_ERR(scaph_copy_d2h_async(r.data(), out.get(), 16, main_->get()));
+  main_->sync();
+  return r;
+}
+
+void cpu_bfs(const HostCsr& g, uint32_t src, uint32_t inf, std::vector<uint32_t>* out) {
+  out->assign(g.n, inf);
+  if (src >= g.n) return;
+  std::vector<uint32_t> cur, next;
+  (*out)[src] = 0;
+  cur.push_back(src);
+  while (!cur.empty()) {
+    for (uint32_t u : cur) {
+      const uint32_t lu = (*out)[u];
+      for (uint64_t k = g.offsets[u]; k < g.offsets[(size_t)u + 1]; ++k) {
+        const uint32_t v = g.ngh[k];
+        if (lu + 1 < (*out)[v]) {
+          (*out)[v] = lu + 1;
+          next.push_back(v);
+        }
+      }
+    }
+    cur.swap(next);
+    next.clear();
+  }
+}
+
+}  // namespace scaph

@@ -1,0 +1,189 @@
+// engine.h — the host BFS engine: graph residency, the per-level loop and
+// the direction schedule.  It plays the role of the reference's App<>
+// (src/app.cuh:72-135): App<> in app.h is a thin template facade over it
+// that keeps the reference's method names.
+#ifndef SCAPH_HOST_ENGINE_H
+#define SCAPH_HOST_ENGINE_H
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gen.h"
+#include "graph_io.h"
+#include "hip_abi.h"
+
+namespace scaph {
+
+// Flags of the reference (src/app.cuh:38-52 values, 57-70 names) plus the
+// additions this engine needs.  FLAGS_threshold defaults to 3, the value
+// app/bfs.cu:129 forces before constructing the app.
+struct Config {
+  int32_t pagesize = 10;      // page size in Mi u32 words
+  int32_t max_memory = 4096;  // device budget in MiB
+  int32_t multi = 1;          // re-execution ranks of cached pages
+  int32_t num_stream = 5;     // exec streams for paged mode
+  int32_t num_blks = 256;     // accepted; grids are sized from the work
+  int32_t num_thds = 256;     // accepted; blocks are 256 threads (4 wave64)
+  int32_t threshold = 3;      // cold-page compaction threshold, in 1/32 pages
+  bool check = false;
+  int32_t tricknum = 0;       // accepted; no effect (no CU-occupying spin kernel)
+  int64_t src = 0;
+  std::string graphfile;
+  // additions
+  std::string gen;               // generator spec used when graphfile is empty
+  double alpha = 14.0;           // push -> pull when m_f > m_u / alpha
+  double beta = 24.0;            // pull -> push when n_f shrinks below n / beta
+  std::string schedule = "fixed";  // fixed | push | pull | learned
+  std::string model;             // learned-schedule weights file
+  int32_t rounds = 10;           // timed rounds of the CLI (app/bfs.cu:137)
+  int32_t device = 0;
+  std::string levels_out;        // write final levels (u32) here
+  bool compat63 = false;         // print/write levels with the reference's cap of 63
+  std::string trace;             // per-iteration JSONL trace file
+  bool build_pull = true;        // keep transposed pages for the pull expand
+  int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
+};
+
+// Registers the flags above (once) against a process-wide Config.
+Config& global_flags();
+// Resets the global flags to defaults, parses args, returns a copy.
+bool parse_config(const std::vector<std::string>& args, Config* out,
+                  std::vector<std::string>* positional, std::string* err);
+
+enum class Dir : int { kPush = 0, kPull = 1 };
+
+// The per-iteration record — also the learned schedule's feature source.
+struct LevelRecord {
+  uint32_t level = 0;
+  int dir = 0;
+  uint64_t n_f = 0, m_f = 0, m_u = 0;   // frontier before the step
+  uint64_t n_next = 0, m_next = 0;      // reached by the step
+  uint64_t scanned = 0;                 // edges examined
+  float ms = 0;                         // device time of the step
+  std::array<float, 12> page_hist{};    // histogram of per-page active counts
+};
+
+struct RunStats {
+  uint32_t src = 0;
+  double ms = 0;               // t_BFS: init .. levels final on device
+  uint64_t n_reached = 0;      // n_c
+  uint64_t m_reached = 0;      // m_c = sum of out-degrees of reached vertices
+  uint32_t depth = 0;          // number of levels
+  std::vector<LevelRecord> levels;
+  // paged (out-of-core) mode
+  uint64_t page_execs = 0, page_execs_cached = 0, pages_full = 0, pages_compacted = 0;
+  uint64_t bytes_h2d = 0;
+  double h2d_ms = 0;
+};
+
+// Direction policy.  The default is Beamer's rule with the flags' alpha
+// and beta; the learned schedule supplies its own.
+struct SchedState {
+  Dir cur;
+  uint32_t level;
+  uint64_t n, m;                 // graph totals
+  uint64_t n_f, m_f, m_u, n_f_prev;
+  const LevelRecord* last;       // record of the previous step (or null)
+};
+using SchedFn = std::function<Dir(const SchedState&)>;
+
+class Engine {
+ public:
+  explicit Engine(const Config& cfg);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // Graph loading (replaces fetch_graph + load_graph, src/graph.h:24-108,
+  // src/app.cuh:267-345).
+  void load_files(const std::string& base);
+  void load_csr(HostCsr g, bool keep_host = true);
+  void gen_rmat_device(const RmatParams& p, bool keep_host_csr);
+  void load_from_config();  // graphfile, else gen spec
+
+  // One BFS from src.  Levels stay on the device (levels_to_host to read).
+  RunStats bfs(uint32_t src);
+  void levels_to_host(uint32_t* out) const;
+  // Device-side BFS invariant check; returns {edge violations, orphans}.
+  std::array<uint64_t, 2> validate(uint32_t src);
+
+  void set_schedule(SchedFn fn) { sched_ = std::move(fn); }
+  void set_config_schedule(const std::string& s, double alpha, double beta) {
+    cfg_.schedule = s;
+    cfg_.alpha = alpha;
+    cfg_.beta = beta;
+  }
+  Dir beamer(const SchedState& s) const;
+
+  uint32_t num_vertices() const { return n_; }
+  uint64_t num_edges() const { return m_; }
+  uint32_t out_degree(uint32_t v);
+  uint32_t in_degree(uint32_t v);
+  bool resident() const { return resident_; }
+  bool has_pull() const { return has_in_; }
+  const HostCsr* host_csr() const { return host_.n ? &host_ : nullptr; }
+  const std::vector<scaph_page>& pages() const { return pages_out_; }
+  const Config& config() const { return cfg_; }
+  uint64_t page_words() const { return P_; }
+  int pool_size() const { return pool_size_; }
+  uint64_t device_bytes() const;
+  // Device pointers (for the C-ABI tests and the batched/learned parts).
+  const scaph_graph_desc& out_desc() const { return g_out_; }
+  const scaph_graph_desc& in_desc() const { return g_in_; }
+  const uint32_t* d_level() const { return level_.get<uint32_t>(); }
+  const uint32_t* d_degree() const { return deg_out_.get<uint32_t>(); }
+  void* stream() const { return main_->get(); }
+
+ private:
+  struct PagedState;
+  void reset_graph();
+  void upload_pages(const HostCsr& g, bool weighted, std::vector<scaph_page>* pages, DevBuf* pool,
+                    DevBuf* d_pages, DevBuf* d_ptr, scaph_graph_desc* desc);
+  void finish_setup(const std::vector<uint32_t>* in_deg_host);
+  void alloc_state();
+  void plan_residency();
+  void setup_paged();
+  RunStats bfs_resident(uint32_t src);
+  RunStats bfs_paged(uint32_t src);
+  void page_histogram(const uint32_t* nodes, LevelRecord* rec);
+
+  Config cfg_;
+  uint64_t P_;  // page size in words
+  uint32_t n_ = 0;
+  uint64_t m_ = 0;
+  bool resident_ = true;
+  bool has_in_ = false;
+  int pool_size_ = 0;
+  HostCsr host_;                  // kept when available (check(), paged mode)
+  std::vector<scaph_page> pages_out_, pages_in_;
+  DevBuf pool_out_, pool_in_;     // packed resident pages
+  DevBuf dpages_out_, dptr_out_, dpages_in_, dptr_in_;
+  scaph_graph_desc g_out_{}, g_in_{};
+  DevBuf deg_out_, deg_in_, static_visited_;
+  // BFS state
+  DevBuf level_, visited_, front_[2], queue_[2], counters_, ws_, page_nodes_, page_datas_, totals_;
+  size_t ws_bytes_ = 0;
+  std::vector<uint32_t> h_deg_out_;
+  std::unique_ptr<Stream> main_;
+  PinnedBuf hcount_;  // pinned landing zone for per-level counters
+  std::vector<std::unique_ptr<Event>> ev_;
+  SchedFn sched_;
+  std::unique_ptr<PagedState> paged_;
+};
+
+// CPU BFS used by bfs::check() (app/bfs.cu:67-117): FIFO level BFS over the
+// out-edges with levels starting at `inf`; a vertex is relaxed when
+// level(u) + 1 < level(v).  With inf = 63 it reproduces the reference's
+// capped levels; with inf = SCAPH_UNREACHED it gives exact BFS depths.
+void cpu_bfs(const HostCsr& g, uint32_t src, uint32_t inf, std::vector<uint32_t>* out);
+
+// Reference cap: depth >= 63 (or unreached) reads as 63 (app/bfs.cu:18).
+inline uint32_t compat63(uint32_t lvl) { return lvl <= 62 ? lvl : 63u; }
+
+}  // namespace scaph
+
+#endif

@@ -1,0 +1,343 @@
+// engine_capi.cpp — extern "C" wrappers of the host engine
+// (include/scaph_engine.h).  Exceptions never cross the boundary.
+#include "scaph_engine.h"
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "learned.h"
+
+struct scaph_engine {
+  scaph::Config cfg;
+  scaph::Engine* eng = nullptr;
+  scaph::RunStats last;
+  scaph::LearnedSchedule learned;
+  ~scaph_engine() { delete eng; }
+};
+
+namespace {
+
+void put_err(char* err, size_t len, const std::string& msg) {
+  if (!err || len == 0) return;
+  std::strncpy(err, msg.c_str(), len - 1);
+  err[len - 1] = '\0';
+}
+
+template <typename F>
+int guarded(char* err, size_t len, F f) {
+  try {
+    f();
+    put_err(err, len, "");
+    return 0;
+  } catch (const scaph::Error& e) {
+    put_err(err, len, e.what());
+    return e.status ? e.status : 1;
+  } catch (const std::exception& e) {
+    put_err(err, len, e.what());
+    return 1;
+  }
+}
+
+void apply_schedule(scaph_engine* e) {
+  if (e->cfg.schedule == "learned") {
+    scaph::LearnedSchedule* ls = &e->learned;
+    scaph::Engine* eng = e->eng;
+    eng->set_schedule([ls, eng](const scaph::SchedState& s) { return ls->decide(*eng, s); });
+  } else {
+    e->eng->set_schedule(nullptr);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int scaph_engine_create(int argc, const char* const* argv, scaph_engine** out, char* err,
+                        size_t errlen) {
+  if (!out) return 1;
+  *out = nullptr;
+  return guarded(err, errlen, [&] {
+    std::vector<std::string> args;
+    for (int i = 1; i < argc; ++i) args.push_back(argv[i] ? argv[i] : "");
+    auto* e = new scaph_engine();
+    std::string msg;
+    if (!scaph::parse_config(args, &e->cfg, nullptr, &msg)) {
+      delete e;
+      throw scaph::Error(1, msg);
+    }
+    try {
+      e->eng = new scaph::Engine(e->cfg);
+      if (!e->cfg.model.empty()) e->learned.load_file(e->cfg.model);
+      apply_schedule(e);
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+void scaph_engine_destroy(scaph_engine* e) { delete e; }
+
+int scaph_engine_load(scaph_engine* e, char* err, size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] { e->eng->load_from_config(); });
+}
+
+int scaph_engine_load_csr(scaph_engine* e, uint32_t n, const uint64_t* offsets, const uint32_t* ngh,
+                          int keep_host, char* err, size_t errlen) {
+  if (!e || !offsets) return 1;
+  return guarded(err, errlen, [&] {
+    scaph::HostCsr g;
+    g.n = n;
+    g.offsets.assign(offsets, offsets + (size_t)n + 1);
+    const uint64_t m = g.offsets[n];
+    if (m && !ngh) throw scaph::Error(1, "null neighbour array");
+    g.ngh.assign(ngh, ngh + m);
+    for (uint64_t i = 0; i < m; ++i)
+      if (g.ngh[i] >= n) throw scaph::Error(1, "neighbour id out of range");
+    for (uint32_t v = 0; v < n; ++v)
+      if (g.offsets[v] > g.offsets[(size_t)v + 1]) throw scaph::Error(1, "offsets decrease");
+    e->eng->load_csr(std::move(g), keep_host != 0);
+  });
+}
+
+int scaph_engine_gen_rmat_device(scaph_engine* e, uint32_t scale, uint32_t edge_factor, double a,
+                                 double b, double c, uint64_t seed, int keep_host, char* err,
+                                 size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] {
+    scaph::RmatParams p;
+    p.scale = scale;
+    p.edge_factor = edge_factor;
+    p.a = a;
+    p.b = b;
+    p.c = c;
+    p.seed = seed;
+    e->eng->gen_rmat_device(p, keep_host != 0);
+  });
+}
+
+int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_pages, int* resident,
+                      int* has_pull) {
+  if (!e) return 1;
+  if (n) *n = e->eng->num_vertices();
+  if (m) *m = e->eng->num_edges();
+  if (num_pages) *num_pages = (uint32_t)e->eng->pages().size();
+  if (resident) *resident = e->eng->resident() ? 1 : 0;
+  if (has_pull) *has_pull = e->eng->has_pull() ? 1 : 0;
+  return 0;
+}
+
+int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* err,
+                     size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] {
+    e->last = e->eng->bfs(src);
+    if (out) {
+      std::memset(out, 0, sizeof(*out));
+      out->src = e->last.src;
+      out->depth = e->last.depth;
+      out->ms = e->last.ms;
+      out->n_reached = e->last.n_reached;
+      out->m_reached = e->last.m_reached;
+      out->page_execs = e->last.page_execs;
+      out->page_execs_cached = e->last.page_execs_cached;
+      out->pages_full = e->last.pages_full;
+      out->pages_compacted = e->last.pages_compacted;
+      out->bytes_h2d = e->last.bytes_h2d;
+      out->h2d_ms = e->last.h2d_ms;
+    }
+  });
+}
+
+int scaph_engine_set_schedule(scaph_engine* e, const char* name, double alpha, double beta,
+                              char* err, size_t errlen) {
+  if (!e || !name) return 1;
+  return guarded(err, errlen, [&] {
+    const std::string s(name);
+    if (s != "fixed" && s != "push" && s != "pull" && s != "learned")
+      throw scaph::Error(1, "schedule must be fixed, push, pull or learned");
+    if (s == "learned" && !e->learned.ready())
+      throw scaph::Error(1, "learned schedule has no model (scaph_engine_set_model / -model)");
+    if (alpha > 0) e->cfg.alpha = alpha;
+    if (beta > 0) e->cfg.beta = beta;
+    e->cfg.schedule = s;
+    e->eng->set_config_schedule(s, e->cfg.alpha, e->cfg.beta);
+    apply_schedule(e);
+  });
+}
+
+int scaph_engine_set_model(scaph_engine* e, const uint16_t* params_bf16, const float* feat_mean,
+                           const float* feat_scale, float prefetch_gain, char* err,
+                           size_t errlen) {
+  if (!e || !params_bf16 || !feat_mean || !feat_scale) return 1;
+  return guarded(err, errlen, [&] {
+    e->learned.set(params_bf16, feat_mean, feat_scale, prefetch_gain);
+  });
+}
+
+int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_t max,
+                               uint32_t* count) {
+  if (!e) return 1;
+  const auto& lv = e->last.levels;
+  const uint32_t k = (uint32_t)std::min<size_t>(lv.size(), max);
+  for (uint32_t i = 0; i < k && out; ++i) {
+    out[i].level = lv[i].level;
+    out[i].dir = lv[i].dir;
+    out[i].n_f = lv[i].n_f;
+    out[i].m_f = lv[i].m_f;
+    out[i].m_u = lv[i].m_u;
+    out[i].n_next = lv[i].n_next;
+    out[i].m_next = lv[i].m_next;
+    out[i].scanned = lv[i].scanned;
+    out[i].ms = lv[i].ms;
+    for (int j = 0; j < 12; ++j) out[i].page_hist[j] = lv[i].page_hist[j];
+  }
+  if (count) *count = (uint32_t)lv.size();
+  return 0;
+}
+
+int scaph_engine_levels(scaph_engine* e, uint32_t* host_out) {
+  if (!e || !host_out) return 1;
+  return guarded(nullptr, 0, [&] { e->eng->levels_to_host(host_out); });
+}
+
+int scaph_engine_validate(scaph_engine* e, uint32_t src, uint64_t* out2) {
+  if (!e || !out2) return 1;
+  return guarded(nullptr, 0, [&] {
+    auto r = e->eng->validate(src);
+    out2[0] = r[0];
+    out2[1] = r[1];
+  });
+}
+
+int scaph_engine_out_degrees(scaph_engine* e, uint32_t* host_out) {
+  if (!e || !host_out) return 1;
+  return guarded(nullptr, 0, [&] {
+    const uint32_t n = e->eng->num_vertices();
+    H_ERR(scaph_copy_d2h_async(host_out, e->eng->d_degree(), (size_t)n * 4, e->eng->stream()));
+    H_ERR(scaph_stream_sync(e->eng->stream()));
+  });
+}
+
+int scaph_engine_host_csr(scaph_engine* e, uint64_t* offsets, uint32_t* ngh) {
+  if (!e) return 1;
+  const scaph::HostCsr* g = e->eng->host_csr();
+  if (!g) return 1;
+  if (offsets) std::memcpy(offsets, g->offsets.data(), g->offsets.size() * 8);
+  if (ngh) std::memcpy(ngh, g->ngh.data(), g->ngh.size() * 4);
+  return 0;
+}
+
+int scaph_engine_check(scaph_engine* e, uint32_t src, uint64_t* errors, char* err, size_t errlen) {
+  if (!e || !errors) return 1;
+  return guarded(err, errlen, [&] {
+    const scaph::HostCsr* g = e->eng->host_csr();
+    if (!g) throw scaph::Error(1, "no host copy of the graph to check against");
+    const uint32_t n = e->eng->num_vertices();
+    std::vector<uint32_t> gpu(n), capped, exact;
+    e->eng->levels_to_host(gpu.data());
+    scaph::cpu_bfs(*g, src, 63u, &capped);
+    scaph::cpu_bfs(*g, src, SCAPH_UNREACHED, &exact);
+    uint64_t bad = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      if (scaph::compat63(gpu[v]) != capped[v]) ++bad;
+      if (gpu[v] != exact[v]) ++bad;
+    }
+    *errors = bad;
+  });
+}
+
+int scaph_engine_graph_desc(scaph_engine* e, int transposed, void* desc_out) {
+  if (!e || !desc_out) return 1;
+  const scaph_graph_desc& d = transposed ? e->eng->in_desc() : e->eng->out_desc();
+  std::memcpy(desc_out, &d, sizeof(d));
+  return 0;
+}
+
+int scaph_engine_device_degree(scaph_engine* e, const uint32_t** degree) {
+  if (!e || !degree) return 1;
+  *degree = e->eng->d_degree();
+  return 0;
+}
+
+int scaph_host_page_cuts(const uint64_t* offsets, uint32_t n, uint64_t pagesize_words, int weighted,
+                         void* pages_out, uint32_t max_pages, uint32_t* num_pages) {
+  if (!offsets || !num_pages) return 1;
+  std::vector<scaph_page> pages;
+  std::string err;
+  if (!scaph::page_cuts_raw(offsets, n, pagesize_words, weighted != 0, &pages, &err)) return 2;
+  *num_pages = (uint32_t)pages.size();
+  if (pages_out) {
+    if (pages.size() > max_pages) return 3;
+    std::memcpy(pages_out, pages.data(), pages.size() * sizeof(scaph_page));
+  }
+  return 0;
+}
+
+static int copy_csr_out(const scaph::HostCsr& g, uint64_t* offsets, uint32_t* ngh, uint64_t cap,
+                        uint64_t* num_edges) {
+  if (num_edges) *num_edges = g.num_edges();
+  if (offsets) std::memcpy(offsets, g.offsets.data(), g.offsets.size() * 8);
+  if (ngh) {
+    if (g.num_edges() > cap) return 3;
+    std::memcpy(ngh, g.ngh.data(), g.ngh.size() * 4);
+  }
+  return 0;
+}
+
+int scaph_host_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                        uint64_t seed, uint64_t* offsets, uint32_t* ngh, uint64_t ngh_cap,
+                        uint64_t* num_edges) {
+  if (scale == 0 || scale > 31) return 1;
+  scaph::RmatParams p;
+  p.scale = scale;
+  p.edge_factor = edge_factor;
+  p.a = a;
+  p.b = b;
+  p.c = c;
+  p.seed = seed;
+  return copy_csr_out(scaph::generate_rmat(p), offsets, ngh, ngh_cap, num_edges);
+}
+
+int scaph_host_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint64_t* offsets, uint32_t* ngh,
+                      uint64_t ngh_cap, uint64_t* num_edges) {
+  if (n == 0) return 1;
+  return copy_csr_out(scaph::generate_er(n, m, seed), offsets, ngh, ngh_cap, num_edges);
+}
+
+int scaph_host_write_graph(const char* base, uint32_t n, const uint64_t* offsets,
+                           const uint32_t* ngh, uint64_t pagesize_words, int idx64, int write_wgh,
+                           char* err, size_t errlen) {
+  if (!base || !offsets) return 1;
+  scaph::HostCsr g;
+  g.n = n;
+  g.offsets.assign(offsets, offsets + (size_t)n + 1);
+  g.ngh.assign(ngh, ngh + g.offsets[n]);
+  std::string msg;
+  if (!scaph::write_csr_files(base, g, idx64 != 0, write_wgh != 0, &msg) ||
+      (pagesize_words && !scaph::write_page_files(base, g, pagesize_words, false, nullptr, &msg))) {
+    put_err(err, errlen, msg);
+    return 2;
+  }
+  put_err(err, errlen, "");
+  return 0;
+}
+
+int scaph_host_cpu_bfs(uint32_t n, const uint64_t* offsets, const uint32_t* ngh, uint32_t src,
+                       uint32_t inf, uint32_t* levels) {
+  if (!offsets || !levels || (offsets[n] && !ngh)) return 1;
+  scaph::HostCsr g;
+  g.n = n;
+  g.offsets.assign(offsets, offsets + (size_t)n + 1);
+  g.ngh.assign(ngh, ngh + offsets[n]);
+  std::vector<uint32_t> out;
+  scaph::cpu_bfs(g, src, inf, &out);
+  std::memcpy(levels, out.data(), (size_t)n * 4);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// pagegencsr — offline page generator with the command line and output of
+// tools/bin2pagecsr.cpp:254-312:  pagegencsr <base> <pagesize MiB words> w|u
+// Reads <base>_csr.{ngh,deg,idx} (and _csr.wgh when weighted) and writes
+// <base>page<N>, <base>page.info and <base>graph.info.
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "../host/graph_io.h"
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    std::printf("usage : ./pagegencsr filename pagesize(MB Vertex) w/u(weighted)\n");
+    return 0;
+  }
+  const std::string base = argv[1];
+  const uint64_t P = (uint64_t)std::atol(argv[2]) * 1024 * 1024;
+  const bool weighted = argv[3][0] == 'w';
+  scaph::HostCsr g;
+  std::string err;
+  if (!scaph::read_csr_files(base, &g, &err)) {
+    std::fprintf(stderr, "%s\n", err.c_str());
+    return 1;
+  }
+  if (weighted && g.wgh.size() != g.ngh.size()) {
+    std::fprintf(stderr, "weighted pages need %s_csr.wgh\n", base.c_str());
+    return 1;
+  }
+  std::vector<scaph_page> pages;
+  if (!scaph::write_page_files(base, g, P, weighted, &pages, &err)) {
+    std::fprintf(stderr, "%s\n", err.c_str());
+    return 1;
+  }
+  for (const auto& p : pages)
+    std::printf("left: %u right: %u\nedgenum: %u\nnodenum: %u\n\n", p.left, p.right, p.edgenum,
+                p.nodenum);
+  std::printf("node: %u\nedge: %llu\n", g.n, (unsigned long long)g.num_edges());
+  return 0;
+}

@@ -1,0 +1,186 @@
+"""GPU parity tests: levels from the HIP path (through the C-ABI) against
+the oracle, bit for bit, for every schedule; plus the device graph builder
+and the per-page count kernel."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import scaph_amd
+from scaph_amd import _lib
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+U = 0xFFFFFFFF
+SCHEDULES = ["fixed", "push", "pull"]
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def engine(*flags):
+    return scaph_amd.BFS("-max_memory=65536", *flags)
+
+
+def cases():
+    z = np.load(os.path.join(GOLDEN, "bfs_cases.npz"), allow_pickle=False)
+    return [(str(n), z[f"{n}__offsets"], z[f"{n}__ngh"], int(z[f"{n}__src"][0]),
+             z[f"{n}__levels"]) for n in z["names"]]
+
+
+@pytest.mark.parametrize("page_words", [0, 4096])
+def test_golden_fixtures_all_schedules(page_words):
+    flags = [f"-page_words={page_words}"] if page_words else []
+    for name, off, ngh, src, want in cases():
+        say("fixture", name, "page_words", page_words)
+        with engine(*flags) as e:
+            e.load_csr(off, ngh)
+            for sched in SCHEDULES:
+                e.set_schedule(sched)
+                st = e.run(src)
+                got = e.levels()
+                np.testing.assert_array_equal(got, want, err_msg=f"{name} {sched}")
+                reached = want != U
+                assert st["n_reached"] == reached.sum()
+                deg = np.diff(off.astype(np.int64))
+                assert st["m_reached"] == deg[reached].sum()
+
+
+@pytest.mark.parametrize("scale,ef,seed,page_words", [(12, 8, 3, 0), (16, 16, 1, 0),
+                                                      (14, 32, 5, 65536)])
+def test_rmat_parity_with_oracle(scale, ef, seed, page_words):
+    off, ngh = O.rmat_csr(scale, ef, seed=seed)
+    deg = np.diff(off.astype(np.int64))
+    srcs = [int(np.argmax(deg)), int(np.nonzero(deg)[0][7]), 0]
+    flags = [f"-page_words={page_words}"] if page_words else []
+    with engine(*flags) as e:
+        e.load_csr(off, ngh)
+        for src in srcs:
+            want = O.bfs(off, ngh, src)
+            for sched in SCHEDULES:
+                say("rmat", scale, ef, seed, "src", src, sched)
+                e.set_schedule(sched)
+                e.run(src)
+                np.testing.assert_array_equal(e.levels(), want, err_msg=f"src {src} {sched}")
+            assert e.check(src) == 0
+
+
+def test_config1_s20_ef16_seed1_from_max_degree():
+    """Config 1: R-MAT s20 ef16 (a,b,c)=(.57,.19,.19) seed 1, source = max out-degree."""
+    off, ngh = O.rmat_csr(20, 16, seed=1)
+    deg = np.diff(off.astype(np.int64))
+    src = int(np.argmax(deg))
+    want = O.bfs(off, ngh, src)
+    want63 = O.bfs(off, ngh, src, inf=63)
+    with engine() as e:
+        e.load_csr(off, ngh)
+        for sched in SCHEDULES:
+            say("config1", sched)
+            e.set_schedule(sched)
+            st = e.run(src)
+            got = e.levels()
+            np.testing.assert_array_equal(got, want)
+            np.testing.assert_array_equal(np.where(got <= 62, got, 63), want63)
+            assert e.validate(src) == (0, 0)
+            say("config1", sched, "ms", round(st["ms"], 3), "depth", st["depth"])
+
+
+def test_many_pages_global_page_lookup():
+    """More pages than the LDS page table holds (global lookup path)."""
+    off, ngh = O.er_csr(200000, 1600000, 21)
+    src = 17
+    want = O.bfs(off, ngh, src)
+    with engine("-page_words=1024") as e:
+        e.load_csr(off, ngh)
+        assert e.info()["pages"] > 1024
+        for sched in SCHEDULES:
+            say("many pages", sched)
+            e.set_schedule(sched)
+            e.run(src)
+            np.testing.assert_array_equal(e.levels(), want)
+
+
+def test_device_builder_equals_oracle_generator():
+    for scale, ef, seed in ((12, 16, 2), (15, 8, 9)):
+        say("device builder", scale, ef, seed)
+        off, ngh = O.rmat_csr(scale, ef, seed=seed)
+        with engine() as e:
+            e.gen_rmat_device(scale, ef, seed=seed, keep_host=True)
+            hoff, hngh = e.host_csr()
+            np.testing.assert_array_equal(hoff, off)
+            np.testing.assert_array_equal(hngh, ngh)
+            np.testing.assert_array_equal(e.out_degrees(), np.diff(off).astype(np.uint32))
+            src = int(np.argmax(np.diff(off.astype(np.int64))))
+            e.run(src)
+            np.testing.assert_array_equal(e.levels(), O.bfs(off, ngh, src))
+
+
+def test_edge_cases():
+    say("edge cases")
+    with engine() as e:
+        # one vertex, no edges
+        e.load_csr(np.array([0, 0], np.uint64), np.zeros(0, np.uint32))
+        e.run(0)
+        np.testing.assert_array_equal(e.levels(), [0])
+        # no edges at all, source in the middle
+        e.load_csr(np.zeros(11, np.uint64), np.zeros(0, np.uint32))
+        e.run(4)
+        want = np.full(10, U, np.uint32)
+        want[4] = 0
+        np.testing.assert_array_equal(e.levels(), want)
+        # source with out-degree 0 in a non-trivial graph
+        off, ngh = O.csr_from_edges(5, [0, 1, 2], [1, 2, 3])
+        e.load_csr(off, ngh)
+        e.run(3)
+        np.testing.assert_array_equal(e.levels(), [U, U, U, 0, U])
+        # out-of-range source is an error, not a fault
+        with pytest.raises(scaph_amd.ScaphError):
+            e.run(5)
+
+
+def test_page_counts_kernel_vs_oracle():
+    """scaph_page_counts (get_page_info/get_total_info) on a random frontier."""
+    hip = _lib.hip()
+    off, ngh = O.er_csr(40000, 320000, 12)
+    n = 40000
+    deg = np.diff(off).astype(np.uint32)
+    pages = O.page_cuts(off, 1 << 14)
+    rng = np.random.default_rng(4)
+    stat = (rng.random(n) < 0.15).astype(np.uint8)
+    words = ((n + 63) // 64) * 2
+    bits = np.zeros(words * 32, np.uint8)
+    bits[:n] = stat
+    bitmap = np.packbits(bits, bitorder="little").view(np.uint32)
+    want_nodes, want_datas, want_tot = O.page_counts(pages, stat, deg)
+    bufs = {}
+
+    def dev(arr):
+        p = ctypes.c_void_p()
+        _lib.check(hip.scaph_dev_alloc(ctypes.byref(p), max(arr.nbytes, 8)))
+        _lib.check(hip.scaph_copy_h2d_async(p, arr.ctypes.data, arr.nbytes, None))
+        bufs[id(arr)] = p
+        return p
+
+    d_pages, d_front, d_deg = dev(np.ascontiguousarray(pages)), dev(bitmap), dev(deg)
+    out_n = np.zeros(len(pages), np.uint32)
+    out_d = np.zeros(len(pages), np.uint32)
+    tot = np.zeros(2, np.uint64)
+    d_n, d_d, d_t = dev(out_n), dev(out_d), dev(tot)
+    for weighted in (0, 1):
+        _lib.check(hip.scaph_page_counts(d_pages, len(pages), d_front, d_deg, weighted, d_n, d_d,
+                                         d_t, None))
+        _lib.check(hip.scaph_device_sync())
+        for arr, p in ((out_n, d_n), (out_d, d_d), (tot, d_t)):
+            _lib.check(hip.scaph_copy_d2h_async(arr.ctypes.data, p, arr.nbytes, None))
+        _lib.check(hip.scaph_device_sync())
+        wn, wd, wt = O.page_counts(pages, stat, deg, weighted=bool(weighted))
+        np.testing.assert_array_equal(out_n, wn)
+        np.testing.assert_array_equal(out_d, wd)
+        np.testing.assert_array_equal(tot, wt)
+    for p in bufs.values():
+        hip.scaph_dev_free(p)

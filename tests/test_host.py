@@ -1,0 +1,144 @@
+"""CPU tests of the product's host side: C-ABI exports, formats, generators,
+flags — each compared with the oracle or the reference tool's output."""
+import ctypes
+import hashlib
+import json
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import scaph_amd
+from scaph_amd import _lib
+from tests import _oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+BIN = os.path.join(ROOT, "scaph_amd", "bin")
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(scaph_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("scaph_hip.h", "hip"), ("scaph_engine.h", "host")])
+def test_c_abi_exports_every_declared_symbol(header, lib):
+    L = _lib.hip() if lib == "hip" else _lib.host()
+    names = declared_functions(header)
+    assert len(names) > 10
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_host_page_cuts_match_oracle():
+    off, ngh = O.rmat_csr(14, 16, seed=4)
+    for P in (1 << 16, 1 << 18, 1 << 20):
+        np.testing.assert_array_equal(scaph_amd.page_cuts(off, P), O.page_cuts(off, P))
+
+
+def test_host_rmat_generator_equals_oracle():
+    for scale, ef, seed in ((10, 8, 1), (13, 16, 9), (12, 4, 2)):
+        a = scaph_amd.generate_rmat(scale, ef, seed=seed)
+        b = O.rmat_csr(scale, ef, seed=seed)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+    a = scaph_amd.generate_rmat(11, 8, a=0.45, b=0.22, c=0.22, seed=5)
+    b = O.rmat_csr(11, 8, a=0.45, b=0.22, c=0.22, seed=5)
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_host_er_generator_equals_oracle():
+    a = scaph_amd.generate_er(3000, 24000, seed=17)
+    b = O.er_csr(3000, 24000, 17)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def sha(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def test_host_page_files_match_reference_hashes():
+    gold = json.load(open(os.path.join(GOLDEN, "pages.json")))
+    for name, g in gold.items():
+        spec = g["spec"]
+        if spec["kind"] == "rmat":
+            off, ngh = scaph_amd.generate_rmat(spec["scale"], spec["ef"], seed=spec["seed"])
+        else:
+            off, ngh = scaph_amd.generate_er(spec["n"], spec["m"], seed=spec["seed"])
+        with tempfile.TemporaryDirectory() as td:
+            base = os.path.join(td, "g")
+            scaph_amd.write_graph(base, off, ngh, pagesize_words=g["pagesize_mb"] << 20,
+                                  idx64=False, write_wgh=True)
+            for fname, h in g["sha256"].items():
+                assert sha(base + fname) == h, (name, fname)
+
+
+def test_pagegencsr_cli_matches_reference_tool_bytes():
+    """bin/pagegencsr (product) vs oracle/_ref/bin2pagecsr (reference source)."""
+    if not os.path.exists(O.REF_TOOL):
+        pytest.skip("reference tool not built here")
+    off, ngh = O.rmat_csr(16, 32, seed=21)
+    with tempfile.TemporaryDirectory() as td:
+        for who in ("ref", "mine"):
+            base = os.path.join(td, who)
+            off.astype(np.uint32).tofile(base + "_csr.idx")
+            ngh.tofile(base + "_csr.ngh")
+            np.diff(off).astype(np.uint32).tofile(base + "_csr.deg")
+            np.zeros(len(ngh), np.uint32).tofile(base + "_csr.wgh")
+        subprocess.run([O.REF_TOOL, os.path.join(td, "ref"), "1", "u"], check=True,
+                       stdout=subprocess.DEVNULL)
+        subprocess.run([os.path.join(BIN, "pagegencsr"), os.path.join(td, "mine"), "1", "u"],
+                       check=True, stdout=subprocess.DEVNULL)
+        names = sorted(f[3:] for f in os.listdir(td) if f.startswith("ref") and "_csr" not in f)
+        assert "page.info" in names and "graph.info" in names and "page1" in names
+        for f in names:
+            assert sha(os.path.join(td, "ref" + f)) == sha(os.path.join(td, "mine" + f)), f
+
+
+def test_flag_syntax_gflags_forms():
+    """-f=v, --f=v, -f v, -nobool, dash/underscore, hex, flagfile, unknown."""
+    with tempfile.TemporaryDirectory() as td:
+        ff = os.path.join(td, "flags")
+        open(ff, "w").write("# comment\n-num_stream=7\n--threshold=4\n")
+        args = ["-max-memory=2048", "--pagesize", "0x2", "-nocheck", "--check=yes",
+                f"--flagfile={ff}", "-src", "5", "--", "-notaflag"]
+        out = subprocess.run([os.path.join(BIN, "bfs"), *args, "-rounds=0"],
+                             capture_output=True, text=True)
+        # parsing succeeds; what fails next is the missing graph (or, on a
+        # machine without a GPU, the device) -- never a flag
+        assert "ERROR" not in out.stdout, out.stdout + out.stderr
+        assert ("no -graphfile and no -gen" in out.stdout or "device" in out.stdout), out.stdout
+        bad = subprocess.run([os.path.join(BIN, "bfs"), "-nosuchflag=1"], capture_output=True,
+                             text=True)
+        assert bad.returncode != 0 and "unknown command line flag 'nosuchflag'" in bad.stdout
+        bad = subprocess.run([os.path.join(BIN, "bfs"), "-num_stream=abc"], capture_output=True,
+                             text=True)
+        assert bad.returncode != 0 and "illegal value" in bad.stdout
+
+
+def test_engine_config_parse_via_capi_rejects_bad_threshold():
+    # threshold > 16 would overflow the half-page compaction tail
+    with pytest.raises(scaph_amd.ScaphError, match="threshold"):
+        scaph_amd.BFS("-threshold=17")
+
+
+def test_host_cpu_bfs_equals_oracle():
+    z = np.load(os.path.join(GOLDEN, "bfs_cases.npz"), allow_pickle=False)
+    L = _lib.host()
+    for name in z["names"]:
+        off = z[f"{name}__offsets"]
+        ngh = z[f"{name}__ngh"]
+        src = int(z[f"{name}__src"][0])
+        n = len(off) - 1
+        out = np.empty(n, np.uint32)
+        g = ngh if len(ngh) else np.zeros(1, np.uint32)
+        assert L.scaph_host_cpu_bfs(n, off.ctypes.data, g.ctypes.data, src, 0xFFFFFFFF,
+                                    out.ctypes.data) == 0
+        np.testing.assert_array_equal(out, z[f"{name}__levels"])
