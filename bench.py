@@ -1,0 +1,210 @@
+"""BFS benchmark on one MI355X: R-MAT scale 26, edge factor 32, seed 2.
+
+One "step" is one full BFS from one source (levels final on the device),
+sources cycling through 64 seeded sources taken from the giant component.
+The graph (paged CSR, out- and in-edge pages) is built on the device
+before timing and stays resident in HBM.
+
+Prints ONE JSON line:
+  value        median GTEPS over the timed BFS runs (m_c / t_BFS)
+  roofline     the dominant kernel (by device time): algorithmic bytes per
+               launch / its average launch time (HIP events around it)
+  bfs_hbm_frac B_alg / t_BFS / 8 TB/s with B_alg = 4 m_c + 12 n_c + 4 n
+  cpu_baseline the oracle's single-core BFS on this host, one source
+"""
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pick_sources(eng, n, count, seed, min_reach_frac):
+    """count sources, uniformly drawn (seeded) among vertices with out-degree
+    >= 1, kept when their BFS reaches >= min_reach_frac of the vertices (the
+    giant component); the reach test is an untimed BFS."""
+    deg = eng.out_degrees()
+    cand = np.nonzero(deg > 0)[0]
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(cand))
+    out = []
+    tried = 0
+    for i in order:
+        v = int(cand[i])
+        tried += 1
+        st = eng.run(v)
+        if st["n_reached"] >= min_reach_frac * n:
+            out.append(v)
+            if len(out) == count:
+                break
+        if tried > 20 * count:
+            break
+    return out, tried
+
+
+def alg_bytes_step(rec, n):
+    """Algorithmic bytes of one step's expand kernels:
+    pull: Node record 12 B per examined vertex, 4 B per scanned in-edge,
+          4 B level write per vertex found, and the visited-read,
+          visited-write and next-frontier-write bitmaps (3 * n/8 B);
+    push: 12 B Node + 4 B queue read per frontier vertex, 4 B per scanned
+          out-edge, 4 B level write + 4 B queue write per vertex reached."""
+    if rec["dir"] == 1:
+        return 12 * rec["examined"] + 4 * rec["scanned"] + 4 * rec["n_next"] + 3 * (n / 8)
+    return 16 * rec["n_f"] + 4 * rec["scanned"] + 8 * rec["n_next"]
+
+
+def cpu_baseline(eng, src, gpu_levels, budget_s):
+    """Oracle BFS (single core) on the same graph for one source."""
+    from tests import _oracle
+    off, ngh = eng.host_csr()
+    cpu = os.sched_getaffinity(0)
+    core = min(cpu)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        lv = _oracle.bfs(off, ngh, src)
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, cpu)
+    deg = np.diff(off.astype(np.int64))
+    reached = lv != 0xFFFFFFFF
+    m_c = int(deg[reached].sum())
+    parity = bool(np.array_equal(lv, gpu_levels))
+    return {"value": m_c / dt / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"1 BFS (source {src}) of the same graph; {dt:.1f} s; "
+                      f"levels bit-exact vs GPU: {parity}",
+            "seconds": dt}, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--ef", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--sources", type=int, default=64)
+    ap.add_argument("--schedule", default="fixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=60.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the BFS would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import scaph_amd
+    from scaph_amd import _lib
+
+    props = scaph_amd.device_props(0)
+    log("device", props)
+    eng = scaph_amd.BFS("-max_memory=262144", "-pagesize=10", f"-schedule={args.schedule}")
+    t0 = time.perf_counter()
+    eng.gen_rmat_device(args.scale, args.ef, 0.57, 0.19, 0.19, seed=args.seed,
+                        keep_host=not args.no_cpu)
+    info = eng.info()
+    log("graph built", info, f"{time.perf_counter() - t0:.1f}s")
+    n = info["n"]
+
+    srcs, tried = pick_sources(eng, n, args.sources, seed=args.seed, min_reach_frac=0.01)
+    log(f"sources: {len(srcs)} of {tried} candidates reach >=1% of V")
+    if not srcs:
+        raise SystemExit("no source reaches 1% of the vertices")
+
+    for i in range(args.warmup):
+        eng.run(srcs[i % len(srcs)])
+
+    hip = _lib.hip()
+    _lib.check(hip.scaph_device_sync())
+    runs = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        st = eng.run(srcs[k % len(srcs)])
+        st["records"] = eng.records()
+        runs.append(st)
+    _lib.check(hip.scaph_device_sync())
+    wall = time.perf_counter() - t_start
+
+    gteps = [r["m_reached"] / (r["ms"] / 1e3) / 1e9 for r in runs]
+    balg = [4 * r["m_reached"] + 12 * r["n_reached"] + 4 * n for r in runs]
+    frac = [b / (r["ms"] / 1e3) / HBM_PEAK for b, r in zip(balg, runs)]
+    ref_gteps = [info["m"] / (r["ms"] / 1e3) / 1e9 for r in runs]
+
+    # dominant kernel: pull or push expand, by summed kernel time
+    agg = {0: [0.0, 0, 0.0], 1: [0.0, 0, 0.0]}  # dir -> [ms, launches, alg bytes]
+    for r in runs:
+        for rec in r["records"]:
+            a = agg[rec["dir"]]
+            a[0] += rec["kernel_ms"]
+            a[1] += 1
+            a[2] += alg_bytes_step(rec, n)
+    dom = max(agg, key=lambda d: agg[d][0])
+    ms_tot, launches, bytes_tot = agg[dom]
+    avg_ms = ms_tot / max(launches, 1)
+    avg_bytes = bytes_tot / max(launches, 1)
+    achieved = avg_bytes / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                "frac": achieved / 8000.0, "traffic": None,
+                "kernel": "k_pull" if dom == 1 else "push (k_prep_queue+k_scan_blocks+k_mark_tiles+k_expand)",
+                "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_bytes, "launches": launches,
+                "share_of_kernel_time": ms_tot / max(agg[0][0] + agg[1][0], 1e-9)}
+
+    # parity at full size: device invariant check on the last source
+    viol = eng.validate(runs[-1]["src"])
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        src0 = srcs[0]
+        eng.run(src0)
+        gl = eng.levels()
+        cpu, parity = cpu_baseline(eng, src0, gl, args.cpu_budget)
+        log("cpu baseline", cpu)
+
+    med = statistics.median(gteps)
+    out = {
+        "metric": "BFS GTEPS, R-MAT s26 ef32, learned vs fixed-alpha schedule; % HBM peak",
+        "value": med,
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic R-MAT built on the device (SplitMix64 stream, seeded relabelling)",
+        "config": {"workload": f"BFS R-MAT scale {args.scale} edge factor {args.ef} "
+                               f"(a,b,c)=(.57,.19,.19) seed {args.seed}, {len(srcs)} sources",
+                   "vertices": n, "edges": info["m"], "pages": info["pages"],
+                   "schedule": args.schedule, "alpha": 14, "beta": 24},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "bfs_hbm_frac_median": statistics.median(frac),
+        "t_bfs_ms_median": statistics.median(r["ms"] for r in runs),
+        "gteps_ref_formula_median": statistics.median(ref_gteps),
+        "gteps_mean_of_runs": sum(r["m_reached"] for r in runs) / sum(r["ms"] for r in runs) / 1e6,
+        "levels_exact_vs_oracle": parity,
+        "invariant_violations": list(viol),
+        "device": props["name"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

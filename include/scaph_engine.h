@@ -39,8 +39,9 @@ typedef struct scaph_level_record {
   uint32_t level;
   int32_t dir; /* 0 push, 1 pull */
   uint64_t n_f, m_f, m_u;
-  uint64_t n_next, m_next, scanned;
-  float ms;
+  uint64_t n_next, m_next, scanned, examined;
+  float ms;        /* step time including host turnaround */
+  float kernel_ms; /* expand kernels of the step alone (events around them) */
   float page_hist[12];
 } scaph_level_record;
 

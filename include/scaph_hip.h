@@ -105,6 +105,8 @@ typedef struct scaph_level_counters {
   unsigned long long m_next;   /* sum of their out-degrees */
   unsigned long long q_len;    /* entries appended to the next queue */
   unsigned long long scanned;  /* edges examined (push: all; pull: until hit) */
+  unsigned long long examined; /* pull: unvisited vertices whose in-list was read */
+  unsigned long long reserved;
 } scaph_level_counters;
 
 /* Level init (app/bfs.cu:20-37): level[v] = SCAPH_UNREACHED for v < n,

@@ -39,7 +39,8 @@ class GraphDesc(ctypes.Structure):
 
 class LevelCounters(ctypes.Structure):
     _fields_ = [("n_next", ctypes.c_ulonglong), ("m_next", ctypes.c_ulonglong),
-                ("q_len", ctypes.c_ulonglong), ("scanned", ctypes.c_ulonglong)]
+                ("q_len", ctypes.c_ulonglong), ("scanned", ctypes.c_ulonglong),
+                ("examined", ctypes.c_ulonglong), ("reserved", ctypes.c_ulonglong)]
 
 
 class DeviceProps(ctypes.Structure):
@@ -63,7 +64,8 @@ class LevelRecord(ctypes.Structure):
     _fields_ = [("level", ctypes.c_uint32), ("dir", ctypes.c_int32),
                 ("n_f", ctypes.c_uint64), ("m_f", ctypes.c_uint64), ("m_u", ctypes.c_uint64),
                 ("n_next", ctypes.c_uint64), ("m_next", ctypes.c_uint64),
-                ("scanned", ctypes.c_uint64), ("ms", ctypes.c_float),
+                ("scanned", ctypes.c_uint64), ("examined", ctypes.c_uint64),
+                ("ms", ctypes.c_float), ("kernel_ms", ctypes.c_float),
                 ("page_hist", ctypes.c_float * 12)]
 
 

@@ -97,6 +97,8 @@ __global__ void k_bfs_set_source(uint32_t* level, uint32_t* visited, uint32_t sr
     counters->m_next = 0;
     counters->q_len = 0;
     counters->scanned = 0;
+    counters->examined = 0;
+    counters->reserved = 0;
   }
 }
 
@@ -364,7 +366,7 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
        const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
        uint32_t next_level) {
   __shared__ PageLds s;
-  __shared__ unsigned long long red[3][kBlock / SCAPH_WAVE];
+  __shared__ unsigned long long red[4][kBlock / SCAPH_WAVE];
   stage_pages(s, g);
   const uint32_t n = g.num_vertices;
   const uint32_t lane = lane_id();
@@ -372,7 +374,7 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
   const uint64_t nchunks = ((uint64_t)n + 63) / 64;
   const uint64_t gw = (uint64_t)blockIdx.x * (kBlock / SCAPH_WAVE) + wid;
   const uint64_t nw = (uint64_t)gridDim.x * (kBlock / SCAPH_WAVE);
-  unsigned long long acc_n = 0, acc_m = 0, acc_s = 0;
+  unsigned long long acc_n = 0, acc_m = 0, acc_s = 0, acc_x = 0;
   for (uint64_t chunk = gw; chunk < nchunks; chunk += nw) {
     const uint32_t v = (uint32_t)(chunk * 64 + lane);
     const size_t word = chunk * 2 + (lane >> 5);
@@ -389,6 +391,7 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
     nr.edges = nullptr;
     nr.len = 0;
     if (unv) {
+      ++acc_x;
       nr = load_node(s, g, v);
       const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
       uint32_t k = 0;
@@ -437,22 +440,26 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
   const unsigned long long sn = wave_sum(acc_n);
   const unsigned long long sm = wave_sum(acc_m);
   const unsigned long long ss = wave_sum(acc_s);
+  const unsigned long long sx = wave_sum(acc_x);
   if (lane == 0) {
     red[0][wid] = sn;
     red[1][wid] = sm;
     red[2][wid] = ss;
+    red[3][wid] = sx;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long a = 0, b = 0, c = 0;
+    unsigned long long a = 0, b = 0, c = 0, x = 0;
     for (uint32_t w = 0; w < kBlock / SCAPH_WAVE; ++w) {
       a += red[0][w];
       b += red[1][w];
       c += red[2][w];
+      x += red[3][w];
     }
     if (a) atomicAdd(&counters->n_next, a);
     if (b) atomicAdd(&counters->m_next, b);
     if (c) atomicAdd(&counters->scanned, c);
+    if (x) atomicAdd(&counters->examined, x);
   }
 }
 

@@ -17,6 +17,7 @@ namespace {
 
 constexpr uint64_t kAlignWords = 64;  // page buffers start on 256-byte boundaries
 constexpr size_t kMaxTimedLevels = 1024;
+constexpr size_t kCounterWords = sizeof(scaph_level_counters) / 4;
 
 uint64_t align_words(uint64_t w) { return (w + kAlignWords - 1) / kAlignWords * kAlignWords; }
 
@@ -485,12 +486,18 @@ RunStats Engine::bfs_resident(uint32_t src) {
       front_ok = true;
     }
     if (d == Dir::kPush && !queue_ok) {
-      H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt + 1), 0u, 8, st));
+      H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt + 1), 0u, kCounterWords, st));
       H_ERR(scaph_frontier_bitmap_to_queue(front_[fa].get<uint32_t>(), n_,
                                            queue_[qa].get<uint32_t>(), cnt + 1, st));
       queue_ok = true;
     }
-    H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, 8, st));
+    H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, kCounterWords, st));
+    const bool timed = rs.levels.size() < kMaxTimedLevels;
+    const size_t eb = 2 + 3 * rs.levels.size();
+    if (timed) {
+      while (ev_.size() < eb + 3) ev_.emplace_back(new Event());
+      ev_[eb]->record(st);
+    }
     if (d == Dir::kPush) {
       H_ERR(scaph_bfs_push(&g_out_, deg, level, visited, queue_[qa].get<uint32_t>(), (uint32_t)n_f,
                            m_f, queue_[qa ^ 1].get<uint32_t>(), cnt, cur, ws_.get(), ws_bytes_,
@@ -505,13 +512,12 @@ RunStats Engine::bfs_resident(uint32_t src) {
       front_ok = true;
       queue_ok = false;
     }
-    if (rs.levels.size() < kMaxTimedLevels) {
-      const size_t e = 2 + rs.levels.size();
-      while (ev_.size() <= e) ev_.emplace_back(new Event());
-      ev_[e]->record(st);
-      lev_ev.push_back(e);
+    if (timed) {
+      ev_[eb + 1]->record(st);
+      lev_ev.push_back(eb);
     }
     H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
+    if (timed) ev_[eb + 2]->record(st);
     main_->sync();
     LevelRecord rec;
     rec.level = cur;
@@ -522,6 +528,7 @@ RunStats Engine::bfs_resident(uint32_t src) {
     rec.n_next = hc->n_next;
     rec.m_next = hc->m_next;
     rec.scanned = hc->scanned;
+    rec.examined = hc->examined;
     rs.levels.push_back(rec);
     n_f_prev = n_f;
     n_f = hc->n_next;
@@ -535,11 +542,11 @@ RunStats Engine::bfs_resident(uint32_t src) {
   ev_[1]->record(st);
   ev_[1]->sync();
   rs.ms = Event::elapsed_ms(*ev_[0], *ev_[1]);
-  size_t prev = 0;
   for (size_t i = 0; i < lev_ev.size(); ++i) {
-    const Event& a = i == 0 ? *ev_[0] : *ev_[lev_ev[i - 1]];
-    rs.levels[i].ms = Event::elapsed_ms(a, *ev_[lev_ev[i]]);
-    (void)prev;
+    const size_t e = lev_ev[i];
+    const Event& prev_end = i == 0 ? *ev_[0] : *ev_[lev_ev[i - 1] + 2];
+    rs.levels[i].ms = Event::elapsed_ms(prev_end, *ev_[e + 2]);
+    rs.levels[i].kernel_ms = Event::elapsed_ms(*ev_[e], *ev_[e + 1]);
   }
   rs.depth = cur;
   return rs;

@@ -63,7 +63,9 @@ struct LevelRecord {
   uint64_t n_f = 0, m_f = 0, m_u = 0;   // frontier before the step
   uint64_t n_next = 0, m_next = 0;      // reached by the step
   uint64_t scanned = 0;                 // edges examined
-  float ms = 0;                         // device time of the step
+  uint64_t examined = 0;                // pull: vertices whose in-list was read
+  float ms = 0;                         // step time incl. host turnaround
+  float kernel_ms = 0;                  // expand kernels alone
   std::array<float, 12> page_hist{};    // histogram of per-page active counts
 };
 

@@ -193,7 +193,9 @@ int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_
     out[i].n_next = lv[i].n_next;
     out[i].m_next = lv[i].m_next;
     out[i].scanned = lv[i].scanned;
+    out[i].examined = lv[i].examined;
     out[i].ms = lv[i].ms;
+    out[i].kernel_ms = lv[i].kernel_ms;
     for (int j = 0; j < 12; ++j) out[i].page_hist[j] = lv[i].page_hist[j];
   }
   if (count) *count = (uint32_t)lv.size();
