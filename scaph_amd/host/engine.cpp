@@ -219,10 +219,12 @@ void Engine::alloc_state() {
   queue_[0].alloc(std::max<size_t>(n_, 1) * 4);
   queue_[1].alloc(std::max<size_t>(n_, 1) * 4);
   counters_.alloc(4 * sizeof(scaph_level_counters));
-  uint64_t max_edges = m_;
-  if (!resident_) max_edges = std::max<uint64_t>(max_edges, P_);
-  ws_bytes_ = scaph_push_workspace_bytes(std::max<uint32_t>(n_, 1), std::max<uint64_t>(max_edges, 1));
-  ws_.alloc(ws_bytes_);
+  // the resident push expands whole frontiers; the paged mode keeps one
+  // page-sized workspace per exec stream instead
+  ws_bytes_ = resident_ ? scaph_push_workspace_bytes(std::max<uint32_t>(n_, 1),
+                                                      std::max<uint64_t>(m_, 1))
+                        : 0;
+  if (ws_bytes_) ws_.alloc(ws_bytes_);
   const size_t np = std::max<size_t>(pages_out_.size(), 1);
   page_nodes_.alloc(np * 4);
   page_datas_.alloc(np * 4);

@@ -1,17 +1,301 @@
-// engine_paged.cpp — out-of-core (paged) BFS: a bounded pool of device page
-// slots caches host pages; per level the host classifies every active page
-// as cached / full copy / compacted copy and streams the missing ones from
-// pinned memory (src/pagehandle.h, src/compress.h, src/app.cuh:395-468).
+// engine_paged.cpp — out-of-core (paged) BFS.  A bounded pool of device
+// page slots caches the pages held in pinned host memory; every level the
+// host classifies the active pages (cached / full copy / compacted copy,
+// src/pagehandle.h:326-379), compacts the cold ones on worker threads
+// (src/compress.h:42-56) and streams copies on a side stream while the
+// exec streams run the push kernel page by page (src/app.cuh:395-468).
+// Levels are synchronous: the frontier of level L is a bitmap, each page
+// runs once per level (more with -multi, which re-runs bound pages), and a
+// vertex is claimed by its first edge exactly as in the resident mode.
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
 #include "paged.h"
 
 namespace scaph {
 
-void Engine::setup_paged() {
-  throw Error(SCAPH_STATUS_INVALID, "out-of-core paged mode is not available in this build");
+namespace {
+
+size_t bitmap_words(uint32_t n) { return (((size_t)n + 63) / 64) * 2; }
+
+// Active Node records of one page plus their adjacency, re-indexed: the
+// layout compress_stat writes (src/compress.h:18-40).  Returns words used.
+uint64_t compact_page(const uint32_t* page, const scaph_page& pg, const uint32_t* front,
+                      uint32_t active_nodes, uint32_t* out) {
+  const uint32_t* edges = page + 3ull * pg.nodenum;
+  uint32_t* eout = out + 3ull * active_nodes;
+  uint64_t off = 0;
+  uint32_t cnt = 0;
+  for (uint32_t i = 0; i < pg.nodenum; ++i) {
+    const uint32_t v = pg.left + i;
+    if (!((front[v >> 5] >> (v & 31)) & 1u)) continue;
+    const uint32_t idx = page[3ull * i + 1], len = page[3ull * i + 2];
+    if (cnt < active_nodes) {
+      out[3ull * cnt] = v;
+      out[3ull * cnt + 1] = (uint32_t)off;
+      out[3ull * cnt + 2] = len;
+      std::memcpy(eout + off, edges + idx, (size_t)len * 4);
+    }
+    off += len;
+    ++cnt;
+  }
+  if (cnt != active_nodes) throw Error(SCAPH_STATUS_INVALID, "compaction count mismatch");
+  return 3ull * active_nodes + off;
 }
 
-RunStats Engine::bfs_paged(uint32_t) {
-  throw Error(SCAPH_STATUS_INVALID, "out-of-core paged mode is not available in this build");
+}  // namespace
+
+void Engine::setup_paged() {
+  paged_.reset(new PagedState());
+  PagedState& ps = *paged_;
+  const size_t np = pages_out_.size();
+  ps.chunk = P_ / 32;
+  ps.thres = (uint64_t)cfg_.threshold * ps.chunk;
+  // pinned host pages (src/graph.h:96-104); the compaction area is sized by
+  // the threshold instead of the reference's fixed half page
+  ps.h_page.resize(np);
+  ps.h_tail.resize(np);
+  for (size_t p = 0; p < np; ++p) {
+    const uint64_t used = page_used_words(pages_out_[p], false);
+    ps.h_page[p].alloc(std::max<uint64_t>(used, 1) * 4);
+    fill_page(host_, pages_out_[p], false, ps.h_page[p].get<uint32_t>());
+    if (ps.thres) ps.h_tail[p].alloc(ps.thres * 4);
+  }
+  ps.pool.alloc((size_t)pool_size_ * P_ * 4);
+  ps.copy.reset(new Stream());
+  ps.copy_beg.reset(new Event());
+  ps.copy_end.reset(new Event());
+  const int ns = cfg_.num_stream;
+  uint32_t max_nodes = 1;
+  for (const auto& pg : pages_out_) max_nodes = std::max(max_nodes, pg.nodenum);
+  ps.ws_bytes = scaph_push_workspace_bytes(max_nodes, P_);
+  for (int s = 0; s < ns; ++s) {
+    ps.exec.emplace_back(new Stream());
+    ps.exec_done.emplace_back(new Event());
+    ps.ws.emplace_back(ps.ws_bytes);
+  }
+  ps.h_nodes.alloc(np * 4 + 64);
+  ps.h_datas.alloc(np * 4 + 64);
+  ps.h_totals.alloc(64);
+  ps.h_front.alloc(bitmap_words(n_) * 4);
+  // device page list for the count kernel
+  dpages_out_.alloc(std::max<size_t>(np, 1) * sizeof(scaph_page));
+  H_ERR(scaph_copy_h2d_async(dpages_out_.get(), pages_out_.data(), np * sizeof(scaph_page),
+                             main_->get()));
+  // initial pool fill: the first pool_size pages (src/app.cuh:330-339)
+  for (int d = 0; d < pool_size_; ++d) {
+    const uint64_t used = page_used_words(pages_out_[d], false);
+    H_ERR(scaph_copy_h2d_async(ps.pool.get<uint32_t>() + (uint64_t)d * P_,
+                               ps.h_page[d].get(), used * 4, ps.copy->get()));
+  }
+  ps.copy->sync();
+  main_->sync();
+  ps.handle.init((int)np, pool_size_, pages_out_, ps.thres, cfg_.multi);
+}
+
+RunStats Engine::bfs_paged(uint32_t src) {
+  PagedState& ps = *paged_;
+  void* st = main_->get();
+  RunStats rs;
+  rs.src = src;
+  const size_t np = pages_out_.size();
+  const size_t words = bitmap_words(n_);
+  auto* cnt = counters_.get<scaph_level_counters>();
+  auto* hc = hcount_.get<scaph_level_counters>();
+  uint32_t* level = level_.get<uint32_t>();
+  uint32_t* visited = visited_.get<uint32_t>();
+  const uint32_t* deg = deg_out_.get<uint32_t>();
+  const auto t0 = std::chrono::steady_clock::now();
+
+  H_ERR(scaph_bfs_init(level, visited, nullptr, n_, src, nullptr, cnt, st));
+  int fa = 0;
+  // level-0 frontier bitmap: the source alone
+  *hcount_.get<uint32_t>() = src;
+  H_ERR(scaph_copy_h2d_async(queue_[0].get(), hcount_.get(), 4, st));
+  H_ERR(scaph_frontier_queue_to_bitmap(queue_[0].get<uint32_t>(), 1, front_[0].get<uint32_t>(), n_,
+                                       st));
+  main_->sync();
+  uint64_t n_f = 1, m_f = out_degree(src), m_u = m_ - m_f;
+  rs.n_reached = 1;
+  rs.m_reached = m_f;
+  uint32_t cur = 0;
+  const int ns = (int)ps.exec.size();
+  while (n_f > 0) {
+    const auto tl = std::chrono::steady_clock::now();
+    // get_page_info / get_total_info + D2H (src/app.cuh:188-207)
+    H_ERR(scaph_page_counts(dpages_out_.get<scaph_page>(), (uint32_t)np, front_[fa].get<uint32_t>(),
+                            deg, 0, page_nodes_.get<uint32_t>(), page_datas_.get<uint32_t>(),
+                            totals_.get<unsigned long long>(), st));
+    H_ERR(scaph_copy_d2h_async(ps.h_nodes.get(), page_nodes_.get(), np * 4, st));
+    H_ERR(scaph_copy_d2h_async(ps.h_datas.get(), page_datas_.get(), np * 4, st));
+    H_ERR(scaph_copy_d2h_async(ps.h_totals.get(), totals_.get(), 16, st));
+    H_ERR(scaph_fill_u32(front_[fa ^ 1].get<uint32_t>(), 0u, words, st));
+    H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, sizeof(scaph_level_counters) / 4, st));
+    main_->sync();
+    const uint32_t* nodes = ps.h_nodes.get<uint32_t>();
+    const uint32_t* datas = ps.h_datas.get<uint32_t>();
+    if (ps.h_totals.get<unsigned long long>()[1] == 0) break;
+    LevelRecord rec;
+    rec.level = cur;
+    rec.dir = 0;
+    rec.n_f = n_f;
+    rec.m_f = m_f;
+    rec.m_u = m_u;
+    page_histogram(nodes, &rec);
+
+    // cold pages need the frontier on the host for compaction
+    std::vector<int> cold;
+    for (size_t p = 0; p < np; ++p)
+      if (!ps.handle.vp[p].cached && datas[p] > 0 && datas[p] <= ps.thres) cold.push_back((int)p);
+    if (!cold.empty()) {
+      H_ERR(scaph_copy_d2h_async(ps.h_front.get(), front_[fa].get(), words * 4, st));
+      main_->sync();
+    }
+    ps.resident_at_reset.assign(np, 0);
+    for (size_t p = 0; p < np; ++p) ps.resident_at_reset[p] = ps.handle.vp[p].cached ? 1 : 0;
+    ps.handle.queue_reset(datas);
+    rs.pages_full += ps.handle.num_full;
+
+    // compaction workers (parallel_compress_stat_async, src/compress.h:42-56)
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> workers;
+    const int nw = (int)std::min<size_t>(cold.size(), 8);
+    std::atomic<bool> failed{false};
+    for (int w = 0; w < nw; ++w) {
+      workers.emplace_back([&] {
+        for (size_t k = next++; k < cold.size(); k = next++) {
+          const int p = cold[k];
+          try {
+            const uint64_t used = compact_page(ps.h_page[p].get<uint32_t>(), pages_out_[p],
+                                               ps.h_front.get<uint32_t>(), nodes[p],
+                                               ps.h_tail[p].get<uint32_t>());
+            VPstat& v = ps.handle.vp[p];
+            v.nodenum = nodes[p];
+            v.datanum = used;
+            v.chunknum = (int)((used + ps.chunk - 1) / ps.chunk);
+            v.shared = true;
+            ps.handle.merge_write(p);
+          } catch (...) {
+            failed = true;
+          }
+        }
+      });
+    }
+    rs.pages_compacted += cold.size();
+
+    // scheduling loop (gpu_push_batch, src/app.cuh:414-459), event driven
+    std::vector<int> last(ns, -1);
+    int copy_last = -1;
+    float copy_ms = 0;
+    bool copy_timed = false;
+    while (!ps.handle.finished()) {
+      if (failed) break;
+      bool progress = false;
+      for (int s = 0; s < ns; ++s) {
+        if (!ps.exec_done[s]->done()) continue;
+        if (last[s] != -1) {
+          ps.handle.release_exec_finished(last[s]);
+          last[s] = -1;
+          progress = true;
+        }
+        int e;
+        if (ps.handle.get_execute(&e)) {
+          const VPstat& v = ps.handle.vp[e];
+          const uint32_t* buf = ps.pool.get<uint32_t>() + (uint64_t)v.dpid * P_ +
+                                (uint64_t)v.dpoff * ps.chunk;
+          const uint64_t edges_bound = v.shared ? v.datanum - 3ull * v.nodenum
+                                                : (uint64_t)pages_out_[e].edgenum;
+          H_ERR(scaph_bfs_push_page(buf, v.nodenum, std::max<uint64_t>(edges_bound, 1), n_, deg,
+                                    level, visited, front_[fa].get<uint32_t>(),
+                                    front_[fa ^ 1].get<uint32_t>(), cnt, cur, ps.ws[s].get(),
+                                    ps.ws_bytes, ps.exec[s]->get()));
+          ps.exec_done[s]->record(ps.exec[s]->get());
+          last[s] = e;
+          ++rs.page_execs;
+          if (ps.resident_at_reset[e]) ++rs.page_execs_cached;
+          progress = true;
+        }
+      }
+      if (ps.copy_end->done()) {
+        if (copy_last != -1) {
+          if (copy_timed) copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
+          ps.handle.release_copy_finished(copy_last);
+          copy_last = -1;
+          progress = true;
+        }
+        int s;
+        if (ps.handle.get_empty(&s)) {
+          // PageCopy (src/app.cuh:371-393): a compacted run into its chunks,
+          // or the whole page into its slot
+          const VPstat& v = ps.handle.vp[s];
+          uint32_t* dst = ps.pool.get<uint32_t>() + (uint64_t)v.dpid * P_ +
+                          (uint64_t)v.dpoff * ps.chunk;
+          const void* srcp = v.shared ? ps.h_tail[s].get() : ps.h_page[s].get();
+          const uint64_t w = v.shared ? v.datanum : page_used_words(pages_out_[s], false);
+          ps.copy_beg->record(ps.copy->get());
+          H_ERR(scaph_copy_h2d_async(dst, srcp, w * 4, ps.copy->get()));
+          ps.copy_end->record(ps.copy->get());
+          copy_timed = true;
+          rs.bytes_h2d += w * 4;
+          copy_last = s;
+          progress = true;
+        }
+      }
+      if (!progress) std::this_thread::yield();
+    }
+    for (auto& t : workers) t.join();
+    if (failed) throw Error(SCAPH_STATUS_INVALID, "page compaction failed");
+    for (int s = 0; s < ns; ++s) ps.exec[s]->sync();
+    ps.copy->sync();
+    if (copy_last != -1) {
+      copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
+      ps.handle.release_copy_finished(copy_last);
+    }
+    rs.h2d_ms += copy_ms;
+    H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
+    main_->sync();
+    rec.n_next = hc->n_next;
+    rec.m_next = hc->m_next;
+    rec.scanned = hc->scanned;
+    rec.ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl)
+                 .count();
+    rec.kernel_ms = rec.ms;
+    rs.levels.push_back(rec);
+    n_f = hc->n_next;
+    m_f = hc->m_next;
+    m_u = m_u > m_f ? m_u - m_f : 0;
+    rs.n_reached += n_f;
+    rs.m_reached += m_f;
+    fa ^= 1;
+    ++cur;
+  }
+  main_->sync();
+  rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  rs.depth = cur;
+  return rs;
+}
+
+// 12-bin histogram of the per-page active-vertex share (fraction of pages
+// per bin; bin 0 = no active vertex, bins 1..11 split (0, 1] evenly in
+// log2 of the share from 2^-10 up).
+void Engine::page_histogram(const uint32_t* nodes, LevelRecord* rec) {
+  rec->page_hist.fill(0.f);
+  const size_t np = pages_out_.size();
+  if (np == 0) return;
+  for (size_t p = 0; p < np; ++p) {
+    const double share = (double)nodes[p] / std::max<uint32_t>(pages_out_[p].nodenum, 1);
+    int bin = 0;
+    if (share > 0) {
+      const double l = std::log2(share);  // <= 0
+      bin = 1 + (int)std::min(10.0, std::max(0.0, std::floor(l + 11.0)));
+      if (bin > 11) bin = 11;
+    }
+    rec->page_hist[bin] += 1.f;
+  }
+  for (auto& x : rec->page_hist) x /= (float)np;
 }
 
 }  // namespace scaph

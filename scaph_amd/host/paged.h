@@ -2,11 +2,31 @@
 #ifndef SCAPH_HOST_PAGED_H
 #define SCAPH_HOST_PAGED_H
 
+#include <memory>
+#include <vector>
+
 #include "engine.h"
+#include "pagehandle.h"
 
 namespace scaph {
 
-struct Engine::PagedState {};
+struct Engine::PagedState {
+  std::vector<PinnedBuf> h_page;  // page content (Node records + neighbours), used words
+  std::vector<PinnedBuf> h_tail;  // compacted copy of a cold page (<= thres words)
+  DevBuf pool;                    // pool_size slots of P words
+  std::vector<std::unique_ptr<Stream>> exec;
+  std::unique_ptr<Stream> copy;
+  std::vector<std::unique_ptr<Event>> exec_done;
+  std::vector<DevBuf> ws;
+  size_t ws_bytes = 0;
+  std::unique_ptr<Event> copy_beg, copy_end;
+  PageHandle handle;
+  PinnedBuf h_nodes, h_datas, h_totals, h_front;
+  uint64_t thres = 0;  // words (threshold * P/32)
+  uint64_t chunk = 0;  // words per chunk (P/32)
+  // pages resident at the start of the iteration (page hit accounting)
+  std::vector<char> resident_at_reset;
+};
 
 }  // namespace scaph
 

@@ -24,6 +24,8 @@ def say(*a):
 
 
 def engine(*flags):
+    if any(f.startswith("-max_memory") for f in flags):
+        return scaph_amd.BFS(*flags)
     return scaph_amd.BFS("-max_memory=65536", *flags)
 
 
@@ -184,3 +186,27 @@ def test_page_counts_kernel_vs_oracle():
         np.testing.assert_array_equal(tot, wt)
     for p in bufs.values():
         hip.scaph_dev_free(p)
+
+
+@pytest.mark.parametrize("page_words,max_mib,threshold,multi,streams", [
+    (65536, 1, 3, 1, 5), (65536, 1, 16, 1, 2), (16384, 1, 0, 1, 5), (16384, 1, 8, 2, 3),
+    (8192, 1, 12, 1, 1)])
+def test_paged_out_of_core_parity(page_words, max_mib, threshold, multi, streams):
+    """Out-of-core mode: device pool smaller than the page set, pages stream
+    from pinned memory (full copies and compacted cold pages)."""
+    off, ngh = O.rmat_csr(14, 16, seed=6)
+    deg = np.diff(off.astype(np.int64))
+    with engine(f"-page_words={page_words}", f"-max_memory={max_mib}",
+                f"-threshold={threshold}", f"-multi={multi}", f"-num_stream={streams}") as e:
+        e.load_csr(off, ngh)
+        inf = e.info()
+        assert not inf["resident"], inf
+        for src in (int(np.argmax(deg)), int(np.nonzero(deg)[0][3])):
+            say("paged", page_words, max_mib, threshold, multi, streams, "src", src)
+            want = O.bfs(off, ngh, src)
+            st = e.run(src)
+            np.testing.assert_array_equal(e.levels(), want)
+            assert st["page_execs"] >= 1 and st["bytes_h2d"] > 0
+            if threshold == 0:
+                assert st["pages_compacted"] == 0
+            assert e.check(src) == 0

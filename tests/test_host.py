@@ -142,3 +142,53 @@ def test_host_cpu_bfs_equals_oracle():
         assert L.scaph_host_cpu_bfs(n, off.ctypes.data, g.ctypes.data, src, 0xFFFFFFFF,
                                     out.ctypes.data) == 0
         np.testing.assert_array_equal(out, z[f"{name}__levels"])
+
+
+# ------------------------------------------------------- page scheduler
+
+def test_chunk_masks_match_oracle():
+    L = _lib.host()
+    Ob = O.lib()
+    for f in ("scaph_host_max_zero_run",):
+        getattr(L, f).argtypes = [ctypes.c_uint32]
+        getattr(L, f).restype = ctypes.c_int
+    L.scaph_host_fit_zero_run.argtypes = [ctypes.c_uint32, ctypes.c_int]
+    L.scaph_host_fit_zero_run.restype = ctypes.c_int
+    L.scaph_host_set_run.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+    L.scaph_host_set_run.restype = ctypes.c_uint32
+    L.scaph_host_clear_run.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+    L.scaph_host_clear_run.restype = ctypes.c_uint32
+    rng = np.random.default_rng(8)
+    masks = list(range(1 << 14)) + [int(x) for x in rng.integers(0, 1 << 32, 30000, dtype=np.int64)]
+    for m in masks:
+        mi = int(np.uint32(m).view(np.int32))
+        assert L.scaph_host_max_zero_run(m) == Ob.oracle_max_zero_run(mi)
+        r = int(m % 32) + 1
+        assert L.scaph_host_fit_zero_run(m, r) == Ob.oracle_fit_zero_run(mi, r), (hex(m), r)
+        off, num = int(m % 29), int((m >> 5) % 4)
+        assert L.scaph_host_set_run(m, off, num) == (Ob.oracle_set_run(mi, off, num) & 0xFFFFFFFF)
+        assert L.scaph_host_clear_run(m, off, num) == (Ob.oracle_clear_run(mi, off, num) &
+                                                       0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("list_size,pool,thres,multi,seed", [
+    (40, 8, 3, 1, 1), (40, 8, 3, 2, 2), (100, 10, 12, 1, 3), (16, 16, 4, 1, 4),
+    (64, 3, 16, 3, 5), (12, 1, 8, 1, 6)])
+def test_page_scheduler_invariants(list_size, pool, thres, multi, seed):
+    """Every active page runs at least once per iteration; a page runs only
+    when resident; chunk runs in shared slots never overlap and match the
+    slot masks (simulated executor, random completion order)."""
+    L = _lib.host()
+    L.scaph_host_pagehandle_simulate.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
+                                                 ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
+                                                 ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    L.scaph_host_pagehandle_simulate.restype = ctypes.c_int
+    rng = np.random.default_rng(seed)
+    iters = 12
+    # mix of inactive, cold (<= thres) and hot (> thres) pages, in chunk units
+    datas = rng.choice([0, 0, 1, 2, thres, thres + 1, 32], size=(iters, list_size)).astype(np.uint32)
+    stats = np.zeros(4, np.uint64)
+    rc = L.scaph_host_pagehandle_simulate(list_size, pool, thres, multi, iters, datas.ctypes.data,
+                                          seed, stats.ctypes.data)
+    assert rc == 0, f"invariant {rc} violated"
+    assert stats[0] >= (datas > 0).sum()
