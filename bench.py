@@ -141,6 +141,14 @@ def main():
     wall = time.perf_counter() - t_start
 
     gteps = [r["m_reached"] / (r["ms"] / 1e3) / 1e9 for r in runs]
+    mid = sorted(range(len(runs)), key=lambda i: gteps[i])[len(runs) // 2]
+    for rec in runs[mid]["records"]:
+        log("level", rec["level"], "dir", "pull" if rec["dir"] else "push", "n_f", rec["n_f"],
+            "m_f", rec["m_f"], "n_next", rec["n_next"], "scanned", rec["scanned"],
+            "examined", rec["examined"], f"kernel_ms {rec['kernel_ms']:.3f}",
+            f"step_ms {rec['ms']:.3f}")
+    log(f"median run: t_bfs {runs[mid]['ms']:.3f} ms, n_c {runs[mid]['n_reached']}, "
+        f"m_c {runs[mid]['m_reached']}")
     balg = [4 * r["m_reached"] + 12 * r["n_reached"] + 4 * n for r in runs]
     frac = [b / (r["ms"] / 1e3) / HBM_PEAK for b, r in zip(balg, runs)]
     ref_gteps = [info["m"] / (r["ms"] / 1e3) / 1e9 for r in runs]
