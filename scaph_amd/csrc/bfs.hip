@@ -156,8 +156,8 @@ __device__ __forceinline__ void prep_store(bool keep, uint64_t ep, uint32_t len,
 __global__ void __launch_bounds__(kBlock)
 k_prep_queue(scaph_graph_desc g, const uint32_t* q, uint32_t n_in, uint64_t* ent_ep,
              uint32_t* ent_len, uint32_t* blk_cnt, uint64_t* blk_edges) {
-  __shared__ PageLds s;
-  stage_pages(s, g);
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  const PageTab s = stage_pages(g, dyn_lds);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t len = 0;
   uint64_t ep = 0;
@@ -279,6 +279,9 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
          scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
   __shared__ unsigned long long s_gx[kTile + 1];
   __shared__ unsigned long long s_ep[kTile + 1];
+  __shared__ uint32_t s_claim[kTile];
+  __shared__ uint32_t s_cnt;
+  __shared__ unsigned long long s_base;
   __shared__ unsigned long long red[3][kBlock / SCAPH_WAVE];
   const unsigned long long n_ent = totals[0];
   const unsigned long long n_edges = totals[1];
@@ -288,13 +291,15 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
     const unsigned long long s0 = t * kTile;
     const unsigned long long e0 = min(s0 + kTile, n_edges);
     const uint32_t i0 = tstart[t];
-    const uint32_t i1 = (t + 1 < n_tiles) ? tstart[t + 1] : (uint32_t)(n_ent - 1);
+    uint32_t i1 = (t + 1 < n_tiles) ? tstart[t + 1] : (uint32_t)(n_ent - 1);
+    if (i1 < i0) i1 = i0;
     uint32_t cnt = i1 - i0 + 1;
     if (cnt > kTile + 1) cnt = kTile + 1;
     for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
       s_gx[k] = gx[i0 + k];
       s_ep[k] = gep[i0 + k];
     }
+    if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     const unsigned long long r0 = s0 + (unsigned long long)threadIdx.x * kPerThread;
     uint32_t lo = 0, hi = cnt;
@@ -305,39 +310,34 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
 #pragma unroll
     for (uint32_t j = 0; j < kPerThread; ++j) {
       const unsigned long long r = r0 + j;
-      bool claimed = false;
-      uint32_t v = 0;
       if (r < e0) {
         while (lo + 1 < cnt && s_gx[lo + 1] <= r) ++lo;
         const uint32_t* ep = reinterpret_cast<const uint32_t*>(s_ep[lo]);
-        v = ep[r - s_gx[lo]];
+        const uint32_t v = ep[r - s_gx[lo]];
         if (v < n) {
           const uint32_t w = v >> 5, bit = 1u << (v & 31);
           if (!(visited[w] & bit)) {
             const uint32_t old = atomicOr(&visited[w], bit);
             if (!(old & bit)) {
-              claimed = true;
               level[v] = next_level;
               acc_n += 1;
               acc_m += degree[v];
               if (MODE == OUT_BITMAP) atomicOr(&front_out[w], bit);
+              else s_claim[atomicAdd(&s_cnt, 1u)] = v;
             }
           }
         }
       }
-      if (MODE == OUT_QUEUE) {
-        const uint64_t m = __ballot(claimed);
-        if (m) {
-          const uint32_t lane = lane_id();
-          const int leader = __ffsll((unsigned long long)m) - 1;
-          unsigned long long base = 0;
-          if ((int)lane == leader) base = atomicAdd(&counters->q_len, (unsigned long long)__popcll(m));
-          base = wave_bcast64(base, leader);
-          if (claimed) q_out[base + lanes_below(m)] = v;
-        }
-      }
     }
     __syncthreads();
+    if (MODE == OUT_QUEUE) {
+      // one global append per tile, then coalesced queue writes
+      const uint32_t c = s_cnt;
+      if (threadIdx.x == 0 && c) s_base = atomicAdd(&counters->q_len, (unsigned long long)c);
+      __syncthreads();
+      for (uint32_t k = threadIdx.x; k < c; k += kBlock) q_out[s_base + k] = s_claim[k];
+      __syncthreads();
+    }
   }
   const unsigned long long sn = wave_sum(acc_n);
   const unsigned long long sm = wave_sum(acc_m);
@@ -360,82 +360,124 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
 }
 
 // ------------------------------------------------------------------ pull
+//
+// Each wave owns a window of kPullWinWords bitmap words (1024 vertices):
+// one coalesced read of the visited words, the unvisited vertices of the
+// window compacted into LDS (ballot/prefix over popcounts), then processed
+// 64 at a time: a lane reads its vertex's in-Node and probes kPullProbe
+// in-neighbours per step against the frontier bitmap (independent loads,
+// so several misses are in flight per lane), stopping at the first hit.
+// Lists longer than kPullSerial entries finish with the whole wave probing
+// 64 in-neighbours per step.  Found vertices set bits in an LDS copy of the
+// window, written back as whole words (visited |= found, next = found).
+
+constexpr uint32_t kPullWinWords = 32;
+constexpr uint32_t kPullWin = kPullWinWords * 32;
+constexpr uint32_t kPullProbe = 4;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __global__ void __launch_bounds__(kBlock)
 k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* visited,
        const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
        uint32_t next_level) {
-  __shared__ PageLds s;
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  __shared__ uint32_t s_q[kBlock / SCAPH_WAVE][kPullWin];
+  __shared__ uint32_t s_fw[kBlock / SCAPH_WAVE][kPullWinWords];
   __shared__ unsigned long long red[4][kBlock / SCAPH_WAVE];
-  stage_pages(s, g);
+  const PageTab s = stage_pages(g, dyn_lds);
   const uint32_t n = g.num_vertices;
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x / SCAPH_WAVE;
-  const uint64_t nchunks = ((uint64_t)n + 63) / 64;
-  const uint64_t gw = (uint64_t)blockIdx.x * (kBlock / SCAPH_WAVE) + wid;
-  const uint64_t nw = (uint64_t)gridDim.x * (kBlock / SCAPH_WAVE);
+  const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
+  const uint64_t per_block = (uint64_t)kPullWinWords * (kBlock / SCAPH_WAVE);
+  const uint64_t iters = (words + per_block - 1) / per_block;
   unsigned long long acc_n = 0, acc_m = 0, acc_s = 0, acc_x = 0;
-  for (uint64_t chunk = gw; chunk < nchunks; chunk += nw) {
-    const uint32_t v = (uint32_t)(chunk * 64 + lane);
-    const size_t word = chunk * 2 + (lane >> 5);
-    const uint32_t vw = visited[word];
-    const bool unv = v < n && !((vw >> (lane & 31)) & 1u);
-    const uint64_t mu = __ballot(unv);
-    if (mu == 0) {
-      if ((lane & 31) == 0) front_out[word] = 0u;
-      continue;
+  uint32_t* q = s_q[wid];
+  uint32_t* fw = s_fw[wid];
+  for (uint64_t bi = blockIdx.x; bi < iters; bi += gridDim.x) {
+    const uint64_t w0 = bi * per_block + (uint64_t)wid * kPullWinWords;
+    const uint64_t w = w0 + lane;
+    const bool own = lane < kPullWinWords && w < words;
+    const uint32_t vw = own ? visited[w] : ~0u;
+    uint32_t unv = ~vw;
+    const uint32_t c = __popc(unv);
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t tot = wave_bcast(incl, SCAPH_WAVE - 1);
+    uint32_t o = incl - c;
+    while (unv) {
+      const uint32_t b = __ffs(unv) - 1;
+      unv &= unv - 1;
+      q[o++] = (lane << 5) | b;
     }
-    bool found = false;
-    bool longl = false;
-    NodeRef nr;
-    nr.edges = nullptr;
-    nr.len = 0;
-    if (unv) {
-      ++acc_x;
-      nr = load_node(s, g, v);
-      const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
-      uint32_t k = 0;
-      for (; k < lim; ++k) {
-        const uint32_t u = nr.edges[k];
-        if (bit_test(front_in, u)) {
-          found = true;
-          ++k;
-          break;
+    if (lane < kPullWinWords) fw[lane] = 0u;
+    wave_lds_sync();
+    for (uint32_t i0 = 0; i0 < tot; i0 += SCAPH_WAVE) {
+      const uint32_t i = i0 + lane;
+      const bool act = i < tot;
+      const uint32_t loc = act ? q[i] : 0u;
+      const uint32_t v = (uint32_t)(w0 * 32) + loc;
+      bool found = false, longl = false;
+      NodeRef nr;
+      nr.edges = nullptr;
+      nr.len = 0;
+      if (act) {
+        ++acc_x;
+        nr = load_node(s, g, v);
+        const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
+        uint32_t k = 0;
+        for (; k < lim; k += kPullProbe) {
+          uint32_t u[kPullProbe];
+#pragma unroll
+          for (uint32_t j = 0; j < kPullProbe; ++j) u[j] = (k + j < lim) ? nr.edges[k + j] : 0u;
+          bool h = false;
+#pragma unroll
+          for (uint32_t j = 0; j < kPullProbe; ++j) h |= (k + j < lim) && bit_test(front_in, u[j]);
+          if (h) {
+            found = true;
+            k += kPullProbe;
+            break;
+          }
+        }
+        acc_s += k < lim ? k : lim;
+        longl = !found && nr.len > kPullSerial;
+      }
+      uint64_t ml = __ballot(longl);
+      while (ml) {
+        const int j = __ffsll((unsigned long long)ml) - 1;
+        ml &= ml - 1;
+        const uint64_t ep = wave_bcast64(reinterpret_cast<uint64_t>(nr.edges), j);
+        const uint32_t len = wave_bcast(nr.len, j);
+        const uint32_t* e = reinterpret_cast<const uint32_t*>(ep);
+        for (uint32_t k = kPullSerial; k < len; k += SCAPH_WAVE) {
+          const uint32_t kk = k + lane;
+          const bool hit = kk < len && bit_test(front_in, e[kk]);
+          const uint64_t mh = __ballot(hit);
+          if (lane == 0) acc_s += min(len - k, (uint32_t)SCAPH_WAVE);
+          if (mh) {
+            if ((int)lane == j) found = true;
+            break;
+          }
         }
       }
-      acc_s += k;
-      longl = !found && nr.len > kPullSerial;
-    }
-    uint64_t ml = __ballot(longl);
-    while (ml) {
-      const int j = __ffsll((unsigned long long)ml) - 1;
-      ml &= ml - 1;
-      const uint64_t ep = wave_bcast64(reinterpret_cast<uint64_t>(nr.edges), j);
-      const uint32_t len = wave_bcast(nr.len, j);
-      const uint32_t* e = reinterpret_cast<const uint32_t*>(ep);
-      for (uint32_t k = kPullSerial; k < len; k += SCAPH_WAVE) {
-        const uint32_t kk = k + lane;
-        const bool hit = kk < len && bit_test(front_in, e[kk]);
-        const uint64_t mh = __ballot(hit);
-        if (mh) {
-          if ((int)lane == j) found = true;
-          if (lane == 0) acc_s += (uint32_t)(__ffsll((unsigned long long)mh)) + (k - kPullSerial);
-          break;
-        }
-        if (lane == 0) acc_s += min(len - k, (uint32_t)SCAPH_WAVE);
+      if (found) {
+        level[v] = next_level;
+        acc_n += 1;
+        acc_m += degree[v];
+        atomicOr(&fw[loc >> 5], 1u << (loc & 31));
       }
     }
-    const uint64_t mf = __ballot(found);
-    if (found) {
-      level[v] = next_level;
-      acc_n += 1;
-      acc_m += degree[v];
+    wave_lds_sync();
+    if (own) {
+      const uint32_t f = fw[lane];
+      visited[w] = vw | f;
+      front_out[w] = f;
     }
-    if ((lane & 31) == 0) {
-      const uint32_t fw = (uint32_t)(mf >> (lane & 32));
-      visited[word] = vw | fw;
-      front_out[word] = fw;
-    }
+    wave_lds_sync();
   }
   const unsigned long long sn = wave_sum(acc_n);
   const unsigned long long sm = wave_sum(acc_m);
@@ -477,36 +519,34 @@ k_queue_to_bitmap(const uint32_t* q, uint32_t nq, uint32_t* bitmap, uint32_t n) 
 __global__ void __launch_bounds__(kBlock)
 k_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* q,
                   scaph_level_counters* counters) {
-  const size_t words = ((size_t)n + 31) / 32;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const uint32_t lane = lane_id();
-  // uniform trip count across the wave so the ballot-free scan is safe
-  const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x - lane;
-  for (size_t w0 = first; w0 < words; w0 += stride) {
-    const size_t w = w0 + lane;
+  __shared__ uint32_t wtot[kBlock / SCAPH_WAVE];
+  __shared__ unsigned long long s_base;
+  const uint64_t words = ((uint64_t)n + 31) / 32;
+  for (uint64_t b0 = (uint64_t)blockIdx.x * kBlock; b0 < words; b0 += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t w = b0 + threadIdx.x;
     uint32_t bits = 0;
     if (w < words) {
       bits = bitmap[w];
-      const size_t base = w * 32;
+      const uint64_t base = w * 32;
       if (base + 32 > n) {
         const uint32_t valid = (uint32_t)(n - base);
         bits &= valid >= 32 ? ~0u : ((1u << valid) - 1u);
       }
     }
     const uint32_t c = __popc(bits);
-    const uint32_t incl = wave_incl_scan(c);
-    const uint32_t tot = wave_bcast(incl, SCAPH_WAVE - 1);
-    unsigned long long base = 0;
-    if (tot) {
-      if (lane == 0) base = atomicAdd(&counters->q_len, (unsigned long long)tot);
-      base = wave_bcast64(base, 0);
+    uint32_t total;
+    const uint32_t incl = block256_incl_scan(c, wtot, &total);
+    if (threadIdx.x == 0 && total) s_base = atomicAdd(&counters->q_len, (unsigned long long)total);
+    __syncthreads();
+    if (total) {
+      unsigned long long o = s_base + incl - c;
+      while (bits) {
+        const uint32_t b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        q[o++] = (uint32_t)(w * 32 + b);
+      }
     }
-    unsigned long long o = base + incl - c;
-    while (bits) {
-      const uint32_t b = __ffs(bits) - 1;
-      bits &= bits - 1;
-      q[o++] = (uint32_t)(w * 32 + b);
-    }
+    __syncthreads();
   }
 }
 
@@ -565,8 +605,8 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
 
 __global__ void __launch_bounds__(kBlock)
 k_validate_edges(scaph_graph_desc g, const uint32_t* level, unsigned long long* out) {
-  __shared__ PageLds s;
-  stage_pages(s, g);
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  const PageTab s = stage_pages(g, dyn_lds);
   const uint32_t n = g.num_vertices;
   unsigned long long bad = 0;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -586,8 +626,8 @@ k_validate_edges(scaph_graph_desc g, const uint32_t* level, unsigned long long* 
 __global__ void __launch_bounds__(kBlock)
 k_validate_parents(scaph_graph_desc gin, const uint32_t* level, uint32_t src,
                    unsigned long long* out) {
-  __shared__ PageLds s;
-  stage_pages(s, gin);
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  const PageTab s = stage_pages(gin, dyn_lds);
   const uint32_t n = gin.num_vertices;
   unsigned long long bad = 0;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -684,7 +724,8 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree, ui
   if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
   hipStream_t st = as_stream(stream);
   const uint32_t nb = (n_in + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_prep_queue, dim3(nb), dim3(kBlock), 0, st, *out_graph, q_in, n_in,
+  hipLaunchKernelGGL(k_prep_queue, dim3(nb), dim3(kBlock), page_lds_bytes(out_graph->num_pages), st,
+                     *out_graph, q_in, n_in,
                      w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
   return launch_push_tail(w, nb, m_in, degree, level, visited, q_out, nullptr, counters,
@@ -716,10 +757,11 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* degree, uin
                    scaph_level_counters* counters, uint32_t cur_level, void* stream) {
   if (!in_graph || !degree || !level || !visited || !front_in || !front_out || !counters)
     return SCAPH_STATUS_INVALID;
-  const uint64_t nchunks = ((uint64_t)in_graph->num_vertices + 63) / 64;
-  const unsigned grid = grid_for(nchunks, kBlock / SCAPH_WAVE, 256 * 16);
-  hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), 0, as_stream(stream), *in_graph, degree,
-                     level, visited, front_in, front_out, counters, cur_level + 1);
+  const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
+  const unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 256 * 16);
+  hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), page_lds_bytes(in_graph->num_pages),
+                     as_stream(stream), *in_graph, degree, level, visited, front_in, front_out,
+                     counters, cur_level + 1);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }
@@ -742,7 +784,7 @@ int scaph_frontier_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t*
                                    scaph_level_counters* counters, void* stream) {
   if (!bitmap || !queue || !counters) return SCAPH_STATUS_INVALID;
   const size_t words = ((size_t)n + 31) / 32;
-  hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid_for(words, kBlock, 8192)), dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid_for(words, kBlock, 1024)), dim3(kBlock), 0,
                      as_stream(stream), bitmap, n, queue, counters);
   SCAPH_LAUNCH_CHECK();
   return 0;
@@ -770,11 +812,12 @@ int scaph_bfs_validate(const scaph_graph_desc* out_graph, const scaph_graph_desc
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, out, 2u);
   SCAPH_LAUNCH_CHECK();
   const unsigned grid = grid_for(out_graph->num_vertices, kBlock, 16384);
-  hipLaunchKernelGGL(k_validate_edges, dim3(grid), dim3(kBlock), 0, st, *out_graph, level, out);
+  hipLaunchKernelGGL(k_validate_edges, dim3(grid), dim3(kBlock), page_lds_bytes(out_graph->num_pages),
+                     st, *out_graph, level, out);
   SCAPH_LAUNCH_CHECK();
   if (in_graph) {
-    hipLaunchKernelGGL(k_validate_parents, dim3(grid), dim3(kBlock), 0, st, *in_graph, level, src,
-                       out);
+    hipLaunchKernelGGL(k_validate_parents, dim3(grid), dim3(kBlock),
+                       page_lds_bytes(in_graph->num_pages), st, *in_graph, level, src, out);
     SCAPH_LAUNCH_CHECK();
   }
   return 0;

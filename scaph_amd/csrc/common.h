@@ -93,40 +93,62 @@ __device__ __forceinline__ bool bit_test(const uint32_t* bm, uint32_t v) {
   return (bm[v >> 5] >> (v & 31)) & 1u;
 }
 
-// Page table staged in LDS: lefts, node counts and buffer addresses.
+// Page table staged in LDS: lefts, node counts and buffer addresses of up
+// to SCAPH_LDS_PAGES pages, in the kernel's dynamic LDS (launch with
+// page_lds_bytes(num_pages) bytes); larger page sets are searched in HBM.
 #define SCAPH_LDS_PAGES 1024
 
-struct PageLds {
-  uint32_t left[SCAPH_LDS_PAGES];
-  uint32_t nodenum[SCAPH_LDS_PAGES];
-  uint64_t ptr[SCAPH_LDS_PAGES];
+static inline size_t page_lds_bytes(uint32_t num_pages) {
+  const size_t s = num_pages < SCAPH_LDS_PAGES ? num_pages : SCAPH_LDS_PAGES;
+  return s * 16;
+}
+
+struct PageTab {
+  const uint64_t* ptr;
+  const uint32_t* left;
+  const uint32_t* nodenum;
+  bool in_lds;
 };
 
-__device__ __forceinline__ void stage_pages(PageLds& s, const scaph_graph_desc& g) {
-  const uint32_t np = g.num_pages < SCAPH_LDS_PAGES ? g.num_pages : SCAPH_LDS_PAGES;
+__device__ __forceinline__ PageTab stage_pages(const scaph_graph_desc& g, void* lds) {
+  PageTab t;
+  t.in_lds = g.num_pages <= SCAPH_LDS_PAGES;
+  if (!t.in_lds) {
+    t.ptr = nullptr;
+    t.left = nullptr;
+    t.nodenum = nullptr;
+    return t;
+  }
+  const uint32_t np = g.num_pages;
+  uint64_t* ptr = static_cast<uint64_t*>(lds);
+  uint32_t* left = reinterpret_cast<uint32_t*>(ptr + np);
+  uint32_t* nn = left + np;
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
-    s.left[p] = g.pages[p].left;
-    s.nodenum[p] = g.pages[p].nodenum;
-    s.ptr[p] = g.page_ptr[p];
+    ptr[p] = g.page_ptr[p];
+    left[p] = g.pages[p].left;
+    nn[p] = g.pages[p].nodenum;
   }
   __syncthreads();
+  t.ptr = ptr;
+  t.left = left;
+  t.nodenum = nn;
+  return t;
 }
 
 // Page holding vertex v: the last page whose left <= v.
-__device__ __forceinline__ uint32_t find_page(const PageLds& s, const scaph_graph_desc& g,
+__device__ __forceinline__ uint32_t find_page(const PageTab& t, const scaph_graph_desc& g,
                                               uint32_t v) {
-  if (g.num_pages <= SCAPH_LDS_PAGES) {
-    uint32_t lo = 0, hi = g.num_pages;  // invariant: left[lo] <= v < left[hi]
+  uint32_t lo = 0, hi = g.num_pages;  // invariant: left[lo] <= v < left[hi]
+  if (t.in_lds) {
     while (hi - lo > 1) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (s.left[mid] <= v) lo = mid; else hi = mid;
+      const uint32_t mid = (lo + hi) >> 1;
+      if (t.left[mid] <= v) lo = mid; else hi = mid;
     }
-    return lo;
-  }
-  uint32_t lo = 0, hi = g.num_pages;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (g.pages[mid].left <= v) lo = mid; else hi = mid;
+  } else {
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (g.pages[mid].left <= v) lo = mid; else hi = mid;
+    }
   }
   return lo;
 }
@@ -136,15 +158,15 @@ struct NodeRef {
   uint32_t len;           // out- (or in-) degree of v
 };
 
-__device__ __forceinline__ NodeRef load_node(const PageLds& s, const scaph_graph_desc& g,
+__device__ __forceinline__ NodeRef load_node(const PageTab& t, const scaph_graph_desc& g,
                                              uint32_t v) {
-  const uint32_t p = find_page(s, g, v);
+  const uint32_t p = find_page(t, g, v);
   uint32_t left, nn;
   const uint32_t* base;
-  if (g.num_pages <= SCAPH_LDS_PAGES) {
-    left = s.left[p];
-    nn = s.nodenum[p];
-    base = reinterpret_cast<const uint32_t*>(s.ptr[p]);
+  if (t.in_lds) {
+    left = t.left[p];
+    nn = t.nodenum[p];
+    base = reinterpret_cast<const uint32_t*>(t.ptr[p]);
   } else {
     left = g.pages[p].left;
     nn = g.pages[p].nodenum;
