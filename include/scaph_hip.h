@@ -144,11 +144,15 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree,
 /* Bottom-up (pull) expand of one level: every unvisited vertex scans its
  * in-neighbours (transposed paged CSR) for one in the frontier bitmap.
  * Writes level/visited for the vertices found and the full next-frontier
- * bitmap (all words, so it needs no clearing).  No reference counterpart
- * (the reference engine is push-only). */
+ * bitmap (all words, so it needs no clearing).  In-lists longer than 512
+ * entries are finished by a second, grid-wide pass.  max_edges bounds the
+ * graph's edges (sizes the workspace: scaph_pull_workspace_bytes).  No
+ * reference counterpart (the reference engine is push-only). */
+size_t scaph_pull_workspace_bytes(unsigned long long max_edges);
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* degree, uint32_t* level,
                    uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
-                   scaph_level_counters* counters, uint32_t cur_level, void* stream);
+                   scaph_level_counters* counters, uint32_t cur_level, unsigned long long max_edges,
+                   void* workspace, size_t workspace_bytes, void* stream);
 
 /* Frontier format conversions (no reference counterpart). */
 int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint32_t* bitmap,

@@ -23,6 +23,8 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kTile = 1024;                   // edges per expand tile
 constexpr uint32_t kPerThread = kTile / kBlock;    // 4 consecutive edges
 constexpr uint32_t kPullSerial = 16;               // per-lane scan before wave help
+constexpr uint32_t kHeavy = 512;                   // longer in-lists go to k_pull_heavy
+constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work item
 constexpr uint32_t kScanThreads = 1024;
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -68,6 +70,11 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
 }
 
 // ------------------------------------------------------------------ init
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 
 __global__ void __launch_bounds__(kBlock)
 k_bfs_init(uint32_t* level, uint32_t* visited, const uint32_t* static_visited, uint32_t n,
@@ -371,6 +378,58 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
 // 64 in-neighbours per step.  Found vertices set bits in an LDS copy of the
 // window, written back as whole words (visited |= found, next = found).
 
+struct HeavyChunk {
+  uint32_t v;
+  uint32_t len;
+  uint64_t ptr;
+};
+
+struct PullWs {
+  unsigned long long* count;  // heavy chunks appended
+  HeavyChunk* chunks;
+  unsigned long long cap;
+  size_t bytes;
+};
+
+PullWs carve_pull_ws(void* base, unsigned long long m) {
+  PullWs w;
+  w.cap = m / kHeavyChunk + m / kHeavy + 64;
+  w.bytes = 256 + w.cap * sizeof(HeavyChunk);
+  char* p = static_cast<char*>(base);
+  w.count = reinterpret_cast<unsigned long long*>(p);
+  w.chunks = p ? reinterpret_cast<HeavyChunk*>(p + 256) : nullptr;
+  return w;
+}
+
+__device__ __forceinline__ uint32_t page_left(const PageTab& t, const scaph_graph_desc& g,
+                                              uint32_t p) {
+  return t.in_lds ? t.left[p] : g.pages[p].left;
+}
+
+// Node of v, starting the page search at `hint` (a page with left <= v).
+__device__ __forceinline__ NodeRef load_node_hint(const PageTab& t, const scaph_graph_desc& g,
+                                                  uint32_t v, uint32_t hint) {
+  uint32_t p = hint;
+  while (p + 1 < g.num_pages && page_left(t, g, p + 1) <= v) ++p;
+  uint32_t left, nn;
+  const uint32_t* base;
+  if (t.in_lds) {
+    left = t.left[p];
+    nn = t.nodenum[p];
+    base = reinterpret_cast<const uint32_t*>(t.ptr[p]);
+  } else {
+    left = g.pages[p].left;
+    nn = g.pages[p].nodenum;
+    base = reinterpret_cast<const uint32_t*>(g.page_ptr[p]);
+  }
+  const uint32_t i = v - left;
+  NodeRef r;
+  const uint32_t idx = base[3u * i + 1u];
+  r.len = base[3u * i + 2u];
+  r.edges = base + 3ull * nn + idx;
+  return r;
+}
+
 constexpr uint32_t kPullWinWords = 32;
 constexpr uint32_t kPullWin = kPullWinWords * 32;
 constexpr uint32_t kPullProbe = 4;
@@ -384,7 +443,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 __global__ void __launch_bounds__(kBlock)
 k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* visited,
        const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
-       uint32_t next_level) {
+       uint32_t next_level, PullWs hw) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
   __shared__ uint32_t s_q[kBlock / SCAPH_WAVE][kPullWin];
   __shared__ uint32_t s_fw[kBlock / SCAPH_WAVE][kPullWinWords];
@@ -416,6 +475,7 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
     }
     if (lane < kPullWinWords) fw[lane] = 0u;
     wave_lds_sync();
+    const uint32_t hint = tot ? find_page(s, g, (uint32_t)(w0 * 32)) : 0u;
     for (uint32_t i0 = 0; i0 < tot; i0 += SCAPH_WAVE) {
       const uint32_t i = i0 + lane;
       const bool act = i < tot;
@@ -425,9 +485,28 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
       NodeRef nr;
       nr.edges = nullptr;
       nr.len = 0;
+      bool heavy = false;
       if (act) {
         ++acc_x;
-        nr = load_node(s, g, v);
+        nr = load_node_hint(s, g, v, hint);
+        heavy = nr.len > kHeavy;
+      }
+      if (heavy) {
+        // grid-wide follow-up: one work item per kHeavyChunk in-edges
+        const uint32_t nch = (nr.len + kHeavyChunk - 1) / kHeavyChunk;
+        const unsigned long long at = atomicAdd(hw.count, (unsigned long long)nch);
+        if (at + nch <= hw.cap) {
+          for (uint32_t c = 0; c < nch; ++c) {
+            HeavyChunk h;
+            h.v = v;
+            h.len = min(kHeavyChunk, nr.len - c * kHeavyChunk);
+            h.ptr = reinterpret_cast<uint64_t>(nr.edges + (uint64_t)c * kHeavyChunk);
+            hw.chunks[at + c] = h;
+          }
+        } else {
+          counters->reserved = 1;  // overflow: the host reports an error
+        }
+      } else if (act) {
         const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
         uint32_t k = 0;
         for (; k < lim; k += kPullProbe) {
@@ -505,6 +584,54 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
   }
 }
 
+// Heavy in-lists deferred by k_pull: each block takes one chunk of
+// kHeavyChunk in-edges of one vertex and probes them 256 at a time against
+// the frontier, stopping at the first hit; the first block to hit claims
+// the vertex (atomicOr on the visited word decides).
+__global__ void __launch_bounds__(kBlock)
+k_pull_heavy(const HeavyChunk* chunks, const unsigned long long* count, const uint32_t* degree,
+             uint32_t* level, uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
+             scaph_level_counters* counters, uint32_t next_level) {
+  __shared__ HeavyChunk s_c;
+  __shared__ int s_skip;
+  const unsigned long long total = *count;
+  unsigned long long scanned = 0;
+  for (unsigned long long e = blockIdx.x; e < total; e += gridDim.x) {
+    if (threadIdx.x == 0) {
+      s_c = chunks[e];
+      s_skip = bit_test(visited, s_c.v) ? 1 : 0;  // another chunk already won
+    }
+    __syncthreads();
+    const HeavyChunk c = s_c;
+    const int skip = s_skip;
+    __syncthreads();
+    if (skip) continue;
+    const uint32_t* ep = reinterpret_cast<const uint32_t*>(c.ptr);
+    bool hit = false;
+    for (uint32_t k0 = 0; k0 < c.len; k0 += kBlock) {
+      const uint32_t k = k0 + threadIdx.x;
+      const bool h = k < c.len && bit_test(front_in, ep[k]);
+      scanned += (k < c.len) ? 1 : 0;
+      if (__syncthreads_or(h)) {
+        hit = true;
+        break;
+      }
+    }
+    if (hit && threadIdx.x == 0) {
+      const uint32_t v = c.v, w = v >> 5, bit = 1u << (v & 31);
+      const uint32_t old = atomicOr(&visited[w], bit);
+      if (!(old & bit)) {
+        level[v] = next_level;
+        atomicOr(&front_out[w], bit);
+        atomicAdd(&counters->n_next, 1ull);
+        atomicAdd(&counters->m_next, (unsigned long long)degree[v]);
+      }
+    }
+  }
+  scanned = wave_sum(scanned);
+  if (lane_id() == 0 && scanned) atomicAdd(&counters->scanned, scanned);
+}
+
 // ------------------------------------------------------- conversions
 
 __global__ void __launch_bounds__(kBlock)
@@ -516,37 +643,49 @@ k_queue_to_bitmap(const uint32_t* q, uint32_t nq, uint32_t* bitmap, uint32_t n) 
   }
 }
 
+// Frontier bitmap -> vertex list.  Each block owns a contiguous range of
+// words: it counts the set bits of its range, takes its slice of the list
+// with a single atomic, then writes the vertices in order.
 __global__ void __launch_bounds__(kBlock)
 k_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* q,
                   scaph_level_counters* counters) {
   __shared__ uint32_t wtot[kBlock / SCAPH_WAVE];
   __shared__ unsigned long long s_base;
   const uint64_t words = ((uint64_t)n + 31) / 32;
-  for (uint64_t b0 = (uint64_t)blockIdx.x * kBlock; b0 < words; b0 += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t w = b0 + threadIdx.x;
-    uint32_t bits = 0;
-    if (w < words) {
-      bits = bitmap[w];
-      const uint64_t base = w * 32;
-      if (base + 32 > n) {
-        const uint32_t valid = (uint32_t)(n - base);
-        bits &= valid >= 32 ? ~0u : ((1u << valid) - 1u);
-      }
+  uint64_t per = (words + gridDim.x - 1) / gridDim.x;
+  per = (per + kBlock - 1) / kBlock * kBlock;
+  const uint64_t b0 = (uint64_t)blockIdx.x * per;
+  const uint64_t b1 = min(b0 + per, words);
+  auto word_bits = [&](uint64_t w) -> uint32_t {
+    uint32_t bits = bitmap[w];
+    const uint64_t base = w * 32;
+    if (base + 32 > n) {
+      const uint32_t valid = (uint32_t)(n - base);
+      bits &= valid >= 32 ? ~0u : ((1u << valid) - 1u);
     }
+    return bits;
+  };
+  uint32_t cnt = 0;
+  for (uint64_t w = b0 + threadIdx.x; w < b1; w += kBlock) cnt += __popc(word_bits(w));
+  uint32_t total;
+  block256_incl_scan(cnt, wtot, &total);
+  if (threadIdx.x == 0) s_base = total ? atomicAdd(&counters->q_len, (unsigned long long)total) : 0;
+  __syncthreads();
+  if (total == 0) return;
+  unsigned long long run = s_base;
+  for (uint64_t c0 = b0; c0 < b1; c0 += kBlock) {
+    const uint64_t w = c0 + threadIdx.x;
+    uint32_t bits = w < b1 ? word_bits(w) : 0u;
     const uint32_t c = __popc(bits);
-    uint32_t total;
-    const uint32_t incl = block256_incl_scan(c, wtot, &total);
-    if (threadIdx.x == 0 && total) s_base = atomicAdd(&counters->q_len, (unsigned long long)total);
-    __syncthreads();
-    if (total) {
-      unsigned long long o = s_base + incl - c;
-      while (bits) {
-        const uint32_t b = __ffs(bits) - 1;
-        bits &= bits - 1;
-        q[o++] = (uint32_t)(w * 32 + b);
-      }
+    uint32_t ctot;
+    const uint32_t incl = block256_incl_scan(c, wtot, &ctot);
+    unsigned long long o = run + incl - c;
+    while (bits) {
+      const uint32_t b = __ffs(bits) - 1;
+      bits &= bits - 1;
+      q[o++] = (uint32_t)(w * 32 + b);
     }
-    __syncthreads();
+    run += ctot;
   }
 }
 
@@ -597,9 +736,6 @@ k_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_t* front
   }
 }
 
-__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
-}
 
 // ------------------------------------------------------------ validation
 
@@ -752,16 +888,30 @@ int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned lon
                           cur_level + 1, n, OUT_BITMAP, st);
 }
 
+size_t scaph_pull_workspace_bytes(unsigned long long max_edges) {
+  return carve_pull_ws(nullptr, max_edges).bytes;
+}
+
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* degree, uint32_t* level,
                    uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
-                   scaph_level_counters* counters, uint32_t cur_level, void* stream) {
-  if (!in_graph || !degree || !level || !visited || !front_in || !front_out || !counters)
+                   scaph_level_counters* counters, uint32_t cur_level, unsigned long long max_edges,
+                   void* workspace, size_t workspace_bytes, void* stream) {
+  if (!in_graph || !degree || !level || !visited || !front_in || !front_out || !counters ||
+      !workspace)
     return SCAPH_STATUS_INVALID;
+  PullWs hw = carve_pull_ws(workspace, max_edges);
+  if (hw.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, hw.count, 1u);
+  SCAPH_LAUNCH_CHECK();
   const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
   const unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 256 * 16);
-  hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), page_lds_bytes(in_graph->num_pages),
-                     as_stream(stream), *in_graph, degree, level, visited, front_in, front_out,
-                     counters, cur_level + 1);
+  hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), page_lds_bytes(in_graph->num_pages), st,
+                     *in_graph, degree, level, visited, front_in, front_out, counters,
+                     cur_level + 1, hw);
+  SCAPH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), 0, st, hw.chunks, hw.count, degree,
+                     level, visited, front_in, front_out, counters, cur_level + 1);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

@@ -225,6 +225,8 @@ void Engine::alloc_state() {
                                                       std::max<uint64_t>(m_, 1))
                         : 0;
   if (ws_bytes_) ws_.alloc(ws_bytes_);
+  pws_bytes_ = has_in_ ? scaph_pull_workspace_bytes(std::max<uint64_t>(m_, 1)) : 0;
+  if (pws_bytes_) pws_.alloc(pws_bytes_);
   const size_t np = std::max<size_t>(pages_out_.size(), 1);
   page_nodes_.alloc(np * 4);
   page_datas_.alloc(np * 4);
@@ -509,7 +511,8 @@ RunStats Engine::bfs_resident(uint32_t src) {
       front_ok = false;
     } else {
       H_ERR(scaph_bfs_pull(&g_in_, deg, level, visited, front_[fa].get<uint32_t>(),
-                           front_[fa ^ 1].get<uint32_t>(), cnt, cur, st));
+                           front_[fa ^ 1].get<uint32_t>(), cnt, cur, m_, pws_.get(), pws_bytes_,
+                           st));
       fa ^= 1;
       front_ok = true;
       queue_ok = false;
@@ -531,6 +534,7 @@ RunStats Engine::bfs_resident(uint32_t src) {
     rec.m_next = hc->m_next;
     rec.scanned = hc->scanned;
     rec.examined = hc->examined;
+    if (hc->reserved) throw Error(SCAPH_STATUS_INVALID, "pull work list overflow");
     rs.levels.push_back(rec);
     n_f_prev = n_f;
     n_f = hc->n_next;

@@ -169,6 +169,8 @@ class Engine {
   // BFS state
   DevBuf level_, visited_, front_[2], queue_[2], counters_, ws_, page_nodes_, page_datas_, totals_;
   size_t ws_bytes_ = 0;
+  DevBuf pws_;  // pull workspace (deferred heavy in-lists)
+  size_t pws_bytes_ = 0;
   std::vector<uint32_t> h_deg_out_;
   std::unique_ptr<Stream> main_;
   PinnedBuf hcount_;  // pinned landing zone for per-level counters

@@ -210,3 +210,26 @@ def test_paged_out_of_core_parity(page_words, max_mib, threshold, multi, streams
             if threshold == 0:
                 assert st["pages_compacted"] == 0
             assert e.check(src) == 0
+
+
+def test_pull_heavy_in_lists():
+    """In-lists far longer than one pull work chunk, with the only frontier
+    in-neighbour at the very end (and a hub with no frontier in-neighbour)."""
+    k = 20000
+    n = k + 4
+    s_, a, t, h = 0, n - 1, 1, 2      # source, its out-neighbour (largest id), targets
+    src = [s_, a]
+    dst = [a, t]
+    xs = np.arange(3, 3 + k)
+    src += list(xs) + list(xs)
+    dst += [t] * k + [h] * k           # t: k + 1 in-edges (a last), h: k in-edges, none reachable
+    off, ngh = O.csr_from_edges(n, src, dst)
+    want = O.bfs(off, ngh, s_)
+    assert want[t] == 2 and want[h] == O.UNREACHED
+    with engine() as e:
+        e.load_csr(off, ngh)
+        for sched in SCHEDULES:
+            say("heavy in-lists", sched)
+            e.set_schedule(sched)
+            e.run(s_)
+            np.testing.assert_array_equal(e.levels(), want)
