@@ -393,7 +393,7 @@ struct PullWs {
 
 PullWs carve_pull_ws(void* base, unsigned long long m) {
   PullWs w;
-  w.cap = m / kHeavyChunk + m / kHeavy + 64;
+  w.cap = m / kHeavyChunk + m / kHeavy + 64;  // >= sum over lists of ceil((len-kHeavy)/chunk)
   w.bytes = 256 + w.cap * sizeof(HeavyChunk);
   char* p = static_cast<char*>(base);
   w.count = reinterpret_cast<unsigned long long*>(p);
@@ -485,28 +485,9 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
       NodeRef nr;
       nr.edges = nullptr;
       nr.len = 0;
-      bool heavy = false;
       if (act) {
         ++acc_x;
         nr = load_node_hint(s, g, v, hint);
-        heavy = nr.len > kHeavy;
-      }
-      if (heavy) {
-        // grid-wide follow-up: one work item per kHeavyChunk in-edges
-        const uint32_t nch = (nr.len + kHeavyChunk - 1) / kHeavyChunk;
-        const unsigned long long at = atomicAdd(hw.count, (unsigned long long)nch);
-        if (at + nch <= hw.cap) {
-          for (uint32_t c = 0; c < nch; ++c) {
-            HeavyChunk h;
-            h.v = v;
-            h.len = min(kHeavyChunk, nr.len - c * kHeavyChunk);
-            h.ptr = reinterpret_cast<uint64_t>(nr.edges + (uint64_t)c * kHeavyChunk);
-            hw.chunks[at + c] = h;
-          }
-        } else {
-          counters->reserved = 1;  // overflow: the host reports an error
-        }
-      } else if (act) {
         const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
         uint32_t k = 0;
         for (; k < lim; k += kPullProbe) {
@@ -525,21 +506,44 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
         acc_s += k < lim ? k : lim;
         longl = !found && nr.len > kPullSerial;
       }
+      // lists past the serial probes: the whole wave probes 64 at a time up
+      // to kHeavy entries; what is left after that is deferred
       uint64_t ml = __ballot(longl);
       while (ml) {
         const int j = __ffsll((unsigned long long)ml) - 1;
         ml &= ml - 1;
         const uint64_t ep = wave_bcast64(reinterpret_cast<uint64_t>(nr.edges), j);
         const uint32_t len = wave_bcast(nr.len, j);
+        const uint32_t stop = len < kHeavy ? len : kHeavy;
         const uint32_t* e = reinterpret_cast<const uint32_t*>(ep);
-        for (uint32_t k = kPullSerial; k < len; k += SCAPH_WAVE) {
+        bool hitj = false;
+        for (uint32_t k = kPullSerial; k < stop; k += SCAPH_WAVE) {
           const uint32_t kk = k + lane;
-          const bool hit = kk < len && bit_test(front_in, e[kk]);
+          const bool hit = kk < stop && bit_test(front_in, e[kk]);
           const uint64_t mh = __ballot(hit);
-          if (lane == 0) acc_s += min(len - k, (uint32_t)SCAPH_WAVE);
+          if (lane == 0) acc_s += min(stop - k, (uint32_t)SCAPH_WAVE);
           if (mh) {
-            if ((int)lane == j) found = true;
+            hitj = true;
             break;
+          }
+        }
+        if (hitj) {
+          if ((int)lane == j) found = true;
+        } else if (len > kHeavy && (int)lane == j) {
+          // grid-wide follow-up: one work item per kHeavyChunk in-edges
+          const uint32_t rest = len - kHeavy;
+          const uint32_t nch = (rest + kHeavyChunk - 1) / kHeavyChunk;
+          const unsigned long long at = atomicAdd(hw.count, (unsigned long long)nch);
+          if (at + nch <= hw.cap) {
+            for (uint32_t c = 0; c < nch; ++c) {
+              HeavyChunk h;
+              h.v = v;
+              h.len = min(kHeavyChunk, rest - c * kHeavyChunk);
+              h.ptr = reinterpret_cast<uint64_t>(e + kHeavy + (uint64_t)c * kHeavyChunk);
+              hw.chunks[at + c] = h;
+            }
+          } else {
+            counters->reserved = 1;  // overflow: the host reports an error
           }
         }
       }
@@ -615,6 +619,14 @@ k_pull_heavy(const HeavyChunk* chunks, const unsigned long long* count, const ui
       if (__syncthreads_or(h)) {
         hit = true;
         break;
+      }
+      if ((k0 / kBlock) % 4 == 3) {
+        // another chunk of the same list may have won meanwhile
+        if (threadIdx.x == 0)
+          s_skip = (int)((__hip_atomic_load(&visited[c.v >> 5], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT) >> (c.v & 31)) & 1u);
+        __syncthreads();
+        if (s_skip) break;
       }
     }
     if (hit && threadIdx.x == 0) {
