@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--schedule", default="fixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=60.0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this size first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -120,6 +122,18 @@ def main():
     info = eng.info()
     log("graph built", info, f"{time.perf_counter() - t0:.1f}s")
     n = info["n"]
+
+    if args.calib_copy_mib:
+        nbytes = args.calib_copy_mib << 20
+        hip = _lib.hip()
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(hip.scaph_dev_alloc(ctypes.byref(a), nbytes))
+        _lib.check(hip.scaph_dev_alloc(ctypes.byref(b), nbytes))
+        _lib.check(hip.scaph_copy_d2d_async(b, a, nbytes, None))
+        _lib.check(hip.scaph_device_sync())
+        hip.scaph_dev_free(a)
+        hip.scaph_dev_free(b)
+        log(f"calibration copy: {nbytes} bytes device-to-device")
 
     srcs, tried = pick_sources(eng, n, args.sources, seed=args.seed, min_reach_frac=0.01)
     log(f"sources: {len(srcs)} of {tried} candidates reach >=1% of V")
