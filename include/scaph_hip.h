@@ -143,16 +143,20 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree,
 
 /* Bottom-up (pull) expand of one level: every unvisited vertex scans its
  * in-neighbours (transposed paged CSR) for one in the frontier bitmap.
- * Writes level/visited for the vertices found and the full next-frontier
- * bitmap (all words, so it needs no clearing).  In-lists longer than 512
- * entries are finished by a second, grid-wide pass.  max_edges bounds the
- * graph's edges (sizes the workspace: scaph_pull_workspace_bytes).  No
- * reference counterpart (the reference engine is push-only). */
+ * In-list entries with the top bit set are hub indices into hub_list
+ * (num_hubs <= 32768; see scaph_csr_from_keys): their frontier bits are
+ * gathered once per level and probed from LDS.  Writes level/visited for
+ * the vertices found and the full next-frontier bitmap (all words, so it
+ * needs no clearing).  In-lists still unmatched after 512 entries are
+ * finished by a second, grid-wide pass.  max_edges bounds the graph's
+ * edges (workspace: scaph_pull_workspace_bytes).  No reference
+ * counterpart (the reference engine is push-only). */
 size_t scaph_pull_workspace_bytes(unsigned long long max_edges);
-int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* degree, uint32_t* level,
-                   uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
-                   scaph_level_counters* counters, uint32_t cur_level, unsigned long long max_edges,
-                   void* workspace, size_t workspace_bytes, void* stream);
+int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* hub_list, uint32_t num_hubs,
+                   const uint32_t* degree, uint32_t* level, uint32_t* visited,
+                   const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
+                   uint32_t cur_level, unsigned long long max_edges, void* workspace,
+                   size_t workspace_bytes, void* stream);
 
 /* Frontier format conversions (no reference counterpart). */
 int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint32_t* bitmap,
@@ -185,11 +189,12 @@ int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned lon
 
 /* Level-array check kernel: counts edges (u,v) violating
  * level[v] <= level[u] + 1 and reached vertices without a parent at
- * level-1 (BFS invariants).  out[0] = violations of the edge rule,
+ * level-1 (BFS invariants; in_graph entries decoded through hub_list).
+ * out[0] = violations of the edge rule,
  * out[1] = reached non-source vertices lacking a parent. */
 int scaph_bfs_validate(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
-                       const uint32_t* level, uint32_t src, unsigned long long* out,
-                       void* stream);
+                       const uint32_t* hub_list, const uint32_t* level, uint32_t src,
+                       unsigned long long* out, void* stream);
 
 /* ------------------------------------------------------------ generators */
 /* R-MAT edge generator (no reference counterpart; the reference ships no
@@ -204,12 +209,15 @@ int scaph_gen_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32
 
 /* Sorts keys, removes duplicates and the self-loop key, and writes a flat
  * CSR: offsets u64[n+1], neighbours u32[*num_edges], out-degree u32[n].
- * transpose != 0 swaps the roles of the key halves (builds in-edges).
- * keys is clobbered; keys_tmp must hold num_keys u64; ngh must hold
- * num_keys u32.  Allocates and frees its own sort scratch. */
+ * transpose != 0 builds in-edges (the key halves swapped); then, when
+ * hub_idx (device u32[n], 0xFFFFFFFF = not a hub) is given, each in-list
+ * holds its hub sources first, as entries 0x80000000 | hub index, then the
+ * other sources by id (the layout scaph_bfs_pull reads).  keys is
+ * clobbered; keys_tmp must hold num_keys u64; ngh must hold num_keys u32.
+ * Allocates and frees its own sort scratch. */
 int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, uint32_t n,
-                        int transpose, uint64_t* offsets, uint32_t* ngh, uint32_t* degree,
-                        uint64_t* num_edges, void* stream);
+                        int transpose, const uint32_t* hub_idx, uint64_t* offsets, uint32_t* ngh,
+                        uint32_t* degree, uint64_t* num_edges, void* stream);
 
 /* Writes the pages of a flat device CSR into page buffers, in the byte
  * layout of tools/bin2pagecsr.cpp:221-242 (Node records then neighbours).

@@ -122,7 +122,7 @@ def hip():
         _sig(lib, "scaph_bfs_push", I32,
              [ctypes.POINTER(GraphDesc), P, P, P, P, U32, ctypes.c_ulonglong, P, P, U32, P, SZ, P])
         _sig(lib, "scaph_pull_workspace_bytes", SZ, [ctypes.c_ulonglong])
-        _sig(lib, "scaph_bfs_pull", I32, [ctypes.POINTER(GraphDesc), P, P, P, P, P, P, U32,
+        _sig(lib, "scaph_bfs_pull", I32, [ctypes.POINTER(GraphDesc), P, U32, P, P, P, P, P, P, U32,
                                           ctypes.c_ulonglong, P, SZ, P])
         _sig(lib, "scaph_frontier_queue_to_bitmap", I32, [P, U32, P, U32, P])
         _sig(lib, "scaph_frontier_bitmap_to_queue", I32, [P, U32, P, P, P])
@@ -130,9 +130,10 @@ def hip():
         _sig(lib, "scaph_bfs_push_page", I32,
              [P, U32, ctypes.c_ulonglong, U32, P, P, P, P, P, P, U32, P, SZ, P])
         _sig(lib, "scaph_bfs_validate", I32,
-             [ctypes.POINTER(GraphDesc), ctypes.POINTER(GraphDesc), P, U32, P, P])
+             [ctypes.POINTER(GraphDesc), ctypes.POINTER(GraphDesc), P, P, U32, P, P])
         _sig(lib, "scaph_gen_rmat_keys", I32, [P, U64, U64, U32, U32, U32, U32, U64, P])
-        _sig(lib, "scaph_csr_from_keys", I32, [P, P, U64, U32, I32, P, P, P, ctypes.POINTER(U64), P])
+        _sig(lib, "scaph_csr_from_keys", I32,
+             [P, P, U64, U32, I32, P, P, P, P, ctypes.POINTER(U64), P])
         _sig(lib, "scaph_pages_from_csr", I32, [P, P, P, P, U32, U32, P])
         _sig(lib, "scaph_graph_capture_end", I32, [P, ctypes.POINTER(P)])
         _sig(lib, "scaph_graph_launch", I32, [P, P])

@@ -25,6 +25,8 @@ constexpr uint32_t kPerThread = kTile / kBlock;    // 4 consecutive edges
 constexpr uint32_t kPullSerial = 16;               // per-lane scan before wave help
 constexpr uint32_t kHeavy = 512;                   // longer in-lists go to k_pull_heavy
 constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work item
+constexpr uint32_t kMaxHubs = 32768;               // hub frontier bits kept in LDS (4 KB)
+constexpr uint32_t kHubTag = 0x80000000u;          // in-list entry = kHubTag | hub index
 constexpr uint32_t kScanThreads = 1024;
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -386,6 +388,7 @@ struct HeavyChunk {
 
 struct PullWs {
   unsigned long long* count;  // heavy chunks appended
+  uint32_t* hub_front;        // frontier bits of the hubs (kMaxHubs bits)
   HeavyChunk* chunks;
   unsigned long long cap;
   size_t bytes;
@@ -394,11 +397,41 @@ struct PullWs {
 PullWs carve_pull_ws(void* base, unsigned long long m) {
   PullWs w;
   w.cap = m / kHeavyChunk + m / kHeavy + 64;  // >= sum over lists of ceil((len-kHeavy)/chunk)
-  w.bytes = 256 + w.cap * sizeof(HeavyChunk);
+  const size_t hub_bytes = kMaxHubs / 8;
+  w.bytes = 256 + hub_bytes + w.cap * sizeof(HeavyChunk);
   char* p = static_cast<char*>(base);
   w.count = reinterpret_cast<unsigned long long*>(p);
-  w.chunks = p ? reinterpret_cast<HeavyChunk*>(p + 256) : nullptr;
+  w.hub_front = p ? reinterpret_cast<uint32_t*>(p + 256) : nullptr;
+  w.chunks = p ? reinterpret_cast<HeavyChunk*>(p + 256 + hub_bytes) : nullptr;
   return w;
+}
+
+// Frontier bits of the hub list, packed (hub i -> bit i).
+__global__ void __launch_bounds__(kBlock)
+k_hub_gather(const uint32_t* hub_list, uint32_t num_hubs, const uint32_t* front,
+             uint32_t* hub_front) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool b = i < num_hubs && bit_test(front, hub_list[i]);
+  const uint64_t m = __ballot(b);
+  const uint32_t lane = lane_id();
+  const uint32_t base = (blockIdx.x * kBlock + threadIdx.x - lane) / 32;
+  if (lane == 0 && base < kMaxHubs / 32) hub_front[base] = (uint32_t)m;
+  if (lane == 32 && base + 1 < kMaxHubs / 32) hub_front[base + 1] = (uint32_t)(m >> 32);
+}
+
+__device__ __forceinline__ void stage_hubs(uint32_t* s_hub, const uint32_t* hub_front,
+                                           uint32_t num_hubs) {
+  for (uint32_t i = threadIdx.x; i < (num_hubs + 31) / 32; i += blockDim.x) s_hub[i] = hub_front[i];
+}
+
+// Is in-list entry u (a vertex id, or kHubTag | hub index) in the frontier?
+__device__ __forceinline__ bool in_front(uint32_t u, const uint32_t* s_hub,
+                                         const uint32_t* front) {
+  if (u & kHubTag) {
+    const uint32_t h = u & ~kHubTag;
+    return (s_hub[h >> 5] >> (h & 31)) & 1u;
+  }
+  return bit_test(front, u);
 }
 
 __device__ __forceinline__ uint32_t page_left(const PageTab& t, const scaph_graph_desc& g,
@@ -443,12 +476,15 @@ __device__ __forceinline__ void wave_lds_sync() {
 __global__ void __launch_bounds__(kBlock)
 k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* visited,
        const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
-       uint32_t next_level, PullWs hw) {
+       uint32_t next_level, PullWs hw, uint32_t num_hubs) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
   __shared__ uint32_t s_q[kBlock / SCAPH_WAVE][kPullWin];
   __shared__ uint32_t s_fw[kBlock / SCAPH_WAVE][kPullWinWords];
+  __shared__ uint32_t s_hub[kMaxHubs / 32];
   __shared__ unsigned long long red[4][kBlock / SCAPH_WAVE];
+  stage_hubs(s_hub, hw.hub_front, num_hubs);
   const PageTab s = stage_pages(g, dyn_lds);
+  __syncthreads();
   const uint32_t n = g.num_vertices;
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x / SCAPH_WAVE;
@@ -496,7 +532,7 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
           for (uint32_t j = 0; j < kPullProbe; ++j) u[j] = (k + j < lim) ? nr.edges[k + j] : 0u;
           bool h = false;
 #pragma unroll
-          for (uint32_t j = 0; j < kPullProbe; ++j) h |= (k + j < lim) && bit_test(front_in, u[j]);
+          for (uint32_t j = 0; j < kPullProbe; ++j) h |= (k + j < lim) && in_front(u[j], s_hub, front_in);
           if (h) {
             found = true;
             k += kPullProbe;
@@ -519,7 +555,7 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
         bool hitj = false;
         for (uint32_t k = kPullSerial; k < stop; k += SCAPH_WAVE) {
           const uint32_t kk = k + lane;
-          const bool hit = kk < stop && bit_test(front_in, e[kk]);
+          const bool hit = kk < stop && in_front(e[kk], s_hub, front_in);
           const uint64_t mh = __ballot(hit);
           if (lane == 0) acc_s += min(stop - k, (uint32_t)SCAPH_WAVE);
           if (mh) {
@@ -595,9 +631,13 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
 __global__ void __launch_bounds__(kBlock)
 k_pull_heavy(const HeavyChunk* chunks, const unsigned long long* count, const uint32_t* degree,
              uint32_t* level, uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
-             scaph_level_counters* counters, uint32_t next_level) {
+             scaph_level_counters* counters, uint32_t next_level, const uint32_t* hub_front,
+             uint32_t num_hubs) {
   __shared__ HeavyChunk s_c;
   __shared__ int s_skip;
+  __shared__ uint32_t s_hub[kMaxHubs / 32];
+  stage_hubs(s_hub, hub_front, num_hubs);
+  __syncthreads();
   const unsigned long long total = *count;
   unsigned long long scanned = 0;
   for (unsigned long long e = blockIdx.x; e < total; e += gridDim.x) {
@@ -614,7 +654,7 @@ k_pull_heavy(const HeavyChunk* chunks, const unsigned long long* count, const ui
     bool hit = false;
     for (uint32_t k0 = 0; k0 < c.len; k0 += kBlock) {
       const uint32_t k = k0 + threadIdx.x;
-      const bool h = k < c.len && bit_test(front_in, ep[k]);
+      const bool h = k < c.len && in_front(ep[k], s_hub, front_in);
       scanned += (k < c.len) ? 1 : 0;
       if (__syncthreads_or(h)) {
         hit = true;
@@ -772,8 +812,8 @@ k_validate_edges(scaph_graph_desc g, const uint32_t* level, unsigned long long* 
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_validate_parents(scaph_graph_desc gin, const uint32_t* level, uint32_t src,
-                   unsigned long long* out) {
+k_validate_parents(scaph_graph_desc gin, const uint32_t* hub_list, const uint32_t* level,
+                   uint32_t src, unsigned long long* out) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
   const PageTab s = stage_pages(gin, dyn_lds);
   const uint32_t n = gin.num_vertices;
@@ -789,7 +829,11 @@ k_validate_parents(scaph_graph_desc gin, const uint32_t* level, uint32_t src,
     if (lv == 0) { ++bad; continue; }
     NodeRef nr = load_node(s, gin, v);
     bool ok = false;
-    for (uint32_t k = 0; k < nr.len && !ok; ++k) ok = level[nr.edges[k]] == lv - 1;
+    for (uint32_t k = 0; k < nr.len && !ok; ++k) {
+      uint32_t u = nr.edges[k];
+      if (u & kHubTag) u = hub_list[u & ~kHubTag];
+      ok = level[u] == lv - 1;
+    }
     if (!ok) ++bad;
   }
   bad = wave_sum(bad);
@@ -904,26 +948,33 @@ size_t scaph_pull_workspace_bytes(unsigned long long max_edges) {
   return carve_pull_ws(nullptr, max_edges).bytes;
 }
 
-int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* degree, uint32_t* level,
-                   uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
-                   scaph_level_counters* counters, uint32_t cur_level, unsigned long long max_edges,
-                   void* workspace, size_t workspace_bytes, void* stream) {
+int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* hub_list, uint32_t num_hubs,
+                   const uint32_t* degree, uint32_t* level, uint32_t* visited,
+                   const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
+                   uint32_t cur_level, unsigned long long max_edges, void* workspace,
+                   size_t workspace_bytes, void* stream) {
   if (!in_graph || !degree || !level || !visited || !front_in || !front_out || !counters ||
-      !workspace)
+      !workspace || num_hubs > kMaxHubs || (num_hubs && !hub_list))
     return SCAPH_STATUS_INVALID;
   PullWs hw = carve_pull_ws(workspace, max_edges);
   if (hw.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, hw.count, 1u);
   SCAPH_LAUNCH_CHECK();
+  if (num_hubs) {
+    hipLaunchKernelGGL(k_hub_gather, dim3((num_hubs + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       hub_list, num_hubs, front_in, hw.hub_front);
+    SCAPH_LAUNCH_CHECK();
+  }
   const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
   const unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 256 * 16);
   hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), page_lds_bytes(in_graph->num_pages), st,
                      *in_graph, degree, level, visited, front_in, front_out, counters,
-                     cur_level + 1, hw);
+                     cur_level + 1, hw, num_hubs);
   SCAPH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), 0, st, hw.chunks, hw.count, degree,
-                     level, visited, front_in, front_out, counters, cur_level + 1);
+                     level, visited, front_in, front_out, counters, cur_level + 1, hw.hub_front,
+                     num_hubs);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }
@@ -967,8 +1018,8 @@ int scaph_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_
 }
 
 int scaph_bfs_validate(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
-                       const uint32_t* level, uint32_t src, unsigned long long* out,
-                       void* stream) {
+                       const uint32_t* hub_list, const uint32_t* level, uint32_t src,
+                       unsigned long long* out, void* stream) {
   if (!out_graph || !level || !out) return SCAPH_STATUS_INVALID;
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, out, 2u);
@@ -979,7 +1030,8 @@ int scaph_bfs_validate(const scaph_graph_desc* out_graph, const scaph_graph_desc
   SCAPH_LAUNCH_CHECK();
   if (in_graph) {
     hipLaunchKernelGGL(k_validate_parents, dim3(grid), dim3(kBlock),
-                       page_lds_bytes(in_graph->num_pages), st, *in_graph, level, src, out);
+                       page_lds_bytes(in_graph->num_pages), st, *in_graph, hub_list, level, src,
+                       out);
     SCAPH_LAUNCH_CHECK();
   }
   return 0;

@@ -37,40 +37,57 @@ k_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t count, uint32_t scale, ui
   }
 }
 
-// (u<<32|v) -> packed (a<<bits|b) with (a,b) = (u,v) or (v,u).  Keys that
-// are self-loops (u == v) all become the largest key (n-1, n-1).
+// (u<<32|v) -> packed sort key.  Out-edges: (u << bits) | v.  In-edges:
+// (v << (bits+1)) | (flag << bits) | low where a hub source u (hub_idx[u]
+// != ~0) gets flag 0 and low = its hub index, any other source flag 1 and
+// low = u — so each in-list sorts hubs first (by hub index), then the rest
+// by id.  Self-loops all become the largest key, dropped after dedup.
 __global__ void __launch_bounds__(kBlock)
-k_pack(uint64_t* keys, uint64_t count, uint32_t bits, int transpose, uint32_t n) {
+k_pack(uint64_t* keys, uint64_t count, uint32_t bits, int transpose, uint32_t n,
+       const uint32_t* hub_idx) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t sentinel_t = ((uint64_t)(n - 1) << (bits + 1)) | (1ull << bits) | (n - 1);
+  const uint64_t sentinel_o = ((uint64_t)(n - 1) << bits) | (n - 1);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
     const uint64_t k = keys[i];
-    uint32_t u = (uint32_t)(k >> 32), v = (uint32_t)k;
-    if (u == v) u = v = n - 1;
-    const uint32_t a = transpose ? v : u;
-    const uint32_t b = transpose ? u : v;
-    keys[i] = ((uint64_t)a << bits) | b;
+    const uint32_t u = (uint32_t)(k >> 32), v = (uint32_t)k;
+    if (!transpose) {
+      keys[i] = u == v ? sentinel_o : (((uint64_t)u << bits) | v);
+    } else if (u == v) {
+      keys[i] = sentinel_t;
+    } else {
+      const uint32_t h = hub_idx ? hub_idx[u] : 0xFFFFFFFFu;
+      const uint64_t low = h != 0xFFFFFFFFu ? (uint64_t)h : ((1ull << bits) | u);
+      keys[i] = ((uint64_t)v << (bits + 1)) | low;
+    }
   }
 }
 
+// Sorted unique keys -> neighbour entries + run starts of each vertex.
+// `abits` is the shift of the owning vertex, `bits` the width of the id
+// field; in-edge keys (tagged) carry the hub flag at bit `bits`.
 __global__ void __launch_bounds__(kBlock)
-k_run_starts(const uint64_t* uniq, uint64_t count, uint32_t bits, uint32_t* ngh, uint64_t* start) {
+k_run_starts(const uint64_t* uniq, uint64_t count, uint32_t abits, uint32_t bits, int tagged,
+             uint32_t* ngh, uint64_t* start) {
   const uint64_t mask = (1ull << bits) - 1;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
     const uint64_t k = uniq[i];
-    const uint32_t a = (uint32_t)(k >> bits);
-    ngh[i] = (uint32_t)(k & mask);
-    if (i == 0 || (uint32_t)(uniq[i - 1] >> bits) != a) start[a] = i;
+    const uint32_t a = (uint32_t)(k >> abits);
+    uint32_t e = (uint32_t)(k & mask);
+    if (tagged && !((k >> bits) & 1ull)) e |= 0x80000000u;  // hub index
+    ngh[i] = e;
+    if (i == 0 || (uint32_t)(uniq[i - 1] >> abits) != a) start[a] = i;
   }
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_run_ends(const uint64_t* uniq, uint64_t count, uint32_t bits, const uint64_t* start,
+k_run_ends(const uint64_t* uniq, uint64_t count, uint32_t abits, const uint64_t* start,
            uint32_t* degree) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-    const uint32_t a = (uint32_t)(uniq[i] >> bits);
-    if (i + 1 == count || (uint32_t)(uniq[i + 1] >> bits) != a)
+    const uint32_t a = (uint32_t)(uniq[i] >> abits);
+    if (i + 1 == count || (uint32_t)(uniq[i + 1] >> abits) != a)
       degree[a] = (uint32_t)(i + 1 - start[a]);
   }
 }
@@ -135,12 +152,16 @@ int scaph_gen_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32
 }
 
 int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, uint32_t n,
-                        int transpose, uint64_t* offsets, uint32_t* ngh, uint32_t* degree,
-                        uint64_t* num_edges, void* stream) {
+                        int transpose, const uint32_t* hub_idx, uint64_t* offsets, uint32_t* ngh,
+                        uint32_t* degree, uint64_t* num_edges, void* stream) {
   if (!keys || !keys_tmp || !offsets || !ngh || !degree || !num_edges || n == 0)
     return SCAPH_STATUS_INVALID;
   hipStream_t st = as_stream(stream);
   const uint32_t bits = bits_for(n);
+  if (transpose && n > 0x7FFFFFFFu) return SCAPH_STATUS_INVALID;  // top bit tags hub entries
+  const uint32_t abits = transpose ? bits + 1 : bits;
+  const uint32_t key_bits = abits + bits;
+  const int tagged = transpose ? 1 : 0;
   SCAPH_TRY(hipMemsetAsync(degree, 0, (size_t)n * 4, st));
   if (num_keys == 0) {
     SCAPH_TRY(hipMemsetAsync(offsets, 0, ((size_t)n + 1) * 8, st));
@@ -149,13 +170,13 @@ int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, u
     return 0;
   }
   hipLaunchKernelGGL(k_pack, dim3(grid_for(num_keys, kBlock * 8, 65536)), dim3(kBlock), 0, st,
-                     keys, num_keys, bits, transpose, n);
+                     keys, num_keys, bits, transpose, n, transpose ? hub_idx : nullptr);
   SCAPH_LAUNCH_CHECK();
 
   // sort (ping-pong between keys and keys_tmp)
   rocprim::double_buffer<uint64_t> db(keys, keys_tmp);
   size_t sort_bytes = 0;
-  SCAPH_TRY(rocprim::radix_sort_keys(nullptr, sort_bytes, db, (size_t)num_keys, 0u, 2u * bits, st));
+  SCAPH_TRY(rocprim::radix_sort_keys(nullptr, sort_bytes, db, (size_t)num_keys, 0u, key_bits, st));
   uint64_t* sorted_count = nullptr;
   void* tmp = nullptr;
   size_t uniq_bytes = 0;
@@ -173,7 +194,7 @@ int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, u
   int rc = 0;
   do {
     size_t b = sort_bytes;
-    if ((rc = (int)rocprim::radix_sort_keys(tmp, b, db, (size_t)num_keys, 0u, 2u * bits, st)))
+    if ((rc = (int)rocprim::radix_sort_keys(tmp, b, db, (size_t)num_keys, 0u, key_bits, st)))
       break;
     uint64_t* in = db.current();
     uint64_t* out = db.alternate();
@@ -188,7 +209,9 @@ int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, u
       if ((rc = (int)hipMemcpyAsync(&last, out + (h_count - 1), 8, hipMemcpyDeviceToHost, st)))
         break;
       if ((rc = (int)hipStreamSynchronize(st))) break;
-      const uint64_t loop_key = ((uint64_t)(n - 1) << bits) | (uint64_t)(n - 1);
+      const uint64_t loop_key =
+          transpose ? (((uint64_t)(n - 1) << (bits + 1)) | (1ull << bits) | (uint64_t)(n - 1))
+                    : (((uint64_t)(n - 1) << bits) | (uint64_t)(n - 1));
       if (last == loop_key) --h_count;
     }
     // `in` is free now: reuse it for the run starts (n u64 <= num_keys u64?)
@@ -199,10 +222,10 @@ int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, u
     }
     if (h_count > 0) {
       hipLaunchKernelGGL(k_run_starts, dim3(grid_for(h_count, kBlock * 8, 65536)), dim3(kBlock),
-                         0, st, out, h_count, bits, ngh, start);
+                         0, st, out, h_count, abits, bits, tagged, ngh, start);
       if ((rc = (int)hipGetLastError())) break;
       hipLaunchKernelGGL(k_run_ends, dim3(grid_for(h_count, kBlock * 8, 65536)), dim3(kBlock), 0,
-                         st, out, h_count, bits, start, degree);
+                         st, out, h_count, abits, start, degree);
       if ((rc = (int)hipGetLastError())) break;
     }
     b = scan_bytes;

@@ -15,6 +15,7 @@ double Total_throughput = 0;
 
 namespace {
 
+constexpr uint32_t kMaxHubs = 32768;  // frontier bits the pull expand keeps in LDS
 constexpr uint64_t kAlignWords = 64;  // page buffers start on 256-byte boundaries
 constexpr size_t kMaxTimedLevels = 1024;
 constexpr size_t kCounterWords = sizeof(scaph_level_counters) / 4;
@@ -129,6 +130,8 @@ void Engine::reset_graph() {
   deg_out_.reset();
   deg_in_.reset();
   static_visited_.reset();
+  hub_list_.reset();
+  num_hubs_ = 0;
   h_deg_out_.clear();
   g_out_ = scaph_graph_desc{};
   g_in_ = scaph_graph_desc{};
@@ -242,18 +245,25 @@ void Engine::load_csr(HostCsr g, bool keep_host) {
   if (!page_cuts(g, P_, false, &pages_out_, &err)) throw Error(SCAPH_STATUS_INVALID, err);
   HostCsr t;
   std::vector<uint32_t> in_deg;
+  h_deg_out_.resize(n_);
+  for (uint32_t v = 0; v < n_; ++v) h_deg_out_[v] = g.degree(v);
+  std::vector<uint32_t> hubs;
   if (cfg_.build_pull) {
+    if (n_ > 0x7FFFFFFFu) throw Error(SCAPH_STATUS_INVALID, "pull needs fewer than 2^31 vertices");
     t = transpose(g);
+    hubs = select_hubs(h_deg_out_, kMaxHubs);
+    encode_hub_in_lists(&t, hubs);
     if (!page_cuts(t, P_, false, &pages_in_, &err)) throw Error(SCAPH_STATUS_INVALID, err);
   }
   plan_residency();
-  h_deg_out_.resize(n_);
-  for (uint32_t v = 0; v < n_; ++v) h_deg_out_[v] = g.degree(v);
   deg_out_.alloc(std::max<size_t>(n_, 1) * 4);
   H_ERR(scaph_copy_h2d_async(deg_out_.get(), h_deg_out_.data(), (size_t)n_ * 4, main_->get()));
   if (resident_) {
     upload_pages(g, false, &pages_out_, &pool_out_, &dpages_out_, &dptr_out_, &g_out_);
     if (has_in_) {
+      num_hubs_ = (uint32_t)hubs.size();
+      hub_list_.alloc(std::max<size_t>(hubs.size(), 1) * 4);
+      H_ERR(scaph_copy_h2d_async(hub_list_.get(), hubs.data(), hubs.size() * 4, main_->get()));
       upload_pages(t, false, &pages_in_, &pool_in_, &dpages_in_, &dptr_in_, &g_in_);
       in_deg.resize(n_);
       for (uint32_t v = 0; v < n_; ++v) in_deg[v] = t.degree(v);
@@ -343,11 +353,24 @@ void Engine::gen_rmat_device(const RmatParams& p, bool keep_host_csr) {
   for (int dir = 0; dir < 2; ++dir) {
     if (dir == 1 && !cfg_.build_pull) break;
     DevBuf deg((size_t)n * 4);
+    DevBuf hub_idx;
+    if (dir == 1) {
+      // hubs from the out-degrees (h_deg_out_ holds them after the first pass)
+      const std::vector<uint32_t> hubs = select_hubs(h_deg_out_, kMaxHubs);
+      std::vector<uint32_t> idx(n, 0xFFFFFFFFu);
+      for (uint32_t i = 0; i < hubs.size(); ++i) idx[hubs[i]] = i;
+      hub_idx.alloc((size_t)n * 4);
+      H_ERR(scaph_copy_h2d_async(hub_idx.get(), idx.data(), (size_t)n * 4, st));
+      num_hubs_ = (uint32_t)hubs.size();
+      hub_list_.alloc(std::max<size_t>(hubs.size(), 1) * 4);
+      H_ERR(scaph_copy_h2d_async(hub_list_.get(), hubs.data(), hubs.size() * 4, st));
+      main_->sync();
+    }
     H_ERR(scaph_gen_rmat_keys(keys.get<uint64_t>(), 0, raw, p.scale, ta, tab, tabc, p.seed, st));
     uint64_t E = 0;
     H_ERR(scaph_csr_from_keys(keys.get<uint64_t>(), tmp.get<uint64_t>(), raw, n, dir,
-                              offs.get<uint64_t>(), ngh.get<uint32_t>(), deg.get<uint32_t>(), &E,
-                              st));
+                              hub_idx.get<uint32_t>(), offs.get<uint64_t>(), ngh.get<uint32_t>(),
+                              deg.get<uint32_t>(), &E, st));
     H_ERR(scaph_copy_d2h_async(h_off.data(), offs.get(), h_off.size() * 8, st));
     main_->sync();
     if (dir == 0) m_ = E;
@@ -411,6 +434,8 @@ void Engine::gen_rmat_device(const RmatParams& p, bool keep_host_csr) {
     dpages_in_.reset();
     dptr_in_.reset();
     deg_in_.reset();
+    hub_list_.reset();
+    num_hubs_ = 0;
     g_in_ = scaph_graph_desc{};
   }
   keys.reset();
@@ -510,7 +535,8 @@ RunStats Engine::bfs_resident(uint32_t src) {
       queue_ok = true;
       front_ok = false;
     } else {
-      H_ERR(scaph_bfs_pull(&g_in_, deg, level, visited, front_[fa].get<uint32_t>(),
+      H_ERR(scaph_bfs_pull(&g_in_, hub_list_.get<uint32_t>(), num_hubs_, deg, level, visited,
+                           front_[fa].get<uint32_t>(),
                            front_[fa ^ 1].get<uint32_t>(), cnt, cur, m_, pws_.get(), pws_bytes_,
                            st));
       fa ^= 1;
@@ -566,7 +592,8 @@ void Engine::levels_to_host(uint32_t* out) const {
 std::array<uint64_t, 2> Engine::validate(uint32_t src) {
   if (!resident_) throw Error(SCAPH_STATUS_INVALID, "validation needs resident pages");
   DevBuf out(64);
-  H_ERR(scaph_bfs_validate(&g_out_, has_in_ ? &g_in_ : nullptr, level_.get<uint32_t>(), src,
+  H_ERR(scaph_bfs_validate(&g_out_, has_in_ ? &g_in_ : nullptr, hub_list_.get<uint32_t>(),
+                           level_.get<uint32_t>(), src,
                            out.get<unsigned long long>(), main_->get()));
   std::array<uint64_t, 2> r{};
   H_ERR(scaph_copy_d2h_async(r.data(), out.get(), 16, main_->get()));

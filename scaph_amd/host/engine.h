@@ -138,6 +138,8 @@ class Engine {
   const scaph_graph_desc& in_desc() const { return g_in_; }
   const uint32_t* d_level() const { return level_.get<uint32_t>(); }
   const uint32_t* d_degree() const { return deg_out_.get<uint32_t>(); }
+  const uint32_t* d_hub_list() const { return hub_list_.get<uint32_t>(); }
+  uint32_t num_hubs() const { return num_hubs_; }
   void* stream() const { return main_->get(); }
 
  private:
@@ -166,6 +168,8 @@ class Engine {
   DevBuf dpages_out_, dptr_out_, dpages_in_, dptr_in_;
   scaph_graph_desc g_out_{}, g_in_{};
   DevBuf deg_out_, deg_in_, static_visited_;
+  DevBuf hub_list_;  // hub vertex ids (pull in-lists refer to them by index)
+  uint32_t num_hubs_ = 0;
   // BFS state
   DevBuf level_, visited_, front_[2], queue_[2], counters_, ws_, page_nodes_, page_datas_, totals_;
   size_t ws_bytes_ = 0;

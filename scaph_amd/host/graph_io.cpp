@@ -267,4 +267,35 @@ HostCsr transpose(const HostCsr& g) {
   return t;
 }
 
+std::vector<uint32_t> select_hubs(const std::vector<uint32_t>& deg, uint32_t max_hubs) {
+  std::vector<uint32_t> ids;
+  for (uint32_t v = 0; v < deg.size(); ++v)
+    if (deg[v] > 0) ids.push_back(v);
+  auto better = [&](uint32_t a, uint32_t b) { return deg[a] != deg[b] ? deg[a] > deg[b] : a < b; };
+  const size_t k = std::min<size_t>(ids.size(), max_hubs);
+  std::partial_sort(ids.begin(), ids.begin() + k, ids.end(), better);
+  ids.resize(k);
+  return ids;
+}
+
+void encode_hub_in_lists(HostCsr* in, const std::vector<uint32_t>& hubs) {
+  std::vector<uint32_t> idx(in->n, 0xFFFFFFFFu);
+  for (uint32_t i = 0; i < hubs.size(); ++i) idx[hubs[i]] = i;
+  std::vector<uint32_t> hub_part, rest;
+  for (uint32_t v = 0; v < in->n; ++v) {
+    hub_part.clear();
+    rest.clear();
+    for (uint64_t k = in->offsets[v]; k < in->offsets[(size_t)v + 1]; ++k) {
+      const uint32_t u = in->ngh[k];
+      if (idx[u] != 0xFFFFFFFFu) hub_part.push_back(idx[u]);
+      else rest.push_back(u);
+    }
+    std::sort(hub_part.begin(), hub_part.end());
+    std::sort(rest.begin(), rest.end());
+    uint64_t k = in->offsets[v];
+    for (uint32_t h : hub_part) in->ngh[k++] = 0x80000000u | h;
+    for (uint32_t u : rest) in->ngh[k++] = u;
+  }
+}
+
 }  // namespace scaph

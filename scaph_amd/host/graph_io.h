@@ -86,6 +86,14 @@ bool read_page_file(const std::string& base, int index, uint64_t pagesize_words,
 // In-edge CSR (counting sort; neighbour lists come out sorted by source).
 HostCsr transpose(const HostCsr& g);
 
+// Hubs: the (at most) max_hubs vertices of largest out-degree (> 0), by
+// degree descending then id.  Entry i of the result has hub index i.
+std::vector<uint32_t> select_hubs(const std::vector<uint32_t>& out_degree, uint32_t max_hubs);
+// Re-encodes an in-edge CSR for the pull expand: in each list the hub
+// sources come first, by hub index, as 0x80000000 | index; the others
+// follow by id.
+void encode_hub_in_lists(HostCsr* in, const std::vector<uint32_t>& hubs);
+
 }  // namespace scaph
 
 #endif
