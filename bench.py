@@ -180,8 +180,15 @@ def main():
     avg_ms = ms_tot / max(launches, 1)
     avg_bytes = bytes_tot / max(launches, 1)
     achieved = avg_bytes / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "bfs_s26_ef32_k_pull_traffic.json")
+    if dom == 1 and os.path.exists(tfile) and args.scale == 26 and args.ef == 32:
+        # HBM-side bytes per k_pull launch from rocprofv3 FETCH_SIZE/WRITE_SIZE
+        # passes over this same command (separate runs; calibrated by a copy)
+        with open(tfile) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                "frac": achieved / 8000.0, "traffic": None,
+                "frac": achieved / 8000.0, "traffic": traffic,
                 "kernel": "k_pull" if dom == 1 else "push (k_prep_queue+k_scan_blocks+k_mark_tiles+k_expand)",
                 "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_bytes, "launches": launches,
                 "share_of_kernel_time": ms_tot / max(agg[0][0] + agg[1][0], 1e-9)}
