@@ -58,6 +58,9 @@ int scaph_engine_load_csr(scaph_engine* e, uint32_t n, const uint64_t* offsets,
 int scaph_engine_gen_rmat_device(scaph_engine* e, uint32_t scale, uint32_t edge_factor, double a,
                                  double b, double c, uint64_t seed, int keep_host, char* err,
                                  size_t errlen);
+/* Build an Erdos-Renyi G(n, m) graph directly in device memory. */
+int scaph_engine_gen_er_device(scaph_engine* e, uint32_t n, uint64_t m, uint64_t seed,
+                               int keep_host, char* err, size_t errlen);
 int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_pages,
                       int* resident, int* has_pull);
 /* run(): one BFS from src.  schedule: NULL keeps the -schedule flag, else
@@ -72,6 +75,20 @@ int scaph_engine_set_model(scaph_engine* e, const uint16_t* params_bf16, const f
                            size_t errlen);
 int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_t max,
                                uint32_t* count);
+
+/* Training traces for the learned schedule: a BFS from src in which every
+ * level runs both directions from the same state (min of `reps` timings
+ * each) and continues with the faster one.  One record per level: the raw
+ * features of the frontier entering it (the decision kernel's), both step
+ * times, and the fraction of pages active at the next level. */
+typedef struct scaph_explore_record {
+  uint64_t n_f, m_f, n_next;
+  uint32_t level;
+  float t_push_ms, t_pull_ms, pages_active_next;
+  float x[16];
+} scaph_explore_record;
+int scaph_engine_explore(scaph_engine* e, uint32_t src, int reps, scaph_explore_record* out,
+                         uint32_t max, uint32_t* count, char* err, size_t errlen);
 int scaph_engine_levels(scaph_engine* e, uint32_t* host_out);
 int scaph_engine_validate(scaph_engine* e, uint32_t src, uint64_t* out2);
 int scaph_engine_out_degrees(scaph_engine* e, uint32_t* host_out);

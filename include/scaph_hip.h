@@ -207,6 +207,12 @@ int scaph_gen_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32
                         uint32_t thr_a, uint32_t thr_ab, uint32_t thr_abc, uint64_t seed,
                         void* stream);
 
+/* Erdos-Renyi G(n, m) edge keys for e in [e_begin, e_end): endpoints are
+ * the high halves of SplitMix64 words 2e and 2e+1 times n.  No reference
+ * counterpart. */
+int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t n,
+                      uint64_t seed, void* stream);
+
 /* Sorts keys, removes duplicates and the self-loop key, and writes a flat
  * CSR: offsets u64[n+1], neighbours u32[*num_edges], out-degree u32[n].
  * transpose != 0 builds in-edges (the key halves swapped); then, when
@@ -225,6 +231,48 @@ int scaph_csr_from_keys(uint64_t* keys, uint64_t* keys_tmp, uint64_t num_keys, u
  * each buffer past the used part are NOT touched. */
 int scaph_pages_from_csr(const uint64_t* offsets, const uint32_t* ngh, const scaph_page* pages,
                          const uint64_t* page_ptr, uint32_t num_pages, uint32_t n, void* stream);
+
+/* ------------------------------------------------------------------- MLP */
+/* 16 -> 64 -> 64 -> 2 ReLU MLP of the learned direction/prefetch schedule
+ * (no reference counterpart).  Weights are bf16 (u16 bit patterns),
+ * accumulation is fp32.  Parameter layout:
+ *   W1[64][16], b1[64], W2[64][64], b2[64], W3[2][64], b3[2].
+ * x: batch x 16 features; y: batch x 2 outputs (0: pull-is-faster logit,
+ * 1: next-level page-activity threshold); act (optional, batch x 128) keeps
+ * the hidden activations for backward. */
+#define SCAPH_MLP_IN 16
+#define SCAPH_MLP_H 64
+#define SCAPH_MLP_OUT 2
+#define SCAPH_MLP_NPARAM (64 * 16 + 64 + 64 * 64 + 64 + 2 * 64 + 2)
+int scaph_mlp_forward(const uint16_t* params_bf16, const float* x, uint32_t batch, float* y,
+                      float* act, void* stream);
+/* Batch-mean loss softplus(y0) - t0*y0 + w1*(y1 - t1)^2 with labels
+ * t = label[2*s..]; grad (fp32, SCAPH_MLP_NPARAM) is overwritten with its
+ * gradient and loss[0] with the loss. */
+int scaph_mlp_backward(const uint16_t* params_bf16, const float* x, const float* act,
+                       const float* y, const float* label, uint32_t batch, float w1, float* grad,
+                       float* loss, void* stream);
+/* momentum = mu*momentum + grad; master -= lr*momentum;
+ * params_bf16 = round-to-nearest-even(master). */
+int scaph_mlp_sgd(float* master, float* momentum, const float* grad, uint32_t nparam, float lr,
+                  float mu, uint16_t* params_bf16, void* stream);
+/* Per-page active counts of a frontier into page_cnt[num_pages]: from a
+ * bitmap, or (bitmap NULL) from a vertex list whose length is read on the
+ * device from *queue_len (e.g. the level counters' q_len). */
+int scaph_frontier_page_counts(const uint32_t* bitmap, const uint32_t* queue,
+                               const unsigned long long* queue_len, const scaph_page* pages,
+                               uint32_t num_pages, uint32_t n, uint32_t* page_cnt, void* stream);
+/* One decision on the device: the level's counters (next frontier size and
+ * edges), the unexplored edges before the level (m_u; the kernel subtracts
+ * the counters' m_next) and the per-page counts become the 16 features
+ * (normalised
+ * by norm = mean[16], scale[16]); out (36 floats) receives the pull logit,
+ * the threshold, the number of active pages, the 12-bin histogram and the
+ * raw features. */
+int scaph_mlp_decide(const uint16_t* params_bf16, const float* norm, const uint32_t* page_cnt,
+                     const scaph_page* pages, uint32_t num_pages,
+                     const scaph_level_counters* counters, unsigned long long m_u,
+                     uint32_t level, float* out, void* stream);
 
 /* ------------------------------------------------------------ hipGraph */
 /* Capture everything launched on `stream` between begin and end into an

@@ -76,6 +76,11 @@ class BFS:
                                                          seed, int(keep_host), e, 1024),
                    e.value.decode())
 
+    def gen_er_device(self, n, m, seed=1, keep_host=False):
+        e = _err()
+        _lib.check(self._lib.scaph_engine_gen_er_device(self._h, n, m, seed, int(keep_host), e,
+                                                       1024), e.value.decode())
+
     def info(self):
         n = ctypes.c_uint32()
         m = ctypes.c_uint64()
@@ -119,6 +124,20 @@ class BFS:
             d = {k: getattr(r, k) for k, _ in _lib.LevelRecord._fields_ if k != "page_hist"}
             d["page_hist"] = list(r.page_hist)
             out.append(d)
+        return out
+
+    def explore(self, src, reps=2):
+        """Training traces: per level, both directions timed from the same state."""
+        arr = (_lib.ExploreRecord * 4096)()
+        cnt = ctypes.c_uint32()
+        e = _err()
+        _lib.check(self._lib.scaph_engine_explore(self._h, src, reps, arr, 4096, ctypes.byref(cnt),
+                                                  e, 1024), e.value.decode())
+        out = []
+        for r in arr[:min(cnt.value, 4096)]:
+            out.append({"level": r.level, "n_f": r.n_f, "m_f": r.m_f, "n_next": r.n_next,
+                        "t_push_ms": r.t_push_ms, "t_pull_ms": r.t_pull_ms,
+                        "pages_active_next": r.pages_active_next, "x": list(r.x)})
         return out
 
     def levels(self):

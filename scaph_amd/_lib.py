@@ -52,6 +52,13 @@ class DeviceProps(ctypes.Structure):
                 ("mem_bus_width", ctypes.c_int)]
 
 
+class ExploreRecord(ctypes.Structure):
+    _fields_ = [("n_f", ctypes.c_uint64), ("m_f", ctypes.c_uint64), ("n_next", ctypes.c_uint64),
+                ("level", ctypes.c_uint32), ("t_push_ms", ctypes.c_float),
+                ("t_pull_ms", ctypes.c_float), ("pages_active_next", ctypes.c_float),
+                ("x", ctypes.c_float * 16)]
+
+
 class BfsStats(ctypes.Structure):
     _fields_ = [("src", ctypes.c_uint32), ("depth", ctypes.c_uint32), ("ms", ctypes.c_double),
                 ("n_reached", ctypes.c_uint64), ("m_reached", ctypes.c_uint64),
@@ -132,9 +139,15 @@ def hip():
         _sig(lib, "scaph_bfs_validate", I32,
              [ctypes.POINTER(GraphDesc), ctypes.POINTER(GraphDesc), P, P, U32, P, P])
         _sig(lib, "scaph_gen_rmat_keys", I32, [P, U64, U64, U32, U32, U32, U32, U64, P])
+        _sig(lib, "scaph_gen_er_keys", I32, [P, U64, U64, U32, U64, P])
         _sig(lib, "scaph_csr_from_keys", I32,
              [P, P, U64, U32, I32, P, P, P, P, ctypes.POINTER(U64), P])
         _sig(lib, "scaph_pages_from_csr", I32, [P, P, P, P, U32, U32, P])
+        _sig(lib, "scaph_mlp_forward", I32, [P, P, U32, P, P, P])
+        _sig(lib, "scaph_mlp_backward", I32, [P, P, P, P, P, U32, ctypes.c_float, P, P, P])
+        _sig(lib, "scaph_mlp_sgd", I32, [P, P, P, U32, ctypes.c_float, ctypes.c_float, P, P])
+        _sig(lib, "scaph_frontier_page_counts", I32, [P, P, U32, P, U32, U32, P, P])
+        _sig(lib, "scaph_mlp_decide", I32, [P, P, P, P, U32, P, ctypes.c_ulonglong, U32, P, P])
         _sig(lib, "scaph_graph_capture_end", I32, [P, ctypes.POINTER(P)])
         _sig(lib, "scaph_graph_launch", I32, [P, P])
         _hip = lib
@@ -157,6 +170,7 @@ def host():
         _sig(lib, "scaph_engine_load_csr", I32, [E, U32, P, P, I32, CHARP, SZ])
         _sig(lib, "scaph_engine_gen_rmat_device", I32,
              [E, U32, U32, DBL, DBL, DBL, U64, I32, CHARP, SZ])
+        _sig(lib, "scaph_engine_gen_er_device", I32, [E, U32, U64, U64, I32, CHARP, SZ])
         _sig(lib, "scaph_engine_info", I32,
              [E, ctypes.POINTER(U32), ctypes.POINTER(U64), ctypes.POINTER(U32),
               ctypes.POINTER(I32), ctypes.POINTER(I32)])
@@ -166,6 +180,8 @@ def host():
         _sig(lib, "scaph_engine_level_records", I32,
              [E, ctypes.POINTER(LevelRecord), U32, ctypes.POINTER(U32)])
         _sig(lib, "scaph_engine_levels", I32, [E, P])
+        _sig(lib, "scaph_engine_explore", I32,
+             [E, U32, I32, ctypes.POINTER(ExploreRecord), U32, ctypes.POINTER(U32), CHARP, SZ])
         _sig(lib, "scaph_engine_validate", I32, [E, U32, P])
         _sig(lib, "scaph_engine_out_degrees", I32, [E, P])
         _sig(lib, "scaph_engine_host_csr", I32, [E, P, P])

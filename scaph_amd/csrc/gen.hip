@@ -37,6 +37,20 @@ k_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t count, uint32_t scale, ui
   }
 }
 
+// Directed G(n, m): endpoints floor(w * n / 2^64) of SplitMix64 words 2e
+// and 2e+1 (Lemire's multiply-high mapping), as the host generator.
+__global__ void __launch_bounds__(kBlock)
+k_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t count, uint32_t n, uint64_t seed) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint64_t e = e_begin + i;
+    const uint64_t wu = scaph_spec::word(seed, 2 * e), wv = scaph_spec::word(seed, 2 * e + 1);
+    const uint32_t u = (uint32_t)__umul64hi(wu, (uint64_t)n);
+    const uint32_t v = (uint32_t)__umul64hi(wv, (uint64_t)n);
+    keys[i] = ((uint64_t)u << 32) | v;
+  }
+}
+
 // (u<<32|v) -> packed sort key.  Out-edges: (u << bits) | v.  In-edges:
 // (v << (bits+1)) | (flag << bits) | low where a hub source u (hub_idx[u]
 // != ~0) gets flag 0 and low = its hub index, any other source flag 1 and
@@ -147,6 +161,17 @@ int scaph_gen_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32
   hipLaunchKernelGGL(k_rmat_keys, dim3(grid_for(count, kBlock * 8, 65536)), dim3(kBlock), 0,
                      as_stream(stream), keys, e_begin, count, scale, thr_a, thr_ab, thr_abc, seed,
                      pk);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t n,
+                      uint64_t seed, void* stream) {
+  if (!keys || n == 0 || e_end < e_begin) return SCAPH_STATUS_INVALID;
+  if (e_end == e_begin) return 0;
+  const uint64_t count = e_end - e_begin;
+  hipLaunchKernelGGL(k_er_keys, dim3(grid_for(count, kBlock * 8, 65536)), dim3(kBlock), 0,
+                     as_stream(stream), keys, e_begin, count, n, seed);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

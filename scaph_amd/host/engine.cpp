@@ -6,8 +6,8 @@
 #include <cstring>
 #include <deque>
 
+#include "engine_internal.h"
 #include "flags.h"
-#include "paged.h"
 
 namespace scaph {
 
@@ -339,13 +339,26 @@ void Engine::load_from_config() {
 }
 
 void Engine::gen_rmat_device(const RmatParams& p, bool keep_host_csr) {
-  reset_graph();
   if (p.scale == 0 || p.scale > 31) throw Error(SCAPH_STATUS_INVALID, "rmat scale must be 1..31");
-  void* st = main_->get();
-  const uint32_t n = (uint32_t)(1ull << p.scale);
-  const uint64_t raw = (uint64_t)p.edge_factor << p.scale;
   uint32_t ta, tab, tabc;
   rmat_thresholds(p, &ta, &tab, &tabc);
+  const uint64_t raw = (uint64_t)p.edge_factor << p.scale;
+  build_device((uint32_t)(1ull << p.scale), raw, keep_host_csr, [&](uint64_t* keys, void* st) {
+    H_ERR(scaph_gen_rmat_keys(keys, 0, raw, p.scale, ta, tab, tabc, p.seed, st));
+  });
+}
+
+void Engine::gen_er_device(uint32_t n, uint64_t m, uint64_t seed, bool keep_host_csr) {
+  if (n == 0) throw Error(SCAPH_STATUS_INVALID, "er needs n > 0");
+  build_device(n, m, keep_host_csr, [&](uint64_t* keys, void* st) {
+    H_ERR(scaph_gen_er_keys(keys, 0, m, n, seed, st));
+  });
+}
+
+void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
+                          const std::function<void(uint64_t*, void*)>& keygen) {
+  reset_graph();
+  void* st = main_->get();
   n_ = n;
   DevBuf keys(raw * 8), tmp(raw * 8), offs(((size_t)n + 1) * 8), ngh(std::max<uint64_t>(raw, 1) * 4);
   std::vector<uint64_t> h_off((size_t)n + 1);
@@ -366,7 +379,7 @@ void Engine::gen_rmat_device(const RmatParams& p, bool keep_host_csr) {
       H_ERR(scaph_copy_h2d_async(hub_list_.get(), hubs.data(), hubs.size() * 4, st));
       main_->sync();
     }
-    H_ERR(scaph_gen_rmat_keys(keys.get<uint64_t>(), 0, raw, p.scale, ta, tab, tabc, p.seed, st));
+    keygen(keys.get<uint64_t>(), st);
     uint64_t E = 0;
     H_ERR(scaph_csr_from_keys(keys.get<uint64_t>(), tmp.get<uint64_t>(), raw, n, dir,
                               hub_idx.get<uint32_t>(), offs.get<uint64_t>(), ngh.get<uint32_t>(),
@@ -479,109 +492,6 @@ RunStats Engine::bfs(uint32_t src) {
   if (n_ == 0) throw Error(SCAPH_STATUS_INVALID, "no graph loaded");
   if (src >= n_) throw Error(SCAPH_STATUS_INVALID, "source vertex out of range");
   return resident_ ? bfs_resident(src) : bfs_paged(src);
-}
-
-RunStats Engine::bfs_resident(uint32_t src) {
-  void* st = main_->get();
-  RunStats rs;
-  rs.src = src;
-  while (ev_.size() < 2) ev_.emplace_back(new Event());
-  const uint64_t deg_src = out_degree(src);
-  auto* cnt = counters_.get<scaph_level_counters>();
-  auto* hc = hcount_.get<scaph_level_counters>();
-  uint32_t* level = level_.get<uint32_t>();
-  uint32_t* visited = visited_.get<uint32_t>();
-  const uint32_t* deg = deg_out_.get<uint32_t>();
-
-  ev_[0]->record(st);
-  H_ERR(scaph_bfs_init(level, visited, has_in_ ? static_visited_.get<uint32_t>() : nullptr, n_,
-                       src, queue_[0].get<uint32_t>(), cnt, st));
-  Dir dir = Dir::kPush;
-  uint64_t n_f = 1, m_f = deg_src, m_u = m_ - deg_src, n_f_prev = 0;
-  int qa = 0, fa = 0;
-  bool queue_ok = true, front_ok = false;
-  uint32_t cur = 0;
-  rs.n_reached = 1;
-  rs.m_reached = deg_src;
-  std::vector<size_t> lev_ev;
-  while (n_f > 0) {
-    SchedState ss{dir, cur, n_, m_, n_f, m_f, m_u, n_f_prev,
-                  rs.levels.empty() ? nullptr : &rs.levels.back()};
-    Dir d = sched_ ? sched_(ss) : beamer(ss);
-    if (d == Dir::kPull && !has_in_) d = Dir::kPush;
-    if (d == Dir::kPull && !front_ok) {
-      H_ERR(scaph_frontier_queue_to_bitmap(queue_[qa].get<uint32_t>(), (uint32_t)n_f,
-                                           front_[fa].get<uint32_t>(), n_, st));
-      front_ok = true;
-    }
-    if (d == Dir::kPush && !queue_ok) {
-      H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt + 1), 0u, kCounterWords, st));
-      H_ERR(scaph_frontier_bitmap_to_queue(front_[fa].get<uint32_t>(), n_,
-                                           queue_[qa].get<uint32_t>(), cnt + 1, st));
-      queue_ok = true;
-    }
-    H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, kCounterWords, st));
-    const bool timed = rs.levels.size() < kMaxTimedLevels;
-    const size_t eb = 2 + 3 * rs.levels.size();
-    if (timed) {
-      while (ev_.size() < eb + 3) ev_.emplace_back(new Event());
-      ev_[eb]->record(st);
-    }
-    if (d == Dir::kPush) {
-      H_ERR(scaph_bfs_push(&g_out_, deg, level, visited, queue_[qa].get<uint32_t>(), (uint32_t)n_f,
-                           m_f, queue_[qa ^ 1].get<uint32_t>(), cnt, cur, ws_.get(), ws_bytes_,
-                           st));
-      qa ^= 1;
-      queue_ok = true;
-      front_ok = false;
-    } else {
-      H_ERR(scaph_bfs_pull(&g_in_, hub_list_.get<uint32_t>(), num_hubs_, deg, level, visited,
-                           front_[fa].get<uint32_t>(),
-                           front_[fa ^ 1].get<uint32_t>(), cnt, cur, m_, pws_.get(), pws_bytes_,
-                           st));
-      fa ^= 1;
-      front_ok = true;
-      queue_ok = false;
-    }
-    if (timed) {
-      ev_[eb + 1]->record(st);
-      lev_ev.push_back(eb);
-    }
-    H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
-    if (timed) ev_[eb + 2]->record(st);
-    main_->sync();
-    LevelRecord rec;
-    rec.level = cur;
-    rec.dir = (int)d;
-    rec.n_f = n_f;
-    rec.m_f = m_f;
-    rec.m_u = m_u;
-    rec.n_next = hc->n_next;
-    rec.m_next = hc->m_next;
-    rec.scanned = hc->scanned;
-    rec.examined = hc->examined;
-    if (hc->reserved) throw Error(SCAPH_STATUS_INVALID, "pull work list overflow");
-    rs.levels.push_back(rec);
-    n_f_prev = n_f;
-    n_f = hc->n_next;
-    m_f = hc->m_next;
-    m_u = m_u > m_f ? m_u - m_f : 0;
-    rs.n_reached += n_f;
-    rs.m_reached += m_f;
-    dir = d;
-    ++cur;
-  }
-  ev_[1]->record(st);
-  ev_[1]->sync();
-  rs.ms = Event::elapsed_ms(*ev_[0], *ev_[1]);
-  for (size_t i = 0; i < lev_ev.size(); ++i) {
-    const size_t e = lev_ev[i];
-    const Event& prev_end = i == 0 ? *ev_[0] : *ev_[lev_ev[i - 1] + 2];
-    rs.levels[i].ms = Event::elapsed_ms(prev_end, *ev_[e + 2]);
-    rs.levels[i].kernel_ms = Event::elapsed_ms(*ev_[e], *ev_[e + 1]);
-  }
-  rs.depth = cur;
-  return rs;
 }
 
 void Engine::levels_to_host(uint32_t* out) const {

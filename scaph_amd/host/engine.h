@@ -105,6 +105,7 @@ class Engine {
   void load_files(const std::string& base);
   void load_csr(HostCsr g, bool keep_host = true);
   void gen_rmat_device(const RmatParams& p, bool keep_host_csr);
+  void gen_er_device(uint32_t n, uint64_t m, uint64_t seed, bool keep_host_csr);
   void load_from_config();  // graphfile, else gen spec
 
   // One BFS from src.  Levels stay on the device (levels_to_host to read).
@@ -114,6 +115,20 @@ class Engine {
   std::array<uint64_t, 2> validate(uint32_t src);
 
   void set_schedule(SchedFn fn) { sched_ = std::move(fn); }
+  // Learned schedule on the device: bf16 MLP parameters and the feature
+  // normalisation; used by the level loop when -schedule=learned.
+  void set_device_model(const uint16_t* params, const float* mean, const float* scale);
+  bool device_model_ready() const;
+  // Training traces: at every level both directions run from the same
+  // state (min of `reps` timings each), the faster one continues.
+  struct ExploreRecord {
+    float x[16];             // raw features of the frontier entering the level
+    float t_push_ms, t_pull_ms;
+    float pages_active_next;  // fraction of pages active at the next level
+    uint32_t level;
+    uint64_t n_f, m_f, n_next;
+  };
+  std::vector<ExploreRecord> explore(uint32_t src, int reps);
   void set_config_schedule(const std::string& s, double alpha, double beta) {
     cfg_.schedule = s;
     cfg_.alpha = alpha;
@@ -151,7 +166,18 @@ class Engine {
   void alloc_state();
   void plan_residency();
   void setup_paged();
+  void build_device(uint32_t n, uint64_t raw_edges, bool keep_host_csr,
+                    const std::function<void(uint64_t*, void*)>& keygen);
   RunStats bfs_resident(uint32_t src);
+  struct Front {  // which buffers hold the current frontier, in which forms
+    int qa = 0, fa = 0;
+    bool queue_ok = true, front_ok = false;
+  };
+  void ensure_forms(Front& f, bool queue, bool bitmap, uint64_t n_f);
+  void launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m_f);
+  struct DeviceModel;
+  std::unique_ptr<DeviceModel> model_;
+  void ensure_model_buffers();
   RunStats bfs_paged(uint32_t src);
   void page_histogram(const uint32_t* nodes, LevelRecord* rec);
 
@@ -181,6 +207,7 @@ class Engine {
   std::vector<std::unique_ptr<Event>> ev_;
   SchedFn sched_;
   std::unique_ptr<PagedState> paged_;
+  DevBuf snap_level_, snap_visited_, snap_queue_, snap_front_;
 };
 
 // CPU BFS used by bfs::check() (app/bfs.cu:67-117): FIFO level BFS over the

@@ -40,6 +40,9 @@ int guarded(char* err, size_t len, F f) {
   }
 }
 
+// The learned schedule decides on the device (scaph_mlp_decide inside the
+// level loop); the host-side model only answers the first level and serves
+// the CPU-side checks.
 void apply_schedule(scaph_engine* e) {
   if (e->cfg.schedule == "learned") {
     scaph::LearnedSchedule* ls = &e->learned;
@@ -69,7 +72,10 @@ int scaph_engine_create(int argc, const char* const* argv, scaph_engine** out, c
     }
     try {
       e->eng = new scaph::Engine(e->cfg);
-      if (!e->cfg.model.empty()) e->learned.load_file(e->cfg.model);
+      if (!e->cfg.model.empty()) {
+        e->learned.load_file(e->cfg.model);
+        e->eng->set_device_model(e->learned.params(), e->learned.mean(), e->learned.scale());
+      }
       apply_schedule(e);
     } catch (...) {
       delete e;
@@ -118,6 +124,12 @@ int scaph_engine_gen_rmat_device(scaph_engine* e, uint32_t scale, uint32_t edge_
     p.seed = seed;
     e->eng->gen_rmat_device(p, keep_host != 0);
   });
+}
+
+int scaph_engine_gen_er_device(scaph_engine* e, uint32_t n, uint64_t m, uint64_t seed,
+                               int keep_host, char* err, size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] { e->eng->gen_er_device(n, m, seed, keep_host != 0); });
 }
 
 int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_pages, int* resident,
@@ -176,6 +188,7 @@ int scaph_engine_set_model(scaph_engine* e, const uint16_t* params_bf16, const f
   if (!e || !params_bf16 || !feat_mean || !feat_scale) return 1;
   return guarded(err, errlen, [&] {
     e->learned.set(params_bf16, feat_mean, feat_scale, prefetch_gain);
+    e->eng->set_device_model(params_bf16, feat_mean, feat_scale);
   });
 }
 
@@ -200,6 +213,26 @@ int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_
   }
   if (count) *count = (uint32_t)lv.size();
   return 0;
+}
+
+int scaph_engine_explore(scaph_engine* e, uint32_t src, int reps, scaph_explore_record* out,
+                         uint32_t max, uint32_t* count, char* err, size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] {
+    const auto recs = e->eng->explore(src, reps);
+    const uint32_t k = (uint32_t)std::min<size_t>(recs.size(), max);
+    for (uint32_t i = 0; i < k && out; ++i) {
+      out[i].n_f = recs[i].n_f;
+      out[i].m_f = recs[i].m_f;
+      out[i].n_next = recs[i].n_next;
+      out[i].level = recs[i].level;
+      out[i].t_push_ms = recs[i].t_push_ms;
+      out[i].t_pull_ms = recs[i].t_pull_ms;
+      out[i].pages_active_next = recs[i].pages_active_next;
+      for (int j = 0; j < 16; ++j) out[i].x[j] = recs[i].x[j];
+    }
+    if (count) *count = (uint32_t)recs.size();
+  });
 }
 
 int scaph_engine_levels(scaph_engine* e, uint32_t* host_out) {

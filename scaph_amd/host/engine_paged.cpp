@@ -13,7 +13,7 @@
 #include <cstring>
 #include <thread>
 
-#include "paged.h"
+#include "engine_internal.h"
 
 namespace scaph {
 

@@ -36,6 +36,9 @@ class LearnedSchedule {
   // fp32-accumulate forward pass; out[0] = pull logit, out[1] = prefetch threshold
   void forward(const float* x, float* out) const;
   Dir decide(const Engine& eng, const SchedState& s) const;
+  const uint16_t* params() const { return p_.data(); }
+  const float* mean() const { return mean_.data(); }
+  const float* scale() const { return scale_.data(); }
 
  private:
   bool ready_ = false;

@@ -331,3 +331,45 @@ def test_rmat_csr_is_simple_and_sorted():
         seg = ngh[off[v]:off[v + 1]].astype(np.int64)
         assert np.all(np.diff(seg) > 0)  # sorted, no duplicates
         assert not np.any(seg == v)      # no self-loops
+
+
+# ------------------------------------------------------------------- MLP
+
+def numpy_mlp(p, x):
+    W1 = p[:1024].reshape(64, 16); b1 = p[1024:1088]
+    W2 = p[1088:5184].reshape(64, 64); b2 = p[5184:5248]
+    W3 = p[5248:5376].reshape(2, 64); b3 = p[5376:5378]
+    h1 = np.maximum(x @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    return h2 @ W3.T + b3
+
+
+def test_oracle_mlp_forward_matches_matmul_form():
+    rng = np.random.default_rng(0)
+    p = rng.normal(0, 0.3, 5378).astype(np.float32)
+    x = rng.normal(0, 1, (37, 16)).astype(np.float32)
+    y, _, _ = O.mlp_forward(p, x)
+    np.testing.assert_allclose(y, numpy_mlp(p.astype(np.float64), x.astype(np.float64)),
+                               rtol=1e-4, atol=1e-4)
+
+
+def test_oracle_mlp_gradient_finite_differences():
+    rng = np.random.default_rng(1)
+    p = rng.normal(0, 0.3, 5378).astype(np.float64)
+    x = rng.normal(0, 1, (8, 16))
+    t = np.stack([rng.uniform(0, 1, 8), rng.uniform(-1, 1, 8)], 1)
+
+    def loss(pp):
+        y = numpy_mlp(pp, x)
+        z0, z1 = y[:, 0], y[:, 1]
+        return np.mean(np.logaddexp(0, z0) - t[:, 0] * z0 + 0.5 * (z1 - t[:, 1]) ** 2)
+
+    l32, g = O.mlp_loss_grad(p.astype(np.float32), x.astype(np.float32), t.astype(np.float32), 0.5)
+    assert abs(l32 - loss(p)) < 1e-3 * max(1.0, abs(loss(p)))
+    idx = rng.choice(5378, 60, replace=False)
+    idx = np.concatenate([idx, [0, 1023, 1024, 1100, 5184, 5300, 5376, 5377]])
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = 1e-4
+        fd = (loss(p + e) - loss(p - e)) / 2e-4
+        assert abs(fd - g[i]) <= 2e-3 * max(1.0, abs(fd)), (i, fd, g[i])
