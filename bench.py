@@ -101,6 +101,10 @@ def main():
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--schedule", default="fixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", dest="learned", action="store_false",
+                    help="skip the learned-schedule pass")
+    ap.add_argument("--learned-check", type=int, default=8,
+                    help="sources whose levels are compared between the schedules")
     ap.add_argument("--cpu-budget", type=float, default=60.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
@@ -193,6 +197,45 @@ def main():
                 "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_bytes, "launches": launches,
                 "share_of_kernel_time": ms_tot / max(agg[0][0] + agg[1][0], 1e-9)}
 
+    # the learned schedule (committed model) on the same sources: same step
+    # count, levels compared with the fixed schedule's for every source
+    learned = None
+    model_path = os.path.join(ROOT, "scaph_amd", "schedule_mlp.bin")
+    if args.learned and os.path.exists(model_path):
+        from scaph_amd import learn as L
+        p, mean, scale, _ = L.load_model(model_path)
+        eng.set_model(p, mean, scale)
+        fixed_levels = {}
+        for src in srcs[:args.learned_check]:
+            eng.set_schedule("fixed")
+            eng.run(src)
+            fixed_levels[src] = eng.levels()
+        eng.set_schedule("learned")
+        for i in range(args.warmup):
+            eng.run(srcs[i % len(srcs)])
+        _lib.check(hip.scaph_device_sync())
+        lruns = []
+        for k in range(args.steps):
+            st = eng.run(srcs[k % len(srcs)])
+            st["dirs"] = [r["dir"] for r in eng.records()]
+            lruns.append(st)
+        _lib.check(hip.scaph_device_sync())
+        equal = True
+        for src, want in fixed_levels.items():
+            eng.run(src)
+            equal &= bool(np.array_equal(eng.levels(), want))
+        eng.set_schedule("fixed")
+        lg = [r["m_reached"] / (r["ms"] / 1e3) / 1e9 for r in lruns]
+        lt = statistics.median(r["ms"] for r in lruns)
+        ft = statistics.median(r["ms"] for r in runs)
+        same_dirs = sum(1 for a, b in zip(runs, lruns)
+                        if [x["dir"] for x in a["records"]] == b["dirs"])
+        learned = {"gteps_median": statistics.median(lg), "t_bfs_ms_median": lt,
+                   "speedup_vs_fixed": ft / lt, "levels_equal_fixed": equal,
+                   "sources_checked": len(fixed_levels),
+                   "runs_with_same_directions": same_dirs, "model": "scaph_amd/schedule_mlp.bin"}
+        log("learned schedule", learned)
+
     # parity at full size: device invariant check on the last source
     viol = eng.validate(runs[-1]["src"])
     cpu = None
@@ -224,6 +267,7 @@ def main():
                    "schedule": args.schedule, "alpha": 14, "beta": 24},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned": learned,
         "bfs_hbm_frac_median": statistics.median(frac),
         "t_bfs_ms_median": statistics.median(r["ms"] for r in runs),
         "gteps_ref_formula_median": statistics.median(ref_gteps),

@@ -265,14 +265,25 @@ int scaph_frontier_page_counts(const uint32_t* bitmap, const uint32_t* queue,
 /* One decision on the device: the level's counters (next frontier size and
  * edges), the unexplored edges before the level (m_u; the kernel subtracts
  * the counters' m_next) and the per-page counts become the 16 features
- * (normalised
- * by norm = mean[16], scale[16]); out (36 floats) receives the pull logit,
- * the threshold, the number of active pages, the 12-bin histogram and the
- * raw features. */
+ * (normalised by norm = mean[16], scale[16]); out (32 floats) receives
+ * [0] the pull logit, [1] the threshold, [2] the number of active pages,
+ * [3] padding, [4..15] the 12-bin histogram, [16..31] the raw features. */
 int scaph_mlp_decide(const uint16_t* params_bf16, const float* norm, const uint32_t* page_cnt,
                      const scaph_page* pages, uint32_t num_pages,
                      const scaph_level_counters* counters, unsigned long long m_u,
                      uint32_t level, float* out, void* stream);
+
+/* scaph_frontier_page_counts and scaph_mlp_decide as the engine uses them
+ * once per level (two launches, no host work between): the frontier is a
+ * bitmap, or (bitmap NULL) a list whose length is counters->q_len.
+ * page_cnt (num_pages u32) is workspace that must be zero before the first
+ * call; the decision kernel leaves it zero again.  Calls sharing that
+ * workspace must not overlap.  out as for scaph_mlp_decide. */
+int scaph_mlp_decide_frontier(const uint16_t* params_bf16, const float* norm,
+                              const uint32_t* bitmap, const uint32_t* queue,
+                              const scaph_page* pages, uint32_t num_pages, uint32_t n,
+                              const scaph_level_counters* counters, unsigned long long m_u,
+                              uint32_t level, uint32_t* page_cnt, float* out, void* stream);
 
 /* ------------------------------------------------------------ hipGraph */
 /* Capture everything launched on `stream` between begin and end into an

@@ -146,8 +146,10 @@ def hip():
         _sig(lib, "scaph_mlp_forward", I32, [P, P, U32, P, P, P])
         _sig(lib, "scaph_mlp_backward", I32, [P, P, P, P, P, U32, ctypes.c_float, P, P, P])
         _sig(lib, "scaph_mlp_sgd", I32, [P, P, P, U32, ctypes.c_float, ctypes.c_float, P, P])
-        _sig(lib, "scaph_frontier_page_counts", I32, [P, P, U32, P, U32, U32, P, P])
+        _sig(lib, "scaph_frontier_page_counts", I32, [P, P, P, P, U32, U32, P, P])
         _sig(lib, "scaph_mlp_decide", I32, [P, P, P, P, U32, P, ctypes.c_ulonglong, U32, P, P])
+        _sig(lib, "scaph_mlp_decide_frontier", I32,
+             [P, P, P, P, P, U32, U32, P, ctypes.c_ulonglong, U32, P, P, P])
         _sig(lib, "scaph_graph_capture_end", I32, [P, ctypes.POINTER(P)])
         _sig(lib, "scaph_graph_launch", I32, [P, P])
         _hip = lib

@@ -58,6 +58,63 @@ __device__ __forceinline__ void forward_wave(const float* w, const float* x, flo
   for (int o = 0; o < kOut; ++o) y[o] = wave_sum(w[kOffW3 + o * kH + j] * h2) + w[kOffB3 + o];
 }
 
+// The same forward pass reading the bf16 parameters straight from global
+// memory: lane j issues all loads of its W1 and W2 rows (16 B vectors) at
+// once, so a single decision costs one memory latency instead of staging
+// all 5378 parameters through LDS.
+__device__ __forceinline__ void unpack8(const uint4 q, float* f) {
+  const uint32_t u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f[2 * k] = __uint_as_float(u[k] << 16);
+    f[2 * k + 1] = __uint_as_float(u[k] & 0xFFFF0000u);
+  }
+}
+
+struct RowRegs {
+  uint4 q1[2], q2[8];
+  float b1, b2, w30, w31, b30, b31;
+};
+
+// Lane j's W1 and W2 rows and the scalars it needs, issued together.
+__device__ __forceinline__ void load_rows(const uint16_t* p, RowRegs& r) {
+  const int j = (int)lane_id();
+  const uint4* r1 = reinterpret_cast<const uint4*>(p + j * kIn);
+  const uint4* r2 = reinterpret_cast<const uint4*>(p + kOffW2 + j * kH);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) r.q1[k] = r1[k];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.q2[k] = r2[k];
+  r.b1 = bf16_to_f32(p[kOffB1 + j]);
+  r.b2 = bf16_to_f32(p[kOffB2 + j]);
+  r.w30 = bf16_to_f32(p[kOffW3 + j]);
+  r.w31 = bf16_to_f32(p[kOffW3 + kH + j]);
+  r.b30 = bf16_to_f32(p[kOffB3]);
+  r.b31 = bf16_to_f32(p[kOffB3 + 1]);
+}
+
+// forward_wave's arithmetic (same order of accumulation) on loaded rows.
+__device__ __forceinline__ void forward_rows(const RowRegs& r, const float* x, float* y) {
+  float w1[kIn];
+  unpack8(r.q1[0], w1);
+  unpack8(r.q1[1], w1 + 8);
+  float a = r.b1;
+#pragma unroll
+  for (int i = 0; i < kIn; ++i) a += w1[i] * x[i];
+  const float h1 = a > 0.f ? a : 0.f;
+  float b = r.b2;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    float w2[8];
+    unpack8(r.q2[k], w2);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b += w2[i] * __shfl(h1, 8 * k + i, SCAPH_WAVE);
+  }
+  const float h2 = b > 0.f ? b : 0.f;
+  y[0] = wave_sum(r.w30 * h2) + r.b30;
+  y[1] = wave_sum(r.w31 * h2) + r.b31;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_mlp_forward(const uint16_t* params, const float* x, uint32_t batch, float* y, float* act) {
   __shared__ float w[kNParam];
@@ -206,24 +263,73 @@ k_frontier_pages(const uint32_t* bitmap, const uint32_t* queue,
   }
 }
 
+// List form with at most kLdsPages pages: page bounds and a block-private
+// histogram in LDS, so the global counters see one add per (block, page)
+// instead of one per vertex (many vertices of a frontier share a page).
+constexpr uint32_t kLdsPages = 4096;
+
 __global__ void __launch_bounds__(kBlock)
-k_decide(const uint16_t* params, const float* norm /* mean[16], scale[16] */,
-         const uint32_t* page_cnt, const scaph_page* pages, uint32_t num_pages,
-         const scaph_level_counters* cnt, unsigned long long m_u, uint32_t level,
-         DecideOut* out) {
-  __shared__ float w[kNParam];
-  __shared__ float hist[12];
-  __shared__ float active;
+k_queue_pages(const uint32_t* queue, const unsigned long long* queue_len,
+              const scaph_page* pages, uint32_t num_pages, uint32_t n, uint32_t* page_cnt) {
+  __shared__ uint32_t left[kLdsPages];
+  __shared__ uint32_t hist[kLdsPages];
+  for (uint32_t p = threadIdx.x; p < num_pages; p += blockDim.x) {
+    left[p] = pages[p].left;
+    hist[p] = 0;
+  }
+  __syncthreads();
+  const unsigned long long n_queue = *queue_len;
+  for (unsigned long long i = blockIdx.x * blockDim.x + threadIdx.x; i < n_queue;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    const uint32_t v = queue[i];
+    if (v >= n) continue;
+    uint32_t lo = 0, hi = num_pages;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (left[mid] <= v) lo = mid; else hi = mid;
+    }
+    atomicAdd(&hist[lo], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < num_pages; p += blockDim.x)
+    if (hist[p]) atomicAdd(&page_cnt[p], hist[p]);
+}
+
+// The decision proper, by one block: per-page shares -> 12-bin histogram,
+// the 16 features, normalisation, forward pass.  `w` holds the staged
+// parameters; page_cnt must be visible to this block.
+__device__ void decide_block(const uint16_t* params, const float* norm, const uint32_t* page_cnt,
+                             const scaph_page* pages, uint32_t num_pages,
+                             const scaph_level_counters* cnt, unsigned long long m_u,
+                             uint32_t level, DecideOut* out, float* hist, float* active) {
+  // everything the forward pass needs is requested before the page loop
+  RowRegs rows;
+  float nrm[2 * kIn];
+  unsigned long long n_next = 0, m_next = 0;
+  if (threadIdx.x < SCAPH_WAVE) {
+    load_rows(params, rows);
+    const float4* n4 = reinterpret_cast<const float4*>(norm);
+#pragma unroll
+    for (int k = 0; k < 2 * kIn / 4; ++k) {
+      const float4 t = n4[k];
+      nrm[4 * k] = t.x;
+      nrm[4 * k + 1] = t.y;
+      nrm[4 * k + 2] = t.z;
+      nrm[4 * k + 3] = t.w;
+    }
+    n_next = cnt->n_next;
+    m_next = cnt->m_next;
+  }
   if (threadIdx.x < 12) hist[threadIdx.x] = 0.f;
-  if (threadIdx.x == 0) active = 0.f;
-  stage_params(w, params);  // ends with a barrier
+  if (threadIdx.x == 0) *active = 0.f;
+  __syncthreads();
   for (uint32_t p = threadIdx.x; p < num_pages; p += blockDim.x) {
     const float share = (float)page_cnt[p] / (float)(pages[p].nodenum ? pages[p].nodenum : 1u);
     int bin = 0;
     if (share > 0.f) {
       const float l = floorf(__log2f(share) + 11.f);
       bin = 1 + (int)fminf(10.f, fmaxf(0.f, l));
-      atomicAdd(&active, 1.f);
+      atomicAdd(active, 1.f);
     }
     atomicAdd(&hist[bin], 1.f);
   }
@@ -231,27 +337,119 @@ k_decide(const uint16_t* params, const float* norm /* mean[16], scale[16] */,
   if (threadIdx.x < SCAPH_WAVE) {
     const float np = (float)(num_pages ? num_pages : 1u);
     float x[kIn];
-    x[0] = __log2f(1.f + (float)cnt->n_next);
-    x[1] = __log2f(1.f + (float)cnt->m_next);
-    const unsigned long long mu = m_u > cnt->m_next ? m_u - cnt->m_next : 0ull;
+    x[0] = __log2f(1.f + (float)n_next);
+    x[1] = __log2f(1.f + (float)m_next);
+    const unsigned long long mu = m_u > m_next ? m_u - m_next : 0ull;
     x[2] = __log2f(1.f + (float)mu);
     x[3] = (float)level;
     for (int i = 0; i < 12; ++i) x[4 + i] = hist[i] / np;
     float xn[kIn];
+#pragma unroll
     for (int i = 0; i < kIn; ++i) {
-      const float sc = norm[kIn + i];
-      xn[i] = (x[i] - norm[i]) / (sc != 0.f ? sc : 1.f);
+      const float sc = nrm[kIn + i];
+      xn[i] = (x[i] - nrm[i]) / (sc != 0.f ? sc : 1.f);
     }
     float y[kOut];
-    forward_wave(w, xn, nullptr, nullptr, y);
+    forward_rows(rows, xn, y);
     if (threadIdx.x == 0) {
       out->logit_pull = y[0];
       out->threshold = y[1];
-      out->pages_active = active;
+      out->pages_active = *active;
+      out->pad = 0.f;
       for (int i = 0; i < 12; ++i) out->hist[i] = hist[i] / np;
       for (int i = 0; i < kIn; ++i) out->feat[i] = x[i];
     }
   }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_decide(const uint16_t* params, const float* norm /* mean[16], scale[16] */,
+         const uint32_t* page_cnt, const scaph_page* pages, uint32_t num_pages,
+         const scaph_level_counters* cnt, unsigned long long m_u, uint32_t level,
+         DecideOut* out) {
+  __shared__ float hist[12];
+  __shared__ float active;
+  decide_block(params, norm, page_cnt, pages, num_pages, cnt, m_u, level, out, hist, &active);
+}
+
+// Per-page frontier counts added into page_cnt (zero on entry): every block
+// counts its share of the frontier (bitmap: 64 vertices per thread, split
+// at page boundaries; list: one vertex per thread) into an LDS histogram
+// and flushes it with one atomic per touched page.
+__global__ void __launch_bounds__(kBlock)
+k_count_frontier(const uint32_t* bitmap, const uint32_t* queue, const scaph_page* pages,
+                 uint32_t num_pages, uint32_t n, const scaph_level_counters* cnt,
+                 uint32_t* page_cnt) {
+  __shared__ uint32_t left[kLdsPages];
+  __shared__ uint32_t lhist[kLdsPages];
+  const bool lds = num_pages <= kLdsPages;
+  if (lds) {
+    for (uint32_t p = threadIdx.x; p < num_pages; p += blockDim.x) {
+      left[p] = pages[p].left;
+      lhist[p] = 0;
+    }
+  }
+  __syncthreads();
+  auto page_of = [&](uint32_t v) {
+    uint32_t lo = 0, hi = num_pages;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if ((lds ? left[mid] : pages[mid].left) <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+  };
+  auto add = [&](uint32_t p, uint32_t c) {
+    if (!c) return;
+    if (lds) atomicAdd(&lhist[p], c);
+    else atomicAdd(&page_cnt[p], c);
+  };
+  if (bitmap) {
+    // one 64-vertex pair of words per thread; a pair that straddles a page
+    // boundary is split at the boundary
+    const uint32_t pairs = (n + 63) >> 6;
+    const uint2* bm2 = reinterpret_cast<const uint2*>(bitmap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < pairs;
+         i += gridDim.x * blockDim.x) {
+      const uint2 q = bm2[i];
+      uint64_t bits = ((uint64_t)q.y << 32) | q.x;
+      const uint32_t v0 = i << 6;
+      if (n - v0 < 64) bits &= (1ull << (n - v0)) - 1ull;
+      if (!bits) continue;
+      uint32_t p = page_of(v0);
+      while (bits) {
+        const uint32_t right = pages[p].right;  // inclusive
+        const uint32_t span = right - v0 + 1;  // vertices of this pair in page p and before
+        const uint64_t in_p = span >= 64 ? bits : bits & ((1ull << span) - 1ull);
+        add(p, (uint32_t)__popcll(in_p));
+        bits &= ~in_p;
+        if (++p >= num_pages) break;
+      }
+    }
+  } else {
+    const unsigned long long n_queue = cnt->q_len;
+    for (unsigned long long i = blockIdx.x * blockDim.x + threadIdx.x; i < n_queue;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+      const uint32_t v = queue[i];
+      if (v >= n) continue;
+      add(page_of(v), 1u);
+    }
+  }
+  __syncthreads();
+  if (lds)
+    for (uint32_t p = threadIdx.x; p < num_pages; p += blockDim.x)
+      if (lhist[p]) atomicAdd(&page_cnt[p], lhist[p]);
+}
+
+// The decision after k_count_frontier: one block; it clears page_cnt
+// behind itself so the next level's counts start from zero.
+__global__ void __launch_bounds__(kBlock)
+k_decide_clear(const uint16_t* params, const float* norm, uint32_t* page_cnt,
+               const scaph_page* pages, uint32_t num_pages, const scaph_level_counters* cnt,
+               unsigned long long m_u, uint32_t level, DecideOut* out) {
+  __shared__ float hist[12];
+  __shared__ float active;
+  decide_block(params, norm, page_cnt, pages, num_pages, cnt, m_u, level, out, hist, &active);
+  for (uint32_t p = threadIdx.x; p < num_pages; p += blockDim.x) page_cnt[p] = 0;
 }
 
 inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
@@ -309,8 +507,12 @@ int scaph_frontier_page_counts(const uint32_t* bitmap, const uint32_t* queue,
     hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(num_pages, kBlock, 64)), dim3(kBlock), 0, st,
                        reinterpret_cast<float*>(page_cnt), num_pages);
     SCAPH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_frontier_pages, dim3(1024), dim3(kBlock), 0, st, bitmap, queue, queue_len,
-                       pages, num_pages, n, page_cnt);
+    if (num_pages <= kLdsPages)
+      hipLaunchKernelGGL(k_queue_pages, dim3(256), dim3(kBlock), 0, st, queue, queue_len, pages,
+                         num_pages, n, page_cnt);
+    else
+      hipLaunchKernelGGL(k_frontier_pages, dim3(1024), dim3(kBlock), 0, st, bitmap, queue,
+                         queue_len, pages, num_pages, n, page_cnt);
   } else {
     hipLaunchKernelGGL(k_frontier_pages, dim3(grid_for(num_pages, 1, 4096)), dim3(kBlock), 0, st,
                        bitmap, queue, queue_len, pages, num_pages, n, page_cnt);
@@ -328,6 +530,26 @@ int scaph_mlp_decide(const uint16_t* params_bf16, const float* norm, const uint3
   hipLaunchKernelGGL(k_decide, dim3(1), dim3(kBlock), 0, as_stream(stream), params_bf16, norm,
                      page_cnt, pages, num_pages, counters, m_u, level,
                      reinterpret_cast<DecideOut*>(out));
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_mlp_decide_frontier(const uint16_t* params_bf16, const float* norm,
+                              const uint32_t* bitmap, const uint32_t* queue,
+                              const scaph_page* pages, uint32_t num_pages, uint32_t n,
+                              const scaph_level_counters* counters, unsigned long long m_u,
+                              uint32_t level, uint32_t* page_cnt, float* out, void* stream) {
+  if (!params_bf16 || !norm || !pages || !counters || !page_cnt || !out || (!bitmap && !queue))
+    return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  // bitmap: one thread per 64 vertices; list: a fixed grid reading the
+  // length on the device
+  const unsigned grid = bitmap ? grid_for(((size_t)n + 63) / 64, kBlock, 512) : 256u;
+  hipLaunchKernelGGL(k_count_frontier, dim3(grid), dim3(kBlock), 0, st, bitmap, queue, pages,
+                     num_pages, n, counters, page_cnt);
+  SCAPH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_decide_clear, dim3(1), dim3(kBlock), 0, st, params_bf16, norm, page_cnt,
+                     pages, num_pages, counters, m_u, level, reinterpret_cast<DecideOut*>(out));
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

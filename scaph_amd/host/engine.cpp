@@ -221,7 +221,10 @@ void Engine::alloc_state() {
   front_[1].alloc(words * 4);
   queue_[0].alloc(std::max<size_t>(n_, 1) * 4);
   queue_[1].alloc(std::max<size_t>(n_, 1) * 4);
-  counters_.alloc(4 * sizeof(scaph_level_counters));
+  // level counters (four records), then the learned decision's output
+  // (word offset 64), all zero to start
+  counters_.alloc(1024);
+  H_ERR(scaph_fill_u32(counters_.get<uint32_t>(), 0u, 1024 / 4, main_->get()));
   // the resident push expands whole frontiers; the paged mode keeps one
   // page-sized workspace per exec stream instead
   ws_bytes_ = resident_ ? scaph_push_workspace_bytes(std::max<uint32_t>(n_, 1),

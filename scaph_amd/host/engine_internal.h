@@ -13,7 +13,7 @@ struct Engine::DeviceModel {
   DevBuf params;    // SCAPH_MLP_NPARAM u16
   DevBuf norm;      // 32 f32
   DevBuf page_cnt;  // u32 per page
-  DevBuf out;       // 36 f32 (logit, threshold, pages active, pad, hist[12], feat[16])
+  DevBuf out;       // 32 f32 (logit, threshold, pages active, pad, hist[12], feat[16])
   DevBuf ccnt;      // a counters record describing the frontier entering a level
   PinnedBuf hout;   // host landing zone for out
   PinnedBuf hccnt;

@@ -12,7 +12,10 @@ namespace {
 
 constexpr size_t kMaxTimedLevels = 1024;
 constexpr size_t kCounterWords = sizeof(scaph_level_counters) / 4;
-constexpr size_t kDecideFloats = 36;
+constexpr size_t kDecideFloats = 32;  // logit, threshold, pages active, pad, hist[12], feat[16]
+// Word offset of the learned decision's output inside the counters buffer
+// (beside the level counters, so one copy returns both).
+constexpr size_t kDecideWord = 64;
 
 size_t bitmap_words(uint32_t n) { return (((size_t)n + 63) / 64) * 2; }
 
@@ -77,8 +80,12 @@ void Engine::ensure_model_buffers() {
     H_ERR(scaph_copy_h2d_async(m.norm.get(), norm.data(), norm.size() * 4, main_->get()));
     main_->sync();
   }
-  if (m.page_cnt.bytes() < std::max<size_t>(pages_out_.size(), 1) * 4)
+  if (m.page_cnt.bytes() < std::max<size_t>(pages_out_.size(), 1) * 4) {
+    // the decision kernels expect (and leave) page_cnt all zero
     m.page_cnt.alloc(std::max<size_t>(pages_out_.size(), 1) * 4);
+    H_ERR(scaph_fill_u32(m.page_cnt.get<uint32_t>(), 0u, m.page_cnt.bytes() / 4, main_->get()));
+    main_->sync();
+  }
 }
 
 void Engine::set_device_model(const uint16_t* params, const float* mean, const float* scale) {
@@ -106,7 +113,7 @@ RunStats Engine::bfs_resident(uint32_t src) {
   auto* hc = hcount_.get<scaph_level_counters>();
   const bool learned = cfg_.schedule == "learned" && device_model_ready() && has_in_;
   if (learned) ensure_model_buffers();
-  const float* hout = learned ? model_->hout.get<float>() : nullptr;
+  const float* hout = learned ? hcount_.get<float>() + kDecideWord : nullptr;
 
   ev_[0]->record(st);
   H_ERR(scaph_bfs_init(level_.get<uint32_t>(), visited_.get<uint32_t>(),
@@ -142,19 +149,19 @@ RunStats Engine::bfs_resident(uint32_t src) {
       lev_ev.push_back(eb);
     }
     if (learned) {
-      // features of the new frontier and the model's choice, on the device
+      // features of the new frontier and the model's choice, on the device;
+      // the decision lands beside the counters so one copy brings both back
       DeviceModel& m = *model_;
       const bool bm = d == Dir::kPull;
-      H_ERR(scaph_frontier_page_counts(bm ? front_[f.fa].get<uint32_t>() : nullptr,
-                                       bm ? nullptr : queue_[f.qa].get<uint32_t>(), &cnt->q_len,
-                                       g_out_.pages, (uint32_t)pages_out_.size(), n_,
-                                       m.page_cnt.get<uint32_t>(), st));
-      H_ERR(scaph_mlp_decide(m.params.get<uint16_t>(), m.norm.get<float>(),
-                             m.page_cnt.get<uint32_t>(), g_out_.pages, (uint32_t)pages_out_.size(),
-                             cnt, m_u, cur + 1, m.out.get<float>(), st));
-      H_ERR(scaph_copy_d2h_async(m.hout.get(), m.out.get(), kDecideFloats * 4, st));
+      H_ERR(scaph_mlp_decide_frontier(
+          m.params.get<uint16_t>(), m.norm.get<float>(),
+          bm ? front_[f.fa].get<uint32_t>() : nullptr, bm ? nullptr : queue_[f.qa].get<uint32_t>(),
+          g_out_.pages, (uint32_t)pages_out_.size(), n_, cnt, m_u, cur + 1,
+          m.page_cnt.get<uint32_t>(), counters_.get<float>() + kDecideWord, st));
+      H_ERR(scaph_copy_d2h_async(hc, cnt, (kDecideWord + kDecideFloats) * 4, st));
+    } else {
+      H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
     }
-    H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
     if (timed) ev_[eb + 2]->record(st);
     main_->sync();
     LevelRecord rec;
@@ -204,8 +211,9 @@ std::vector<Engine::ExploreRecord> Engine::explore(uint32_t src, int reps) {
   DeviceModel& m = *model_;
   void* st = main_->get();
   const size_t words = bitmap_words(n_);
-  if (!snap_level_.get()) {
-    snap_level_.alloc((size_t)n_ * 4);
+  // snapshot buffers follow the loaded graph's size
+  if (snap_level_.bytes() < std::max<size_t>(n_, 1) * 4) {
+    snap_level_.alloc(std::max<size_t>(n_, 1) * 4);
     snap_visited_.alloc(words * 4);
     snap_queue_.alloc(std::max<size_t>(n_, 1) * 4);
     snap_front_.alloc(words * 4);
@@ -257,7 +265,7 @@ std::vector<Engine::ExploreRecord> Engine::explore(uint32_t src, int reps) {
     snapshot(true);
     main_->sync();
     ExploreRecord r;
-    for (int i = 0; i < 16; ++i) r.x[i] = hout[20 + i];
+    for (int i = 0; i < 16; ++i) r.x[i] = hout[16 + i];
     if (!recs.empty()) recs.back().pages_active_next = hout[2] / (float)pages_out_.size();
     r.level = cur;
     r.n_f = n_f;

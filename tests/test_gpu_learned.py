@@ -97,3 +97,121 @@ def test_learned_schedule_levels_bit_exact_and_explore():
             for s in srcs:
                 e.run(s)
                 np.testing.assert_array_equal(e.levels(), O.bfs(off, ngh, s))
+        # a second, larger graph on the same engine: explore and the learned
+        # schedule follow the new sizes
+        off2, ngh2 = O.rmat_csr(17, 8, seed=12)
+        e.load_csr(off2, ngh2)
+        s2 = int(np.argmax(np.diff(off2.astype(np.int64))))
+        say("explore on a larger graph", s2)
+        r2 = e.explore(s2, 1)
+        assert sum(1 for _ in r2) >= 2
+        np.testing.assert_array_equal(e.levels(), O.bfs(off2, ngh2, s2))
+        e.set_model(mlp.params_bf16(), mean, scale)
+        e.set_schedule("learned")
+        e.run(s2)
+        np.testing.assert_array_equal(e.levels(), O.bfs(off2, ngh2, s2))
+
+
+@pytest.mark.parametrize("page_words", [1 << 14, 128, 48])
+def test_frontier_page_counts_bitmap_and_list(page_words):
+    """Active vertices per page (the decision features) from a bitmap and
+    from a vertex list, for page tables that fit LDS and that do not."""
+    say("frontier page counts", page_words)
+    n = 40000
+    off, ngh = O.er_csr(n, 320000, 12)
+    deg = np.diff(off).astype(np.uint32)
+    pages = np.ascontiguousarray(O.page_cuts(off, page_words))
+    rng = np.random.default_rng(page_words)
+    stat = (rng.random(n) < 0.2).astype(np.uint8)
+    want, _, _ = O.page_counts(pages, stat, deg)
+    words = ((n + 63) // 64) * 2
+    bits = np.zeros(words * 32, np.uint8)
+    bits[:n] = stat
+    bitmap = np.packbits(bits, bitorder="little").view(np.uint32)
+    queue = rng.permutation(np.nonzero(stat)[0]).astype(np.uint32)
+    d_pages = L.DevArray(pages)
+    d_bm = L.DevArray(bitmap)
+    d_q = L.DevArray(queue)
+    d_len = L.DevArray(np.array([len(queue)], np.uint64))
+    d_cnt = L.DevArray(np.full(len(pages), 7, np.uint32))
+    hip = _lib.hip()
+    for form in ("bitmap", "list"):
+        _lib.check(hip.scaph_frontier_page_counts(d_bm.ptr if form == "bitmap" else None,
+                                                  None if form == "bitmap" else d_q.ptr,
+                                                  None if form == "bitmap" else d_len.ptr,
+                                                  d_pages.ptr, len(pages), n, d_cnt.ptr, None))
+        got = d_cnt.get(np.uint32, len(pages))
+        np.testing.assert_array_equal(got, want, err_msg=f"{form} pages={len(pages)}")
+        d_cnt.put(np.full(len(pages), 7, np.uint32))
+    # the fused entry (counts + decision in one launch) equals the two-call
+    # path, call after call (it leaves its workspace zero)
+    params = L.f32_to_bf16(L.init_params(seed=3))
+    norm = np.concatenate([rng.normal(0, 1, 16), rng.uniform(0.5, 2, 16)]).astype(np.float32)
+    d_par, d_norm = L.DevArray(params), L.DevArray(norm)
+    cnt = np.zeros(6, np.uint64)
+    cnt[0], cnt[1], cnt[2] = len(queue), int(deg[queue].sum()), len(queue)
+    d_c = L.DevArray(cnt)
+    d_ws = L.DevArray(np.zeros(len(pages) + 1, np.uint32))
+    d_out, d_out2 = L.DevArray(nbytes=128), L.DevArray(nbytes=128)
+    m_u = int(deg.sum())
+    for form in ("bitmap", "list", "bitmap", "list"):
+        bm, q = (d_bm.ptr, None) if form == "bitmap" else (None, d_q.ptr)
+        _lib.check(hip.scaph_frontier_page_counts(bm, q, d_c.ptr if q else None, d_pages.ptr,
+                                                  len(pages), n, d_cnt.ptr, None))
+        _lib.check(hip.scaph_mlp_decide(d_par.ptr, d_norm.ptr, d_cnt.ptr, d_pages.ptr, len(pages),
+                                        d_c.ptr, m_u, 3, d_out.ptr, None))
+        _lib.check(hip.scaph_mlp_decide_frontier(d_par.ptr, d_norm.ptr, bm, q, d_pages.ptr,
+                                                 len(pages), n, d_c.ptr, m_u, 3, d_ws.ptr,
+                                                 d_out2.ptr, None))
+        a, b = d_out.get(np.float32, 32), d_out2.get(np.float32, 32)
+        np.testing.assert_array_equal(a, b, err_msg=form)
+        assert np.all(d_ws.get(np.uint32, len(pages)) == 0)
+        hist = np.bincount(np.where(want > 0, 1 + np.clip(np.floor(
+            np.log2(np.maximum(want, 1) / np.maximum(pages[:, 2], 1)) + 11), 0, 10), 0).astype(int),
+            minlength=12) / len(pages)
+        np.testing.assert_allclose(a[4:16], hist, rtol=1e-6)
+        assert a[2] == np.count_nonzero(want)
+        np.testing.assert_allclose(a[16:20], [np.log2(1 + len(queue)), np.log2(1 + cnt[1]),
+                                              np.log2(1 + m_u - cnt[1]), 3.0], rtol=1e-5)
+
+
+def test_trained_model_on_held_out_graph():
+    """The committed model, on a graph outside its training set: per-level
+    step times of both directions (explore) price its choices against the
+    per-level best and against the fixed alpha/beta rule; the learned
+    schedule's levels equal the fixed schedule's."""
+    p, mean, scale, _ = L.load_model()
+    pf = L.bf16_to_f32(p)
+    with scaph_amd.BFS("-max_memory=65536") as e:
+        e.gen_rmat_device(19, 16, 0.6, 0.16, 0.16, seed=777)
+        n = e.info()["n"]
+        srcs = L.pick_sources(e, 2, seed=777)
+        t_model = t_best = t_fixed = 0.0
+        for s in srcs:
+            say("held-out explore", s)
+            recs = e.explore(s, 2)
+            X = np.array([r["x"] for r in recs], np.float32)
+            y, _, _ = O.mlp_forward(pf, ((X - mean) / scale).astype(np.float32))
+            prev_pull, prev_nf = False, 0
+            for r, yy in zip(recs, y):
+                tp, tq = r["t_push_ms"], r["t_pull_ms"]
+                t_model += tq if yy[0] > 0 else tp
+                t_best += min(tp, tq)
+                m_u = 2.0 ** r["x"][2] - 1
+                if not prev_pull:
+                    pull = r["m_f"] > m_u / 14
+                else:
+                    pull = not (r["n_f"] < prev_nf and r["n_f"] < n / 24)
+                t_fixed += tq if pull else tp
+                prev_pull, prev_nf = tq < tp, r["n_f"]
+            e.set_schedule("fixed")
+            e.run(s)
+            want = e.levels()
+            e.set_model(p, mean, scale)
+            e.set_schedule("learned")
+            e.run(s)
+            np.testing.assert_array_equal(e.levels(), want)
+        say(f"step time: model {t_model:.3f} ms, fixed rule {t_fixed:.3f} ms, "
+            f"per-level best {t_best:.3f} ms")
+        # the model picks (nearly) the faster direction at every level
+        assert t_model <= 1.15 * t_best

@@ -192,3 +192,13 @@ def test_page_scheduler_invariants(list_size, pool, thres, multi, seed):
                                           seed, stats.ctypes.data)
     assert rc == 0, f"invariant {rc} violated"
     assert stats[0] >= (datas > 0).sum()
+
+
+def test_committed_schedule_model_loads():
+    from scaph_amd import learn as L
+    p, mean, scale, gain = L.load_model()
+    assert p.shape == (L.NPARAM,) and mean.shape == (16,) and scale.shape == (16,)
+    w = L.bf16_to_f32(p)
+    assert np.all(np.isfinite(w)) and np.all(scale > 0) and np.isfinite(gain)
+    # bf16 round trip of the stored parameters is exact
+    np.testing.assert_array_equal(L.f32_to_bf16(w), p)
