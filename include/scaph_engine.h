@@ -15,6 +15,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "scaph_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -90,6 +92,37 @@ typedef struct scaph_explore_record {
 int scaph_engine_explore(scaph_engine* e, uint32_t src, int reps, scaph_explore_record* out,
                          uint32_t max, uint32_t* count, char* err, size_t errlen);
 int scaph_engine_levels(scaph_engine* e, uint32_t* host_out);
+
+/* Multi-source BFS (no reference counterpart): up to 256 queries over the
+ * loaded resident graph in one traversal (scaph_ms_* kernels).  A NULL tune
+ * uses the last autotuned choice, else the defaults.  Levels of each query
+ * equal a single-source BFS from its source. */
+typedef struct scaph_ms_tune {
+  uint32_t push_div, push_mdiv; /* push while n_active*push_div < n and m_active*push_mdiv < m */
+  uint32_t pull_block;          /* 256, 512 or 1024 */
+  uint32_t use_graph;           /* replay hipGraph-captured level segments */
+  uint32_t segment;             /* levels per captured segment */
+} scaph_ms_tune;
+typedef struct scaph_ms_stats {
+  double ms;                    /* device time: init and every level */
+  uint32_t depth, queries;
+  unsigned long long pairs;     /* (vertex, query) pairs reached */
+} scaph_ms_stats;
+int scaph_engine_msbfs(scaph_engine* e, const uint32_t* sources, uint32_t count,
+                       const scaph_ms_tune* tune, scaph_ms_stats* out, char* err, size_t errlen);
+/* uint32 levels of query `query` of the last multi-source run. */
+int scaph_engine_msbfs_levels(scaph_engine* e, uint32_t query, uint32_t* host_out, char* err,
+                              size_t errlen);
+/* Per-level counters of the last run: records 0..depth. */
+int scaph_engine_msbfs_counters(scaph_engine* e, scaph_ms_counters* out, uint32_t max,
+                                uint32_t* count);
+/* Times each candidate tuning on these sources and keeps the fastest for
+ * later NULL-tune runs; `log` (may be NULL) receives up to log_max
+ * (tune, ms) pairs. */
+int scaph_engine_msbfs_autotune(scaph_engine* e, const uint32_t* sources, uint32_t count,
+                                scaph_ms_tune* best, double* best_ms, scaph_ms_tune* log_tunes,
+                                double* log_ms, uint32_t log_max, uint32_t* log_count,
+                                char* err, size_t errlen);
 int scaph_engine_validate(scaph_engine* e, uint32_t src, uint64_t* out2);
 int scaph_engine_out_degrees(scaph_engine* e, uint32_t* host_out);
 /* Copy of the host CSR kept at load (offsets n+1, ngh m); 1 if none kept. */
@@ -115,6 +148,35 @@ int scaph_host_write_graph(const char* base, uint32_t n, const uint64_t* offsets
                            int write_wgh, char* err, size_t errlen);
 int scaph_host_cpu_bfs(uint32_t n, const uint64_t* offsets, const uint32_t* ngh, uint32_t src,
                        uint32_t inf, uint32_t* levels);
+
+/* Batch of small graphs (no reference counterpart): `count` R-MAT graphs
+ * generated on the device (graph g: scale and edge factor uniform in the
+ * given ranges, its own seed and source, from SplitMix64 words of `seed`),
+ * all queried by one persistent launch. */
+typedef struct scaph_batch scaph_batch;
+typedef struct scaph_batch_tune {
+  uint32_t block;       /* threads per workgroup: 256, 512, 1024 */
+  uint32_t wgs_per_cu;  /* persistent workgroups per compute unit */
+  uint32_t use_graph;   /* replay a captured hipGraph */
+} scaph_batch_tune;
+int scaph_batch_create_rmat(uint32_t count, uint32_t min_scale, uint32_t max_scale,
+                            uint32_t min_ef, uint32_t max_ef, uint64_t seed, int device,
+                            scaph_batch** out, char* err, size_t errlen);
+void scaph_batch_destroy(scaph_batch* b);
+int scaph_batch_info(scaph_batch* b, uint32_t* count, unsigned long long* vertices,
+                     unsigned long long* edges);
+int scaph_batch_graph_info(scaph_batch* b, uint32_t g, scaph_batch_entry* out);
+/* One batch of queries (every graph from its source); NULL tune = the
+ * autotuned choice (or the defaults).  *ms = device time. */
+int scaph_batch_run(scaph_batch* b, const scaph_batch_tune* tune, double* ms, char* err,
+                    size_t errlen);
+int scaph_batch_levels(scaph_batch* b, uint32_t g, uint32_t* host_out /* n_g */);
+/* Graph g's CSR with local ids; ngh may be NULL to get the edge count. */
+int scaph_batch_csr(scaph_batch* b, uint32_t g, uint64_t* offsets /* n_g + 1 */, uint32_t* ngh,
+                    unsigned long long cap, unsigned long long* num_edges);
+int scaph_batch_autotune(scaph_batch* b, scaph_batch_tune* best, double* best_ms,
+                         scaph_batch_tune* log_tunes, double* log_ms, uint32_t log_max,
+                         uint32_t* log_count, char* err, size_t errlen);
 
 /* Page-slot chunk masks (src/tools.h:10-85). */
 int scaph_host_max_zero_run(uint32_t mask);

@@ -285,6 +285,98 @@ int scaph_mlp_decide_frontier(const uint16_t* params_bf16, const float* norm,
                               const scaph_level_counters* counters, unsigned long long m_u,
                               uint32_t level, uint32_t* page_cnt, float* out, void* stream);
 
+/* ------------------------------------------------- multi-source bitset BFS */
+/* Up to 256 BFS queries over one graph at once (no reference counterpart):
+ * each vertex carries 4 x 64-bit words, one bit per query, for the vertices
+ * it has been reached from ("seen") and, per level, for the queries whose
+ * frontier it is in at that level.  The per-level frontier words are kept
+ * (front[L]), so the level of vertex v in query q is the L whose front[L][v]
+ * holds bit q; the decode entry points turn that into uint32 levels.
+ * Every kernel of a level reads the level's counters on the device to pick
+ * push or pull and to stop once the frontier is empty, so a sequence of
+ * levels can be captured into a hipGraph and replayed without host work. */
+#define SCAPH_MS_WORDS 4
+#define SCAPH_MS_MAX_QUERIES 256
+typedef struct scaph_ms_counters {
+  unsigned long long n_active;  /* vertices in the level's frontier (any query) */
+  unsigned long long bits;      /* (vertex, query) pairs first reached at the level */
+  unsigned long long m_active;  /* out-edges of the frontier's vertices */
+  unsigned long long q_len;     /* push: frontier list length */
+  unsigned long long heavy_len; /* push: long out-lists cut into chunks */
+  unsigned long long overflow;  /* non-zero: a workspace bound was exceeded */
+  unsigned long long pad[2];
+} scaph_ms_counters; /* 64 bytes */
+typedef struct scaph_ms_params {
+  uint32_t num_queries;       /* 1..256 */
+  uint32_t push_div;          /* push while n_active * push_div < n ... */
+  uint32_t push_mdiv;         /* ... and m_active * push_mdiv < m */
+  uint32_t pull_block;        /* threads per pull block: 256, 512 or 1024 */
+  unsigned long long m;       /* edges of the graph */
+} scaph_ms_params;
+typedef struct scaph_ms_state {
+  uint64_t* seen;             /* n x 4 words */
+  uint64_t* front;            /* max_levels x n x 4 words */
+  uint64_t* act;              /* max_levels x act_words: frontier membership bits */
+  scaph_ms_counters* ctr;     /* max_levels + 1 records */
+  uint32_t* queue;            /* n entries */
+  uint64_t* heavy;            /* heavy_cap chunk items */
+  uint32_t n, max_levels;
+  unsigned long long act_words; /* ceil(n / 64) */
+  unsigned long long heavy_cap;
+} scaph_ms_state;
+/* Device bytes of the state for n vertices, m edges and max_levels levels
+ * (the caller carves the buffers; see scaph_ms_state_carve). */
+size_t scaph_ms_state_bytes(uint32_t n, unsigned long long m, uint32_t max_levels);
+/* Fill *st's pointers from one allocation of scaph_ms_state_bytes bytes. */
+int scaph_ms_state_carve(void* base, uint32_t n, unsigned long long m, uint32_t max_levels,
+                         scaph_ms_state* st);
+/* Level 0: clears the state and sets query q's bit at sources[q]. */
+int scaph_ms_init(const scaph_ms_state* st, const uint32_t* sources, uint32_t count,
+                  const uint32_t* degree, void* stream);
+/* One level (level -> level+1): push (out_graph) or pull (in_graph with
+ * hub-tagged in-lists, as scaph_bfs_pull) chosen on the device. */
+int scaph_ms_level(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
+                   const uint32_t* hub_list, const uint32_t* degree, const scaph_ms_state* st,
+                   uint32_t level, const scaph_ms_params* prm, void* stream);
+/* uint32 levels of one query (SCAPH_UNREACHED where not reached) from the
+ * first `depth` levels. */
+int scaph_ms_levels_u32(const scaph_ms_state* st, uint32_t depth, uint32_t query, uint32_t* out,
+                        void* stream);
+/* Levels of all queries as bytes, out[v * 256 + q] (0xFF = not reached);
+ * needs depth < 255. */
+int scaph_ms_levels_u8(const scaph_ms_state* st, uint32_t depth, uint8_t* out, void* stream);
+
+/* ------------------------------------------------ batch of small graphs */
+/* Many independent small graphs (n <= 65536 each) stored as one disjoint-
+ * union CSR and traversed by one persistent launch, one graph per
+ * workgroup at a time (no reference counterpart). */
+typedef struct scaph_batch_entry {
+  unsigned long long vbase;   /* first global vertex id of the graph */
+  unsigned long long ebase;   /* first raw edge index (generator stream) */
+  unsigned long long seed;    /* the graph's R-MAT seed */
+  uint32_t n;                 /* vertices (<= 65536) */
+  uint32_t scale;             /* n = 2^scale for generated graphs */
+  uint32_t src;               /* BFS source (local id) */
+  uint32_t edge_factor;
+} scaph_batch_entry;
+/* Device workspace of scaph_batch_gen_rmat_keys for `count` graphs. */
+size_t scaph_batch_perm_ws_bytes(uint32_t count);
+/* Raw R-MAT edge keys (global u << 32 | global v) of every graph of the
+ * batch, graph g's edges at [ebase_g, ebase_g + edge_factor_g * 2^scale_g):
+ * each graph is the same edge stream as scaph_gen_rmat_keys with its seed.
+ * Build the union CSR from them with scaph_csr_from_keys. */
+int scaph_batch_gen_rmat_keys(uint64_t* keys, const scaph_batch_entry* graphs, uint32_t count,
+                              unsigned long long total_edges, uint32_t thr_a, uint32_t thr_ab,
+                              uint32_t thr_abc, void* perm_ws, void* stream);
+/* BFS of every graph from its source: levels (global vertex order) get the
+ * uint32 levels (SCAPH_UNREACHED where not reached).  order lists graph
+ * indices in the order workgroups take them; `work` is one u32 of scratch,
+ * qbuf grid * qcap u32 (qcap >= the largest n).  block: 256, 512 or 1024
+ * threads; grid: persistent workgroups.  Capturable (no memset). */
+int scaph_batch_bfs(const uint64_t* offsets, const uint32_t* ngh, const scaph_batch_entry* graphs,
+                    const uint32_t* order, uint32_t count, uint32_t* levels, uint32_t* work,
+                    uint32_t* qbuf, uint32_t qcap, uint32_t block, uint32_t grid, void* stream);
+
 /* ------------------------------------------------------------ hipGraph */
 /* Capture everything launched on `stream` between begin and end into an
  * executable graph (opaque handle); launch replays it on `stream`. */

@@ -16,7 +16,7 @@ import numpy as np
 from . import _lib
 from ._lib import UNREACHED, LibraryMissing, ScaphError  # noqa: F401
 
-__all__ = ["BFS", "generate_rmat", "generate_er", "page_cuts", "write_graph", "UNREACHED",
+__all__ = ["BFS", "Batch", "generate_rmat", "generate_er", "page_cuts", "write_graph", "UNREACHED",
            "LibraryMissing", "ScaphError", "device_props"]
 
 
@@ -140,6 +140,52 @@ class BFS:
                         "pages_active_next": r.pages_active_next, "x": list(r.x)})
         return out
 
+    # -- multi-source (batched) BFS ------------------------------------------
+    def msbfs(self, sources, tune=None):
+        """Up to 256 BFS queries in one traversal; returns the run's stats."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        st = _lib.MsStats()
+        t = None
+        if tune is not None:
+            t = _lib.MsTune(tune.get("push_div", 64), tune.get("push_mdiv", 16),
+                            tune.get("pull_block", 256), tune.get("use_graph", 0),
+                            tune.get("segment", 8))
+        e = _err()
+        _lib.check(self._lib.scaph_engine_msbfs(self._h, _ptr(src), len(src),
+                                                ctypes.byref(t) if t is not None else None,
+                                                ctypes.byref(st), e, 1024), e.value.decode())
+        return {"ms": st.ms, "depth": st.depth, "queries": st.queries, "pairs": st.pairs}
+
+    def msbfs_levels(self, query):
+        n = self.info()["n"]
+        out = np.empty(n, np.uint32)
+        e = _err()
+        _lib.check(self._lib.scaph_engine_msbfs_levels(self._h, query, _ptr(out), e, 1024),
+                   e.value.decode())
+        return out
+
+    def msbfs_counters(self):
+        arr = (_lib.MsCounters * 256)()
+        cnt = ctypes.c_uint32()
+        _lib.check(self._lib.scaph_engine_msbfs_counters(self._h, arr, 256, ctypes.byref(cnt)))
+        return [{k: getattr(r, k) for k in ("n_active", "bits", "m_active", "q_len", "heavy_len",
+                                             "overflow")} for r in arr[:min(cnt.value, 256)]]
+
+    def msbfs_autotune(self, sources):
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        best = _lib.MsTune()
+        bms = ctypes.c_double()
+        logt = (_lib.MsTune * 64)()
+        logm = (ctypes.c_double * 64)()
+        cnt = ctypes.c_uint32()
+        e = _err()
+        _lib.check(self._lib.scaph_engine_msbfs_autotune(self._h, _ptr(src), len(src),
+                                                         ctypes.byref(best), ctypes.byref(bms),
+                                                         logt, logm, 64, ctypes.byref(cnt), e,
+                                                         1024), e.value.decode())
+        as_d = lambda t: {k: getattr(t, k) for k, _ in _lib.MsTune._fields_}  # noqa: E731
+        return as_d(best), bms.value, [(as_d(logt[i]), logm[i]) for i in range(min(cnt.value, 64))]
+
     def levels(self):
         n = self.info()["n"]
         out = np.empty(n, np.uint32)
@@ -181,6 +227,86 @@ class BFS:
         p = ctypes.c_void_p()
         _lib.check(self._lib.scaph_engine_device_degree(self._h, ctypes.byref(p)))
         return p.value
+
+
+class Batch:
+    """A batch of small R-MAT graphs on the device, all queried by one
+    persistent launch (one BFS per graph from its own source)."""
+
+    def __init__(self, count=4096, min_scale=10, max_scale=16, min_ef=8, max_ef=32, seed=1,
+                 device=0):
+        lib = _lib.host()
+        h = ctypes.c_void_p()
+        e = _err()
+        _lib.check(lib.scaph_batch_create_rmat(count, min_scale, max_scale, min_ef, max_ef, seed,
+                                               device, ctypes.byref(h), e, 1024),
+                   e.value.decode())
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.scaph_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        c, v, m = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.check(self._lib.scaph_batch_info(self._h, ctypes.byref(c), ctypes.byref(v),
+                                              ctypes.byref(m)))
+        return {"graphs": c.value, "vertices": v.value, "edges": m.value}
+
+    def graph(self, g):
+        r = _lib.BatchEntry()
+        _lib.check(self._lib.scaph_batch_graph_info(self._h, g, ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in _lib.BatchEntry._fields_}
+
+    def run(self, tune=None):
+        t = None
+        if tune is not None:
+            t = _lib.BatchTune(tune.get("block", 256), tune.get("wgs_per_cu", 4),
+                               tune.get("use_graph", 0))
+        ms = ctypes.c_double()
+        e = _err()
+        _lib.check(self._lib.scaph_batch_run(self._h, ctypes.byref(t) if t is not None else None,
+                                             ctypes.byref(ms), e, 1024), e.value.decode())
+        return ms.value
+
+    def levels(self, g):
+        out = np.empty(self.graph(g)["n"], np.uint32)
+        _lib.check(self._lib.scaph_batch_levels(self._h, g, _ptr(out)))
+        return out
+
+    def csr(self, g):
+        n = self.graph(g)["n"]
+        off = np.empty(n + 1, np.uint64)
+        m = ctypes.c_uint64()
+        _lib.check(self._lib.scaph_batch_csr(self._h, g, _ptr(off), None, 0, ctypes.byref(m)))
+        ngh = np.empty(max(m.value, 1), np.uint32)
+        _lib.check(self._lib.scaph_batch_csr(self._h, g, _ptr(off), _ptr(ngh), len(ngh),
+                                             ctypes.byref(m)))
+        return off, ngh[:m.value]
+
+    def autotune(self):
+        best = _lib.BatchTune()
+        bms = ctypes.c_double()
+        logt = (_lib.BatchTune * 64)()
+        logm = (ctypes.c_double * 64)()
+        cnt = ctypes.c_uint32()
+        e = _err()
+        _lib.check(self._lib.scaph_batch_autotune(self._h, ctypes.byref(best), ctypes.byref(bms),
+                                                  logt, logm, 64, ctypes.byref(cnt), e, 1024),
+                   e.value.decode())
+        as_d = lambda t: {k: getattr(t, k) for k, _ in _lib.BatchTune._fields_}  # noqa: E731
+        return as_d(best), bms.value, [(as_d(logt[i]), logm[i]) for i in range(min(cnt.value, 64))]
 
 
 def generate_rmat(scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1):

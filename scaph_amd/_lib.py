@@ -59,6 +59,35 @@ class ExploreRecord(ctypes.Structure):
                 ("x", ctypes.c_float * 16)]
 
 
+class MsTune(ctypes.Structure):
+    _fields_ = [("push_div", ctypes.c_uint32), ("push_mdiv", ctypes.c_uint32),
+                ("pull_block", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
+                ("segment", ctypes.c_uint32)]
+
+
+class MsStats(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double), ("depth", ctypes.c_uint32), ("queries", ctypes.c_uint32),
+                ("pairs", ctypes.c_uint64)]
+
+
+class MsCounters(ctypes.Structure):
+    _fields_ = [("n_active", ctypes.c_uint64), ("bits", ctypes.c_uint64),
+                ("m_active", ctypes.c_uint64), ("q_len", ctypes.c_uint64),
+                ("heavy_len", ctypes.c_uint64), ("overflow", ctypes.c_uint64),
+                ("pad", ctypes.c_uint64 * 2)]
+
+
+class BatchEntry(ctypes.Structure):
+    _fields_ = [("vbase", ctypes.c_uint64), ("ebase", ctypes.c_uint64), ("seed", ctypes.c_uint64),
+                ("n", ctypes.c_uint32), ("scale", ctypes.c_uint32), ("src", ctypes.c_uint32),
+                ("edge_factor", ctypes.c_uint32)]
+
+
+class BatchTune(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
+                ("use_graph", ctypes.c_uint32)]
+
+
 class BfsStats(ctypes.Structure):
     _fields_ = [("src", ctypes.c_uint32), ("depth", ctypes.c_uint32), ("ms", ctypes.c_double),
                 ("n_reached", ctypes.c_uint64), ("m_reached", ctypes.c_uint64),
@@ -185,11 +214,32 @@ def host():
         _sig(lib, "scaph_engine_explore", I32,
              [E, U32, I32, ctypes.POINTER(ExploreRecord), U32, ctypes.POINTER(U32), CHARP, SZ])
         _sig(lib, "scaph_engine_validate", I32, [E, U32, P])
+        _sig(lib, "scaph_engine_msbfs", I32,
+             [E, P, U32, ctypes.POINTER(MsTune), ctypes.POINTER(MsStats), CHARP, SZ])
+        _sig(lib, "scaph_engine_msbfs_levels", I32, [E, U32, P, CHARP, SZ])
+        _sig(lib, "scaph_engine_msbfs_counters", I32,
+             [E, ctypes.POINTER(MsCounters), U32, ctypes.POINTER(U32)])
+        _sig(lib, "scaph_engine_msbfs_autotune", I32,
+             [E, P, U32, ctypes.POINTER(MsTune), ctypes.POINTER(DBL), ctypes.POINTER(MsTune),
+              ctypes.POINTER(DBL), U32, ctypes.POINTER(U32), CHARP, SZ])
         _sig(lib, "scaph_engine_out_degrees", I32, [E, P])
         _sig(lib, "scaph_engine_host_csr", I32, [E, P, P])
         _sig(lib, "scaph_engine_check", I32, [E, U32, ctypes.POINTER(U64), CHARP, SZ])
         _sig(lib, "scaph_engine_graph_desc", I32, [E, I32, ctypes.POINTER(GraphDesc)])
         _sig(lib, "scaph_engine_device_degree", I32, [E, ctypes.POINTER(P)])
+        _sig(lib, "scaph_batch_create_rmat", I32,
+             [U32, U32, U32, U32, U32, U64, I32, ctypes.POINTER(P), CHARP, SZ])
+        _sig(lib, "scaph_batch_destroy", None, [P])
+        _sig(lib, "scaph_batch_info", I32,
+             [P, ctypes.POINTER(U32), ctypes.POINTER(U64), ctypes.POINTER(U64)])
+        _sig(lib, "scaph_batch_graph_info", I32, [P, U32, ctypes.POINTER(BatchEntry)])
+        _sig(lib, "scaph_batch_run", I32,
+             [P, ctypes.POINTER(BatchTune), ctypes.POINTER(DBL), CHARP, SZ])
+        _sig(lib, "scaph_batch_levels", I32, [P, U32, P])
+        _sig(lib, "scaph_batch_csr", I32, [P, U32, P, P, U64, ctypes.POINTER(U64)])
+        _sig(lib, "scaph_batch_autotune", I32,
+             [P, ctypes.POINTER(BatchTune), ctypes.POINTER(DBL), ctypes.POINTER(BatchTune),
+              ctypes.POINTER(DBL), U32, ctypes.POINTER(U32), CHARP, SZ])
         _sig(lib, "scaph_host_page_cuts", I32, [P, U32, U64, I32, P, U32, ctypes.POINTER(U32)])
         _sig(lib, "scaph_host_gen_rmat", I32,
              [U32, U32, DBL, DBL, DBL, U64, P, P, U64, ctypes.POINTER(U64)])

@@ -116,6 +116,7 @@ Engine::Engine(const Config& cfg)
 Engine::~Engine() = default;
 
 void Engine::reset_graph() {
+  ++graph_gen_;
   n_ = 0;
   m_ = 0;
   host_ = HostCsr();

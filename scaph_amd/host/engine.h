@@ -156,6 +156,8 @@ class Engine {
   const uint32_t* d_hub_list() const { return hub_list_.get<uint32_t>(); }
   uint32_t num_hubs() const { return num_hubs_; }
   void* stream() const { return main_->get(); }
+  // Changes whenever a graph is (re)loaded; lets companions drop state.
+  uint64_t graph_generation() const { return graph_gen_; }
 
  private:
   struct PagedState;
@@ -208,6 +210,7 @@ class Engine {
   SchedFn sched_;
   std::unique_ptr<PagedState> paged_;
   DevBuf snap_level_, snap_visited_, snap_queue_, snap_front_;
+  uint64_t graph_gen_ = 0;
 };
 
 // CPU BFS used by bfs::check() (app/bfs.cu:67-117): FIFO level BFS over the

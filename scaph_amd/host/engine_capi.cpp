@@ -7,6 +7,8 @@
 #include <vector>
 
 #include "engine.h"
+#include "msbfs.h"
+#include "batch.h"
 #include "learned.h"
 
 struct scaph_engine {
@@ -14,7 +16,19 @@ struct scaph_engine {
   scaph::Engine* eng = nullptr;
   scaph::RunStats last;
   scaph::LearnedSchedule learned;
-  ~scaph_engine() { delete eng; }
+  std::unique_ptr<scaph::MultiSource> ms;
+  scaph::MsTune ms_tune;
+  ~scaph_engine() {
+    ms.reset();
+    delete eng;
+  }
+  scaph::MultiSource& multi() {
+    if (!ms || ms->stale()) {
+      ms.reset();
+      ms.reset(new scaph::MultiSource(*eng, 32));
+    }
+    return *ms;
+  }
 };
 
 namespace {
@@ -235,6 +249,84 @@ int scaph_engine_explore(scaph_engine* e, uint32_t src, int reps, scaph_explore_
   });
 }
 
+namespace {
+scaph::MsTune from_c(const scaph_ms_tune& t) {
+  scaph::MsTune o;
+  o.push_div = t.push_div;
+  o.push_mdiv = t.push_mdiv;
+  o.pull_block = t.pull_block;
+  o.use_graph = t.use_graph;
+  o.segment = t.segment ? t.segment : 8;
+  return o;
+}
+scaph_ms_tune to_c(const scaph::MsTune& t) {
+  scaph_ms_tune o;
+  o.push_div = t.push_div;
+  o.push_mdiv = t.push_mdiv;
+  o.pull_block = t.pull_block;
+  o.use_graph = t.use_graph;
+  o.segment = t.segment;
+  return o;
+}
+}  // namespace
+
+int scaph_engine_msbfs(scaph_engine* e, const uint32_t* sources, uint32_t count,
+                       const scaph_ms_tune* tune, scaph_ms_stats* out, char* err, size_t errlen) {
+  if (!e || !sources) return 1;
+  return guarded(err, errlen, [&] {
+    const scaph::MsTune t = tune ? from_c(*tune) : e->ms_tune;
+    const scaph::MsStats s = e->multi().run(sources, count, t);
+    if (out) {
+      out->ms = s.ms;
+      out->depth = s.depth;
+      out->queries = s.queries;
+      out->pairs = s.pairs;
+    }
+  });
+}
+
+int scaph_engine_msbfs_levels(scaph_engine* e, uint32_t query, uint32_t* host_out, char* err,
+                              size_t errlen) {
+  if (!e || !host_out) return 1;
+  return guarded(err, errlen, [&] {
+    if (!e->ms || e->ms->stale()) throw scaph::Error(SCAPH_STATUS_INVALID, "no multi-source run");
+    e->ms->levels(query, host_out);
+  });
+}
+
+int scaph_engine_msbfs_counters(scaph_engine* e, scaph_ms_counters* out, uint32_t max,
+                                uint32_t* count) {
+  if (!e) return 1;
+  return guarded(nullptr, 0, [&] {
+    if (!e->ms || e->ms->stale()) throw scaph::Error(SCAPH_STATUS_INVALID, "no multi-source run");
+    const auto v = e->ms->counters();
+    for (uint32_t i = 0; i < v.size() && i < max && out; ++i) out[i] = v[i];
+    if (count) *count = (uint32_t)v.size();
+  });
+}
+
+int scaph_engine_msbfs_autotune(scaph_engine* e, const uint32_t* sources, uint32_t count,
+                                scaph_ms_tune* best, double* best_ms, scaph_ms_tune* log_tunes,
+                                double* log_ms, uint32_t log_max, uint32_t* log_count,
+                                char* err, size_t errlen) {
+  if (!e || !sources) return 1;
+  return guarded(err, errlen, [&] {
+    std::vector<std::pair<scaph::MsTune, double>> log;
+    const scaph::MsTune t = e->multi().autotune(sources, count, &log);
+    e->ms_tune = t;
+    double bm = 0;
+    for (const auto& x : log)
+      if (x.first == t) bm = x.second;
+    if (best) *best = to_c(t);
+    if (best_ms) *best_ms = bm;
+    for (uint32_t i = 0; i < log.size() && i < log_max; ++i) {
+      if (log_tunes) log_tunes[i] = to_c(log[i].first);
+      if (log_ms) log_ms[i] = log[i].second;
+    }
+    if (log_count) *log_count = (uint32_t)log.size();
+  });
+}
+
 int scaph_engine_levels(scaph_engine* e, uint32_t* host_out) {
   if (!e || !host_out) return 1;
   return guarded(nullptr, 0, [&] { e->eng->levels_to_host(host_out); });
@@ -373,6 +465,88 @@ int scaph_host_cpu_bfs(uint32_t n, const uint64_t* offsets, const uint32_t* ngh,
   scaph::cpu_bfs(g, src, inf, &out);
   std::memcpy(levels, out.data(), (size_t)n * 4);
   return 0;
+}
+
+struct scaph_batch {
+  std::unique_ptr<scaph::Batch> b;
+  bool tuned = false;
+};
+
+int scaph_batch_create_rmat(uint32_t count, uint32_t min_scale, uint32_t max_scale,
+                            uint32_t min_ef, uint32_t max_ef, uint64_t seed, int device,
+                            scaph_batch** out, char* err, size_t errlen) {
+  if (!out) return 1;
+  *out = nullptr;
+  return guarded(err, errlen, [&] {
+    std::unique_ptr<scaph_batch> h(new scaph_batch());
+    h->b.reset(new scaph::Batch(count, min_scale, max_scale, min_ef, max_ef, seed, device));
+    *out = h.release();
+  });
+}
+
+void scaph_batch_destroy(scaph_batch* b) { delete b; }
+
+int scaph_batch_info(scaph_batch* b, uint32_t* count, unsigned long long* vertices,
+                     unsigned long long* edges) {
+  if (!b) return 1;
+  if (count) *count = b->b->count();
+  if (vertices) *vertices = b->b->vertices();
+  if (edges) *edges = b->b->edges();
+  return 0;
+}
+
+int scaph_batch_graph_info(scaph_batch* b, uint32_t g, scaph_batch_entry* out) {
+  if (!b || !out || g >= b->b->count()) return 1;
+  *out = b->b->graph(g);
+  return 0;
+}
+
+int scaph_batch_run(scaph_batch* b, const scaph_batch_tune* tune, double* ms, char* err,
+                    size_t errlen) {
+  if (!b) return 1;
+  return guarded(err, errlen, [&] {
+    scaph::BatchTune t = b->b->tuned();
+    if (tune) {
+      t.block = tune->block;
+      t.wgs_per_cu = tune->wgs_per_cu;
+      t.use_graph = tune->use_graph;
+    }
+    const double v = b->b->run(t);
+    if (ms) *ms = v;
+  });
+}
+
+int scaph_batch_levels(scaph_batch* b, uint32_t g, uint32_t* host_out) {
+  if (!b || !host_out || g >= b->b->count()) return 1;
+  return guarded(nullptr, 0, [&] { b->b->levels(g, host_out); });
+}
+
+int scaph_batch_csr(scaph_batch* b, uint32_t g, uint64_t* offsets, uint32_t* ngh,
+                    unsigned long long cap, unsigned long long* num_edges) {
+  if (!b || g >= b->b->count()) return 1;
+  return guarded(nullptr, 0, [&] {
+    uint64_t m = 0;
+    b->b->csr(g, offsets, ngh, cap, &m);
+    if (num_edges) *num_edges = m;
+  });
+}
+
+int scaph_batch_autotune(scaph_batch* b, scaph_batch_tune* best, double* best_ms,
+                         scaph_batch_tune* log_tunes, double* log_ms, uint32_t log_max,
+                         uint32_t* log_count, char* err, size_t errlen) {
+  if (!b) return 1;
+  return guarded(err, errlen, [&] {
+    std::vector<std::pair<scaph::BatchTune, double>> log;
+    const scaph::BatchTune t = b->b->autotune(&log);
+    for (uint32_t i = 0; i < log.size() && i < log_max; ++i) {
+      if (log_tunes) log_tunes[i] = {log[i].first.block, log[i].first.wgs_per_cu,
+                                     log[i].first.use_graph};
+      if (log_ms) log_ms[i] = log[i].second;
+      if (best_ms && log[i].first == t) *best_ms = log[i].second;
+    }
+    if (best) *best = {t.block, t.wgs_per_cu, t.use_graph};
+    if (log_count) *log_count = (uint32_t)log.size();
+  });
 }
 
 }  // extern "C"
