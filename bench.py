@@ -90,6 +90,70 @@ def cpu_baseline(eng, src, gpu_levels, budget_s):
             "seconds": dt}, parity
 
 
+def bench_msbfs(scale, ef, queries, seed):
+    """Multi-source bitset BFS on R-MAT scale/ef: `queries` sources from the
+    giant component in one traversal, autotuned, against the same queries
+    run one at a time by the single-source path."""
+    import scaph_amd
+    eng = scaph_amd.BFS("-max_memory=262144", "-pagesize=10")
+    eng.gen_rmat_device(scale, ef, 0.57, 0.19, 0.19, seed=seed)
+    info = eng.info()
+    srcs, _ = pick_sources(eng, info["n"], queries, seed=seed, min_reach_frac=0.01)
+    src = np.array(srcs, np.uint32)
+    best, best_ms, log_ = eng.msbfs_autotune(src)
+    times = [eng.msbfs(src)["ms"] for _ in range(5)]
+    st = eng.msbfs(src)
+    single = []
+    for s_ in srcs:
+        single.append(eng.run(int(s_))["ms"])
+    # spot parity: a few queries against the single-source levels
+    exact = True
+    for q in range(0, len(srcs), max(1, len(srcs) // 8)):
+        eng.run(int(srcs[q]))
+        exact &= bool(np.array_equal(eng.msbfs_levels(q), eng.levels()))
+    ms = statistics.median(times)
+    out = {"workload": f"multi-source BFS, R-MAT scale {scale} edge factor {ef} seed {seed}, "
+                       f"{len(srcs)} sources from the giant component",
+           "queries": len(srcs), "ms": ms, "queries_per_s": len(srcs) / (ms / 1e3),
+           "depth": st["depth"], "pairs_reached": st["pairs"],
+           "single_source_total_ms": sum(single),
+           "speedup_vs_single_source": sum(single) / ms,
+           "tune": best, "autotune_candidates": len(log_),
+           "levels_equal_single_source": exact}
+    eng.close()
+    return out
+
+
+def bench_small_batch(count, seed):
+    """4096 R-MAT graphs (scale 10-16, edge factor 8-32), one query each, in
+    one persistent launch; a sample checked against the oracle."""
+    import scaph_amd
+    b = scaph_amd.Batch(count=count, min_scale=10, max_scale=16, min_ef=8, max_ef=32, seed=seed)
+    info = b.info()
+    best, best_ms, log_ = b.autotune()
+    times = [b.run() for _ in range(5)]
+    ms = statistics.median(times)
+    exact = None
+    try:
+        sys.path.insert(0, ROOT)
+        from tests import _oracle as O
+        exact = True
+        for g in range(0, count, max(1, count // 16)):
+            off, ngh = b.csr(g)
+            exact &= bool(np.array_equal(b.levels(g), O.bfs(off, ngh, b.graph(g)["src"])))
+    except ImportError:
+        pass
+    out = {"workload": f"{count} R-MAT graphs, scale 10-16, edge factor 8-32, seed {seed}, "
+                       "one source each, one persistent launch",
+           "graphs": info["graphs"], "vertices": info["vertices"], "edges": info["edges"],
+           "ms": ms, "queries_per_s": count / (ms / 1e3),
+           "gteps": info["edges"] / (ms / 1e3) / 1e9,
+           "tune": best, "autotune_candidates": len(log_),
+           "levels_exact_vs_oracle_sample": exact}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +165,8 @@ def main():
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--schedule", default="fixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", dest="batched", action="store_false",
+                    help="skip the batched BFS measurements")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip the learned-schedule pass")
     ap.add_argument("--learned-check", type=int, default=8,
@@ -247,6 +313,17 @@ def main():
         cpu, parity = cpu_baseline(eng, src0, gl, args.cpu_budget)
         log("cpu baseline", cpu)
 
+    batched = None
+    if args.batched:
+        eng.close()
+        batched = {}
+        log("batched: multi-source")
+        batched["multi_source"] = bench_msbfs(24, 16, 256, seed=3)
+        log("batched:", batched["multi_source"])
+        log("batched: small graphs")
+        batched["small_graphs"] = bench_small_batch(4096, seed=4)
+        log("batched:", batched["small_graphs"])
+
     med = statistics.median(gteps)
     out = {
         "metric": "BFS GTEPS, R-MAT s26 ef32, learned vs fixed-alpha schedule; % HBM peak",
@@ -268,6 +345,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned": learned,
+        "batched": batched,
         "bfs_hbm_frac_median": statistics.median(frac),
         "t_bfs_ms_median": statistics.median(r["ms"] for r in runs),
         "gteps_ref_formula_median": statistics.median(ref_gteps),
