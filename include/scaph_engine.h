@@ -63,6 +63,13 @@ int scaph_engine_gen_rmat_device(scaph_engine* e, uint32_t scale, uint32_t edge_
 /* Build an Erdos-Renyi G(n, m) graph directly in device memory. */
 int scaph_engine_gen_er_device(scaph_engine* e, uint32_t n, uint64_t m, uint64_t seed,
                                int keep_host, char* err, size_t errlen);
+/* Power-law digraph: out-degree of v = min(cap, floor(x_min * (1 -
+ * u_v)^(-1/(alpha - 1)))) with u_v from SplitMix64 word v of seed (53 bits),
+ * uniform destinations, self-loops and duplicates removed.  Built on the
+ * device, then loaded like a host CSR: beyond -max_memory it runs out of
+ * core.  No reference counterpart (the reference's graphs come from files). */
+int scaph_engine_gen_zipf(scaph_engine* e, uint32_t n, double x_min, double alpha, uint32_t cap,
+                          uint64_t seed, char* err, size_t errlen);
 int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_pages,
                       int* resident, int* has_pull);
 /* run(): one BFS from src.  schedule: NULL keeps the -schedule flag, else

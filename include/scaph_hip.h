@@ -213,6 +213,15 @@ int scaph_gen_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32
 int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t n,
                       uint64_t seed, void* stream);
 
+/* offsets[0..n] = exclusive prefix of degree[0..n) (offsets[n] = total);
+ * allocates its own scan storage and returns after the stream drains. */
+int scaph_degrees_to_offsets(const uint32_t* degree, uint64_t* offsets, uint32_t n, void* stream);
+/* Power-law digraph edges [e_begin, e_end) as keys (u << 32 | v): the
+ * source is the vertex whose range [offsets[u], offsets[u+1]) holds e (the
+ * out-degrees come from the caller), the destination floor(w * n / 2^64)
+ * for SplitMix64 word e of (seed ^ "zipf-dst").  No reference counterpart. */
+int scaph_gen_zipf_keys(uint64_t* keys, const uint64_t* offsets, uint32_t n, uint64_t e_begin,
+                        uint64_t e_end, uint64_t seed, void* stream);
 /* Sorts keys, removes duplicates and the self-loop key, and writes a flat
  * CSR: offsets u64[n+1], neighbours u32[*num_edges], out-degree u32[n].
  * transpose != 0 builds in-edges (the key halves swapped); then, when

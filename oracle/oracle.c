@@ -33,6 +33,14 @@
  *                         Pinned by quadrant frequencies, bijectivity of the
  *                         relabelling, and agreement with the product's host
  *                         and device generators.
+ *   oracle_zipf_degree,   the project's power-law digraph: out-degree
+ *   oracle_zipf_csr       min(cap, floor(x_min * (1 - u)^(-1/(alpha - 1))))
+ *                         with u from SplitMix64 word v (53 bits), uniform
+ *                         destinations (multiply-high of word e of the
+ *                         salted seed).  Pinned by the Pareto tail
+ *                         P(d >= k) = (x_min / k)^(alpha - 1), the capped
+ *                         mean in closed form, and agreement with the
+ *                         product's host and device generators.
  *   oracle_mlp_*          fp32 restatement of the 16-64-64-2 MLP; pinned
  *                         by finite differences and a numpy matmul form.
  */
@@ -428,6 +436,38 @@ ORACLE_API int oracle_er_csr(uint32_t n, uint64_t m, uint64_t seed, uint64_t* of
     keys[e] = ((uint64_t)u << 32) | v;
   }
   int rc = keys_to_csr(keys, m, n, offsets, ngh, cap, num_edges);
+  free(keys);
+  return rc;
+}
+
+/* Power-law out-degrees (continuous Pareto with density ~ x^-alpha for
+ * x >= x_min, floored and capped) and uniform destinations. */
+#define ZIPF_DST_SALT 0x7A6970662D647374ull /* "zipf-dst" */
+
+ORACLE_API uint32_t oracle_zipf_degree(uint64_t seed, uint32_t v, double x_min, double alpha,
+                                       uint32_t cap) {
+  const double u = (double)(oracle_splitmix64(seed, v) >> 11) * (1.0 / 9007199254740992.0);
+  const double x = x_min * pow(1.0 - u, -1.0 / (alpha - 1.0));
+  return x >= (double)cap ? cap : (uint32_t)x;
+}
+
+ORACLE_API int oracle_zipf_csr(uint32_t n, double x_min, double alpha, uint32_t cap, uint64_t seed,
+                               uint64_t* offsets, uint32_t* ngh, uint64_t ngh_cap,
+                               uint64_t* num_edges) {
+  uint64_t m = 0;
+  for (uint32_t v = 0; v < n; ++v) m += oracle_zipf_degree(seed, v, x_min, alpha, cap);
+  uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * (m ? m : 1));
+  if (!keys) return 2;
+  uint64_t e = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint32_t d = oracle_zipf_degree(seed, v, x_min, alpha, cap);
+    for (uint32_t j = 0; j < d; ++j, ++e) {
+      const uint64_t w = oracle_splitmix64(seed ^ ZIPF_DST_SALT, e);
+      const uint32_t t = (uint32_t)(((unsigned __int128)w * n) >> 64);
+      keys[e] = ((uint64_t)v << 32) | t;
+    }
+  }
+  int rc = keys_to_csr(keys, m, n, offsets, ngh, ngh_cap, num_edges);
   free(keys);
   return rc;
 }

@@ -81,6 +81,12 @@ class BFS:
         _lib.check(self._lib.scaph_engine_gen_er_device(self._h, n, m, seed, int(keep_host), e,
                                                        1024), e.value.decode())
 
+    def gen_zipf(self, n, x_min=1.2, alpha=1.9, cap=1000000, seed=3):
+        """Power-law digraph (Pareto out-degrees, uniform destinations)."""
+        e = _err()
+        _lib.check(self._lib.scaph_engine_gen_zipf(self._h, n, x_min, alpha, cap, seed, e, 1024),
+                   e.value.decode())
+
     def info(self):
         n = ctypes.c_uint32()
         m = ctypes.c_uint64()

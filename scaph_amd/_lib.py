@@ -202,6 +202,7 @@ def host():
         _sig(lib, "scaph_engine_gen_rmat_device", I32,
              [E, U32, U32, DBL, DBL, DBL, U64, I32, CHARP, SZ])
         _sig(lib, "scaph_engine_gen_er_device", I32, [E, U32, U64, U64, I32, CHARP, SZ])
+        _sig(lib, "scaph_engine_gen_zipf", I32, [E, U32, DBL, DBL, U32, U64, CHARP, SZ])
         _sig(lib, "scaph_engine_info", I32,
              [E, ctypes.POINTER(U32), ctypes.POINTER(U64), ctypes.POINTER(U32),
               ctypes.POINTER(I32), ctypes.POINTER(I32)])

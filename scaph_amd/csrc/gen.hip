@@ -51,6 +51,33 @@ k_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t count, uint32_t n, uint64_t
   }
 }
 
+// Power-law digraph edges: edge e belongs to the last vertex v with
+// offsets[v] <= e (offsets = exclusive prefix of the out-degrees, n + 1
+// entries); its destination is the multiply-high of SplitMix64 word e of the
+// salted seed (the host and oracle generators use the same stream).
+constexpr uint64_t kZipfDstSalt = 0x7A6970662D647374ull;  // "zipf-dst"
+
+__global__ void __launch_bounds__(kBlock)
+k_zipf_keys(uint64_t* keys, const uint64_t* offsets, uint32_t n, uint64_t e_begin, uint64_t count,
+            uint64_t seed) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint64_t e = e_begin + i;
+    uint32_t lo = 0, hi = n;  // offsets[lo] <= e < offsets[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (offsets[mid] <= e) lo = mid; else hi = mid;
+    }
+    const uint64_t w = scaph_spec::word(seed ^ kZipfDstSalt, e);
+    const uint32_t t = (uint32_t)__umul64hi(w, (uint64_t)n);
+    keys[i] = ((uint64_t)lo << 32) | t;
+  }
+}
+
+__global__ void k_offsets_tail(const uint32_t* deg, uint64_t* offsets, uint32_t n) {
+  offsets[n] = offsets[n - 1] + deg[n - 1];
+}
+
 // (u<<32|v) -> packed sort key.  Out-edges: (u << bits) | v.  In-edges:
 // (v << (bits+1)) | (flag << bits) | low where a hub source u (hub_idx[u]
 // != ~0) gets flag 0 and low = its hub index, any other source flag 1 and
@@ -172,6 +199,37 @@ int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t
   const uint64_t count = e_end - e_begin;
   hipLaunchKernelGGL(k_er_keys, dim3(grid_for(count, kBlock * 8, 65536)), dim3(kBlock), 0,
                      as_stream(stream), keys, e_begin, count, n, seed);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_degrees_to_offsets(const uint32_t* degree, uint64_t* offsets, uint32_t n,
+                             void* stream) {
+  if (!degree || !offsets || n == 0) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  size_t bytes = 0;
+  SCAPH_TRY(rocprim::exclusive_scan(nullptr, bytes, degree, offsets, (uint64_t)0, (size_t)n,
+                                    rocprim::plus<uint64_t>(), st));
+  void* tmp = nullptr;
+  SCAPH_TRY(hipMalloc(&tmp, bytes + 16));
+  int rc = (int)rocprim::exclusive_scan(tmp, bytes, degree, offsets, (uint64_t)0, (size_t)n,
+                                        rocprim::plus<uint64_t>(), st);
+  if (!rc) {
+    hipLaunchKernelGGL(k_offsets_tail, dim3(1), dim3(1), 0, st, degree, offsets, n);
+    rc = (int)hipGetLastError();
+  }
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  hipFree(tmp);
+  return rc;
+}
+
+int scaph_gen_zipf_keys(uint64_t* keys, const uint64_t* offsets, uint32_t n, uint64_t e_begin,
+                        uint64_t e_end, uint64_t seed, void* stream) {
+  if (!keys || !offsets || n == 0 || e_end < e_begin) return SCAPH_STATUS_INVALID;
+  if (e_end == e_begin) return 0;
+  const uint64_t count = e_end - e_begin;
+  hipLaunchKernelGGL(k_zipf_keys, dim3(grid_for(count, kBlock * 8, 65536)), dim3(kBlock), 0,
+                     as_stream(stream), keys, offsets, n, e_begin, count, seed);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

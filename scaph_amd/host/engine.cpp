@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <deque>
+#include <thread>
 
 #include "engine_internal.h"
 #include "flags.h"
@@ -357,6 +358,46 @@ void Engine::gen_er_device(uint32_t n, uint64_t m, uint64_t seed, bool keep_host
   build_device(n, m, keep_host_csr, [&](uint64_t* keys, void* st) {
     H_ERR(scaph_gen_er_keys(keys, 0, m, n, seed, st));
   });
+}
+
+void Engine::gen_zipf(uint32_t n, double x_min, double alpha, uint32_t cap, uint64_t seed) {
+  if (n == 0 || !(alpha > 1.0) || !(x_min > 0.0) || cap == 0)
+    throw Error(SCAPH_STATUS_INVALID, "zipf needs n > 0, alpha > 1, x_min > 0, cap > 0");
+  std::vector<uint32_t> deg(n);
+  {
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        const uint64_t lo = (uint64_t)n * t / nt, hi = (uint64_t)n * (t + 1) / nt;
+        for (uint64_t v = lo; v < hi; ++v)
+          deg[v] = zipf_degree(seed, (uint32_t)v, x_min, alpha, cap);
+      });
+    for (auto& x : th) x.join();
+  }
+  uint64_t raw = 0;
+  for (uint32_t d : deg) raw += d;
+  void* st = main_->get();
+  HostCsr g;
+  g.n = n;
+  g.offsets.resize((size_t)n + 1);
+  {
+    DevBuf d_deg((size_t)n * 4), offs(((size_t)n + 1) * 8), keys(std::max<uint64_t>(raw, 1) * 8),
+        tmp(std::max<uint64_t>(raw, 1) * 8), ngh(std::max<uint64_t>(raw, 1) * 4),
+        deg2((size_t)n * 4);
+    H_ERR(scaph_copy_h2d_async(d_deg.get(), deg.data(), (size_t)n * 4, st));
+    H_ERR(scaph_degrees_to_offsets(d_deg.get<uint32_t>(), offs.get<uint64_t>(), n, st));
+    H_ERR(scaph_gen_zipf_keys(keys.get<uint64_t>(), offs.get<uint64_t>(), n, 0, raw, seed, st));
+    uint64_t E = 0;
+    H_ERR(scaph_csr_from_keys(keys.get<uint64_t>(), tmp.get<uint64_t>(), raw, n, 0, nullptr,
+                              offs.get<uint64_t>(), ngh.get<uint32_t>(), deg2.get<uint32_t>(), &E,
+                              st));
+    g.ngh.resize(E);
+    H_ERR(scaph_copy_d2h_async(g.offsets.data(), offs.get(), g.offsets.size() * 8, st));
+    if (E) H_ERR(scaph_copy_d2h_async(g.ngh.data(), ngh.get(), E * 4, st));
+    main_->sync();
+  }
+  load_csr(std::move(g), true);
 }
 
 void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,

@@ -106,6 +106,10 @@ class Engine {
   void load_csr(HostCsr g, bool keep_host = true);
   void gen_rmat_device(const RmatParams& p, bool keep_host_csr);
   void gen_er_device(uint32_t n, uint64_t m, uint64_t seed, bool keep_host_csr);
+  // Power-law digraph (zipf_degree out-degrees, uniform destinations):
+  // edges built and sorted on the device, then loaded like a host CSR, so
+  // a graph beyond the -max_memory budget runs out of core.
+  void gen_zipf(uint32_t n, double x_min, double alpha, uint32_t cap, uint64_t seed);
   void load_from_config();  // graphfile, else gen spec
 
   // One BFS from src.  Levels stay on the device (levels_to_host to read).

@@ -146,6 +146,12 @@ int scaph_engine_gen_er_device(scaph_engine* e, uint32_t n, uint64_t m, uint64_t
   return guarded(err, errlen, [&] { e->eng->gen_er_device(n, m, seed, keep_host != 0); });
 }
 
+int scaph_engine_gen_zipf(scaph_engine* e, uint32_t n, double x_min, double alpha, uint32_t cap,
+                          uint64_t seed, char* err, size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] { e->eng->gen_zipf(n, x_min, alpha, cap, seed); });
+}
+
 int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_pages, int* resident,
                       int* has_pull) {
   if (!e) return 1;

@@ -67,6 +67,12 @@ uint32_t to_thr(double x) {
 
 }  // namespace
 
+uint32_t zipf_degree(uint64_t seed, uint32_t v, double x_min, double alpha, uint32_t cap) {
+  const double u = (double)(scaph_spec::word(seed, v) >> 11) * (1.0 / 9007199254740992.0);
+  const double x = x_min * std::pow(1.0 - u, -1.0 / (alpha - 1.0));
+  return x >= (double)cap ? cap : (uint32_t)x;
+}
+
 void rmat_thresholds(const RmatParams& p, uint32_t* thr_a, uint32_t* thr_ab, uint32_t* thr_abc) {
   *thr_a = to_thr(p.a);
   *thr_ab = to_thr(p.a + p.b);

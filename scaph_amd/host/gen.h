@@ -20,6 +20,11 @@ struct RmatParams {
   uint64_t seed = 1;
 };
 
+// Power-law out-degree of vertex v: min(cap, floor(x_min * (1 - u)^(-1 /
+// (alpha - 1)))) with u the top 53 bits of SplitMix64 word v of seed over
+// 2^53 (a floored, capped continuous Pareto with density ~ x^-alpha).
+uint32_t zipf_degree(uint64_t seed, uint32_t v, double x_min, double alpha, uint32_t cap);
+
 // u32 quadrant thresholds floor(x * 2^32) (clamped to 2^32 - 1).
 void rmat_thresholds(const RmatParams& p, uint32_t* thr_a, uint32_t* thr_ab, uint32_t* thr_abc);
 

@@ -1,0 +1,119 @@
+"""Out-of-core BFS measurement on the power-law digraph: 105M vertices,
+~3.7B edges (out-degree Pareto with density ~ x^-1.9, x_min 1.2, capped at
+10^6; uniform destinations; seed 3), device page pool capped by
+-max_memory, pages streamed from pinned host memory.
+
+    python -m scaph_amd.outofcore_bench [--budgets 15258,8192] [--sources 4]
+
+Prints one JSON line: per budget the median GTEPS, page hit rate
+(executions of already-resident pages / all page executions), host-to-device
+bytes and PCIe rates, and parity of the levels with a resident (push-only)
+run of the same graph and, for one source, with the oracle BFS.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+from . import BFS
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def log(*a):
+    print("[c3]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=105_000_000)
+    ap.add_argument("--x-min", type=float, default=1.2)
+    ap.add_argument("--alpha", type=float, default=1.9)
+    ap.add_argument("--cap", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--budgets", default="15258,8192",
+                    help="-max_memory values in MiB (15258 MiB = 16e9 bytes)")
+    ap.add_argument("--sources", type=int, default=4)
+    ap.add_argument("--oracle", action="store_true", help="also run the CPU oracle BFS once")
+    args = ap.parse_args()
+
+    # resident reference run first: builds the graph, picks the sources
+    t0 = time.perf_counter()
+    res = BFS("-max_memory=262144", "-build_pull=false")
+    res.gen_zipf(args.n, args.x_min, args.alpha, args.cap, args.seed)
+    info = res.info()
+    log("graph", info, f"{time.perf_counter() - t0:.1f}s")
+    deg = res.out_degrees()
+    rng = np.random.default_rng(args.seed)
+    cand = np.nonzero(deg > 0)[0]
+    sources, want = [], {}
+    for v in rng.permutation(cand)[:50 * args.sources]:
+        st = res.run(int(v))
+        if st["n_reached"] >= 0.01 * info["n"]:
+            sources.append(int(v))
+            want[int(v)] = (res.levels(), st)
+            if len(sources) == args.sources:
+                break
+    log("sources", sources, {s: round(want[s][1]["ms"], 3) for s in sources})
+    oracle_ok = None
+    if args.oracle:
+        sys.path.insert(0, ROOT)
+        from tests import _oracle as O
+        off, ngh = res.host_csr()
+        t1 = time.perf_counter()
+        oracle_ok = bool(np.array_equal(O.bfs(off, ngh, sources[0]), want[sources[0]][0]))
+        log("oracle BFS", oracle_ok, f"{time.perf_counter() - t1:.1f}s")
+        del off, ngh
+    res_ms = [want[s][1]["ms"] for s in sources]
+    res.close()
+
+    out = {"workload": f"power-law digraph n={args.n} x_min={args.x_min} alpha={args.alpha} "
+                       f"cap={args.cap} seed={args.seed}",
+           "vertices": info["n"], "edges": info["m"],
+           "resident_gteps_median": statistics.median(
+               want[s][1]["m_reached"] / (want[s][1]["ms"] / 1e3) / 1e9 for s in sources),
+           "resident_ms_median": statistics.median(res_ms),
+           "levels_resident_vs_oracle": oracle_ok, "budgets": []}
+    for b in [int(x) for x in args.budgets.split(",") if x]:
+        eng = BFS(f"-max_memory={b}", "-build_pull=false")
+        t0 = time.perf_counter()
+        eng.gen_zipf(args.n, args.x_min, args.alpha, args.cap, args.seed)
+        inf = eng.info()
+        log("budget", b, inf, f"{time.perf_counter() - t0:.1f}s")
+        runs, exact = [], True
+        eng.run(sources[0])  # warm: first touches of the pinned pages
+        for s in sources:
+            st = eng.run(s)
+            exact &= bool(np.array_equal(eng.levels(), want[s][0]))
+            runs.append(st)
+            log(" ", s, {k: st[k] for k in ("ms", "page_execs", "page_execs_cached",
+                                             "pages_full", "pages_compacted", "bytes_h2d",
+                                             "h2d_ms")})
+        ms = statistics.median(r["ms"] for r in runs)
+        execs = sum(r["page_execs"] for r in runs)
+        cached = sum(r["page_execs_cached"] for r in runs)
+        h2d = sum(r["bytes_h2d"] for r in runs)
+        h2d_ms = sum(r["h2d_ms"] for r in runs)
+        tot_ms = sum(r["ms"] for r in runs)
+        out["budgets"].append({
+            "max_memory_mib": b, "resident": inf["resident"], "pages": inf["pages"],
+            "gteps_median": statistics.median(r["m_reached"] / (r["ms"] / 1e3) / 1e9
+                                              for r in runs),
+            "t_bfs_ms_median": ms,
+            "page_hit_rate": cached / execs if execs else None,
+            "page_execs": execs, "pages_full_copies": sum(r["pages_full"] for r in runs),
+            "pages_compacted": sum(r["pages_compacted"] for r in runs),
+            "h2d_bytes_per_bfs": h2d / len(runs),
+            "pcie_gbps_over_bfs": h2d / (tot_ms / 1e3) / 1e9 if tot_ms else None,
+            "pcie_gbps_during_copies": h2d / (h2d_ms / 1e3) / 1e9 if h2d_ms else None,
+            "levels_equal_resident": exact})
+        eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

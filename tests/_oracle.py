@@ -52,6 +52,11 @@ def lib():
         L.oracle_rmat_csr.restype = I32
         L.oracle_er_csr.argtypes = [U32, U64, U64, P, P, U64, ctypes.POINTER(U64)]
         L.oracle_er_csr.restype = I32
+        D = ctypes.c_double
+        L.oracle_zipf_degree.argtypes = [U64, U32, D, D, U32]
+        L.oracle_zipf_degree.restype = U32
+        L.oracle_zipf_csr.argtypes = [U32, D, D, U32, U64, P, P, U64, ctypes.POINTER(U64)]
+        L.oracle_zipf_csr.restype = I32
         L.oracle_mlp_forward.argtypes = [P, P, U32, P, P, P]
         L.oracle_mlp_forward.restype = None
         L.oracle_mlp_loss_grad.argtypes = [P, P, P, U32, ctypes.c_float, P]
@@ -151,6 +156,22 @@ def er_csr(n, m, seed):
     rc = lib().oracle_er_csr(n, m, seed, _p(off), _p(ngh), m, ctypes.byref(E))
     if rc:
         raise ValueError(f"oracle_er_csr rc={rc}")
+    return off, ngh[:E.value].copy()
+
+
+def zipf_degrees(n, x_min, alpha, cap, seed):
+    f = lib().oracle_zipf_degree
+    return np.array([f(seed, v, x_min, alpha, cap) for v in range(n)], np.uint32)
+
+
+def zipf_csr(n, x_min, alpha, cap, seed):
+    m = int(zipf_degrees(n, x_min, alpha, cap, seed).astype(np.uint64).sum())
+    off = np.empty(n + 1, np.uint64)
+    ngh = np.empty(max(m, 1), np.uint32)
+    E = ctypes.c_uint64()
+    rc = lib().oracle_zipf_csr(n, x_min, alpha, cap, seed, _p(off), _p(ngh), m, ctypes.byref(E))
+    if rc:
+        raise ValueError(f"oracle_zipf_csr rc={rc}")
     return off, ngh[:E.value].copy()
 
 

@@ -373,3 +373,34 @@ def test_oracle_mlp_gradient_finite_differences():
         e[i] = 1e-4
         fd = (loss(p + e) - loss(p - e)) / 2e-4
         assert abs(fd - g[i]) <= 2e-3 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+# ------------------------------------------------------------ power law
+
+def test_oracle_zipf_degree_tail_and_mean():
+    """Pareto tail P(d >= k) = (x_min/k)^(alpha-1) for k below the cap, and
+    the capped mean E[min(floor(X), cap)] in closed form."""
+    n, x_min, alpha, cap = 200000, 1.2, 1.9, 1000
+    d = O.zipf_degrees(n, x_min, alpha, cap, seed=3).astype(np.float64)
+    a = alpha - 1
+    for k in (2, 5, 20, 100):
+        p = (d >= k).mean()
+        want = (x_min / k) ** a
+        assert abs(p - want) < 4 * np.sqrt(want * (1 - want) / n) + 1e-3, (k, p, want)
+    assert d.min() >= 1 and d.max() == cap
+    # E[min(floor X, C)] = sum_{k=1..C} P(X >= k) with P(X >= k) = min(1, (x_min/k)^a)
+    ks = np.arange(1, cap + 1, dtype=np.float64)
+    mean = np.minimum(1.0, (x_min / ks) ** a).sum()
+    assert abs(d.mean() - mean) < 0.03 * mean, (d.mean(), mean)
+
+
+def test_oracle_zipf_csr_is_the_degree_stream():
+    n = 3000
+    off, ngh = O.zipf_csr(n, 1.2, 1.9, 500, seed=5)
+    deg = O.zipf_degrees(n, 1.2, 1.9, 500, seed=5)
+    got = np.diff(off.astype(np.int64))
+    # dedup and self loops only ever remove edges
+    assert np.all(got <= deg) and got.sum() > 0.97 * deg.sum()
+    for v in range(0, n, 97):
+        row = ngh[off[v]:off[v + 1]]
+        assert np.all(np.diff(row.astype(np.int64)) > 0) and v not in row
