@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "learned.h"
 
 namespace scaph {
 
@@ -56,6 +57,15 @@ class App {
   // src/app.cuh:267-345
   void load_graph() {
     engine->load_from_config();
+    if (cfg.schedule == "learned") {
+      // -model names the weights file (scaph_amd/schedule_mlp.bin is the
+      // trained one); the decision itself runs on the device
+      if (cfg.model.empty())
+        throw Error(SCAPH_STATUS_INVALID, "-schedule=learned needs -model=<file>");
+      LearnedSchedule ls;
+      ls.load_file(cfg.model);
+      engine->set_device_model(ls.params(), ls.mean(), ls.scale());
+    }
     vert_count = engine->num_vertices();
     edge_count = (index_t)engine->num_edges();
     list_size = (int)engine->pages().size();
