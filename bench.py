@@ -165,6 +165,8 @@ def main():
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--schedule", default="fixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--flag", action="append", default=[],
+                    help="extra engine flag, e.g. --flag=-pull_summary=false")
     ap.add_argument("--no-batched", dest="batched", action="store_false",
                     help="skip the batched BFS measurements")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
@@ -185,7 +187,8 @@ def main():
 
     props = scaph_amd.device_props(0)
     log("device", props)
-    eng = scaph_amd.BFS("-max_memory=262144", "-pagesize=10", f"-schedule={args.schedule}")
+    eng = scaph_amd.BFS("-max_memory=262144", "-pagesize=10", f"-schedule={args.schedule}",
+                        *args.flag)
     t0 = time.perf_counter()
     eng.gen_rmat_device(args.scale, args.ef, 0.57, 0.19, 0.19, seed=args.seed,
                         keep_host=not args.no_cpu)
