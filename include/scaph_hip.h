@@ -149,10 +149,13 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree,
  * the vertices found and the full next-frontier bitmap (all words, so it
  * needs no clearing).  In-lists still unmatched after 512 entries are
  * finished by a second, grid-wide pass.  max_edges bounds the graph's
- * edges (workspace: scaph_pull_workspace_bytes).  No reference
- * counterpart (the reference engine is push-only). */
+ * edges (workspace: scaph_pull_workspace_bytes).  in_head (optional,
+ * u32[4n]) holds per vertex its first three in-list entries and its
+ * in-degree (scaph_list_heads_from_csr): vertices found there skip their
+ * page.  No reference counterpart (the reference engine is push-only). */
 size_t scaph_pull_workspace_bytes(unsigned long long max_edges);
-int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* hub_list, uint32_t num_hubs,
+int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
+                   const uint32_t* hub_list, uint32_t num_hubs,
                    const uint32_t* degree, uint32_t* level, uint32_t* visited,
                    const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
                    uint32_t cur_level, unsigned long long max_edges, void* workspace,
@@ -213,6 +216,10 @@ int scaph_gen_rmat_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32
 int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t n,
                       uint64_t seed, void* stream);
 
+/* head[4v..4v+3] = the first three entries of v's list (0xFFFFFFFF past
+ * its end) and its length, from a flat device CSR. */
+int scaph_list_heads_from_csr(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
+                              uint32_t* head, void* stream);
 /* offsets[0..n] = exclusive prefix of degree[0..n) (offsets[n] = total);
  * allocates its own scan storage and returns after the stream drains. */
 int scaph_degrees_to_offsets(const uint32_t* degree, uint64_t* offsets, uint32_t n, void* stream);

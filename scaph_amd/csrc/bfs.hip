@@ -474,9 +474,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* visited,
-       const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
-       uint32_t next_level, PullWs hw, uint32_t num_hubs) {
+k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* degree, uint32_t* level,
+       uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
+       scaph_level_counters* counters, uint32_t next_level, PullWs hw, uint32_t num_hubs) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
   __shared__ uint32_t s_q[kBlock / SCAPH_WAVE][kPullWin];
   __shared__ uint32_t s_fw[kBlock / SCAPH_WAVE][kPullWinWords];
@@ -523,10 +523,27 @@ k_pull(scaph_graph_desc g, const uint32_t* degree, uint32_t* level, uint32_t* vi
       nr.len = 0;
       if (act) {
         ++acc_x;
-        nr = load_node_hint(s, g, v, hint);
-        const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
         uint32_t k = 0;
-        for (; k < lim; k += kPullProbe) {
+        if (head) {
+          // the first three in-list entries and the in-degree sit in a
+          // 16-byte record read in vertex order: a vertex found there (the
+          // lists start with their hubs) never touches its page
+          const uint4 hd = head[v];
+          nr.len = hd.w;
+          const uint32_t l0 = hd.w < 3u ? hd.w : 3u;
+          const bool h0 = l0 > 0 && in_front(hd.x, s_hub, front_in);
+          const bool h1 = l0 > 1 && in_front(hd.y, s_hub, front_in);
+          const bool h2 = l0 > 2 && in_front(hd.z, s_hub, front_in);
+          k = l0;
+          found = h0 | h1 | h2;
+          if (!found && hd.w > 3u) {
+            nr = load_node_hint(s, g, v, hint);
+          }
+        } else {
+          nr = load_node_hint(s, g, v, hint);
+        }
+        const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
+        for (; !found && k < lim; k += kPullProbe) {
           uint32_t u[kPullProbe];
 #pragma unroll
           for (uint32_t j = 0; j < kPullProbe; ++j) u[j] = (k + j < lim) ? nr.edges[k + j] : 0u;
@@ -948,7 +965,8 @@ size_t scaph_pull_workspace_bytes(unsigned long long max_edges) {
   return carve_pull_ws(nullptr, max_edges).bytes;
 }
 
-int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* hub_list, uint32_t num_hubs,
+int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
+                   const uint32_t* hub_list, uint32_t num_hubs,
                    const uint32_t* degree, uint32_t* level, uint32_t* visited,
                    const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
                    uint32_t cur_level, unsigned long long max_edges, void* workspace,
@@ -969,8 +987,8 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* hub_list, u
   const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
   const unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 256 * 16);
   hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), page_lds_bytes(in_graph->num_pages), st,
-                     *in_graph, degree, level, visited, front_in, front_out, counters,
-                     cur_level + 1, hw, num_hubs);
+                     *in_graph, reinterpret_cast<const uint4*>(in_head), degree, level, visited,
+                     front_in, front_out, counters, cur_level + 1, hw, num_hubs);
   SCAPH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), 0, st, hw.chunks, hw.count, degree,
                      level, visited, front_in, front_out, counters, cur_level + 1, hw.hub_front,

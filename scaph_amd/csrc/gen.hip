@@ -51,6 +51,23 @@ k_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t count, uint32_t n, uint64_t
   }
 }
 
+// Per vertex: the first three entries of its list (0xFFFFFFFF past the end)
+// and its length, as one 16-byte record.
+__global__ void __launch_bounds__(kBlock)
+k_list_heads(const uint64_t* offsets, const uint32_t* ngh, uint32_t n, uint4* head) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const uint64_t o = offsets[v];
+    const uint32_t d = (uint32_t)(offsets[v + 1] - o);
+    uint4 r;
+    r.x = d > 0 ? ngh[o] : 0xFFFFFFFFu;
+    r.y = d > 1 ? ngh[o + 1] : 0xFFFFFFFFu;
+    r.z = d > 2 ? ngh[o + 2] : 0xFFFFFFFFu;
+    r.w = d;
+    head[v] = r;
+  }
+}
+
 // Power-law digraph edges: edge e belongs to the last vertex v with
 // offsets[v] <= e (offsets = exclusive prefix of the out-degrees, n + 1
 // entries); its destination is the multiply-high of SplitMix64 word e of the
@@ -199,6 +216,15 @@ int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t
   const uint64_t count = e_end - e_begin;
   hipLaunchKernelGGL(k_er_keys, dim3(grid_for(count, kBlock * 8, 65536)), dim3(kBlock), 0,
                      as_stream(stream), keys, e_begin, count, n, seed);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_list_heads_from_csr(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
+                              uint32_t* head, void* stream) {
+  if (!offsets || !head || n == 0) return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_list_heads, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0,
+                     as_stream(stream), offsets, ngh, n, reinterpret_cast<uint4*>(head));
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

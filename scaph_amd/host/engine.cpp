@@ -57,6 +57,8 @@ Config& global_flags() {
     r.add("compat63", FlagType::kBool, &c.compat63, "report levels capped at 63 like the reference");
     r.add("trace", FlagType::kString, &c.trace, "per-iteration JSONL trace file");
     r.add("build_pull", FlagType::kBool, &c.build_pull, "keep transposed pages for pull");
+    r.add("pull_head", FlagType::kBool, &c.pull_head,
+          "keep each in-list's first entries beside its in-degree for the pull probes");
     r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");
   }
   return c;
@@ -118,6 +120,7 @@ Engine::~Engine() = default;
 
 void Engine::reset_graph() {
   ++graph_gen_;
+  in_head_.reset();
   n_ = 0;
   m_ = 0;
   host_ = HostCsr();
@@ -272,6 +275,18 @@ void Engine::load_csr(HostCsr g, bool keep_host) {
       upload_pages(t, false, &pages_in_, &pool_in_, &dpages_in_, &dptr_in_, &g_in_);
       in_deg.resize(n_);
       for (uint32_t v = 0; v < n_; ++v) in_deg[v] = t.degree(v);
+      if (cfg_.pull_head) {
+        std::vector<uint32_t> head((size_t)n_ * 4);
+        for (uint32_t v = 0; v < n_; ++v) {
+          const uint64_t o = t.offsets[v];
+          const uint32_t d = t.degree(v);
+          for (uint32_t j = 0; j < 3; ++j) head[4ull * v + j] = j < d ? t.ngh[o + j] : 0xFFFFFFFFu;
+          head[4ull * v + 3] = d;
+        }
+        in_head_.alloc(std::max<size_t>(head.size(), 4) * 4);
+        H_ERR(scaph_copy_h2d_async(in_head_.get(), head.data(), head.size() * 4, main_->get()));
+        main_->sync();
+      }
     }
     finish_setup(&in_deg);
     if (keep_host) host_ = std::move(g);
@@ -473,6 +488,11 @@ void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
     H_ERR(scaph_pages_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(),
                                dpages.get<scaph_page>(), dptr.get<uint64_t>(),
                                (uint32_t)pages.size(), n, st));
+    if (dir == 1 && cfg_.pull_head) {
+      in_head_.alloc((size_t)n * 16);
+      H_ERR(scaph_list_heads_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
+                                      in_head_.get<uint32_t>(), st));
+    }
     main_->sync();
     desc.page_ptr = dptr.get<const uint64_t>();
     desc.pages = dpages.get<const scaph_page>();
@@ -488,6 +508,7 @@ void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
     }
   }
   if (!has_in_) {
+    in_head_.reset();
     pool_in_.reset();
     dpages_in_.reset();
     dptr_in_.reset();

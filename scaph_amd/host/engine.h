@@ -45,6 +45,7 @@ struct Config {
   bool compat63 = false;         // print/write levels with the reference's cap of 63
   std::string trace;             // per-iteration JSONL trace file
   bool build_pull = true;        // keep transposed pages for the pull expand
+  bool pull_head = true;         // 16-byte in-list head records in front of the in-pages
   int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
 };
 
@@ -201,6 +202,7 @@ class Engine {
   scaph_graph_desc g_out_{}, g_in_{};
   DevBuf deg_out_, deg_in_, static_visited_;
   DevBuf hub_list_;  // hub vertex ids (pull in-lists refer to them by index)
+  DevBuf in_head_;   // per vertex: first three in-list entries and in-degree
   uint32_t num_hubs_ = 0;
   // BFS state
   DevBuf level_, visited_, front_[2], queue_[2], counters_, ws_, page_nodes_, page_datas_, totals_;
