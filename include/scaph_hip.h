@@ -97,6 +97,17 @@ int scaph_copy_d2h_async(void* dst, const void* src, size_t bytes, void* stream)
 int scaph_copy_d2d_async(void* dst, const void* src, size_t bytes, void* stream);
 /* Fill n u32 words with value (a kernel, safe to capture in a hipGraph). */
 int scaph_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream);
+/* Host memory that kernels write directly (pinned, mapped, coherent):
+ * *host_ptr for the CPU, *device_ptr for kernels; free with
+ * scaph_host_pinned_free(host_ptr). */
+int scaph_host_mapped_alloc(void** host_ptr, void** device_ptr, size_t bytes);
+/* Level hand-off without a copy engine: copies `words` u32 of device
+ * memory into mapped host memory, clears the first `clear_words` of the
+ * source, then writes `seq` to the mapped flag (system-scope release), so a
+ * host spinning on the flag reads complete data.  Replaces the per-level
+ * counter copy of the reference's iter_init (src/app.cuh:188-207). */
+int scaph_publish_words(uint32_t* src, uint32_t words, uint32_t clear_words, uint32_t* dst_mapped,
+                        uint32_t* flag_mapped, uint32_t seq, void* stream);
 
 /* ------------------------------------------------------------- BFS state */
 /* Counters written by the expand kernels, one record per level. */

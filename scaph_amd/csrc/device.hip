@@ -139,6 +139,49 @@ extern "C" int scaph_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* str
   return 0;
 }
 
+// Copies `words` u32 from device memory into host-mapped memory, clears
+// the first `clear_words` of the source, then stores `seq` into the host
+// flag after a system-scope fence: the host sees the flag only once the data
+// is visible.  One wave.
+__global__ void __launch_bounds__(64)
+k_publish(uint32_t* src, uint32_t words, uint32_t clear_words, uint32_t* dst, uint32_t* flag,
+          uint32_t seq) {
+  for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) {
+    dst[i] = src[i];
+    if (i < clear_words) src[i] = 0u;
+  }
+  __threadfence_system();
+  __builtin_amdgcn_wave_barrier();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+extern "C" {
+
+int scaph_host_mapped_alloc(void** host_ptr, void** device_ptr, size_t bytes) {
+  if (!host_ptr || !device_ptr || bytes == 0) return SCAPH_STATUS_INVALID;
+  *host_ptr = *device_ptr = nullptr;
+  SCAPH_TRY(hipHostMalloc(host_ptr, bytes, hipHostMallocMapped | hipHostMallocCoherent));
+  const hipError_t e = hipHostGetDevicePointer(device_ptr, *host_ptr, 0);
+  if (e != hipSuccess) {
+    hipHostFree(*host_ptr);
+    *host_ptr = nullptr;
+    return (int)e;
+  }
+  return 0;
+}
+
+int scaph_publish_words(uint32_t* src, uint32_t words, uint32_t clear_words, uint32_t* dst_mapped,
+                        uint32_t* flag_mapped, uint32_t seq, void* stream) {
+  if (!src || !dst_mapped || !flag_mapped || clear_words > words) return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, as_stream(stream), src, words, clear_words,
+                     dst_mapped, flag_mapped, seq);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------- hipGraph
 extern "C" {
 

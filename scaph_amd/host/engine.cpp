@@ -114,6 +114,8 @@ Engine::Engine(const Config& cfg)
   H_ERR(scaph_init(cfg.device));
   main_.reset(new Stream());
   hcount_.alloc(4096);
+  hand_.alloc(4096);
+  std::memset(hand_.host(), 0, hand_.bytes());
 }
 
 Engine::~Engine() = default;

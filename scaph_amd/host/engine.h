@@ -212,6 +212,9 @@ class Engine {
   std::vector<uint32_t> h_deg_out_;
   std::unique_ptr<Stream> main_;
   PinnedBuf hcount_;  // pinned landing zone for per-level counters
+  MappedBuf hand_;    // resident level hand-off: counters (+ decision) and a flag, kernel-written
+  uint32_t hand_seq_ = 0;
+  void wait_handoff(uint32_t seq);
   std::vector<std::unique_ptr<Event>> ev_;
   SchedFn sched_;
   std::unique_ptr<PagedState> paged_;

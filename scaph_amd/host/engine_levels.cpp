@@ -16,6 +16,8 @@ constexpr size_t kDecideFloats = 32;  // logit, threshold, pages active, pad, hi
 // Word offset of the learned decision's output inside the counters buffer
 // (beside the level counters, so one copy returns both).
 constexpr size_t kDecideWord = 64;
+// Word of the hand-off flag in the mapped hand-off buffer.
+constexpr size_t kHandFlagWord = 256;
 
 size_t bitmap_words(uint32_t n) { return (((size_t)n + 63) / 64) * 2; }
 
@@ -60,6 +62,25 @@ void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m
     f.fa ^= 1;
     f.front_ok = true;
     f.queue_ok = false;
+  }
+}
+
+// Spins on the mapped flag the level's publish kernel writes last; checks
+// the stream now and then so that a failed launch raises instead of hanging.
+void Engine::wait_handoff(uint32_t seq) {
+  const volatile uint32_t* flag = hand_.host<uint32_t>() + kHandFlagWord;
+  for (uint64_t spins = 1;; ++spins) {
+    if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) return;
+    if ((spins & 0x3FF) == 0) {
+      const int st = scaph_stream_query(main_->get());
+      if (st == 0) {
+        // the stream drained: the flag is visible now or never will be
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) return;
+        throw Error(SCAPH_STATUS_INVALID, "level hand-off flag not written");
+      }
+      if (st != SCAPH_STATUS_NOT_READY) raise_status(st, __FILE__, __LINE__);
+    }
+    __builtin_ia32_pause();
   }
 }
 
@@ -111,10 +132,10 @@ RunStats Engine::bfs_resident(uint32_t src) {
   while (ev_.size() < 2) ev_.emplace_back(new Event());
   const uint64_t deg_src = out_degree(src);
   auto* cnt = counters_.get<scaph_level_counters>();
-  auto* hc = hcount_.get<scaph_level_counters>();
   const bool learned = cfg_.schedule == "learned" && device_model_ready() && has_in_;
   if (learned) ensure_model_buffers();
-  const float* hout = learned ? hcount_.get<float>() + kDecideWord : nullptr;
+  const float* hout = learned ? hand_.host<float>() + kDecideWord : nullptr;
+  auto* hc = hand_.host<scaph_level_counters>();
 
   ev_[0]->record(st);
   H_ERR(scaph_bfs_init(level_.get<uint32_t>(), visited_.get<uint32_t>(),
@@ -137,7 +158,8 @@ RunStats Engine::bfs_resident(uint32_t src) {
     else d = sched_ ? sched_(ss) : beamer(ss);
     if (d == Dir::kPull && !has_in_) d = Dir::kPush;
     ensure_forms(f, d == Dir::kPush, d == Dir::kPull, n_f);
-    H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, kCounterWords, st));
+    // later levels find the counters cleared by the previous hand-off
+    if (cur == 0) H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, kCounterWords, st));
     const bool timed = rs.levels.size() < kMaxTimedLevels;
     const size_t eb = 2 + 3 * rs.levels.size();
     if (timed) {
@@ -159,12 +181,16 @@ RunStats Engine::bfs_resident(uint32_t src) {
           bm ? front_[f.fa].get<uint32_t>() : nullptr, bm ? nullptr : queue_[f.qa].get<uint32_t>(),
           g_out_.pages, (uint32_t)pages_out_.size(), n_, cnt, m_u, cur + 1,
           m.page_cnt.get<uint32_t>(), counters_.get<float>() + kDecideWord, st));
-      H_ERR(scaph_copy_d2h_async(hc, cnt, (kDecideWord + kDecideFloats) * 4, st));
-    } else {
-      H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
     }
+    // the counters (and the decision) land in mapped host memory, the
+    // counters are cleared for the next level, and a flag says so
+    const uint32_t seq = ++hand_seq_;
+    H_ERR(scaph_publish_words(counters_.get<uint32_t>(),
+                              (uint32_t)(learned ? kDecideWord + kDecideFloats : kCounterWords),
+                              (uint32_t)kCounterWords, hand_.dev<uint32_t>(),
+                              hand_.dev<uint32_t>() + kHandFlagWord, seq, st));
     if (timed) ev_[eb + 2]->record(st);
-    main_->sync();
+    wait_handoff(seq);
     LevelRecord rec;
     rec.level = cur;
     rec.dir = (int)d;

@@ -111,6 +111,39 @@ class PinnedBuf {
   size_t n_ = 0;
 };
 
+// Pinned, mapped, coherent host memory that kernels write directly.
+class MappedBuf {
+ public:
+  MappedBuf() = default;
+  ~MappedBuf() { reset(); }
+  MappedBuf(const MappedBuf&) = delete;
+  MappedBuf& operator=(const MappedBuf&) = delete;
+  void alloc(size_t bytes) {
+    reset();
+    H_ERR(scaph_host_mapped_alloc(&h_, &d_, bytes));
+    n_ = bytes;
+  }
+  void reset() {
+    if (h_) scaph_host_pinned_free(h_);
+    h_ = d_ = nullptr;
+    n_ = 0;
+  }
+  template <typename T = void>
+  T* host() const {
+    return static_cast<T*>(h_);
+  }
+  template <typename T = void>
+  T* dev() const {
+    return static_cast<T*>(d_);
+  }
+  size_t bytes() const { return n_; }
+
+ private:
+  void* h_ = nullptr;
+  void* d_ = nullptr;
+  size_t n_ = 0;
+};
+
 class Stream {
  public:
   Stream() { H_ERR(scaph_stream_create(&s_)); }
