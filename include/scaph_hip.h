@@ -165,12 +165,17 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree,
  * in-degree (scaph_list_heads_from_csr): vertices found there skip their
  * page.  No reference counterpart (the reference engine is push-only). */
 size_t scaph_pull_workspace_bytes(unsigned long long max_edges);
+/* Launch shape of the pull expand (NULL or 0 fields = built-in choice). */
+typedef struct scaph_pull_tuning {
+  uint32_t grid_blocks; /* cap on 256-thread blocks (each owns 4 x 1024-vertex windows per pass) */
+  uint32_t reserved[3];
+} scaph_pull_tuning;
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
                    const uint32_t* hub_list, uint32_t num_hubs,
                    const uint32_t* degree, uint32_t* level, uint32_t* visited,
                    const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
                    uint32_t cur_level, unsigned long long max_edges, void* workspace,
-                   size_t workspace_bytes, void* stream);
+                   size_t workspace_bytes, const scaph_pull_tuning* tune, void* stream);
 
 /* Frontier format conversions (no reference counterpart). */
 int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint32_t* bitmap,

@@ -857,6 +857,20 @@ k_validate_parents(scaph_graph_desc gin, const uint32_t* hub_list, const uint32_
   if (lane_id() == 0 && bad) atomicAdd(&out[1], bad);
 }
 
+// Compute units of the current device (cached per process).
+inline unsigned cu_count() {
+  static unsigned cus = 0;
+  if (!cus) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      cus = (unsigned)v;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
 inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
   size_t b = (items + per_block - 1) / per_block;
   if (b < 1) b = 1;
@@ -970,7 +984,7 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
                    const uint32_t* degree, uint32_t* level, uint32_t* visited,
                    const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
                    uint32_t cur_level, unsigned long long max_edges, void* workspace,
-                   size_t workspace_bytes, void* stream) {
+                   size_t workspace_bytes, const scaph_pull_tuning* tune, void* stream) {
   if (!in_graph || !degree || !level || !visited || !front_in || !front_out || !counters ||
       !workspace || num_hubs > kMaxHubs || (num_hubs && !hub_list))
     return SCAPH_STATUS_INVALID;
@@ -985,7 +999,11 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
     SCAPH_LAUNCH_CHECK();
   }
   const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
-  const unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 256 * 16);
+  // persistent-style: four resident blocks per CU, each walking its share
+  // of the window groups (a launch larger than what is resident leaves a
+  // tail of late blocks; measured best on s26 among 512..4096 blocks)
+  unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 4u * cu_count());
+  if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
   hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), page_lds_bytes(in_graph->num_pages), st,
                      *in_graph, reinterpret_cast<const uint4*>(in_head), degree, level, visited,
                      front_in, front_out, counters, cur_level + 1, hw, num_hubs);

@@ -57,6 +57,7 @@ Config& global_flags() {
     r.add("compat63", FlagType::kBool, &c.compat63, "report levels capped at 63 like the reference");
     r.add("trace", FlagType::kString, &c.trace, "per-iteration JSONL trace file");
     r.add("build_pull", FlagType::kBool, &c.build_pull, "keep transposed pages for pull");
+    r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
     r.add("pull_head", FlagType::kBool, &c.pull_head,
           "keep each in-list's first entries beside its in-degree for the pull probes");
     r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");
@@ -116,6 +117,7 @@ Engine::Engine(const Config& cfg)
   hcount_.alloc(4096);
   hand_.alloc(4096);
   std::memset(hand_.host(), 0, hand_.bytes());
+  pull_tune_.grid_blocks = (uint32_t)std::max(0, cfg.pull_grid);
 }
 
 Engine::~Engine() = default;

@@ -46,6 +46,7 @@ struct Config {
   std::string trace;             // per-iteration JSONL trace file
   bool build_pull = true;        // keep transposed pages for the pull expand
   bool pull_head = true;         // 16-byte in-list head records in front of the in-pages
+  int32_t pull_grid = 0;         // cap on pull blocks (0: four per compute unit)
   int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
 };
 
@@ -214,6 +215,7 @@ class Engine {
   PinnedBuf hcount_;  // pinned landing zone for per-level counters
   MappedBuf hand_;    // resident level hand-off: counters (+ decision) and a flag, kernel-written
   uint32_t hand_seq_ = 0;
+  scaph_pull_tuning pull_tune_{};
   void wait_handoff(uint32_t seq);
   std::vector<std::unique_ptr<Event>> ev_;
   SchedFn sched_;
