@@ -180,6 +180,12 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
 /* Frontier format conversions (no reference counterpart). */
 int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint32_t* bitmap,
                                    uint32_t n, void* stream);
+/* The frontier a push level produced, as a bitmap: visited & ~snapshot,
+ * where snapshot is the visited bitmap copied before the push (16-byte
+ * aligned buffers).  Costs three bitmap passes instead of one atomic per
+ * frontier vertex. */
+int scaph_frontier_from_visited(const uint32_t* visited, const uint32_t* snapshot,
+                                uint32_t* bitmap, uint32_t n, void* stream);
 int scaph_frontier_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* queue,
                                    scaph_level_counters* counters, void* stream);
 

@@ -125,6 +125,23 @@ k_zero_degree_bitmap(uint32_t* bitmap, const uint32_t* in_degree, uint32_t n, si
   }
 }
 
+// front = visited & ~snapshot: the vertices claimed since the snapshot,
+// i.e. the frontier a push level produced, without one atomic per vertex.
+__global__ void __launch_bounds__(kBlock)
+k_visited_diff(const uint32_t* visited, const uint32_t* snapshot, uint32_t* front, size_t words) {
+  const size_t n4 = words / 4;
+  const uint4* a4 = reinterpret_cast<const uint4*>(visited);
+  const uint4* b4 = reinterpret_cast<const uint4*>(snapshot);
+  uint4* f4 = reinterpret_cast<uint4*>(front);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t i = tid; i < n4; i += stride) {
+    const uint4 a = a4[i], b = b4[i];
+    f4[i] = make_uint4(a.x & ~b.x, a.y & ~b.y, a.z & ~b.z, a.w & ~b.w);
+  }
+  for (size_t w = n4 * 4 + tid; w < words; w += stride) front[w] = visited[w] & ~snapshot[w];
+}
+
 // ------------------------------------------------------------ push: prep
 
 // Block-local compaction of the entries with len > 0.
@@ -1025,6 +1042,19 @@ int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint
   if (!queue) return SCAPH_STATUS_INVALID;
   hipLaunchKernelGGL(k_queue_to_bitmap, dim3(grid_for(n_queue, kBlock, 16384)), dim3(kBlock), 0,
                      as_stream(stream), queue, n_queue, bitmap, n);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_frontier_from_visited(const uint32_t* visited, const uint32_t* snapshot,
+                                uint32_t* bitmap, uint32_t n, void* stream) {
+  if (!visited || !snapshot || !bitmap) return SCAPH_STATUS_INVALID;
+  if ((reinterpret_cast<uintptr_t>(visited) | reinterpret_cast<uintptr_t>(snapshot) |
+       reinterpret_cast<uintptr_t>(bitmap)) & 15)
+    return SCAPH_STATUS_INVALID;  // the bulk uses 16-byte accesses
+  const size_t words = (((size_t)n + 63) / 64) * 2;
+  hipLaunchKernelGGL(k_visited_diff, dim3(grid_for(words / 4 + 1, kBlock, 16384)), dim3(kBlock), 0,
+                     as_stream(stream), visited, snapshot, bitmap, words);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

@@ -226,6 +226,7 @@ void Engine::alloc_state() {
   const size_t words = bitmap_words(n_);
   level_.alloc((size_t)n_ * 4);
   visited_.alloc(words * 4);
+  if (has_in_) vsnap_.alloc(words * 4);
   front_[0].alloc(words * 4);
   front_[1].alloc(words * 4);
   queue_[0].alloc(std::max<size_t>(n_, 1) * 4);

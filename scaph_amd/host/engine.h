@@ -180,6 +180,7 @@ class Engine {
   struct Front {  // which buffers hold the current frontier, in which forms
     int qa = 0, fa = 0;
     bool queue_ok = true, front_ok = false;
+    bool snap_ok = false;  // vsnap_ holds visited from before the push that made the queue
   };
   void ensure_forms(Front& f, bool queue, bool bitmap, uint64_t n_f);
   void launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m_f);
@@ -209,6 +210,7 @@ class Engine {
   DevBuf level_, visited_, front_[2], queue_[2], counters_, ws_, page_nodes_, page_datas_, totals_;
   size_t ws_bytes_ = 0;
   DevBuf pws_;  // pull workspace (deferred heavy in-lists)
+  DevBuf vsnap_;  // visited before the latest push level (its frontier = visited & ~vsnap_)
   size_t pws_bytes_ = 0;
   std::vector<uint32_t> h_deg_out_;
   std::unique_ptr<Stream> main_;

@@ -29,8 +29,12 @@ void Engine::ensure_forms(Front& f, bool queue, bool bitmap, uint64_t n_f) {
   void* st = main_->get();
   auto* cnt = counters_.get<scaph_level_counters>();
   if (bitmap && !f.front_ok) {
-    H_ERR(scaph_frontier_queue_to_bitmap(queue_[f.qa].get<uint32_t>(), (uint32_t)n_f,
-                                         front_[f.fa].get<uint32_t>(), n_, st));
+    if (f.snap_ok)
+      H_ERR(scaph_frontier_from_visited(visited_.get<uint32_t>(), vsnap_.get<uint32_t>(),
+                                        front_[f.fa].get<uint32_t>(), n_, st));
+    else
+      H_ERR(scaph_frontier_queue_to_bitmap(queue_[f.qa].get<uint32_t>(), (uint32_t)n_f,
+                                           front_[f.fa].get<uint32_t>(), n_, st));
     f.front_ok = true;
   }
   if (queue && !f.queue_ok) {
@@ -49,11 +53,16 @@ void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m
   uint32_t* visited = visited_.get<uint32_t>();
   const uint32_t* deg = deg_out_.get<uint32_t>();
   if (d == Dir::kPush) {
+    // with a pull step possible next, keep visited as it was before this
+    // push: the next frontier's bitmap is then visited & ~snapshot
+    const bool snap = vsnap_.get() != nullptr;
+    if (snap) H_ERR(scaph_copy_d2d_async(vsnap_.get(), visited, vsnap_.bytes(), st));
     H_ERR(scaph_bfs_push(&g_out_, deg, level, visited, queue_[f.qa].get<uint32_t>(), (uint32_t)n_f,
                          m_f, queue_[f.qa ^ 1].get<uint32_t>(), cnt, cur, ws_.get(), ws_bytes_, st));
     f.qa ^= 1;
     f.queue_ok = true;
     f.front_ok = false;
+    f.snap_ok = snap;
   } else {
     H_ERR(scaph_bfs_pull(&g_in_, in_head_.get<uint32_t>(), hub_list_.get<uint32_t>(), num_hubs_,
                          deg, level, visited,
@@ -62,6 +71,7 @@ void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m
     f.fa ^= 1;
     f.front_ok = true;
     f.queue_ok = false;
+    f.snap_ok = false;
   }
 }
 
