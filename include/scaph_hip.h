@@ -139,6 +139,12 @@ int scaph_bitmap_from_zero_degree(uint32_t* bitmap, const uint32_t* in_degree, u
  * out-edges total at most max_edges. */
 size_t scaph_push_workspace_bytes(uint32_t max_entries, unsigned long long max_edges);
 
+/* Launch shape of the push expand (NULL or 0 fields = built-in choice). */
+typedef struct scaph_push_tuning {
+  uint32_t grid_blocks; /* cap on expand blocks (one 1024-edge tile per block per pass) */
+  uint32_t edge_rows;   /* 1: a wave reads 256 consecutive edges per step (coalesced rows) */
+  uint32_t reserved[2];
+} scaph_push_tuning;
 /* Top-down (push) expand of one level from a frontier queue — the resident
  * counterpart of gpu_push_csr (src/gpu_engine.cuh:41-169) with the BFS
  * relax of app/bfs_kernel.cuh:6-16 turned into a level-synchronous claim:
@@ -150,7 +156,8 @@ size_t scaph_push_workspace_bytes(uint32_t max_entries, unsigned long long max_e
 int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree,
                    uint32_t* level, uint32_t* visited, const uint32_t* q_in, uint32_t n_in,
                    unsigned long long m_in, uint32_t* q_out, scaph_level_counters* counters,
-                   uint32_t cur_level, void* workspace, size_t workspace_bytes, void* stream);
+                   uint32_t cur_level, void* workspace, size_t workspace_bytes,
+                   const scaph_push_tuning* tune, void* stream);
 
 /* Bottom-up (pull) expand of one level: every unvisited vertex scans its
  * in-neighbours (transposed paged CSR) for one in the frontier bitmap.

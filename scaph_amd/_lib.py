@@ -156,7 +156,8 @@ def hip():
         _sig(lib, "scaph_bitmap_from_zero_degree", I32, [P, P, U32, P])
         _sig(lib, "scaph_push_workspace_bytes", SZ, [U32, ctypes.c_ulonglong])
         _sig(lib, "scaph_bfs_push", I32,
-             [ctypes.POINTER(GraphDesc), P, P, P, P, U32, ctypes.c_ulonglong, P, P, U32, P, SZ, P])
+             [ctypes.POINTER(GraphDesc), P, P, P, P, U32, ctypes.c_ulonglong, P, P, U32, P, SZ, P,
+              P])
         _sig(lib, "scaph_pull_workspace_bytes", SZ, [ctypes.c_ulonglong])
         _sig(lib, "scaph_bfs_pull", I32, [ctypes.POINTER(GraphDesc), P, P, U32, P, P, P, P, P, P,
                                           U32, ctypes.c_ulonglong, P, SZ, P, P])

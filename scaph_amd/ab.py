@@ -1,7 +1,7 @@
 """A/B comparison of two engine flag sets on one device-built R-MAT graph
 and the same sources (min of `reps` runs per source), e.g.
 
-    python -m scaph_amd.ab "-pull_head=false" "-pull_head=true"
+    python -m scaph_amd.ab --a="-pull_head=false" --b="-pull_head=true"
 
 Prints one JSON line with both medians, the median per-source ratio B/A and
 whether the two engines' final levels agree.
@@ -31,16 +31,16 @@ def pick_sources(eng, count, seed, min_frac=0.01):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("flags_a")
-    ap.add_argument("flags_b")
+    ap.add_argument("--a", default="", help="engine flags of A (space separated)")
+    ap.add_argument("--b", default="", help="engine flags of B (space separated)")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--ef", type=int, default=32)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--sources", type=int, default=32)
     args = ap.parse_args()
-    fa = args.flags_a.split()
-    fb = args.flags_b.split()
+    fa = args.a.split()
+    fb = args.b.split()
     common = ["-max_memory=262144", "-pagesize=10"]
     ea = BFS(*common, *fa)
     ea.gen_rmat_device(args.scale, args.ef, 0.57, 0.19, 0.19, seed=args.seed)

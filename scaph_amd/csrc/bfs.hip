@@ -297,7 +297,9 @@ k_mark_tiles(const uint64_t* ent_ep, const uint32_t* ent_len, const uint32_t* bl
 
 enum { OUT_QUEUE = 0, OUT_BITMAP = 1 };
 
-template <int MODE>
+// LAYOUT 0: each thread takes kPerThread consecutive edges of the tile;
+// LAYOUT 1: edge j*kBlock + thread (one coalesced 256-edge row per step).
+template <int MODE, int LAYOUT>
 __global__ void __launch_bounds__(kBlock)
 k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
          const unsigned long long* totals, const uint32_t* degree, uint32_t* level,
@@ -327,7 +329,8 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
     }
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
-    const unsigned long long r0 = s0 + (unsigned long long)threadIdx.x * kPerThread;
+    const unsigned long long r0 = LAYOUT == 0 ? s0 + (unsigned long long)threadIdx.x * kPerThread
+                                              : s0 + threadIdx.x;
     uint32_t lo = 0, hi = cnt;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -335,9 +338,17 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
     }
 #pragma unroll
     for (uint32_t j = 0; j < kPerThread; ++j) {
-      const unsigned long long r = r0 + j;
+      const unsigned long long r = LAYOUT == 0 ? r0 + j : r0 + (unsigned long long)j * kBlock;
       if (r < e0) {
-        while (lo + 1 < cnt && s_gx[lo + 1] <= r) ++lo;
+        if (LAYOUT == 0) {
+          while (lo + 1 < cnt && s_gx[lo + 1] <= r) ++lo;
+        } else {
+          uint32_t h = cnt;  // rows only move forward: search [lo, cnt)
+          while (h - lo > 1) {
+            const uint32_t mid = (lo + h) >> 1;
+            if (s_gx[mid] <= r) lo = mid; else h = mid;
+          }
+        }
         const uint32_t* ep = reinterpret_cast<const uint32_t*>(s_ep[lo]);
         const uint32_t v = ep[r - s_gx[lo]];
         if (v < n) {
@@ -898,7 +909,7 @@ inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
 int launch_push_tail(const PushWs& w, uint32_t nb, unsigned long long max_edges,
                      const uint32_t* degree, uint32_t* level, uint32_t* visited, uint32_t* q_out,
                      uint32_t* front_out, scaph_level_counters* counters, uint32_t next_level,
-                     uint32_t n, int mode, hipStream_t st) {
+                     uint32_t n, int mode, const scaph_push_tuning* tune, hipStream_t st) {
   const unsigned long long tile_cap = (max_edges + kTile - 1) / kTile;
   hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
                      nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap);
@@ -906,16 +917,20 @@ int launch_push_tail(const PushWs& w, uint32_t nb, unsigned long long max_edges,
   hipLaunchKernelGGL(k_mark_tiles, dim3(nb), dim3(kBlock), 0, st, w.ent_ep, w.ent_len, w.blk_cnt,
                      w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
   SCAPH_LAUNCH_CHECK();
-  const unsigned grid = grid_for((size_t)((max_edges + kTile - 1) / kTile), 1, 8192);
+  unsigned grid = grid_for((size_t)((max_edges + kTile - 1) / kTile), 1, 8192);
+  if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
+  const bool rows = tune && tune->edge_rows;
+#define SCAPH_EXPAND(M, L)                                                                       \
+  hipLaunchKernelGGL((k_expand<M, L>), dim3(grid), dim3(kBlock), 0, st, w.gx, w.gep, w.tstart,  \
+                     w.totals, degree, level, visited, q_out, front_out, counters, next_level, n)
   if (mode == OUT_QUEUE) {
-    hipLaunchKernelGGL(k_expand<OUT_QUEUE>, dim3(grid), dim3(kBlock), 0, st, w.gx, w.gep,
-                       w.tstart, w.totals, degree, level, visited, q_out, front_out, counters,
-                       next_level, n);
+    if (rows) SCAPH_EXPAND(OUT_QUEUE, 1);
+    else SCAPH_EXPAND(OUT_QUEUE, 0);
   } else {
-    hipLaunchKernelGGL(k_expand<OUT_BITMAP>, dim3(grid), dim3(kBlock), 0, st, w.gx, w.gep,
-                       w.tstart, w.totals, degree, level, visited, q_out, front_out, counters,
-                       next_level, n);
+    if (rows) SCAPH_EXPAND(OUT_BITMAP, 1);
+    else SCAPH_EXPAND(OUT_BITMAP, 0);
   }
+#undef SCAPH_EXPAND
   SCAPH_LAUNCH_CHECK();
   return 0;
 }
@@ -955,7 +970,8 @@ size_t scaph_push_workspace_bytes(uint32_t max_entries, unsigned long long max_e
 int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree, uint32_t* level,
                    uint32_t* visited, const uint32_t* q_in, uint32_t n_in,
                    unsigned long long m_in, uint32_t* q_out, scaph_level_counters* counters,
-                   uint32_t cur_level, void* workspace, size_t workspace_bytes, void* stream) {
+                   uint32_t cur_level, void* workspace, size_t workspace_bytes,
+                   const scaph_push_tuning* tune, void* stream) {
   if (!out_graph || !degree || !level || !visited || !q_out || !counters || !workspace)
     return SCAPH_STATUS_INVALID;
   if (n_in == 0) return 0;
@@ -969,7 +985,7 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree, ui
                      w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
   return launch_push_tail(w, nb, m_in, degree, level, visited, q_out, nullptr, counters,
-                          cur_level + 1, out_graph->num_vertices, OUT_QUEUE, st);
+                          cur_level + 1, out_graph->num_vertices, OUT_QUEUE, tune, st);
 }
 
 int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned long long edgenum,
@@ -989,7 +1005,7 @@ int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned lon
                      w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
   return launch_push_tail(w, nb, edgenum, degree, level, visited, nullptr, front_out, counters,
-                          cur_level + 1, n, OUT_BITMAP, st);
+                          cur_level + 1, n, OUT_BITMAP, nullptr, st);
 }
 
 size_t scaph_pull_workspace_bytes(unsigned long long max_edges) {

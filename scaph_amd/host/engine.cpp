@@ -58,6 +58,8 @@ Config& global_flags() {
     r.add("trace", FlagType::kString, &c.trace, "per-iteration JSONL trace file");
     r.add("build_pull", FlagType::kBool, &c.build_pull, "keep transposed pages for pull");
     r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
+    r.add("push_grid", FlagType::kInt32, &c.push_grid, "cap on push expand blocks (0 = default)");
+    r.add("push_rows", FlagType::kBool, &c.push_rows, "push expand reads coalesced edge rows");
     r.add("pull_head", FlagType::kBool, &c.pull_head,
           "keep each in-list's first entries beside its in-degree for the pull probes");
     r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");
@@ -118,6 +120,8 @@ Engine::Engine(const Config& cfg)
   hand_.alloc(4096);
   std::memset(hand_.host(), 0, hand_.bytes());
   pull_tune_.grid_blocks = (uint32_t)std::max(0, cfg.pull_grid);
+  push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
+  push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
 }
 
 Engine::~Engine() = default;

@@ -47,6 +47,8 @@ struct Config {
   bool build_pull = true;        // keep transposed pages for the pull expand
   bool pull_head = true;         // 16-byte in-list head records in front of the in-pages
   int32_t pull_grid = 0;         // cap on pull blocks (0: four per compute unit)
+  int32_t push_grid = 0;         // cap on push expand blocks (0: one per tile, at most 8192)
+  bool push_rows = true;         // push expand reads coalesced 256-edge rows
   int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
 };
 
@@ -218,6 +220,7 @@ class Engine {
   MappedBuf hand_;    // resident level hand-off: counters (+ decision) and a flag, kernel-written
   uint32_t hand_seq_ = 0;
   scaph_pull_tuning pull_tune_{};
+  scaph_push_tuning push_tune_{};
   void wait_handoff(uint32_t seq);
   std::vector<std::unique_ptr<Event>> ev_;
   SchedFn sched_;

@@ -58,7 +58,8 @@ void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m
     const bool snap = vsnap_.get() != nullptr;
     if (snap) H_ERR(scaph_copy_d2d_async(vsnap_.get(), visited, vsnap_.bytes(), st));
     H_ERR(scaph_bfs_push(&g_out_, deg, level, visited, queue_[f.qa].get<uint32_t>(), (uint32_t)n_f,
-                         m_f, queue_[f.qa ^ 1].get<uint32_t>(), cnt, cur, ws_.get(), ws_bytes_, st));
+                         m_f, queue_[f.qa ^ 1].get<uint32_t>(), cnt, cur, ws_.get(), ws_bytes_,
+                         &push_tune_, st));
     f.qa ^= 1;
     f.queue_ok = true;
     f.front_ok = false;

@@ -255,3 +255,20 @@ def test_pull_heavy_in_lists():
             e.set_schedule(sched)
             e.run(s_)
             np.testing.assert_array_equal(e.levels(), want)
+
+
+@pytest.mark.parametrize("flags", [["-push_rows=false"], ["-push_rows=true", "-push_grid=64"],
+                                   ["-pull_grid=16"]])
+def test_launch_shape_variants(flags):
+    """The push/pull launch-shape switches change timing only."""
+    off, ngh = O.rmat_csr(16, 16, seed=4)
+    deg = np.diff(off.astype(np.int64))
+    srcs = [int(np.argmax(deg)), int(np.nonzero(deg)[0][9])]
+    with engine(*flags) as e:
+        e.load_csr(off, ngh)
+        for src in srcs:
+            want = O.bfs(off, ngh, src)
+            for sched in SCHEDULES:
+                e.set_schedule(sched)
+                e.run(src)
+                np.testing.assert_array_equal(e.levels(), want, err_msg=f"{flags} {src} {sched}")
