@@ -109,6 +109,7 @@ typedef struct scaph_ms_tune {
   uint32_t pull_block;          /* 256, 512 or 1024 */
   uint32_t use_graph;           /* replay hipGraph-captured level segments */
   uint32_t segment;             /* levels per captured segment */
+  uint32_t pull_grid;           /* cap on pull blocks (0 = one per block of vertices) */
 } scaph_ms_tune;
 typedef struct scaph_ms_stats {
   double ms;                    /* device time: init and every level */

@@ -356,6 +356,8 @@ typedef struct scaph_ms_params {
   uint32_t push_div;          /* push while n_active * push_div < n ... */
   uint32_t push_mdiv;         /* ... and m_active * push_mdiv < m */
   uint32_t pull_block;        /* threads per pull block: 256, 512 or 1024 */
+  uint32_t pull_grid;         /* cap on pull blocks (0: one per block of vertices, at most 16384) */
+  uint32_t pad;
   unsigned long long m;       /* edges of the graph */
 } scaph_ms_params;
 typedef struct scaph_ms_state {

@@ -531,7 +531,9 @@ int scaph_ms_level(const scaph_graph_desc* out_graph, const scaph_graph_desc* in
                      level, p, degree);
   SCAPH_LAUNCH_CHECK();
   const size_t li = page_lds_bytes(in_graph->num_pages);
-  hipLaunchKernelGGL(k_ms_pull, dim3(grid_for(s.n, pb, 16384)), dim3(pb), li, sm, *in_graph,
+  unsigned pgrid = grid_for(s.n, pb, 16384);
+  if (p.pull_grid && p.pull_grid < pgrid) pgrid = p.pull_grid;
+  hipLaunchKernelGGL(k_ms_pull, dim3(pgrid), dim3(pb), li, sm, *in_graph,
                      hub_list, s, level, p, degree);
   SCAPH_LAUNCH_CHECK();
   return 0;

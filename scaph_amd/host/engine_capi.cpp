@@ -263,6 +263,7 @@ scaph::MsTune from_c(const scaph_ms_tune& t) {
   o.pull_block = t.pull_block;
   o.use_graph = t.use_graph;
   o.segment = t.segment ? t.segment : 8;
+  o.pull_grid = t.pull_grid;
   return o;
 }
 scaph_ms_tune to_c(const scaph::MsTune& t) {
@@ -272,6 +273,7 @@ scaph_ms_tune to_c(const scaph::MsTune& t) {
   o.pull_block = t.pull_block;
   o.use_graph = t.use_graph;
   o.segment = t.segment;
+  o.pull_grid = t.pull_grid;
   return o;
 }
 }  // namespace

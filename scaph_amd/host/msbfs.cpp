@@ -74,6 +74,7 @@ MsStats MultiSource::run(const uint32_t* sources, uint32_t count, const MsTune& 
   p.push_div = t.push_div;
   p.push_mdiv = t.push_mdiv;
   p.pull_block = t.pull_block;
+  p.pull_grid = t.pull_grid;
   p.m = e_.num_edges();
   void* st = e_.stream();
   std::memcpy(hsrc_.get(), sources, (size_t)count * 4);
@@ -151,16 +152,22 @@ MsTune MultiSource::autotune(const uint32_t* sources, uint32_t count,
                              std::vector<std::pair<MsTune, double>>* log) {
   std::vector<MsTune> cand;
   const uint32_t thr[][2] = {{64, 16}, {256, 64}, {16, 8}};
+  scaph_device_props props;
+  const uint32_t cus =
+      scaph_get_device_props(0, &props) == 0 && props.compute_units > 0 ? props.compute_units : 256;
   for (uint32_t pb : {256u, 512u, 1024u})
     for (const auto& th : thr)
-      for (uint32_t g : {0u, 1u}) {
-        MsTune t;
-        t.pull_block = pb;
-        t.push_div = th[0];
-        t.push_mdiv = th[1];
-        t.use_graph = g;
-        cand.push_back(t);
-      }
+      for (uint32_t g : {0u, 1u})
+        // pull grid: one block per vertex block, or 1024 threads per CU resident
+        for (uint32_t pg : {0u, 4u * cus * 256u / pb}) {
+          MsTune t;
+          t.pull_block = pb;
+          t.push_div = th[0];
+          t.push_mdiv = th[1];
+          t.use_graph = g;
+          t.pull_grid = pg;
+          cand.push_back(t);
+        }
   MsTune best;
   double best_ms = 1e300;
   for (const MsTune& t : cand) {

@@ -42,7 +42,8 @@ def _check_all(e, off, ngh, src, label):
 @pytest.mark.parametrize("tune", [
     None, PUSH_ALWAYS, PULL_ALWAYS,
     {"pull_block": 1024, "use_graph": 1, "segment": 3},
-    {"pull_block": 512, "use_graph": 1, "segment": 8, "push_div": 16, "push_mdiv": 8}])
+    {"pull_block": 512, "use_graph": 1, "segment": 8, "push_div": 16, "push_mdiv": 8},
+    {"pull_block": 256, "pull_grid": 8}])
 def test_msbfs_256_queries_vs_oracle(tune):
     off, ngh = O.rmat_csr(14, 16, seed=21)
     src = _sources(off, 256, 5)
