@@ -143,7 +143,9 @@ size_t scaph_push_workspace_bytes(uint32_t max_entries, unsigned long long max_e
 typedef struct scaph_push_tuning {
   uint32_t grid_blocks; /* cap on expand blocks (one 1024-edge tile per block per pass) */
   uint32_t edge_rows;   /* 1: a wave reads 256 consecutive edges per step (coalesced rows) */
-  uint32_t reserved[2];
+  uint32_t small_edges; /* frontiers with at most this many out-edges run in one launch
+                           (0: 8192; 1: effectively never) */
+  uint32_t reserved;
 } scaph_push_tuning;
 /* Top-down (push) expand of one level from a frontier queue — the resident
  * counterpart of gpu_push_csr (src/gpu_engine.cuh:41-169) with the BFS

@@ -22,6 +22,7 @@ namespace {
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kTile = 1024;                   // edges per expand tile
 constexpr uint32_t kPerThread = kTile / kBlock;    // 4 consecutive edges
+constexpr unsigned long long kSmallEdges = 8192;  // push frontiers this small run in one launch
 constexpr uint32_t kPullSerial = 16;               // per-lane scan before wave help
 constexpr uint32_t kHeavy = 512;                   // longer in-lists go to k_pull_heavy
 constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work item
@@ -140,6 +141,61 @@ k_visited_diff(const uint32_t* visited, const uint32_t* snapshot, uint32_t* fron
     f4[i] = make_uint4(a.x & ~b.x, a.y & ~b.y, a.z & ~b.z, a.w & ~b.w);
   }
   for (size_t w = n4 * 4 + tid; w < words; w += stride) front[w] = visited[w] & ~snapshot[w];
+}
+
+// A small frontier in one launch: a wave per frontier vertex, its lanes
+// striding the out-edges; claims as in k_expand, list appends aggregated
+// per wave.  Used when the frontier's out-edges are few (the four-kernel
+// tiled expand costs more in launches than in work there).
+__global__ void __launch_bounds__(kBlock)
+k_push_small(scaph_graph_desc g, const uint32_t* q_in, uint32_t n_in, const uint32_t* degree,
+             uint32_t* level, uint32_t* visited, uint32_t* q_out, scaph_level_counters* counters,
+             uint32_t next_level) {
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  const PageTab t = stage_pages(g, dyn_lds);
+  __syncthreads();
+  const uint32_t n = g.num_vertices;
+  const uint32_t lane = lane_id();
+  const uint32_t nw = blockDim.x / SCAPH_WAVE;
+  unsigned long long acc_n = 0, acc_m = 0, acc_s = 0;
+  for (uint32_t i = blockIdx.x * nw + threadIdx.x / SCAPH_WAVE; i < n_in; i += gridDim.x * nw) {
+    const uint32_t u = q_in[i];
+    if (u >= n) continue;
+    const NodeRef nr = load_node(t, g, u);
+    for (uint32_t k0 = 0; k0 < nr.len; k0 += SCAPH_WAVE) {
+      const uint32_t k = k0 + lane;
+      bool claimed = false;
+      uint32_t v = 0;
+      if (k < nr.len) {
+        ++acc_s;
+        v = nr.edges[k];
+        if (v < n) {
+          const uint32_t w = v >> 5, bit = 1u << (v & 31);
+          if (!(visited[w] & bit)) claimed = !(atomicOr(&visited[w], bit) & bit);
+        }
+      }
+      const uint64_t m = __ballot(claimed);
+      if (m) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&counters->q_len, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0, SCAPH_WAVE);
+        if (claimed) {
+          level[v] = next_level;
+          q_out[base + lanes_below(m)] = v;
+          acc_n += 1;
+          acc_m += degree[v];
+        }
+      }
+    }
+  }
+  acc_n = wave_sum(acc_n);
+  acc_m = wave_sum(acc_m);
+  acc_s = wave_sum(acc_s);
+  if (lane == 0) {
+    if (acc_n) atomicAdd(&counters->n_next, acc_n);
+    if (acc_m) atomicAdd(&counters->m_next, acc_m);
+    if (acc_s) atomicAdd(&counters->scanned, acc_s);
+  }
 }
 
 // ------------------------------------------------------------ push: prep
@@ -976,9 +1032,18 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree, ui
     return SCAPH_STATUS_INVALID;
   if (n_in == 0) return 0;
   if (!q_in) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
+  if (m_in <= small && n_in <= 65536) {
+    const unsigned grid = grid_for(n_in, kBlock / SCAPH_WAVE, 64);
+    hipLaunchKernelGGL(k_push_small, dim3(grid), dim3(kBlock), page_lds_bytes(out_graph->num_pages),
+                       st, *out_graph, q_in, n_in, degree, level, visited, q_out, counters,
+                       cur_level + 1);
+    SCAPH_LAUNCH_CHECK();
+    return 0;
+  }
   PushWs w = carve_push_ws(workspace, n_in, m_in);
   if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
-  hipStream_t st = as_stream(stream);
   const uint32_t nb = (n_in + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_prep_queue, dim3(nb), dim3(kBlock), page_lds_bytes(out_graph->num_pages), st,
                      *out_graph, q_in, n_in,

@@ -60,6 +60,8 @@ Config& global_flags() {
     r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
     r.add("push_grid", FlagType::kInt32, &c.push_grid, "cap on push expand blocks (0 = default)");
     r.add("push_rows", FlagType::kBool, &c.push_rows, "push expand reads coalesced edge rows");
+    r.add("push_small", FlagType::kInt32, &c.push_small,
+          "out-edges below which a push level runs as one launch (0 = 8192, 1 = never)");
     r.add("pull_head", FlagType::kBool, &c.pull_head,
           "keep each in-list's first entries beside its in-degree for the pull probes");
     r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");
@@ -122,6 +124,7 @@ Engine::Engine(const Config& cfg)
   pull_tune_.grid_blocks = (uint32_t)std::max(0, cfg.pull_grid);
   push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
+  push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);
 }
 
 Engine::~Engine() = default;

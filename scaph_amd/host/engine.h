@@ -49,6 +49,7 @@ struct Config {
   int32_t pull_grid = 0;         // cap on pull blocks (0: four per compute unit)
   int32_t push_grid = 0;         // cap on push expand blocks (0: one per tile, at most 8192)
   bool push_rows = true;         // push expand reads coalesced 256-edge rows
+  int32_t push_small = 0;        // out-edges below which a push level is one launch (0: 8192)
   int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
 };
 

@@ -18,6 +18,9 @@ constexpr size_t kDecideFloats = 32;  // logit, threshold, pages active, pad, hi
 constexpr size_t kDecideWord = 64;
 // Word of the hand-off flag in the mapped hand-off buffer.
 constexpr size_t kHandFlagWord = 256;
+// Push levels whose frontier has more out-edges than this snapshot visited
+// first, so that a pull next can take the frontier as visited & ~snapshot.
+constexpr uint64_t kSnapEdges = 1u << 22;
 
 size_t bitmap_words(uint32_t n) { return (((size_t)n + 63) / 64) * 2; }
 
@@ -55,7 +58,8 @@ void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m
   if (d == Dir::kPush) {
     // with a pull step possible next, keep visited as it was before this
     // push: the next frontier's bitmap is then visited & ~snapshot
-    const bool snap = vsnap_.get() != nullptr;
+    // (small frontiers skip it: their list converts cheaply with atomics)
+    const bool snap = vsnap_.get() != nullptr && m_f > kSnapEdges;
     if (snap) H_ERR(scaph_copy_d2d_async(vsnap_.get(), visited, vsnap_.bytes(), st));
     H_ERR(scaph_bfs_push(&g_out_, deg, level, visited, queue_[f.qa].get<uint32_t>(), (uint32_t)n_f,
                          m_f, queue_[f.qa ^ 1].get<uint32_t>(), cnt, cur, ws_.get(), ws_bytes_,
@@ -169,8 +173,8 @@ RunStats Engine::bfs_resident(uint32_t src) {
     else d = sched_ ? sched_(ss) : beamer(ss);
     if (d == Dir::kPull && !has_in_) d = Dir::kPush;
     ensure_forms(f, d == Dir::kPush, d == Dir::kPull, n_f);
-    // later levels find the counters cleared by the previous hand-off
-    if (cur == 0) H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, kCounterWords, st));
+    // the counters start cleared: by scaph_bfs_init for level 0, by the
+    // previous level's hand-off after that
     const bool timed = rs.levels.size() < kMaxTimedLevels;
     const size_t eb = 2 + 3 * rs.levels.size();
     if (timed) {
