@@ -177,7 +177,8 @@ size_t scaph_pull_workspace_bytes(unsigned long long max_edges);
 /* Launch shape of the pull expand (NULL or 0 fields = built-in choice). */
 typedef struct scaph_pull_tuning {
   uint32_t grid_blocks; /* cap on 256-thread blocks (each owns 4 x 1024-vertex windows per pass) */
-  uint32_t reserved[3];
+  uint32_t head_outdeg; /* in_head records are {e0, e1, in-degree, out-degree} */
+  uint32_t reserved[2];
 } scaph_pull_tuning;
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
                    const uint32_t* hub_list, uint32_t num_hubs,
@@ -248,9 +249,11 @@ int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t
                       uint64_t seed, void* stream);
 
 /* head[4v..4v+3] = the first three entries of v's list (0xFFFFFFFF past
- * its end) and its length, from a flat device CSR. */
+ * its end) and its length, from a flat device CSR; with out_degree (u32[n])
+ * the record is {first two entries, length, out_degree[v]} instead
+ * (scaph_pull_tuning.head_outdeg). */
 int scaph_list_heads_from_csr(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
-                              uint32_t* head, void* stream);
+                              const uint32_t* out_degree, uint32_t* head, void* stream);
 /* offsets[0..n] = exclusive prefix of degree[0..n) (offsets[n] = total);
  * allocates its own scan storage and returns after the stream drains. */
 int scaph_degrees_to_offsets(const uint32_t* degree, uint64_t* offsets, uint32_t n, void* stream);

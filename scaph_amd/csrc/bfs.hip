@@ -84,7 +84,13 @@ k_bfs_init(uint32_t* level, uint32_t* visited, const uint32_t* static_visited, u
            size_t words) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (size_t v = tid; v < n; v += stride) level[v] = SCAPH_UNREACHED;
+  // 16-byte stores for the aligned bulk of the level array
+  const bool al = (reinterpret_cast<uintptr_t>(level) & 15) == 0;
+  const size_t n4 = al ? n / 4 : 0;
+  uint4* l4 = reinterpret_cast<uint4*>(level);
+  const uint4 u4 = make_uint4(SCAPH_UNREACHED, SCAPH_UNREACHED, SCAPH_UNREACHED, SCAPH_UNREACHED);
+  for (size_t i = tid; i < n4; i += stride) l4[i] = u4;
+  for (size_t v = n4 * 4 + tid; v < n; v += stride) level[v] = SCAPH_UNREACHED;
   for (size_t w = tid; w < words; w += stride) {
     uint32_t x = static_visited ? static_visited[w] : 0u;
     // bits of vertices >= n never become active
@@ -558,7 +564,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* degree, uint32_t* level,
+k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32_t* degree,
+       uint32_t* level,
        uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
        scaph_level_counters* counters, uint32_t next_level, PullWs hw, uint32_t num_hubs) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
@@ -605,6 +612,7 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* degree, uint32_t* 
       NodeRef nr;
       nr.edges = nullptr;
       nr.len = 0;
+      uint32_t outdeg = 0xFFFFFFFFu;
       if (act) {
         ++acc_x;
         uint32_t k = 0;
@@ -612,15 +620,20 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* degree, uint32_t* 
           // the first three in-list entries and the in-degree sit in a
           // 16-byte record read in vertex order: a vertex found there (the
           // lists start with their hubs) never touches its page
+          // (head_outdeg: {e0, e1, in-degree, out-degree} instead, so a
+          // found vertex also needs no degree[] load)
           const uint4 hd = head[v];
-          nr.len = hd.w;
-          const uint32_t l0 = hd.w < 3u ? hd.w : 3u;
+          const uint32_t cap = head_outdeg ? 2u : 3u;
+          const uint32_t len = head_outdeg ? hd.z : hd.w;
+          if (head_outdeg) outdeg = hd.w;
+          nr.len = len;
+          const uint32_t l0 = len < cap ? len : cap;
           const bool h0 = l0 > 0 && in_front(hd.x, s_hub, front_in);
           const bool h1 = l0 > 1 && in_front(hd.y, s_hub, front_in);
           const bool h2 = l0 > 2 && in_front(hd.z, s_hub, front_in);
           k = l0;
           found = h0 | h1 | h2;
-          if (!found && hd.w > 3u) {
+          if (!found && len > cap) {
             nr = load_node_hint(s, g, v, hint);
           }
         } else {
@@ -687,7 +700,7 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* degree, uint32_t* 
       if (found) {
         level[v] = next_level;
         acc_n += 1;
-        acc_m += degree[v];
+        acc_m += outdeg != 0xFFFFFFFFu ? outdeg : degree[v];
         atomicOr(&fw[loc >> 5], 1u << (loc & 31));
       }
     }
@@ -1000,7 +1013,7 @@ int scaph_bfs_init(uint32_t* level, uint32_t* visited, const uint32_t* static_vi
   if (!level || !visited || n == 0 || src >= n) return SCAPH_STATUS_INVALID;
   const size_t words = (((size_t)n + 63) / 64) * 2;
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(n, kBlock * 4, 16384)), dim3(kBlock), 0, st, level,
+  hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(n, kBlock * 16, 16384)), dim3(kBlock), 0, st, level,
                      visited, static_visited, n, words);
   SCAPH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_bfs_set_source, dim3(1), dim3(1), 0, st, level, visited, src, queue,
@@ -1103,7 +1116,8 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
   unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 4u * cu_count());
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
   hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), page_lds_bytes(in_graph->num_pages), st,
-                     *in_graph, reinterpret_cast<const uint4*>(in_head), degree, level, visited,
+                     *in_graph, reinterpret_cast<const uint4*>(in_head),
+                     tune ? tune->head_outdeg : 0u, degree, level, visited,
                      front_in, front_out, counters, cur_level + 1, hw, num_hubs);
   SCAPH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), 0, st, hw.chunks, hw.count, degree,

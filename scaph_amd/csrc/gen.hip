@@ -52,9 +52,11 @@ k_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t count, uint32_t n, uint64_t
 }
 
 // Per vertex: the first three entries of its list (0xFFFFFFFF past the end)
-// and its length, as one 16-byte record.
+// and its length, as one 16-byte record; with out_degree, the first two
+// entries, the length and the vertex's out-degree.
 __global__ void __launch_bounds__(kBlock)
-k_list_heads(const uint64_t* offsets, const uint32_t* ngh, uint32_t n, uint4* head) {
+k_list_heads(const uint64_t* offsets, const uint32_t* ngh, uint32_t n, const uint32_t* out_degree,
+             uint4* head) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
     const uint64_t o = offsets[v];
@@ -62,8 +64,13 @@ k_list_heads(const uint64_t* offsets, const uint32_t* ngh, uint32_t n, uint4* he
     uint4 r;
     r.x = d > 0 ? ngh[o] : 0xFFFFFFFFu;
     r.y = d > 1 ? ngh[o + 1] : 0xFFFFFFFFu;
-    r.z = d > 2 ? ngh[o + 2] : 0xFFFFFFFFu;
-    r.w = d;
+    if (out_degree) {
+      r.z = d;
+      r.w = out_degree[v];
+    } else {
+      r.z = d > 2 ? ngh[o + 2] : 0xFFFFFFFFu;
+      r.w = d;
+    }
     head[v] = r;
   }
 }
@@ -221,10 +228,11 @@ int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t
 }
 
 int scaph_list_heads_from_csr(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
-                              uint32_t* head, void* stream) {
+                              const uint32_t* out_degree, uint32_t* head, void* stream) {
   if (!offsets || !head || n == 0) return SCAPH_STATUS_INVALID;
   hipLaunchKernelGGL(k_list_heads, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0,
-                     as_stream(stream), offsets, ngh, n, reinterpret_cast<uint4*>(head));
+                     as_stream(stream), offsets, ngh, n, out_degree,
+                     reinterpret_cast<uint4*>(head));
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

@@ -58,6 +58,8 @@ Config& global_flags() {
     r.add("trace", FlagType::kString, &c.trace, "per-iteration JSONL trace file");
     r.add("build_pull", FlagType::kBool, &c.build_pull, "keep transposed pages for pull");
     r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
+    r.add("pull_head_outdeg", FlagType::kBool, &c.pull_head_outdeg,
+          "head records carry the out-degree instead of a third in-list entry");
     r.add("push_grid", FlagType::kInt32, &c.push_grid, "cap on push expand blocks (0 = default)");
     r.add("push_rows", FlagType::kBool, &c.push_rows, "push expand reads coalesced edge rows");
     r.add("push_small", FlagType::kInt32, &c.push_small,
@@ -122,6 +124,7 @@ Engine::Engine(const Config& cfg)
   hand_.alloc(4096);
   std::memset(hand_.host(), 0, hand_.bytes());
   pull_tune_.grid_blocks = (uint32_t)std::max(0, cfg.pull_grid);
+  pull_tune_.head_outdeg = cfg.pull_head_outdeg ? 1u : 0u;
   push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
   push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);
@@ -294,6 +297,10 @@ void Engine::load_csr(HostCsr g, bool keep_host) {
           const uint32_t d = t.degree(v);
           for (uint32_t j = 0; j < 3; ++j) head[4ull * v + j] = j < d ? t.ngh[o + j] : 0xFFFFFFFFu;
           head[4ull * v + 3] = d;
+          if (cfg_.pull_head_outdeg) {
+            head[4ull * v + 2] = d;
+            head[4ull * v + 3] = h_deg_out_[v];
+          }
         }
         in_head_.alloc(std::max<size_t>(head.size(), 4) * 4);
         H_ERR(scaph_copy_h2d_async(in_head_.get(), head.data(), head.size() * 4, main_->get()));
@@ -503,6 +510,7 @@ void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
     if (dir == 1 && cfg_.pull_head) {
       in_head_.alloc((size_t)n * 16);
       H_ERR(scaph_list_heads_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
+                                      cfg_.pull_head_outdeg ? deg_out_.get<uint32_t>() : nullptr,
                                       in_head_.get<uint32_t>(), st));
     }
     main_->sync();

@@ -72,20 +72,21 @@ def test_rmat_parity_with_oracle(scale, ef, seed, page_words):
             assert e.check(src) == 0
 
 
-@pytest.mark.parametrize("head", ["true", "false"])
+@pytest.mark.parametrize("head", [["-pull_head=true"], ["-pull_head=false"],
+                                  ["-pull_head=true", "-pull_head_outdeg=true"]])
 def test_pull_head_records(head):
     """The pull expand with and without the 16-byte in-list head records,
     on the host-loaded and the device-built graph."""
     off, ngh = O.rmat_csr(15, 16, seed=8)
     deg = np.diff(off.astype(np.int64))
     srcs = [int(np.argmax(deg)), int(np.nonzero(deg)[0][3])]
-    with engine(f"-pull_head={head}") as e:
+    with engine(*head) as e:
         e.load_csr(off, ngh)
         for src in srcs:
             e.set_schedule("pull")
             e.run(src)
             np.testing.assert_array_equal(e.levels(), O.bfs(off, ngh, src))
-    with engine(f"-pull_head={head}") as e:
+    with engine(*head) as e:
         e.gen_rmat_device(15, 16, seed=8)
         for src in srcs:
             for sched in ("pull", "fixed"):
