@@ -178,7 +178,8 @@ size_t scaph_pull_workspace_bytes(unsigned long long max_edges);
 typedef struct scaph_pull_tuning {
   uint32_t grid_blocks; /* cap on 256-thread blocks (each owns 4 x 1024-vertex windows per pass) */
   uint32_t head_outdeg; /* in_head records are {e0, e1, in-degree, out-degree} */
-  uint32_t reserved[2];
+  uint32_t no_prefetch; /* do not request the next batch's head records early */
+  uint32_t reserved;
 } scaph_pull_tuning;
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
                    const uint32_t* hub_list, uint32_t num_hubs,

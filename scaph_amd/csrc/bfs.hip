@@ -563,6 +563,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+template <int PREFETCH>
 __global__ void __launch_bounds__(kBlock)
 k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32_t* degree,
        uint32_t* level,
@@ -603,11 +604,18 @@ k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32
     if (lane < kPullWinWords) fw[lane] = 0u;
     wave_lds_sync();
     const uint32_t hint = tot ? find_page(s, g, (uint32_t)(w0 * 32)) : 0u;
+    // the head records of the next batch are requested before this batch
+    // is probed (one extra load in flight per lane)
+    uint4 hd_next = make_uint4(0u, 0u, 0u, 0u);
+    if (PREFETCH && head && lane < tot) hd_next = head[(uint32_t)(w0 * 32) + q[lane]];
     for (uint32_t i0 = 0; i0 < tot; i0 += SCAPH_WAVE) {
       const uint32_t i = i0 + lane;
       const bool act = i < tot;
       const uint32_t loc = act ? q[i] : 0u;
       const uint32_t v = (uint32_t)(w0 * 32) + loc;
+      const uint4 hd_cur = hd_next;
+      if (PREFETCH && head && i + SCAPH_WAVE < tot)
+        hd_next = head[(uint32_t)(w0 * 32) + q[i + SCAPH_WAVE]];
       bool found = false, longl = false;
       NodeRef nr;
       nr.edges = nullptr;
@@ -622,7 +630,7 @@ k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32
           // lists start with their hubs) never touches its page
           // (head_outdeg: {e0, e1, in-degree, out-degree} instead, so a
           // found vertex also needs no degree[] load)
-          const uint4 hd = head[v];
+          const uint4 hd = PREFETCH ? hd_cur : head[v];
           const uint32_t cap = head_outdeg ? 2u : 3u;
           const uint32_t len = head_outdeg ? hd.z : hd.w;
           if (head_outdeg) outdeg = hd.w;
@@ -1115,10 +1123,17 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
   // tail of late blocks; measured best on s26 among 512..4096 blocks)
   unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 4u * cu_count());
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
-  hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), page_lds_bytes(in_graph->num_pages), st,
-                     *in_graph, reinterpret_cast<const uint4*>(in_head),
-                     tune ? tune->head_outdeg : 0u, degree, level, visited,
-                     front_in, front_out, counters, cur_level + 1, hw, num_hubs);
+  const uint32_t outdeg_heads = tune ? tune->head_outdeg : 0u;
+  const size_t lds = page_lds_bytes(in_graph->num_pages);
+  const uint4* heads = reinterpret_cast<const uint4*>(in_head);
+  if (!tune || !tune->no_prefetch)
+    hipLaunchKernelGGL(k_pull<1>, dim3(grid), dim3(kBlock), lds, st, *in_graph, heads, outdeg_heads,
+                       degree, level, visited, front_in, front_out, counters, cur_level + 1, hw,
+                       num_hubs);
+  else
+    hipLaunchKernelGGL(k_pull<0>, dim3(grid), dim3(kBlock), lds, st, *in_graph, heads, outdeg_heads,
+                       degree, level, visited, front_in, front_out, counters, cur_level + 1, hw,
+                       num_hubs);
   SCAPH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), 0, st, hw.chunks, hw.count, degree,
                      level, visited, front_in, front_out, counters, cur_level + 1, hw.hub_front,

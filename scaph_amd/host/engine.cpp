@@ -58,6 +58,8 @@ Config& global_flags() {
     r.add("trace", FlagType::kString, &c.trace, "per-iteration JSONL trace file");
     r.add("build_pull", FlagType::kBool, &c.build_pull, "keep transposed pages for pull");
     r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
+    r.add("pull_prefetch", FlagType::kBool, &c.pull_prefetch,
+          "request the next batch's head records before probing the current one");
     r.add("pull_head_outdeg", FlagType::kBool, &c.pull_head_outdeg,
           "head records carry the out-degree instead of a third in-list entry");
     r.add("push_grid", FlagType::kInt32, &c.push_grid, "cap on push expand blocks (0 = default)");
@@ -125,6 +127,7 @@ Engine::Engine(const Config& cfg)
   std::memset(hand_.host(), 0, hand_.bytes());
   pull_tune_.grid_blocks = (uint32_t)std::max(0, cfg.pull_grid);
   pull_tune_.head_outdeg = cfg.pull_head_outdeg ? 1u : 0u;
+  pull_tune_.no_prefetch = cfg.pull_prefetch ? 0u : 1u;
   push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
   push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);

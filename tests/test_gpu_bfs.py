@@ -259,7 +259,8 @@ def test_pull_heavy_in_lists():
 
 
 @pytest.mark.parametrize("flags", [["-push_rows=false"], ["-push_rows=true", "-push_grid=64"],
-                                   ["-pull_grid=16"], ["-push_small=1"], ["-push_small=1000000"]])
+                                   ["-pull_grid=16"], ["-push_small=1"], ["-push_small=1000000"],
+                                   ["-pull_prefetch=false"]])
 def test_launch_shape_variants(flags):
     """The push/pull launch-shape switches change timing only."""
     off, ngh = O.rmat_csr(16, 16, seed=4)
