@@ -586,11 +586,24 @@ k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32
   unsigned long long acc_n = 0, acc_m = 0, acc_s = 0, acc_x = 0;
   uint32_t* q = s_q[wid];
   uint32_t* fw = s_fw[wid];
+  // visited word of this lane in the wave's window of iteration bi
+  auto visited_word = [&](uint64_t bi) -> uint32_t {
+    const uint64_t ww = bi * per_block + (uint64_t)wid * kPullWinWords + lane;
+    return (lane < kPullWinWords && ww < words) ? visited[ww] : ~0u;
+  };
+  uint32_t vw_next = PREFETCH && blockIdx.x < iters ? visited_word(blockIdx.x) : 0u;
   for (uint64_t bi = blockIdx.x; bi < iters; bi += gridDim.x) {
     const uint64_t w0 = bi * per_block + (uint64_t)wid * kPullWinWords;
     const uint64_t w = w0 + lane;
     const bool own = lane < kPullWinWords && w < words;
-    const uint32_t vw = own ? visited[w] : ~0u;
+    uint32_t vw;
+    if (PREFETCH) {
+      // this window's word was requested one window earlier; request the next
+      vw = vw_next;
+      if (bi + gridDim.x < iters) vw_next = visited_word(bi + gridDim.x);
+    } else {
+      vw = own ? visited[w] : ~0u;
+    }
     uint32_t unv = ~vw;
     const uint32_t c = __popc(unv);
     const uint32_t incl = wave_incl_scan(c);
