@@ -210,18 +210,36 @@ int scaph_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_
                       const uint32_t* degree, int weighted, uint32_t* nodes, uint32_t* datas,
                       unsigned long long* totals, void* stream);
 
+/* What scaph_page_counts already says about one page push (optional).
+ * first_vertex: Node i of a full page is vertex first_vertex + i (the
+ * page's left); SCAPH_PAGE_COMPACTED for a compacted buffer, whose Nodes
+ * are all active.  active_edges: datas[p] - 3 * nodes[p] for the same
+ * frontier, i.e. the exact out-edge count of the active Nodes — it sizes
+ * the launch, and an under-count truncates the level. */
+#define SCAPH_PAGE_COMPACTED 0xFFFFFFFFu
+typedef struct scaph_page_hint {
+  uint32_t first_vertex;
+  uint32_t active_nodes;
+  unsigned long long active_edges;
+} scaph_page_hint;
+
 /* Push expand of ONE page buffer (full page, or a compacted cold page as
  * written by compress_stat, src/compress.h:18-40) — gpu_push_csr's unit of
  * work (src/app.cuh:438).  A Node is active when the frontier bitmap has
  * its vtx bit.  Newly reached vertices get level/visited and a bit in
  * front_out (atomic OR; front_out must be cleared before the level).
  * edgenum bounds the page's edges (sizes the launch), n = vertex count.
+ * With a hint, a full page is read through its frontier bits (only active
+ * Nodes are loaded), the launch is sized by active_edges, and a page with
+ * at most tune->small_edges (default 8192) active out-edges runs as one
+ * launch.  tune: NULL for the built-in launch shape.
  * Counters accumulate across calls: clear them at the start of a level. */
 int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned long long edgenum,
                         uint32_t n, const uint32_t* degree, uint32_t* level, uint32_t* visited,
                         const uint32_t* front_in, uint32_t* front_out,
                         scaph_level_counters* counters, uint32_t cur_level, void* workspace,
-                        size_t workspace_bytes, void* stream);
+                        size_t workspace_bytes, const scaph_page_hint* hint,
+                        const scaph_push_tuning* tune, void* stream);
 
 /* Level-array check kernel: counts edges (u,v) violating
  * level[v] <= level[u] + 1 and reached vertices without a parent at

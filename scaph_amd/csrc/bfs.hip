@@ -15,6 +15,8 @@
 // (block-local compaction + scan of the out-degrees), cut into tiles of
 // kTile edges, and every 256-thread block expands whole tiles — a hub with a
 // million out-edges is spread over ~1000 blocks instead of one CTA.
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
@@ -29,12 +31,14 @@ constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work it
 constexpr uint32_t kMaxHubs = 32768;               // hub frontier bits kept in LDS (4 KB)
 constexpr uint32_t kHubTag = 0x80000000u;          // in-list entry = kHubTag | hub index
 constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kBitsNodes = kBlock * 32;       // Nodes per bitmap-driven page prep block
+constexpr uint32_t kSmallChunk = SCAPH_WAVE * 32;  // Nodes per wave step of k_push_page_small
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct PushWs {
-  uint64_t* ent_ep;     // [nb*256] edge-list address of each kept entry (block-compacted)
-  uint32_t* ent_len;    // [nb*256]
+  uint64_t* ent_ep;     // edge-list address of each kept entry (block-compacted)
+  uint32_t* ent_len;    // its length
   uint32_t* blk_cnt;    // [nb]
   uint64_t* blk_edges;  // [nb]
   uint32_t* blk_coff;   // [nb] exclusive entry offset
@@ -51,6 +55,9 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
   const size_t nb = (max_entries + kBlock - 1) / kBlock + 1;
   const size_t m = (size_t)max_entries + 1;
   const size_t tiles = (size_t)((max_edges + kTile - 1) / kTile) + 2;
+  // entry slots: kBlock per queue-prep block, kBitsNodes per bitmap-prep block
+  const size_t ent = std::max<size_t>(nb * kBlock,
+                                      ((size_t)max_entries + kBitsNodes - 1) / kBitsNodes * kBitsNodes);
   size_t off = 0;
   char* p = static_cast<char*>(base);
   auto take = [&](size_t bytes) {
@@ -58,8 +65,8 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
     off += align256(bytes);
     return r;
   };
-  w.ent_ep = reinterpret_cast<uint64_t*>(take(nb * kBlock * 8));
-  w.ent_len = reinterpret_cast<uint32_t*>(take(nb * kBlock * 4));
+  w.ent_ep = reinterpret_cast<uint64_t*>(take(ent * 8));
+  w.ent_len = reinterpret_cast<uint32_t*>(take(ent * 4));
   w.blk_cnt = reinterpret_cast<uint32_t*>(take(nb * 4));
   w.blk_edges = reinterpret_cast<uint64_t*>(take(nb * 8));
   w.blk_coff = reinterpret_cast<uint32_t*>(take(nb * 4));
@@ -204,6 +211,73 @@ k_push_small(scaph_graph_desc g, const uint32_t* q_in, uint32_t n_in, const uint
   }
 }
 
+// Frontier bits of vertices [v, v + c), c in 1..32: bit k is vertex v + k.
+__device__ __forceinline__ uint32_t bits_at(const uint32_t* bm, uint32_t v, uint32_t c) {
+  const uint32_t w = v >> 5, sh = v & 31;
+  uint32_t b = bm[w] >> sh;
+  if (sh && c > 32 - sh) b |= bm[w + 1] << (32 - sh);
+  return c >= 32 ? b : (b & ((1u << c) - 1u));
+}
+
+// One-launch push of a page whose active Nodes have few out-edges.  A full
+// page (Node i = vertex first + i) is walked kSmallChunk Nodes per wave
+// step: each lane takes 32 frontier bits, and the wave visits the active
+// Nodes one after another, its lanes covering the Node's out-list.  A
+// compacted buffer (first == SCAPH_PAGE_COMPACTED) holds only active Nodes:
+// one per wave step.
+__global__ void __launch_bounds__(kBlock)
+k_push_page_small(const uint32_t* buf, uint32_t nodenum, uint32_t first, const uint32_t* front,
+                  const uint32_t* degree, uint32_t* level, uint32_t* visited, uint32_t* front_out,
+                  scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
+  const uint32_t lane = lane_id();
+  const uint32_t nw = kBlock / SCAPH_WAVE;
+  const uint32_t gw = blockIdx.x * nw + threadIdx.x / SCAPH_WAVE;
+  const uint32_t waves = gridDim.x * nw;
+  const uint32_t* eb = buf + 3ull * nodenum;
+  unsigned long long acc_n = 0, acc_m = 0, acc_s = 0;
+  auto expand = [&](uint32_t i) {
+    const uint32_t idx = buf[3ull * i + 1], len = buf[3ull * i + 2];
+    for (uint32_t k0 = 0; k0 < len; k0 += SCAPH_WAVE) {
+      const uint32_t k = k0 + lane;
+      if (k < len) {
+        ++acc_s;
+        const uint32_t v = eb[idx + k];
+        if (v < n) {
+          const uint32_t w = v >> 5, bit = 1u << (v & 31);
+          if (!(visited[w] & bit) && !(atomicOr(&visited[w], bit) & bit)) {
+            level[v] = next_level;
+            acc_n += 1;
+            acc_m += degree[v];
+            atomicOr(&front_out[w], bit);
+          }
+        }
+      }
+    }
+  };
+  if (first == SCAPH_PAGE_COMPACTED) {
+    for (uint32_t i = gw; i < nodenum; i += waves) expand(i);
+  } else {
+    const uint32_t chunks = (nodenum + kSmallChunk - 1) / kSmallChunk;
+    for (uint32_t c = gw; c < chunks; c += waves) {
+      const uint32_t i0 = c * kSmallChunk + lane * 32;
+      const uint32_t bits = i0 < nodenum ? bits_at(front, first + i0, min(32u, nodenum - i0)) : 0u;
+      for (uint64_t any = __ballot(bits != 0); any; any &= any - 1) {
+        const uint32_t src = (uint32_t)__ffsll((unsigned long long)any) - 1;
+        const uint32_t sb = __shfl(bits, (int)src, SCAPH_WAVE);
+        for (uint32_t m = sb; m; m &= m - 1) expand(c * kSmallChunk + src * 32 + __ffs(m) - 1);
+      }
+    }
+  }
+  acc_n = wave_sum(acc_n);
+  acc_m = wave_sum(acc_m);
+  acc_s = wave_sum(acc_s);
+  if (lane == 0) {
+    if (acc_n) atomicAdd(&counters->n_next, acc_n);
+    if (acc_m) atomicAdd(&counters->m_next, acc_m);
+    if (acc_s) atomicAdd(&counters->scanned, acc_s);
+  }
+}
+
 // ------------------------------------------------------------ push: prep
 
 // Block-local compaction of the entries with len > 0.
@@ -277,6 +351,48 @@ k_prep_page(const uint32_t* buf, uint32_t nodenum, const uint32_t* front, uint64
   prep_store(keep, ep, keep ? len : 0u, ent_ep, ent_len, blk_cnt, blk_edges);
 }
 
+// Prep of a full page (Node i = vertex first + i) driven by the frontier
+// bitmap: each thread takes 32 Nodes through one funnel-shifted bitmap
+// word and loads only the records of active Nodes, so a sparse frontier
+// costs nodenum/8 bytes of bitmap instead of 12 bytes per Node.  Block b
+// keeps its entries (non-empty lists) at ent[b * kBitsNodes ...].
+__global__ void __launch_bounds__(kBlock)
+k_prep_page_bits(const uint32_t* buf, uint32_t nodenum, uint32_t first, const uint32_t* front,
+                 uint64_t* ent_ep, uint32_t* ent_len, uint32_t* blk_cnt, uint64_t* blk_edges) {
+  __shared__ uint32_t wc[kBlock / SCAPH_WAVE];
+  __shared__ unsigned long long we[kBlock / SCAPH_WAVE];
+  const uint32_t i0 = blockIdx.x * kBitsNodes + threadIdx.x * 32;
+  uint32_t bits = 0;
+  if (i0 < nodenum) bits = bits_at(front, first + i0, min(32u, nodenum - i0));
+  uint32_t keep = 0, cnt = 0;
+  unsigned long long edges = 0;
+  for (uint32_t m = bits; m; m &= m - 1) {
+    const uint32_t k = __ffs(m) - 1;
+    const uint32_t len = buf[3ull * (i0 + k) + 2];
+    if (len) {
+      keep |= 1u << k;
+      ++cnt;
+      edges += len;
+    }
+  }
+  uint32_t ctot;
+  const uint32_t off = block256_incl_scan(cnt, wc, &ctot) - cnt;
+  unsigned long long etot;
+  block256_incl_scan(edges, we, &etot);
+  const size_t base = (size_t)blockIdx.x * kBitsNodes + off;
+  const uint32_t* eb = buf + 3ull * nodenum;
+  uint32_t r = 0;
+  for (uint32_t m = keep; m; m &= m - 1, ++r) {
+    const uint32_t i = i0 + __ffs(m) - 1;
+    ent_ep[base + r] = reinterpret_cast<uint64_t>(eb + buf[3ull * i + 1]);
+    ent_len[base + r] = buf[3ull * i + 2];
+  }
+  if (threadIdx.x == 0) {
+    blk_cnt[blockIdx.x] = ctot;
+    blk_edges[blockIdx.x] = etot;
+  }
+}
+
 // Exclusive scan of the per-block counts (one 1024-thread block).
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, uint32_t* blk_coff,
@@ -330,28 +446,35 @@ k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, u
   }
 }
 
-// Dense entry arrays + the first entry of every tile.
+// Dense entry arrays + the first entry of every tile.  Prep block b left
+// blk_cnt[b] entries at ent[b * per_block ...] (per_block = kBlock or
+// kBitsNodes); they are taken kBlock at a time with a running edge prefix.
 __global__ void __launch_bounds__(kBlock)
 k_mark_tiles(const uint64_t* ent_ep, const uint32_t* ent_len, const uint32_t* blk_cnt,
              const uint32_t* blk_coff, const uint64_t* blk_eoff, uint64_t* gx, uint64_t* gep,
-             uint32_t* tstart, const unsigned long long* totals) {
+             uint32_t* tstart, const unsigned long long* totals, uint32_t per_block) {
   __shared__ unsigned long long wtot[kBlock / SCAPH_WAVE];
   const uint32_t b = blockIdx.x;
-  const uint32_t j = threadIdx.x;
   const uint32_t cnt = blk_cnt[b];
-  const size_t src = (size_t)b * kBlock + j;
-  const unsigned long long len = j < cnt ? ent_len[src] : 0ull;
-  unsigned long long total;
-  const unsigned long long incl = block256_incl_scan(len, wtot, &total);
-  if (j < cnt) {
-    const uint32_t gi = blk_coff[b] + j;
-    const unsigned long long g0 = blk_eoff[b] + incl - len;
-    gx[gi] = g0;
-    gep[gi] = ent_ep[src];
-    const unsigned long long t0 = (g0 + kTile - 1) / kTile;
-    unsigned long long t1 = (g0 + len + kTile - 1) / kTile;
-    if (t1 > totals[2]) t1 = totals[2];
-    for (unsigned long long t = t0; t < t1; ++t) tstart[t] = gi;
+  const unsigned long long n_tiles = totals[2];
+  unsigned long long carry = blk_eoff[b];
+  for (uint32_t c0 = 0; c0 < cnt; c0 += kBlock) {
+    const uint32_t j = c0 + threadIdx.x;
+    const size_t src = (size_t)b * per_block + j;
+    const unsigned long long len = j < cnt ? ent_len[src] : 0ull;
+    unsigned long long total;
+    const unsigned long long incl = block256_incl_scan(len, wtot, &total);
+    if (j < cnt) {
+      const uint32_t gi = blk_coff[b] + j;
+      const unsigned long long g0 = carry + incl - len;
+      gx[gi] = g0;
+      gep[gi] = ent_ep[src];
+      const unsigned long long t0 = (g0 + kTile - 1) / kTile;
+      unsigned long long t1 = (g0 + len + kTile - 1) / kTile;
+      if (t1 > n_tiles) t1 = n_tiles;
+      for (unsigned long long t = t0; t < t1; ++t) tstart[t] = gi;
+    }
+    carry += total;
   }
 }
 
@@ -996,16 +1119,17 @@ inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
   return (unsigned)b;
 }
 
-int launch_push_tail(const PushWs& w, uint32_t nb, unsigned long long max_edges,
-                     const uint32_t* degree, uint32_t* level, uint32_t* visited, uint32_t* q_out,
-                     uint32_t* front_out, scaph_level_counters* counters, uint32_t next_level,
-                     uint32_t n, int mode, const scaph_push_tuning* tune, hipStream_t st) {
+int launch_push_tail(const PushWs& w, uint32_t nb, uint32_t per_block,
+                     unsigned long long max_edges, const uint32_t* degree, uint32_t* level,
+                     uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
+                     scaph_level_counters* counters, uint32_t next_level, uint32_t n, int mode,
+                     const scaph_push_tuning* tune, hipStream_t st) {
   const unsigned long long tile_cap = (max_edges + kTile - 1) / kTile;
   hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
                      nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap);
   SCAPH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_mark_tiles, dim3(nb), dim3(kBlock), 0, st, w.ent_ep, w.ent_len, w.blk_cnt,
-                     w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
+                     w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals, per_block);
   SCAPH_LAUNCH_CHECK();
   unsigned grid = grid_for((size_t)((max_edges + kTile - 1) / kTile), 1, 8192);
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
@@ -1083,7 +1207,7 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree, ui
                      *out_graph, q_in, n_in,
                      w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
-  return launch_push_tail(w, nb, m_in, degree, level, visited, q_out, nullptr, counters,
+  return launch_push_tail(w, nb, kBlock, m_in, degree, level, visited, q_out, nullptr, counters,
                           cur_level + 1, out_graph->num_vertices, OUT_QUEUE, tune, st);
 }
 
@@ -1091,20 +1215,47 @@ int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned lon
                           uint32_t n, const uint32_t* degree, uint32_t* level, uint32_t* visited,
                           const uint32_t* front_in, uint32_t* front_out,
                           scaph_level_counters* counters, uint32_t cur_level, void* workspace,
-                          size_t workspace_bytes, void* stream) {
+                          size_t workspace_bytes, const scaph_page_hint* hint,
+                          const scaph_push_tuning* tune, void* stream) {
   if (!page_buf || !degree || !level || !visited || !front_in || !front_out || !counters ||
       !workspace)
     return SCAPH_STATUS_INVALID;
   if (nodenum == 0) return 0;
+  hipStream_t st = as_stream(stream);
+  if (hint) {
+    const bool full = hint->first_vertex != SCAPH_PAGE_COMPACTED;
+    if (full && ((unsigned long long)hint->first_vertex + nodenum > n)) return SCAPH_STATUS_INVALID;
+    if (hint->active_nodes == 0 || hint->active_edges == 0) return 0;
+    edgenum = hint->active_edges;
+    const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
+    if (edgenum <= small) {
+      const uint32_t units = full ? (nodenum + kSmallChunk - 1) / kSmallChunk : nodenum;
+      hipLaunchKernelGGL(k_push_page_small, dim3(grid_for(units, kBlock / SCAPH_WAVE, 64)),
+                         dim3(kBlock), 0, st, page_buf, nodenum, hint->first_vertex, front_in,
+                         degree, level, visited, front_out, counters, cur_level + 1, n);
+      SCAPH_LAUNCH_CHECK();
+      return 0;
+    }
+    if (full) {
+      PushWs w = carve_push_ws(workspace, nodenum, edgenum);
+      if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
+      const uint32_t nb = (nodenum + kBitsNodes - 1) / kBitsNodes;
+      hipLaunchKernelGGL(k_prep_page_bits, dim3(nb), dim3(kBlock), 0, st, page_buf, nodenum,
+                         hint->first_vertex, front_in, w.ent_ep, w.ent_len, w.blk_cnt,
+                         w.blk_edges);
+      SCAPH_LAUNCH_CHECK();
+      return launch_push_tail(w, nb, kBitsNodes, edgenum, degree, level, visited, nullptr,
+                              front_out, counters, cur_level + 1, n, OUT_BITMAP, tune, st);
+    }
+  }
   PushWs w = carve_push_ws(workspace, nodenum, edgenum);
   if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
-  hipStream_t st = as_stream(stream);
   const uint32_t nb = (nodenum + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_prep_page, dim3(nb), dim3(kBlock), 0, st, page_buf, nodenum, front_in,
                      w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
-  return launch_push_tail(w, nb, edgenum, degree, level, visited, nullptr, front_out, counters,
-                          cur_level + 1, n, OUT_BITMAP, nullptr, st);
+  return launch_push_tail(w, nb, kBlock, edgenum, degree, level, visited, nullptr, front_out,
+                          counters, cur_level + 1, n, OUT_BITMAP, tune, st);
 }
 
 size_t scaph_pull_workspace_bytes(unsigned long long max_edges) {

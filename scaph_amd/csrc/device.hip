@@ -164,7 +164,7 @@ int scaph_host_mapped_alloc(void** host_ptr, void** device_ptr, size_t bytes) {
   SCAPH_TRY(hipHostMalloc(host_ptr, bytes, hipHostMallocMapped | hipHostMallocCoherent));
   const hipError_t e = hipHostGetDevicePointer(device_ptr, *host_ptr, 0);
   if (e != hipSuccess) {
-    hipHostFree(*host_ptr);
+    (void)hipHostFree(*host_ptr);  // the mapping error is the one reported
     *host_ptr = nullptr;
     return (int)e;
   }

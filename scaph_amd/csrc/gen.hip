@@ -253,8 +253,8 @@ int scaph_degrees_to_offsets(const uint32_t* degree, uint64_t* offsets, uint32_t
     rc = (int)hipGetLastError();
   }
   if (!rc) rc = (int)hipStreamSynchronize(st);
-  hipFree(tmp);
-  return rc;
+  const int rf = (int)hipFree(tmp);
+  return rc ? rc : rf;
 }
 
 int scaph_gen_zipf_keys(uint64_t* keys, const uint64_t* offsets, uint32_t n, uint64_t e_begin,

@@ -66,6 +66,8 @@ Config& global_flags() {
     r.add("push_rows", FlagType::kBool, &c.push_rows, "push expand reads coalesced edge rows");
     r.add("push_small", FlagType::kInt32, &c.push_small,
           "out-edges below which a push level runs as one launch (0 = 8192, 1 = never)");
+    r.add("page_hint", FlagType::kBool, &c.page_hint,
+          "paged: size page pushes by their page counts and read full pages through the frontier bits");
     r.add("pull_head", FlagType::kBool, &c.pull_head,
           "keep each in-list's first entries beside its in-degree for the pull probes");
     r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");

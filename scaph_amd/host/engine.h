@@ -52,6 +52,8 @@ struct Config {
   int32_t push_grid = 0;         // cap on push expand blocks (0: one per tile, at most 8192)
   bool push_rows = true;         // push expand reads coalesced 256-edge rows
   int32_t push_small = 0;        // out-edges below which a push level is one launch (0: 8192)
+  bool page_hint = true;         // paged: size page pushes by their page counts, read full
+                                 // pages through the frontier bits
   int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
 };
 

@@ -2,6 +2,7 @@
 // (include/scaph_engine.h).  Exceptions never cross the boundary.
 #include "scaph_engine.h"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -35,8 +36,9 @@ namespace {
 
 void put_err(char* err, size_t len, const std::string& msg) {
   if (!err || len == 0) return;
-  std::strncpy(err, msg.c_str(), len - 1);
-  err[len - 1] = '\0';
+  const size_t k = std::min(msg.size(), len - 1);
+  std::memcpy(err, msg.data(), k);
+  err[k] = '\0';
 }
 
 template <typename F>

@@ -46,6 +46,18 @@ uint64_t compact_page(const uint32_t* page, const scaph_page& pg, const uint32_t
   return 3ull * active_nodes + off;
 }
 
+// The compaction workers of one level, joined on every exit from it: an
+// error raised by the scheduling loop then reaches the caller instead of
+// destroying joinable threads (std::terminate).
+struct Workers {
+  std::vector<std::thread> t;
+  void join() {
+    for (auto& x : t)
+      if (x.joinable()) x.join();
+  }
+  ~Workers() { join(); }
+};
+
 }  // namespace
 
 void Engine::setup_paged() {
@@ -161,11 +173,11 @@ RunStats Engine::bfs_paged(uint32_t src) {
 
     // compaction workers (parallel_compress_stat_async, src/compress.h:42-56)
     std::atomic<size_t> next{0};
-    std::vector<std::thread> workers;
-    const int nw = (int)std::min<size_t>(cold.size(), 8);
     std::atomic<bool> failed{false};
+    Workers workers;
+    const int nw = (int)std::min<size_t>(cold.size(), 8);
     for (int w = 0; w < nw; ++w) {
-      workers.emplace_back([&] {
+      workers.t.emplace_back([&] {
         for (size_t k = next++; k < cold.size(); k = next++) {
           const int p = cold[k];
           try {
@@ -208,10 +220,16 @@ RunStats Engine::bfs_paged(uint32_t src) {
                                 (uint64_t)v.dpoff * ps.chunk;
           const uint64_t edges_bound = v.shared ? v.datanum - 3ull * v.nodenum
                                                 : (uint64_t)pages_out_[e].edgenum;
+          // page counts of this frontier: the exact active Nodes and out-edges
+          scaph_page_hint hint;
+          hint.first_vertex = v.shared ? SCAPH_PAGE_COMPACTED : pages_out_[e].left;
+          hint.active_nodes = nodes[e];
+          hint.active_edges = datas[e] - 3ull * nodes[e];
           H_ERR(scaph_bfs_push_page(buf, v.nodenum, std::max<uint64_t>(edges_bound, 1), n_, deg,
                                     level, visited, front_[fa].get<uint32_t>(),
                                     front_[fa ^ 1].get<uint32_t>(), cnt, cur, ps.ws[s].get(),
-                                    ps.ws_bytes, ps.exec[s]->get()));
+                                    ps.ws_bytes, cfg_.page_hint ? &hint : nullptr, &push_tune_,
+                                    ps.exec[s]->get()));
           ps.exec_done[s]->record(ps.exec[s]->get());
           last[s] = e;
           ++rs.page_execs;
@@ -246,7 +264,7 @@ RunStats Engine::bfs_paged(uint32_t src) {
       }
       if (!progress) std::this_thread::yield();
     }
-    for (auto& t : workers) t.join();
+    workers.join();
     if (failed) throw Error(SCAPH_STATUS_INVALID, "page compaction failed");
     for (int s = 0; s < ns; ++s) ps.exec[s]->sync();
     ps.copy->sync();

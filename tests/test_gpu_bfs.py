@@ -235,6 +235,28 @@ def test_paged_out_of_core_parity(page_words, max_mib, threshold, multi, streams
             assert e.check(src) == 0
 
 
+@pytest.mark.parametrize("flags", [[], ["-page_hint=false"], ["-push_small=1"],
+                                   ["-push_small=1000000"], ["-push_small=1", "-push_rows=false"]])
+def test_paged_page_push_variants(flags):
+    """Page pushes sized by the page counts: one-launch small pages, full
+    pages read through their frontier bits (pages of ~14K Nodes, so two
+    8192-Node prep blocks, at unaligned first vertices), compacted pages, and
+    the unhinted path, all against the oracle."""
+    off, ngh = O.rmat_csr(16, 16, seed=8)
+    deg = np.diff(off.astype(np.int64))
+    with engine("-page_words=262144", "-max_memory=3", "-threshold=3", "-num_stream=3",
+                *flags) as e:
+        e.load_csr(off, ngh)
+        inf = e.info()
+        assert not inf["resident"] and inf["pages"] >= 4, inf
+        for src in (int(np.argmax(deg)), int(np.nonzero(deg)[0][5])):
+            say("paged variants", flags, "src", src)
+            want = O.bfs(off, ngh, src)
+            st = e.run(src)
+            np.testing.assert_array_equal(e.levels(), want, err_msg=f"{flags} {src}")
+            assert st["bytes_h2d"] > 0
+
+
 def test_pull_heavy_in_lists():
     """In-lists far longer than one pull work chunk, with the only frontier
     in-neighbour at the very end (and a hub with no frontier in-neighbour)."""
