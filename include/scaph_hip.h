@@ -241,6 +241,38 @@ int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned lon
                         size_t workspace_bytes, const scaph_page_hint* hint,
                         const scaph_push_tuning* tune, void* stream);
 
+/* Push expand of a GROUP of resident full pages in one launch sequence —
+ * several of gpu_push_batch's per-page gpu_push_csr launches
+ * (src/app.cuh:414-459) fused.  runs (device memory) lists the pages in
+ * any order: run j is the buffer at runs[j].buf (Nodes, then edges), Node
+ * i is vertex first_vertex + i, and seg0 = the sum of
+ * ceil(nodenum / SCAPH_RUN_SEGMENT) over the runs before j; num_segs is that
+ * sum over all runs.  Each segment is read through its frontier bits, so
+ * only active Nodes are loaded.  active_nodes and active_edges are the
+ * runs' nodes[p] and datas[p] - 3 * nodes[p] sums of scaph_page_counts for
+ * the same frontier: they size the launch and the workspace; an
+ * under-count yields a truncated or empty (wrong) level, never an access
+ * outside the workspace.  Groups with at most tune->small_edges (default
+ * 8192) active out-edges run as one launch.  Output as scaph_bfs_push_page.
+ * Workspace: scaph_push_runs_workspace_bytes(num_segs, active_nodes,
+ * active_edges). */
+#define SCAPH_RUN_SEGMENT 8192u
+typedef struct scaph_page_run {
+  unsigned long long buf;
+  uint32_t first_vertex;
+  uint32_t nodenum;
+  uint32_t seg0;
+  uint32_t reserved;
+} scaph_page_run;
+size_t scaph_push_runs_workspace_bytes(uint32_t num_segs, unsigned long long active_nodes,
+                                       unsigned long long active_edges);
+int scaph_bfs_push_runs(const scaph_page_run* runs, uint32_t num_runs, uint32_t num_segs,
+                        unsigned long long active_nodes, unsigned long long active_edges,
+                        uint32_t n, const uint32_t* degree, uint32_t* level, uint32_t* visited,
+                        const uint32_t* front_in, uint32_t* front_out,
+                        scaph_level_counters* counters, uint32_t cur_level, void* workspace,
+                        size_t workspace_bytes, const scaph_push_tuning* tune, void* stream);
+
 /* Level-array check kernel: counts edges (u,v) violating
  * level[v] <= level[u] + 1 and reached vertices without a parent at
  * level-1 (BFS invariants; in_graph entries decoded through hub_list).

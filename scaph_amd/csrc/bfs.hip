@@ -15,8 +15,6 @@
 // (block-local compaction + scan of the out-degrees), cut into tiles of
 // kTile edges, and every 256-thread block expands whole tiles — a hub with a
 // million out-edges is spread over ~1000 blocks instead of one CTA.
-#include <algorithm>
-
 #include "common.h"
 
 namespace {
@@ -31,14 +29,14 @@ constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work it
 constexpr uint32_t kMaxHubs = 32768;               // hub frontier bits kept in LDS (4 KB)
 constexpr uint32_t kHubTag = 0x80000000u;          // in-list entry = kHubTag | hub index
 constexpr uint32_t kScanThreads = 1024;
-constexpr uint32_t kBitsNodes = kBlock * 32;       // Nodes per bitmap-driven page prep block
-constexpr uint32_t kSmallChunk = SCAPH_WAVE * 32;  // Nodes per wave step of k_push_page_small
+constexpr uint32_t kBitsNodes = SCAPH_RUN_SEGMENT;  // Nodes per run segment (32 per thread)
+constexpr uint32_t kSmallChunk = SCAPH_WAVE * 32;   // Nodes per wave step of k_push_runs_small
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct PushWs {
-  uint64_t* ent_ep;     // edge-list address of each kept entry (block-compacted)
-  uint32_t* ent_len;    // its length
+  uint64_t* ent_ep;     // [nb*256] edge-list address of each kept entry (block-compacted)
+  uint32_t* ent_len;    // [nb*256]
   uint32_t* blk_cnt;    // [nb]
   uint64_t* blk_edges;  // [nb]
   uint32_t* blk_coff;   // [nb] exclusive entry offset
@@ -55,9 +53,6 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
   const size_t nb = (max_entries + kBlock - 1) / kBlock + 1;
   const size_t m = (size_t)max_entries + 1;
   const size_t tiles = (size_t)((max_edges + kTile - 1) / kTile) + 2;
-  // entry slots: kBlock per queue-prep block, kBitsNodes per bitmap-prep block
-  const size_t ent = std::max<size_t>(nb * kBlock,
-                                      ((size_t)max_entries + kBitsNodes - 1) / kBitsNodes * kBitsNodes);
   size_t off = 0;
   char* p = static_cast<char*>(base);
   auto take = [&](size_t bytes) {
@@ -65,8 +60,8 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
     off += align256(bytes);
     return r;
   };
-  w.ent_ep = reinterpret_cast<uint64_t*>(take(ent * 8));
-  w.ent_len = reinterpret_cast<uint32_t*>(take(ent * 4));
+  w.ent_ep = reinterpret_cast<uint64_t*>(take(nb * kBlock * 8));
+  w.ent_len = reinterpret_cast<uint32_t*>(take(nb * kBlock * 4));
   w.blk_cnt = reinterpret_cast<uint32_t*>(take(nb * 4));
   w.blk_edges = reinterpret_cast<uint64_t*>(take(nb * 8));
   w.blk_coff = reinterpret_cast<uint32_t*>(take(nb * 4));
@@ -219,65 +214,6 @@ __device__ __forceinline__ uint32_t bits_at(const uint32_t* bm, uint32_t v, uint
   return c >= 32 ? b : (b & ((1u << c) - 1u));
 }
 
-// One-launch push of a page whose active Nodes have few out-edges.  A full
-// page (Node i = vertex first + i) is walked kSmallChunk Nodes per wave
-// step: each lane takes 32 frontier bits, and the wave visits the active
-// Nodes one after another, its lanes covering the Node's out-list.  A
-// compacted buffer (first == SCAPH_PAGE_COMPACTED) holds only active Nodes:
-// one per wave step.
-__global__ void __launch_bounds__(kBlock)
-k_push_page_small(const uint32_t* buf, uint32_t nodenum, uint32_t first, const uint32_t* front,
-                  const uint32_t* degree, uint32_t* level, uint32_t* visited, uint32_t* front_out,
-                  scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
-  const uint32_t lane = lane_id();
-  const uint32_t nw = kBlock / SCAPH_WAVE;
-  const uint32_t gw = blockIdx.x * nw + threadIdx.x / SCAPH_WAVE;
-  const uint32_t waves = gridDim.x * nw;
-  const uint32_t* eb = buf + 3ull * nodenum;
-  unsigned long long acc_n = 0, acc_m = 0, acc_s = 0;
-  auto expand = [&](uint32_t i) {
-    const uint32_t idx = buf[3ull * i + 1], len = buf[3ull * i + 2];
-    for (uint32_t k0 = 0; k0 < len; k0 += SCAPH_WAVE) {
-      const uint32_t k = k0 + lane;
-      if (k < len) {
-        ++acc_s;
-        const uint32_t v = eb[idx + k];
-        if (v < n) {
-          const uint32_t w = v >> 5, bit = 1u << (v & 31);
-          if (!(visited[w] & bit) && !(atomicOr(&visited[w], bit) & bit)) {
-            level[v] = next_level;
-            acc_n += 1;
-            acc_m += degree[v];
-            atomicOr(&front_out[w], bit);
-          }
-        }
-      }
-    }
-  };
-  if (first == SCAPH_PAGE_COMPACTED) {
-    for (uint32_t i = gw; i < nodenum; i += waves) expand(i);
-  } else {
-    const uint32_t chunks = (nodenum + kSmallChunk - 1) / kSmallChunk;
-    for (uint32_t c = gw; c < chunks; c += waves) {
-      const uint32_t i0 = c * kSmallChunk + lane * 32;
-      const uint32_t bits = i0 < nodenum ? bits_at(front, first + i0, min(32u, nodenum - i0)) : 0u;
-      for (uint64_t any = __ballot(bits != 0); any; any &= any - 1) {
-        const uint32_t src = (uint32_t)__ffsll((unsigned long long)any) - 1;
-        const uint32_t sb = __shfl(bits, (int)src, SCAPH_WAVE);
-        for (uint32_t m = sb; m; m &= m - 1) expand(c * kSmallChunk + src * 32 + __ffs(m) - 1);
-      }
-    }
-  }
-  acc_n = wave_sum(acc_n);
-  acc_m = wave_sum(acc_m);
-  acc_s = wave_sum(acc_s);
-  if (lane == 0) {
-    if (acc_n) atomicAdd(&counters->n_next, acc_n);
-    if (acc_m) atomicAdd(&counters->m_next, acc_m);
-    if (acc_s) atomicAdd(&counters->scanned, acc_s);
-  }
-}
-
 // ------------------------------------------------------------ push: prep
 
 // Block-local compaction of the entries with len > 0.
@@ -351,52 +287,200 @@ k_prep_page(const uint32_t* buf, uint32_t nodenum, const uint32_t* front, uint64
   prep_store(keep, ep, keep ? len : 0u, ent_ep, ent_len, blk_cnt, blk_edges);
 }
 
-// Prep of a full page (Node i = vertex first + i) driven by the frontier
-// bitmap: each thread takes 32 Nodes through one funnel-shifted bitmap
-// word and loads only the records of active Nodes, so a sparse frontier
-// costs nodenum/8 bytes of bitmap instead of 12 bytes per Node.  Block b
-// keeps its entries (non-empty lists) at ent[b * kBitsNodes ...].
+// ------------------------------------------------------ push: page runs
+//
+// A run is a full page buffer on the device (Node i = vertex first + i).
+// A multi-page push takes a table of runs; segment s (kBitsNodes Nodes) of
+// the table belongs to the run j with runs[j].seg0 <= s < runs[j+1].seg0.
+// The single-page entry passes its one run by value (runs == nullptr).
+// Both passes read a segment through its frontier bits, 32 Nodes per
+// thread, and load only the records of active Nodes: a sparse frontier
+// costs nodenum/8 bytes of bitmap instead of 12 bytes per Node.
+
+__device__ __forceinline__ scaph_page_run run_of(const scaph_page_run* runs, uint32_t num_runs,
+                                                 const scaph_page_run& one, uint32_t seg) {
+  if (!runs) return one;
+  uint32_t lo = 0, hi = num_runs;  // runs[lo].seg0 <= seg < runs[hi].seg0
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (runs[mid].seg0 <= seg) lo = mid; else hi = mid;
+  }
+  return runs[lo];
+}
+
+// This thread's 32 Nodes of segment b: first Node index and frontier bits.
+struct SegBits {
+  const uint32_t* buf;
+  uint32_t nodenum;
+  uint32_t i0;
+  uint32_t bits;
+};
+
+__device__ __forceinline__ SegBits seg_bits(const scaph_page_run& r, uint32_t b,
+                                            const uint32_t* front) {
+  SegBits s;
+  s.buf = reinterpret_cast<const uint32_t*>(r.buf);
+  s.nodenum = r.nodenum;
+  s.i0 = (b - r.seg0) * kBitsNodes + threadIdx.x * 32;
+  s.bits = s.i0 < r.nodenum ? bits_at(front, r.first_vertex + s.i0, min(32u, r.nodenum - s.i0))
+                            : 0u;
+  return s;
+}
+
+// Pass 1: per segment, the active Nodes with a non-empty out-list and their
+// out-edges.
 __global__ void __launch_bounds__(kBlock)
-k_prep_page_bits(const uint32_t* buf, uint32_t nodenum, uint32_t first, const uint32_t* front,
-                 uint64_t* ent_ep, uint32_t* ent_len, uint32_t* blk_cnt, uint64_t* blk_edges) {
+k_runs_count(const scaph_page_run* runs, uint32_t num_runs, scaph_page_run one,
+             const uint32_t* front, uint32_t* blk_cnt, uint64_t* blk_edges) {
   __shared__ uint32_t wc[kBlock / SCAPH_WAVE];
   __shared__ unsigned long long we[kBlock / SCAPH_WAVE];
-  const uint32_t i0 = blockIdx.x * kBitsNodes + threadIdx.x * 32;
-  uint32_t bits = 0;
-  if (i0 < nodenum) bits = bits_at(front, first + i0, min(32u, nodenum - i0));
-  uint32_t keep = 0, cnt = 0;
+  const SegBits s = seg_bits(run_of(runs, num_runs, one, blockIdx.x), blockIdx.x, front);
+  uint32_t cnt = 0;
   unsigned long long edges = 0;
-  for (uint32_t m = bits; m; m &= m - 1) {
-    const uint32_t k = __ffs(m) - 1;
-    const uint32_t len = buf[3ull * (i0 + k) + 2];
-    if (len) {
-      keep |= 1u << k;
-      ++cnt;
-      edges += len;
-    }
+  for (uint32_t m = s.bits; m; m &= m - 1) {
+    const uint32_t len = s.buf[3ull * (s.i0 + __ffs(m) - 1) + 2];
+    cnt += len ? 1u : 0u;
+    edges += len;
   }
   uint32_t ctot;
-  const uint32_t off = block256_incl_scan(cnt, wc, &ctot) - cnt;
+  block256_incl_scan(cnt, wc, &ctot);
   unsigned long long etot;
   block256_incl_scan(edges, we, &etot);
-  const size_t base = (size_t)blockIdx.x * kBitsNodes + off;
-  const uint32_t* eb = buf + 3ull * nodenum;
-  uint32_t r = 0;
-  for (uint32_t m = keep; m; m &= m - 1, ++r) {
-    const uint32_t i = i0 + __ffs(m) - 1;
-    ent_ep[base + r] = reinterpret_cast<uint64_t>(eb + buf[3ull * i + 1]);
-    ent_len[base + r] = buf[3ull * i + 2];
-  }
   if (threadIdx.x == 0) {
     blk_cnt[blockIdx.x] = ctot;
     blk_edges[blockIdx.x] = etot;
   }
 }
 
+// Pass 2: the same Nodes, written densely in segment order — gx (position
+// of the first out-edge in the level's edge sequence), gep (address of the
+// out-list) — and the first entry of every tile.  Nothing is written when
+// k_scan_blocks found more entries than the workspace holds (totals[0] = 0).
+__global__ void __launch_bounds__(kBlock)
+k_runs_mark(const scaph_page_run* runs, uint32_t num_runs, scaph_page_run one,
+            const uint32_t* front, const uint32_t* blk_coff, const uint64_t* blk_eoff,
+            uint64_t* gx, uint64_t* gep, uint32_t* tstart, const unsigned long long* totals) {
+  __shared__ uint32_t wc[kBlock / SCAPH_WAVE];
+  __shared__ unsigned long long we[kBlock / SCAPH_WAVE];
+  const unsigned long long n_ent = totals[0];
+  const unsigned long long n_tiles = totals[2];
+  if (n_ent == 0) return;
+  const SegBits s = seg_bits(run_of(runs, num_runs, one, blockIdx.x), blockIdx.x, front);
+  uint32_t cnt = 0;
+  unsigned long long edges = 0;
+  for (uint32_t m = s.bits; m; m &= m - 1) {
+    const uint32_t len = s.buf[3ull * (s.i0 + __ffs(m) - 1) + 2];
+    cnt += len ? 1u : 0u;
+    edges += len;
+  }
+  uint32_t ctot;
+  unsigned long long etot;
+  uint32_t gi = blk_coff[blockIdx.x] + block256_incl_scan(cnt, wc, &ctot) - cnt;
+  unsigned long long g = blk_eoff[blockIdx.x] + block256_incl_scan(edges, we, &etot) - edges;
+  const uint32_t* eb = s.buf + 3ull * s.nodenum;
+  for (uint32_t m = s.bits; m; m &= m - 1) {
+    const uint32_t i = s.i0 + __ffs(m) - 1;
+    const uint32_t len = s.buf[3ull * i + 2];
+    if (!len || gi >= n_ent) continue;
+    gx[gi] = g;
+    gep[gi] = reinterpret_cast<uint64_t>(eb + s.buf[3ull * i + 1]);
+    const unsigned long long t0 = (g + kTile - 1) / kTile;
+    unsigned long long t1 = (g + len + kTile - 1) / kTile;
+    if (t1 > n_tiles) t1 = n_tiles;
+    for (unsigned long long t = t0; t < t1; ++t) tstart[t] = gi;
+    ++gi;
+    g += len;
+  }
+}
+
+// One wave expands Node i of a page buffer: its lanes cover the out-list.
+struct SmallAcc {
+  unsigned long long n = 0, m = 0, s = 0;
+};
+
+__device__ __forceinline__ void push_node_wave(const uint32_t* buf, uint32_t nodenum, uint32_t i,
+                                               const uint32_t* degree, uint32_t* level,
+                                               uint32_t* visited, uint32_t* front_out,
+                                               uint32_t next_level, uint32_t n, SmallAcc& acc) {
+  const uint32_t lane = lane_id();
+  const uint32_t idx = buf[3ull * i + 1], len = buf[3ull * i + 2];
+  const uint32_t* eb = buf + 3ull * nodenum + idx;
+  for (uint32_t k0 = 0; k0 < len; k0 += SCAPH_WAVE) {
+    const uint32_t k = k0 + lane;
+    if (k >= len) continue;
+    ++acc.s;
+    const uint32_t v = eb[k];
+    if (v >= n) continue;
+    const uint32_t w = v >> 5, bit = 1u << (v & 31);
+    if (!(visited[w] & bit) && !(atomicOr(&visited[w], bit) & bit)) {
+      level[v] = next_level;
+      acc.n += 1;
+      acc.m += degree[v];
+      atomicOr(&front_out[w], bit);
+    }
+  }
+}
+
+__device__ __forceinline__ void small_acc_flush(SmallAcc acc, scaph_level_counters* counters) {
+  const unsigned long long a = wave_sum(acc.n), b = wave_sum(acc.m), c = wave_sum(acc.s);
+  if (lane_id() == 0) {
+    if (a) atomicAdd(&counters->n_next, a);
+    if (b) atomicAdd(&counters->m_next, b);
+    if (c) atomicAdd(&counters->scanned, c);
+  }
+}
+
+// One-launch push of runs whose active Nodes have few out-edges: a wave
+// step covers kSmallChunk Nodes (32 frontier bits per lane), and the wave
+// visits that chunk's active Nodes one after another.
+__global__ void __launch_bounds__(kBlock)
+k_push_runs_small(const scaph_page_run* runs, uint32_t num_runs, scaph_page_run one,
+                  uint32_t num_segs, const uint32_t* front, const uint32_t* degree,
+                  uint32_t* level, uint32_t* visited, uint32_t* front_out,
+                  scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
+  constexpr uint32_t kPerSeg = kBitsNodes / kSmallChunk;
+  const uint32_t lane = lane_id();
+  const uint32_t nw = kBlock / SCAPH_WAVE;
+  const uint32_t waves = gridDim.x * nw;
+  const uint32_t units = num_segs * kPerSeg;
+  SmallAcc acc;
+  for (uint32_t u = blockIdx.x * nw + threadIdx.x / SCAPH_WAVE; u < units; u += waves) {
+    const uint32_t seg = u / kPerSeg;
+    const scaph_page_run r = run_of(runs, num_runs, one, seg);
+    const uint32_t* buf = reinterpret_cast<const uint32_t*>(r.buf);
+    const uint32_t c0 = (seg - r.seg0) * kBitsNodes + (u % kPerSeg) * kSmallChunk;
+    const uint32_t i0 = c0 + lane * 32;
+    const uint32_t bits =
+        i0 < r.nodenum ? bits_at(front, r.first_vertex + i0, min(32u, r.nodenum - i0)) : 0u;
+    for (uint64_t any = __ballot(bits != 0); any; any &= any - 1) {
+      const uint32_t src = (uint32_t)__ffsll((unsigned long long)any) - 1;
+      const uint32_t sb = __shfl(bits, (int)src, SCAPH_WAVE);
+      for (uint32_t m = sb; m; m &= m - 1)
+        push_node_wave(buf, r.nodenum, c0 + src * 32 + __ffs(m) - 1, degree, level, visited,
+                       front_out, next_level, n, acc);
+    }
+  }
+  small_acc_flush(acc, counters);
+}
+
+// One-launch push of a compacted buffer (every Node active), one Node per
+// wave step.
+__global__ void __launch_bounds__(kBlock)
+k_push_compact_small(const uint32_t* buf, uint32_t nodenum, const uint32_t* degree,
+                     uint32_t* level, uint32_t* visited, uint32_t* front_out,
+                     scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
+  const uint32_t nw = kBlock / SCAPH_WAVE;
+  SmallAcc acc;
+  for (uint32_t i = blockIdx.x * nw + threadIdx.x / SCAPH_WAVE; i < nodenum; i += gridDim.x * nw)
+    push_node_wave(buf, nodenum, i, degree, level, visited, front_out, next_level, n, acc);
+  small_acc_flush(acc, counters);
+}
+
 // Exclusive scan of the per-block counts (one 1024-thread block).
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, uint32_t* blk_coff,
-              uint64_t* blk_eoff, unsigned long long* totals, unsigned long long tile_cap) {
+              uint64_t* blk_eoff, unsigned long long* totals, unsigned long long tile_cap,
+              unsigned long long ent_cap) {
   __shared__ uint32_t wc[kScanThreads / SCAPH_WAVE];
   __shared__ unsigned long long we[kScanThreads / SCAPH_WAVE];
   const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
@@ -439,6 +523,8 @@ k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, u
     // A caller that under-reports the frontier's edges gets a truncated
     // (wrong, detectable) level instead of writes past the tile table.
     if (etot > tile_cap * kTile) etot = tile_cap * kTile;
+    // More entries than the dense arrays hold: nothing is expanded.
+    if (ctot > ent_cap) ctot = 0, etot = 0;
     totals[0] = ctot;
     totals[1] = etot;
     totals[2] = (etot + kTile - 1) / kTile;
@@ -446,35 +532,28 @@ k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, u
   }
 }
 
-// Dense entry arrays + the first entry of every tile.  Prep block b left
-// blk_cnt[b] entries at ent[b * per_block ...] (per_block = kBlock or
-// kBitsNodes); they are taken kBlock at a time with a running edge prefix.
+// Dense entry arrays + the first entry of every tile.
 __global__ void __launch_bounds__(kBlock)
 k_mark_tiles(const uint64_t* ent_ep, const uint32_t* ent_len, const uint32_t* blk_cnt,
              const uint32_t* blk_coff, const uint64_t* blk_eoff, uint64_t* gx, uint64_t* gep,
-             uint32_t* tstart, const unsigned long long* totals, uint32_t per_block) {
+             uint32_t* tstart, const unsigned long long* totals) {
   __shared__ unsigned long long wtot[kBlock / SCAPH_WAVE];
   const uint32_t b = blockIdx.x;
+  const uint32_t j = threadIdx.x;
   const uint32_t cnt = blk_cnt[b];
-  const unsigned long long n_tiles = totals[2];
-  unsigned long long carry = blk_eoff[b];
-  for (uint32_t c0 = 0; c0 < cnt; c0 += kBlock) {
-    const uint32_t j = c0 + threadIdx.x;
-    const size_t src = (size_t)b * per_block + j;
-    const unsigned long long len = j < cnt ? ent_len[src] : 0ull;
-    unsigned long long total;
-    const unsigned long long incl = block256_incl_scan(len, wtot, &total);
-    if (j < cnt) {
-      const uint32_t gi = blk_coff[b] + j;
-      const unsigned long long g0 = carry + incl - len;
-      gx[gi] = g0;
-      gep[gi] = ent_ep[src];
-      const unsigned long long t0 = (g0 + kTile - 1) / kTile;
-      unsigned long long t1 = (g0 + len + kTile - 1) / kTile;
-      if (t1 > n_tiles) t1 = n_tiles;
-      for (unsigned long long t = t0; t < t1; ++t) tstart[t] = gi;
-    }
-    carry += total;
+  const size_t src = (size_t)b * kBlock + j;
+  const unsigned long long len = j < cnt ? ent_len[src] : 0ull;
+  unsigned long long total;
+  const unsigned long long incl = block256_incl_scan(len, wtot, &total);
+  if (j < cnt) {
+    const uint32_t gi = blk_coff[b] + j;
+    const unsigned long long g0 = blk_eoff[b] + incl - len;
+    gx[gi] = g0;
+    gep[gi] = ent_ep[src];
+    const unsigned long long t0 = (g0 + kTile - 1) / kTile;
+    unsigned long long t1 = (g0 + len + kTile - 1) / kTile;
+    if (t1 > totals[2]) t1 = totals[2];
+    for (unsigned long long t = t0; t < t1; ++t) tstart[t] = gi;
   }
 }
 
@@ -1119,18 +1198,10 @@ inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
   return (unsigned)b;
 }
 
-int launch_push_tail(const PushWs& w, uint32_t nb, uint32_t per_block,
-                     unsigned long long max_edges, const uint32_t* degree, uint32_t* level,
-                     uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
-                     scaph_level_counters* counters, uint32_t next_level, uint32_t n, int mode,
-                     const scaph_push_tuning* tune, hipStream_t st) {
-  const unsigned long long tile_cap = (max_edges + kTile - 1) / kTile;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
-                     nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap);
-  SCAPH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mark_tiles, dim3(nb), dim3(kBlock), 0, st, w.ent_ep, w.ent_len, w.blk_cnt,
-                     w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals, per_block);
-  SCAPH_LAUNCH_CHECK();
+int launch_expand(const PushWs& w, unsigned long long max_edges, const uint32_t* degree,
+                  uint32_t* level, uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
+                  scaph_level_counters* counters, uint32_t next_level, uint32_t n, int mode,
+                  const scaph_push_tuning* tune, hipStream_t st) {
   unsigned grid = grid_for((size_t)((max_edges + kTile - 1) / kTile), 1, 8192);
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
   const bool rows = tune && tune->edge_rows;
@@ -1147,6 +1218,84 @@ int launch_push_tail(const PushWs& w, uint32_t nb, uint32_t per_block,
 #undef SCAPH_EXPAND
   SCAPH_LAUNCH_CHECK();
   return 0;
+}
+
+// Entries prepared by k_prep_queue / k_prep_page: block scan, dense arrays
+// and tiles, expand.
+int launch_push_tail(const PushWs& w, uint32_t nb, unsigned long long max_edges,
+                     const uint32_t* degree, uint32_t* level, uint32_t* visited, uint32_t* q_out,
+                     uint32_t* front_out, scaph_level_counters* counters, uint32_t next_level,
+                     uint32_t n, int mode, const scaph_push_tuning* tune, hipStream_t st) {
+  const unsigned long long tile_cap = (max_edges + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
+                     nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap, ~0ull);
+  SCAPH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mark_tiles, dim3(nb), dim3(kBlock), 0, st, w.ent_ep, w.ent_len, w.blk_cnt,
+                     w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
+  SCAPH_LAUNCH_CHECK();
+  return launch_expand(w, max_edges, degree, level, visited, q_out, front_out, counters,
+                       next_level, n, mode, tune, st);
+}
+
+// Workspace of a run push: per-segment counts and offsets, the dense entry
+// arrays (one entry per active Node with a non-empty out-list) and tiles.
+PushWs carve_runs_ws(void* base, uint32_t num_segs, unsigned long long max_entries,
+                     unsigned long long max_edges) {
+  PushWs w{};
+  const size_t nb = (size_t)num_segs + 1;
+  const size_t m = (size_t)max_entries + 1;
+  const size_t tiles = (size_t)((max_edges + kTile - 1) / kTile) + 2;
+  size_t off = 0;
+  char* p = static_cast<char*>(base);
+  auto take = [&](size_t bytes) {
+    char* r = p ? p + off : nullptr;
+    off += align256(bytes);
+    return r;
+  };
+  w.blk_cnt = reinterpret_cast<uint32_t*>(take(nb * 4));
+  w.blk_edges = reinterpret_cast<uint64_t*>(take(nb * 8));
+  w.blk_coff = reinterpret_cast<uint32_t*>(take(nb * 4));
+  w.blk_eoff = reinterpret_cast<uint64_t*>(take(nb * 8));
+  w.gx = reinterpret_cast<uint64_t*>(take(m * 8));
+  w.gep = reinterpret_cast<uint64_t*>(take(m * 8));
+  w.tstart = reinterpret_cast<uint32_t*>(take(tiles * 4));
+  w.totals = reinterpret_cast<unsigned long long*>(take(64));
+  w.bytes = off;
+  return w;
+}
+
+// Push of full page runs: the table in device memory (runs != nullptr) or
+// one run passed by value.
+int push_runs(const scaph_page_run* runs, uint32_t num_runs, const scaph_page_run& one,
+              uint32_t num_segs, unsigned long long active_nodes,
+              unsigned long long active_edges, uint32_t n, const uint32_t* degree,
+              uint32_t* level, uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
+              scaph_level_counters* counters, uint32_t next_level, void* workspace,
+              size_t workspace_bytes, const scaph_push_tuning* tune, hipStream_t st) {
+  if (num_segs == 0 || active_nodes == 0 || active_edges == 0) return 0;
+  const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
+  if (active_edges <= small) {
+    const size_t units = (size_t)num_segs * (kBitsNodes / kSmallChunk);
+    hipLaunchKernelGGL(k_push_runs_small, dim3(grid_for(units, kBlock / SCAPH_WAVE, 1024)),
+                       dim3(kBlock), 0, st, runs, num_runs, one, num_segs, front_in, degree, level,
+                       visited, front_out, counters, next_level, n);
+    SCAPH_LAUNCH_CHECK();
+    return 0;
+  }
+  PushWs w = carve_runs_ws(workspace, num_segs, active_nodes, active_edges);
+  if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_runs_count, dim3(num_segs), dim3(kBlock), 0, st, runs, num_runs, one,
+                     front_in, w.blk_cnt, w.blk_edges);
+  SCAPH_LAUNCH_CHECK();
+  const unsigned long long tile_cap = (active_edges + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
+                     num_segs, w.blk_coff, w.blk_eoff, w.totals, tile_cap, active_nodes);
+  SCAPH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_runs_mark, dim3(num_segs), dim3(kBlock), 0, st, runs, num_runs, one,
+                     front_in, w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
+  SCAPH_LAUNCH_CHECK();
+  return launch_expand(w, active_edges, degree, level, visited, nullptr, front_out, counters,
+                       next_level, n, OUT_BITMAP, tune, st);
 }
 
 }  // namespace
@@ -1207,7 +1356,7 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree, ui
                      *out_graph, q_in, n_in,
                      w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
-  return launch_push_tail(w, nb, kBlock, m_in, degree, level, visited, q_out, nullptr, counters,
+  return launch_push_tail(w, nb, m_in, degree, level, visited, q_out, nullptr, counters,
                           cur_level + 1, out_graph->num_vertices, OUT_QUEUE, tune, st);
 }
 
@@ -1222,30 +1371,27 @@ int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned lon
     return SCAPH_STATUS_INVALID;
   if (nodenum == 0) return 0;
   hipStream_t st = as_stream(stream);
+  if (hint && hint->first_vertex != SCAPH_PAGE_COMPACTED) {
+    if ((unsigned long long)hint->first_vertex + nodenum > n) return SCAPH_STATUS_INVALID;
+    scaph_page_run one{};
+    one.buf = reinterpret_cast<unsigned long long>(page_buf);
+    one.first_vertex = hint->first_vertex;
+    one.nodenum = nodenum;
+    return push_runs(nullptr, 1, one, (nodenum + kBitsNodes - 1) / kBitsNodes,
+                     hint->active_nodes, hint->active_edges, n, degree, level, visited, front_in,
+                     front_out, counters, cur_level + 1, workspace, workspace_bytes, tune, st);
+  }
   if (hint) {
-    const bool full = hint->first_vertex != SCAPH_PAGE_COMPACTED;
-    if (full && ((unsigned long long)hint->first_vertex + nodenum > n)) return SCAPH_STATUS_INVALID;
-    if (hint->active_nodes == 0 || hint->active_edges == 0) return 0;
+    // compacted buffer: every Node is active
+    if (hint->active_edges == 0) return 0;
     edgenum = hint->active_edges;
     const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
     if (edgenum <= small) {
-      const uint32_t units = full ? (nodenum + kSmallChunk - 1) / kSmallChunk : nodenum;
-      hipLaunchKernelGGL(k_push_page_small, dim3(grid_for(units, kBlock / SCAPH_WAVE, 64)),
-                         dim3(kBlock), 0, st, page_buf, nodenum, hint->first_vertex, front_in,
-                         degree, level, visited, front_out, counters, cur_level + 1, n);
+      hipLaunchKernelGGL(k_push_compact_small, dim3(grid_for(nodenum, kBlock / SCAPH_WAVE, 64)),
+                         dim3(kBlock), 0, st, page_buf, nodenum, degree, level, visited, front_out,
+                         counters, cur_level + 1, n);
       SCAPH_LAUNCH_CHECK();
       return 0;
-    }
-    if (full) {
-      PushWs w = carve_push_ws(workspace, nodenum, edgenum);
-      if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
-      const uint32_t nb = (nodenum + kBitsNodes - 1) / kBitsNodes;
-      hipLaunchKernelGGL(k_prep_page_bits, dim3(nb), dim3(kBlock), 0, st, page_buf, nodenum,
-                         hint->first_vertex, front_in, w.ent_ep, w.ent_len, w.blk_cnt,
-                         w.blk_edges);
-      SCAPH_LAUNCH_CHECK();
-      return launch_push_tail(w, nb, kBitsNodes, edgenum, degree, level, visited, nullptr,
-                              front_out, counters, cur_level + 1, n, OUT_BITMAP, tune, st);
     }
   }
   PushWs w = carve_push_ws(workspace, nodenum, edgenum);
@@ -1254,8 +1400,29 @@ int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned lon
   hipLaunchKernelGGL(k_prep_page, dim3(nb), dim3(kBlock), 0, st, page_buf, nodenum, front_in,
                      w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
-  return launch_push_tail(w, nb, kBlock, edgenum, degree, level, visited, nullptr, front_out,
-                          counters, cur_level + 1, n, OUT_BITMAP, tune, st);
+  return launch_push_tail(w, nb, edgenum, degree, level, visited, nullptr, front_out, counters,
+                          cur_level + 1, n, OUT_BITMAP, tune, st);
+}
+
+size_t scaph_push_runs_workspace_bytes(uint32_t num_segs, unsigned long long active_nodes,
+                                       unsigned long long active_edges) {
+  return carve_runs_ws(nullptr, num_segs, active_nodes, active_edges).bytes;
+}
+
+int scaph_bfs_push_runs(const scaph_page_run* runs, uint32_t num_runs, uint32_t num_segs,
+                        unsigned long long active_nodes, unsigned long long active_edges,
+                        uint32_t n, const uint32_t* degree, uint32_t* level, uint32_t* visited,
+                        const uint32_t* front_in, uint32_t* front_out,
+                        scaph_level_counters* counters, uint32_t cur_level, void* workspace,
+                        size_t workspace_bytes, const scaph_push_tuning* tune, void* stream) {
+  if (!runs || !degree || !level || !visited || !front_in || !front_out || !counters ||
+      num_runs == 0)
+    return SCAPH_STATUS_INVALID;
+  if (active_edges > 0 && !workspace) return SCAPH_STATUS_INVALID;
+  const scaph_page_run none{};
+  return push_runs(runs, num_runs, none, num_segs, active_nodes, active_edges, n, degree, level,
+                   visited, front_in, front_out, counters, cur_level + 1, workspace,
+                   workspace_bytes, tune, as_stream(stream));
 }
 
 size_t scaph_pull_workspace_bytes(unsigned long long max_edges) {

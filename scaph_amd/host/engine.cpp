@@ -68,6 +68,8 @@ Config& global_flags() {
           "out-edges below which a push level runs as one launch (0 = 8192, 1 = never)");
     r.add("page_hint", FlagType::kBool, &c.page_hint,
           "paged: size page pushes by their page counts and read full pages through the frontier bits");
+    r.add("page_groups", FlagType::kInt32, &c.page_groups,
+          "paged: grouped launches per level for the cached active pages (0 = one per page)");
     r.add("pull_head", FlagType::kBool, &c.pull_head,
           "keep each in-list's first entries beside its in-degree for the pull probes");
     r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");
@@ -98,6 +100,10 @@ bool parse_config(const std::vector<std::string>& args, Config* out,
   }
   if (g.num_stream < 1 || g.num_stream > 64 || g.multi < 1 || g.multi > 64) {
     if (err) *err = "ERROR: -num_stream and -multi must be in 1..64";
+    return false;
+  }
+  if (g.page_groups < 0 || g.page_groups > 4096) {
+    if (err) *err = "ERROR: -page_groups must be in 0..4096";
     return false;
   }
   if (g.page_words < 0 || (g.page_words > 0 && g.page_words < 64)) {

@@ -54,6 +54,8 @@ struct Config {
   int32_t push_small = 0;        // out-edges below which a push level is one launch (0: 8192)
   bool page_hint = true;         // paged: size page pushes by their page counts, read full
                                  // pages through the frontier bits
+  int32_t page_groups = 8;       // paged: cached active pages of a level go out in about this
+                                 // many grouped launches (0: one launch per page)
   int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
 };
 

@@ -89,6 +89,20 @@ void Engine::setup_paged() {
     ps.exec_done.emplace_back(new Event());
     ps.ws.emplace_back(ps.ws_bytes);
   }
+  if (cfg_.page_groups > 0) {
+    // one workspace for the group stream: every segment of the graph, up to
+    // 16 M active Nodes per group, and tiles for all edges
+    uint64_t segs = 0;
+    for (const auto& pg : pages_out_) segs += (pg.nodenum + SCAPH_RUN_SEGMENT - 1) / SCAPH_RUN_SEGMENT;
+    ps.group_nodes_cap = std::max<uint64_t>(std::min<uint64_t>(n_, 16ull << 20), max_nodes);
+    ps.group_ws_bytes = scaph_push_runs_workspace_bytes((uint32_t)segs, ps.group_nodes_cap,
+                                                        std::max<uint64_t>(m_, 1));
+    ps.group_ws.alloc(ps.group_ws_bytes);
+    ps.group.reset(new Stream());
+    ps.group_done.reset(new Event());
+    ps.h_runs.alloc(np * sizeof(scaph_page_run) + 64);
+    ps.d_runs.alloc(np * sizeof(scaph_page_run) + 64);
+  }
   ps.h_nodes.alloc(np * 4 + 64);
   ps.h_datas.alloc(np * 4 + 64);
   ps.h_totals.alloc(64);
@@ -168,8 +182,20 @@ RunStats Engine::bfs_paged(uint32_t src) {
     }
     ps.resident_at_reset.assign(np, 0);
     for (size_t p = 0; p < np; ++p) ps.resident_at_reset[p] = ps.handle.vp[p].cached ? 1 : 0;
-    ps.handle.queue_reset(datas);
+    const bool grouping = cfg_.page_groups > 0;
+    ps.handle.queue_reset(datas, grouping);
     rs.pages_full += ps.handle.num_full;
+    // grouped launches: about page_groups of them over the cached active
+    // pages; the first is smaller when copies wait for slots
+    uint64_t group_target = 0;
+    bool first_group = true;
+    std::vector<int> group_pages;
+    const bool copies_pending = ps.handle.num_full > 0 || !cold.empty();
+    if (grouping) {
+      uint64_t e = 0;
+      for (int p : ps.handle.group_queue()) e += datas[p] - 3ull * nodes[p];
+      group_target = std::max<uint64_t>(e / (uint64_t)cfg_.page_groups, 1);
+    }
 
     // compaction workers (parallel_compress_stat_async, src/compress.h:42-56)
     std::atomic<size_t> next{0};
@@ -237,6 +263,53 @@ RunStats Engine::bfs_paged(uint32_t src) {
           progress = true;
         }
       }
+      if (grouping && ps.group_done->done()) {
+        if (!group_pages.empty()) {
+          for (int p : group_pages) ps.handle.release_exec_finished(p);
+          group_pages.clear();
+          progress = true;
+        }
+        auto& gq = ps.handle.group_queue();
+        if (!gq.empty()) {
+          uint64_t target = group_target;
+          if (first_group && copies_pending) target = std::max<uint64_t>(target / 4, 1);
+          first_group = false;
+          auto* tab = ps.h_runs.get<scaph_page_run>();
+          uint64_t g_edges = 0, g_nodes = 0;
+          uint32_t segs = 0;
+          while (!gq.empty()) {
+            const int p = gq.front();
+            if (!group_pages.empty() &&
+                (g_edges >= target || g_nodes + nodes[p] > ps.group_nodes_cap))
+              break;
+            ps.handle.take_group();
+            const VPstat& v = ps.handle.vp[p];
+            scaph_page_run& r = tab[group_pages.size()];
+            r.buf = reinterpret_cast<unsigned long long>(ps.pool.get<uint32_t>() +
+                                                         (uint64_t)v.dpid * P_);
+            r.first_vertex = pages_out_[p].left;
+            r.nodenum = pages_out_[p].nodenum;
+            r.seg0 = segs;
+            r.reserved = 0;
+            segs += (r.nodenum + SCAPH_RUN_SEGMENT - 1) / SCAPH_RUN_SEGMENT;
+            g_edges += datas[p] - 3ull * nodes[p];
+            g_nodes += nodes[p];
+            group_pages.push_back(p);
+            ++rs.page_execs;
+            if (ps.resident_at_reset[p]) ++rs.page_execs_cached;
+          }
+          void* gs = ps.group->get();
+          H_ERR(scaph_copy_h2d_async(ps.d_runs.get(), tab,
+                                     group_pages.size() * sizeof(scaph_page_run), gs));
+          H_ERR(scaph_bfs_push_runs(ps.d_runs.get<scaph_page_run>(), (uint32_t)group_pages.size(),
+                                    segs, g_nodes, g_edges, n_, deg, level, visited,
+                                    front_[fa].get<uint32_t>(), front_[fa ^ 1].get<uint32_t>(),
+                                    cnt, cur, ps.group_ws.get(), ps.group_ws_bytes, &push_tune_,
+                                    gs));
+          ps.group_done->record(gs);
+          progress = true;
+        }
+      }
       if (ps.copy_end->done()) {
         if (copy_last != -1) {
           if (copy_timed) copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
@@ -267,6 +340,7 @@ RunStats Engine::bfs_paged(uint32_t src) {
     workers.join();
     if (failed) throw Error(SCAPH_STATUS_INVALID, "page compaction failed");
     for (int s = 0; s < ns; ++s) ps.exec[s]->sync();
+    if (ps.group) ps.group->sync();
     ps.copy->sync();
     if (copy_last != -1) {
       copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);

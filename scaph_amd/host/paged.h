@@ -26,6 +26,13 @@ struct Engine::PagedState {
   uint64_t chunk = 0;  // words per chunk (P/32)
   // pages resident at the start of the iteration (page hit accounting)
   std::vector<char> resident_at_reset;
+  // grouped launches of the cached active pages (-page_groups)
+  std::unique_ptr<Stream> group;
+  std::unique_ptr<Event> group_done;
+  DevBuf group_ws, d_runs;
+  PinnedBuf h_runs;
+  size_t group_ws_bytes = 0;
+  uint64_t group_nodes_cap = 0;  // active Nodes one group may hold (workspace size)
 };
 
 }  // namespace scaph

@@ -83,18 +83,26 @@ void PageHandle::init(int list_size, int pool_size, const std::vector<scaph_page
   sharedq_.clear();
   expectq_.clear();
   hold_.clear();
+  groupq_.clear();
   mergeq_.clear();
   merge_reader_ = 0;
 }
 
 bool PageHandle::finished() const {
-  if (!expectq_.empty() || !hold_.empty()) return false;
+  if (!expectq_.empty() || !hold_.empty() || !groupq_.empty()) return false;
   for (const auto& q : execq_)
     if (!q.empty()) return false;
   return true;
 }
 
-void PageHandle::queue_reset(const uint32_t* datas) {
+int PageHandle::take_group() {
+  const int p = groupq_.front();
+  groupq_.pop_front();
+  hold_[p] = 0;
+  return p;
+}
+
+void PageHandle::queue_reset(const uint32_t* datas, bool group_cached) {
   emptyq_.clear();
   sharedq_.clear();
   expectq_.clear();
@@ -104,6 +112,7 @@ void PageHandle::queue_reset(const uint32_t* datas) {
     merge_reader_ = 0;
   }
   for (auto& q : execq_) q.clear();
+  groupq_.clear();
   hold_.clear();
   num_cached = 0;
   for (size_t d = 0; d < dp.size(); ++d) {
@@ -112,7 +121,7 @@ void PageHandle::queue_reset(const uint32_t* datas) {
     } else {
       const int pid = dp[d].vpid;
       if (pid != -1 && datas[pid] > 0) {
-        execq_[0].push_back(pid);
+        (group_cached ? groupq_ : execq_[0]).push_back(pid);
         ++num_cached;
       } else {
         emptyq_.push_back((int)d);

@@ -59,8 +59,13 @@ class PageHandle {
   void init(int list_size, int pool_size, const std::vector<scaph_page>& pages, uint64_t thres,
             int maxentry);
   bool finished() const;
-  // Start of an iteration: datas[p] = active data words of page p.
-  void queue_reset(const uint32_t* datas);
+  // Start of an iteration: datas[p] = active data words of page p.  With
+  // group_cached, the cached active pages wait in group_queue() for grouped
+  // launches instead of joining the rank-0 queue.
+  void queue_reset(const uint32_t* datas, bool group_cached = false);
+  std::deque<int>& group_queue() { return groupq_; }
+  // Moves the front of group_queue() into execution at rank 0.
+  int take_group();
   void release_exec_finished(int vpid);
   void release_copy_finished(int vpid);
   bool get_execute(int* vpid);
@@ -88,6 +93,7 @@ class PageHandle {
   uint64_t thres_ = 0;
   int maxentry_ = 1;
   std::vector<std::deque<int>> execq_;  // by rank
+  std::deque<int> groupq_;              // cached active pages for grouped launches
   std::deque<int> emptyq_;
   std::set<int> sharedq_;
   std::set<int> expectq_;

@@ -235,13 +235,16 @@ def test_paged_out_of_core_parity(page_words, max_mib, threshold, multi, streams
             assert e.check(src) == 0
 
 
-@pytest.mark.parametrize("flags", [[], ["-page_hint=false"], ["-push_small=1"],
-                                   ["-push_small=1000000"], ["-push_small=1", "-push_rows=false"]])
+@pytest.mark.parametrize("flags", [[], ["-page_groups=0"], ["-page_groups=0", "-page_hint=false"],
+                                   ["-push_small=1"], ["-push_small=1000000"],
+                                   ["-push_small=1", "-push_rows=false"], ["-page_groups=1"],
+                                   ["-page_groups=64", "-multi=2"]])
 def test_paged_page_push_variants(flags):
-    """Page pushes sized by the page counts: one-launch small pages, full
-    pages read through their frontier bits (pages of ~14K Nodes, so two
-    8192-Node prep blocks, at unaligned first vertices), compacted pages, and
-    the unhinted path, all against the oracle."""
+    """Page pushes sized by the page counts: grouped launches over several
+    resident pages, one-launch small pages and groups, full pages read
+    through their frontier bits (pages of ~14K Nodes, so two 8192-Node
+    segments, at unaligned first vertices), compacted pages, and the
+    per-page and unhinted paths, all against the oracle."""
     off, ngh = O.rmat_csr(16, 16, seed=8)
     deg = np.diff(off.astype(np.int64))
     with engine("-page_words=262144", "-max_memory=3", "-threshold=3", "-num_stream=3",
