@@ -38,6 +38,10 @@ def main():
     ap.add_argument("--budgets", default="15258,8192",
                     help="-max_memory values in MiB (15258 MiB = 16e9 bytes)")
     ap.add_argument("--sources", type=int, default=4)
+    ap.add_argument("--threshold", type=int, default=8,
+                    help="-threshold of the paged runs (1/32 pages; 8 is the reference's "
+                         "FLAGS_threshold, app/bfs.cu forces 3)")
+    ap.add_argument("--flags", default="", help="more engine flags for the paged runs")
     ap.add_argument("--oracle", action="store_true", help="also run the CPU oracle BFS once")
     args = ap.parse_args()
 
@@ -79,7 +83,9 @@ def main():
            "resident_ms_median": statistics.median(res_ms),
            "levels_resident_vs_oracle": oracle_ok, "budgets": []}
     for b in [int(x) for x in args.budgets.split(",") if x]:
-        eng = BFS(f"-max_memory={b}", "-build_pull=false")
+        paged_flags = [f"-max_memory={b}", "-build_pull=false", f"-threshold={args.threshold}"]
+        paged_flags += args.flags.split()
+        eng = BFS(*paged_flags)
         t0 = time.perf_counter()
         eng.gen_zipf(args.n, args.x_min, args.alpha, args.cap, args.seed)
         inf = eng.info()
@@ -100,7 +106,8 @@ def main():
         h2d_ms = sum(r["h2d_ms"] for r in runs)
         tot_ms = sum(r["ms"] for r in runs)
         out["budgets"].append({
-            "max_memory_mib": b, "resident": inf["resident"], "pages": inf["pages"],
+            "max_memory_mib": b, "flags": paged_flags, "resident": inf["resident"],
+            "pages": inf["pages"],
             "gteps_median": statistics.median(r["m_reached"] / (r["ms"] / 1e3) / 1e9
                                               for r in runs),
             "t_bfs_ms_median": ms,
