@@ -145,7 +145,8 @@ typedef struct scaph_push_tuning {
   uint32_t edge_rows;   /* 1: a wave reads 256 consecutive edges per step (coalesced rows) */
   uint32_t small_edges; /* frontiers with at most this many out-edges run in one launch
                            (0: 8192; 1: effectively never) */
-  uint32_t reserved;
+  uint32_t scan_single; /* at most this many prep blocks are scanned by one 1024-thread block;
+                           more take three coalesced launches (0: 1024) */
 } scaph_push_tuning;
 /* Top-down (push) expand of one level from a frontier queue — the resident
  * counterpart of gpu_push_csr (src/gpu_engine.cuh:41-169) with the BFS

@@ -44,9 +44,22 @@ struct PushWs {
   uint64_t* gx;         // [M] global exclusive edge prefix (dense)
   uint64_t* gep;        // [M] edge-list address (dense)
   uint32_t* tstart;     // [tiles+1] entry containing the tile's first edge
+  uint32_t* part_cnt;   // [nb/1024+2] per-chunk sums and offsets of the parallel block scan
+  uint64_t* part_edges;
+  uint32_t* part_coff;
+  uint64_t* part_eoff;
   unsigned long long* totals;  // [4] entries, edges, tiles
   size_t bytes;
 };
+
+template <typename Take>
+void carve_parts(PushWs& w, size_t nb, Take& take) {
+  const size_t parts = nb / kScanThreads + 2;
+  w.part_cnt = reinterpret_cast<uint32_t*>(take(parts * 4));
+  w.part_edges = reinterpret_cast<uint64_t*>(take(parts * 8));
+  w.part_coff = reinterpret_cast<uint32_t*>(take(parts * 4));
+  w.part_eoff = reinterpret_cast<uint64_t*>(take(parts * 8));
+}
 
 PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_edges) {
   PushWs w;
@@ -70,6 +83,7 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
   w.gep = reinterpret_cast<uint64_t*>(take(m * 8));
   w.tstart = reinterpret_cast<uint32_t*>(take(tiles * 4));
   w.totals = reinterpret_cast<unsigned long long*>(take(64));
+  carve_parts(w, nb, take);
   w.bytes = off;
   return w;
 }
@@ -529,6 +543,68 @@ k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, u
     totals[1] = etot;
     totals[2] = (etot + kTile - 1) / kTile;
     totals[3] = tile_cap;
+  }
+}
+
+// Exclusive scan over a 1024-thread block (16 waves); *total = block sum.
+template <typename T>
+__device__ __forceinline__ T block1024_excl_scan(T v, T* wtot, T* total) {
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x / SCAPH_WAVE;
+  const T x = wave_incl_scan(v);
+  if (lane == SCAPH_WAVE - 1) wtot[wid] = x;
+  __syncthreads();
+  T off = 0, all = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kScanThreads / SCAPH_WAVE; ++w) {
+    const T t = wtot[w];
+    if (w < wid) off += t;
+    all += t;
+  }
+  __syncthreads();
+  *total = all;
+  return off + x - v;
+}
+
+// Parallel block scan for many prep blocks, in three launches: the sums of
+// every 1024 consecutive block counts (k_scan_chunk_sums), k_scan_blocks
+// over those sums, then each block's offset (k_scan_chunk_down).  All loads
+// are coalesced; the single-block k_scan_blocks alone reads nb/1024
+// strided values per thread.
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_chunk_sums(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb,
+                  uint32_t* part_cnt, uint64_t* part_edges) {
+  __shared__ uint32_t wc[kScanThreads / SCAPH_WAVE];
+  __shared__ unsigned long long we[kScanThreads / SCAPH_WAVE];
+  const uint32_t b = blockIdx.x * kScanThreads + threadIdx.x;
+  const uint32_t c = b < nb ? blk_cnt[b] : 0u;
+  const unsigned long long e = b < nb ? blk_edges[b] : 0ull;
+  uint32_t ct;
+  unsigned long long et;
+  block1024_excl_scan(c, wc, &ct);
+  block1024_excl_scan(e, we, &et);
+  if (threadIdx.x == 0) {
+    part_cnt[blockIdx.x] = ct;
+    part_edges[blockIdx.x] = et;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_chunk_down(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb,
+                  const uint32_t* part_coff, const uint64_t* part_eoff, uint32_t* blk_coff,
+                  uint64_t* blk_eoff) {
+  __shared__ uint32_t wc[kScanThreads / SCAPH_WAVE];
+  __shared__ unsigned long long we[kScanThreads / SCAPH_WAVE];
+  const uint32_t b = blockIdx.x * kScanThreads + threadIdx.x;
+  const uint32_t c = b < nb ? blk_cnt[b] : 0u;
+  const unsigned long long e = b < nb ? blk_edges[b] : 0ull;
+  uint32_t ct;
+  unsigned long long et;
+  const uint32_t xc = block1024_excl_scan(c, wc, &ct);
+  const unsigned long long xe = block1024_excl_scan(e, we, &et);
+  if (b < nb) {
+    blk_coff[b] = part_coff[blockIdx.x] + xc;
+    blk_eoff[b] = part_eoff[blockIdx.x] + xe;
   }
 }
 
@@ -1198,6 +1274,26 @@ inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
   return (unsigned)b;
 }
 
+// Offsets of the prep blocks' entries and edges, and the level totals.
+hipError_t launch_block_scan(const PushWs& w, uint32_t nb, unsigned long long tile_cap,
+                             unsigned long long ent_cap, const scaph_push_tuning* tune,
+                             hipStream_t st) {
+  const uint32_t single = tune && tune->scan_single ? tune->scan_single : kScanThreads;
+  if (nb <= single) {
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
+                       nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap, ent_cap);
+    return hipGetLastError();
+  }
+  const uint32_t parts = (nb + kScanThreads - 1) / kScanThreads;
+  hipLaunchKernelGGL(k_scan_chunk_sums, dim3(parts), dim3(kScanThreads), 0, st, w.blk_cnt,
+                     w.blk_edges, nb, w.part_cnt, w.part_edges);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.part_cnt, w.part_edges,
+                     parts, w.part_coff, w.part_eoff, w.totals, tile_cap, ent_cap);
+  hipLaunchKernelGGL(k_scan_chunk_down, dim3(parts), dim3(kScanThreads), 0, st, w.blk_cnt,
+                     w.blk_edges, nb, w.part_coff, w.part_eoff, w.blk_coff, w.blk_eoff);
+  return hipGetLastError();
+}
+
 int launch_expand(const PushWs& w, unsigned long long max_edges, const uint32_t* degree,
                   uint32_t* level, uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
                   scaph_level_counters* counters, uint32_t next_level, uint32_t n, int mode,
@@ -1227,9 +1323,7 @@ int launch_push_tail(const PushWs& w, uint32_t nb, unsigned long long max_edges,
                      uint32_t* front_out, scaph_level_counters* counters, uint32_t next_level,
                      uint32_t n, int mode, const scaph_push_tuning* tune, hipStream_t st) {
   const unsigned long long tile_cap = (max_edges + kTile - 1) / kTile;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
-                     nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap, ~0ull);
-  SCAPH_LAUNCH_CHECK();
+  SCAPH_TRY(launch_block_scan(w, nb, tile_cap, ~0ull, tune, st));
   hipLaunchKernelGGL(k_mark_tiles, dim3(nb), dim3(kBlock), 0, st, w.ent_ep, w.ent_len, w.blk_cnt,
                      w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
   SCAPH_LAUNCH_CHECK();
@@ -1260,6 +1354,7 @@ PushWs carve_runs_ws(void* base, uint32_t num_segs, unsigned long long max_entri
   w.gep = reinterpret_cast<uint64_t*>(take(m * 8));
   w.tstart = reinterpret_cast<uint32_t*>(take(tiles * 4));
   w.totals = reinterpret_cast<unsigned long long*>(take(64));
+  carve_parts(w, nb, take);
   w.bytes = off;
   return w;
 }
@@ -1288,9 +1383,7 @@ int push_runs(const scaph_page_run* runs, uint32_t num_runs, const scaph_page_ru
                      front_in, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
   const unsigned long long tile_cap = (active_edges + kTile - 1) / kTile;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
-                     num_segs, w.blk_coff, w.blk_eoff, w.totals, tile_cap, active_nodes);
-  SCAPH_LAUNCH_CHECK();
+  SCAPH_TRY(launch_block_scan(w, num_segs, tile_cap, active_nodes, tune, st));
   hipLaunchKernelGGL(k_runs_mark, dim3(num_segs), dim3(kBlock), 0, st, runs, num_runs, one,
                      front_in, w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
   SCAPH_LAUNCH_CHECK();

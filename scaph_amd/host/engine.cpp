@@ -66,6 +66,8 @@ Config& global_flags() {
     r.add("push_rows", FlagType::kBool, &c.push_rows, "push expand reads coalesced edge rows");
     r.add("push_small", FlagType::kInt32, &c.push_small,
           "out-edges below which a push level runs as one launch (0 = 8192, 1 = never)");
+    r.add("push_scan", FlagType::kInt32, &c.push_scan,
+          "prep blocks scanned by one block before the scan takes three launches (0 = 1024)");
     r.add("page_hint", FlagType::kBool, &c.page_hint,
           "paged: size page pushes by their page counts and read full pages through the frontier bits");
     r.add("page_groups", FlagType::kInt32, &c.page_groups,
@@ -139,6 +141,7 @@ Engine::Engine(const Config& cfg)
   push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
   push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);
+  push_tune_.scan_single = (uint32_t)std::max(0, cfg.push_scan);
 }
 
 Engine::~Engine() = default;
