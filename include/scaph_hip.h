@@ -147,6 +147,10 @@ typedef struct scaph_push_tuning {
                            (0: 8192; 1: effectively never) */
   uint32_t scan_single; /* at most this many prep blocks are scanned by one 1024-thread block;
                            more take three coalesced launches (0: 1024) */
+  uint32_t xcd_slices;  /* levels with at least this many 1024-edge tiles expand XCD-sliced:
+                           tiles inside one out-list go to the XCD owning their targets'
+                           eighth of the vertex ids (0: off) */
+  uint32_t reserved[3];
 } scaph_push_tuning;
 /* Top-down (push) expand of one level from a frontier queue — the resident
  * counterpart of gpu_push_csr (src/gpu_engine.cuh:41-169) with the BFS

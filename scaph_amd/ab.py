@@ -53,9 +53,13 @@ def main():
         ta.append(min(ea.run(s)["ms"] for _ in range(args.reps)))
         tb.append(min(eb.run(s)["ms"] for _ in range(args.reps)))
         equal &= bool(np.array_equal(ea.levels(), eb.levels()))
+    ratios = [b / a for a, b in zip(ta, tb)]
     print(json.dumps({"A": fa, "B": fb, "sources": len(srcs), "median_A_ms": statistics.median(ta),
                       "median_B_ms": statistics.median(tb),
-                      "median_ratio_B_over_A": statistics.median(b / a for a, b in zip(ta, tb)),
+                      "median_ratio_B_over_A": statistics.median(ratios),
+                      "total_ratio_B_over_A": sum(tb) / sum(ta),
+                      "ratio_quartiles": statistics.quantiles(ratios, n=4) if len(ratios) > 1
+                      else ratios,
                       "levels_equal": equal}), flush=True)
 
 

@@ -31,6 +31,8 @@ constexpr uint32_t kHubTag = 0x80000000u;          // in-list entry = kHubTag | 
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kBitsNodes = SCAPH_RUN_SEGMENT;  // Nodes per run segment (32 per thread)
 constexpr uint32_t kSmallChunk = SCAPH_WAVE * 32;   // Nodes per wave step of k_push_runs_small
+constexpr uint32_t kSlices = 8;                     // target slices of the sliced expand, one per XCD
+constexpr uint32_t kMixed = kSlices;                // tiles whose targets span slices
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -44,6 +46,9 @@ struct PushWs {
   uint64_t* gx;         // [M] global exclusive edge prefix (dense)
   uint64_t* gep;        // [M] edge-list address (dense)
   uint32_t* tstart;     // [tiles+1] entry containing the tile's first edge
+  uint8_t* tslice;      // [tiles] target slice of each tile (XCD-sliced expand)
+  uint32_t* tlist;      // [tiles] tiles grouped by slice (mixed last)
+  uint32_t* tbin;       // [32] tiles per slice, then the scatter cursors
   uint32_t* part_cnt;   // [nb/1024+2] per-chunk sums and offsets of the parallel block scan
   uint64_t* part_edges;
   uint32_t* part_coff;
@@ -82,6 +87,9 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
   w.gx = reinterpret_cast<uint64_t*>(take(m * 8));
   w.gep = reinterpret_cast<uint64_t*>(take(m * 8));
   w.tstart = reinterpret_cast<uint32_t*>(take(tiles * 4));
+  w.tslice = reinterpret_cast<uint8_t*>(take(tiles));
+  w.tlist = reinterpret_cast<uint32_t*>(take(tiles * 4));
+  w.tbin = reinterpret_cast<uint32_t*>(take(32 * 4));
   w.totals = reinterpret_cast<unsigned long long*>(take(64));
   carve_parts(w, nb, take);
   w.bytes = off;
@@ -89,6 +97,10 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
 }
 
 // ------------------------------------------------------------------ init
+
+__global__ void k_fill_words(uint32_t* p, uint32_t v, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
 
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
@@ -637,13 +649,78 @@ k_mark_tiles(const uint64_t* ent_ep, const uint32_t* ent_len, const uint32_t* bl
 
 enum { OUT_QUEUE = 0, OUT_BITMAP = 1 };
 
+// Target slice of every tile (slice x = vertex ids [x n/8, (x+1) n/8)): the
+// slice of its first and last target when the tile lies inside one out-list
+// (lists are sorted by target) and the two agree, else kMixed.  Unsorted
+// lists only mislabel tiles, which costs locality, not correctness.
+__global__ void __launch_bounds__(kBlock)
+k_tile_slices(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
+              const unsigned long long* totals, uint8_t* tslice, uint32_t* tbin, uint32_t n) {
+  __shared__ uint32_t hist[kSlices + 1];
+  if (threadIdx.x <= kSlices) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const unsigned long long n_ent = totals[0];
+  const unsigned long long n_edges = totals[1];
+  const unsigned long long n_tiles = totals[2];
+  const unsigned long long stride = (unsigned long long)gridDim.x * kBlock;
+  for (unsigned long long t = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; t < n_tiles;
+       t += stride) {
+    const unsigned long long r0 = t * kTile;
+    const unsigned long long r1 = min(r0 + kTile, n_edges) - 1;
+    const uint32_t i = tstart[t];
+    uint8_t sl = kMixed;
+    if (i + 1ull >= n_ent || gx[i + 1] > r1) {
+      const uint32_t* e = reinterpret_cast<const uint32_t*>(gep[i]);
+      const unsigned long long b = gx[i];
+      const uint32_t a = e[r0 - b], z = e[r1 - b];
+      if (a < n && z < n) {
+        const uint32_t sa = (uint32_t)((uint64_t)a * kSlices / n);
+        const uint32_t sz = (uint32_t)((uint64_t)z * kSlices / n);
+        if (sa == sz) sl = (uint8_t)sa;
+      }
+    }
+    tslice[t] = sl;
+    atomicAdd(&hist[sl], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x <= kSlices && hist[threadIdx.x]) atomicAdd(&tbin[threadIdx.x], hist[threadIdx.x]);
+}
+
+// Counting-sort scatter of the tiles into tlist, slice 0 first and the
+// mixed tiles last (tbin[0..8] = counts, tbin[16..24] = cursors, cleared
+// before k_tile_slices).
+__global__ void __launch_bounds__(kBlock)
+k_tile_scatter(const unsigned long long* totals, const uint8_t* tslice, uint32_t* tbin,
+               uint32_t* tlist) {
+  __shared__ uint32_t hist[kSlices + 1], base[kSlices + 1];
+  const unsigned long long n_tiles = totals[2];
+  const unsigned long long t0 = (unsigned long long)blockIdx.x * kBlock;
+  if (t0 >= n_tiles) return;  // uniform per block
+  if (threadIdx.x <= kSlices) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const unsigned long long t = t0 + threadIdx.x;
+  const bool live = t < n_tiles;
+  const uint32_t sl = live ? tslice[t] : 0u;
+  const uint32_t rank = live ? atomicAdd(&hist[sl], 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x <= kSlices) {
+    uint32_t off = 0;
+    for (uint32_t b = 0; b < threadIdx.x; ++b) off += tbin[b];
+    base[threadIdx.x] =
+        off + (hist[threadIdx.x] ? atomicAdd(&tbin[16 + threadIdx.x], hist[threadIdx.x]) : 0u);
+  }
+  __syncthreads();
+  if (live) tlist[base[sl] + rank] = (uint32_t)t;
+}
+
 // LAYOUT 0: each thread takes kPerThread consecutive edges of the tile;
 // LAYOUT 1: edge j*kBlock + thread (one coalesced 256-edge row per step).
 template <int MODE, int LAYOUT>
 __global__ void __launch_bounds__(kBlock)
 k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
-         const unsigned long long* totals, const uint32_t* degree, uint32_t* level,
-         uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
+         const unsigned long long* totals, const uint32_t* tlist, const uint32_t* tbin,
+         const uint32_t* degree,
+         uint32_t* level, uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
          scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
   __shared__ unsigned long long s_gx[kTile + 1];
   __shared__ unsigned long long s_ep[kTile + 1];
@@ -655,7 +732,34 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
   const unsigned long long n_edges = totals[1];
   const unsigned long long n_tiles = totals[2];
   unsigned long long acc_n = 0, acc_m = 0;
-  for (unsigned long long t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+  // Sliced (tlist != nullptr): block b = 8j + x takes entries j, j + G/8,
+  // ... of slice x's part of tlist, then entries b, b + G, ... of the mixed
+  // part.  Workgroups are dispatched to the eight XCDs round-robin, so slice
+  // x's visited words stay in one XCD's L2.  Unsliced: tiles b, b + G, ...
+  uint32_t sl_beg = 0, sl_cnt = 0, mx_beg = 0, mx_cnt = 0;
+  if (tlist) {
+    const uint32_t xs = blockIdx.x % kSlices;
+    for (uint32_t b = 0; b < kMixed; ++b) {
+      const uint32_t c = tbin[b];
+      if (b < xs) sl_beg += c;
+      if (b == xs) sl_cnt = c;
+      mx_beg += c;
+    }
+    mx_cnt = tbin[kMixed];
+  }
+  const unsigned long long sl_steps =
+      tlist ? (sl_cnt > blockIdx.x / kSlices
+                   ? (sl_cnt - blockIdx.x / kSlices + gridDim.x / kSlices - 1) / (gridDim.x / kSlices)
+                   : 0)
+            : 0;
+  const unsigned long long steps =
+      tlist ? sl_steps + (mx_cnt > blockIdx.x ? (mx_cnt - blockIdx.x + gridDim.x - 1) / gridDim.x : 0)
+            : (n_tiles > blockIdx.x ? (n_tiles - blockIdx.x + gridDim.x - 1) / gridDim.x : 0);
+  for (unsigned long long k = 0; k < steps; ++k) {
+    unsigned long long t;
+    if (!tlist) t = blockIdx.x + k * gridDim.x;
+    else if (k < sl_steps) t = tlist[sl_beg + blockIdx.x / kSlices + k * (gridDim.x / kSlices)];
+    else t = tlist[mx_beg + blockIdx.x + (k - sl_steps) * gridDim.x];
     const unsigned long long s0 = t * kTile;
     const unsigned long long e0 = min(s0 + kTile, n_edges);
     const uint32_t i0 = tstart[t];
@@ -1298,12 +1402,27 @@ int launch_expand(const PushWs& w, unsigned long long max_edges, const uint32_t*
                   uint32_t* level, uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
                   scaph_level_counters* counters, uint32_t next_level, uint32_t n, int mode,
                   const scaph_push_tuning* tune, hipStream_t st) {
-  unsigned grid = grid_for((size_t)((max_edges + kTile - 1) / kTile), 1, 8192);
+  const unsigned long long tiles = (max_edges + kTile - 1) / kTile;
+  unsigned grid = grid_for((size_t)tiles, 1, 8192);
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
   const bool rows = tune && tune->edge_rows;
+  const uint32_t* tlist = nullptr;
+  if (tune && tune->xcd_slices && tiles >= tune->xcd_slices) {
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, w.tbin, 0u, 32u);
+    SCAPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tile_slices, dim3(grid_for((size_t)tiles, kBlock, 4096)), dim3(kBlock), 0,
+                       st, w.gx, w.gep, w.tstart, w.totals, w.tslice, w.tbin, n);
+    SCAPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for((size_t)tiles, kBlock, 1u << 30)),
+                       dim3(kBlock), 0, st, w.totals, w.tslice, w.tbin, w.tlist);
+    SCAPH_LAUNCH_CHECK();
+    tlist = w.tlist;
+    grid = grid < kSlices ? kSlices : grid / kSlices * kSlices;
+  }
 #define SCAPH_EXPAND(M, L)                                                                       \
   hipLaunchKernelGGL((k_expand<M, L>), dim3(grid), dim3(kBlock), 0, st, w.gx, w.gep, w.tstart,  \
-                     w.totals, degree, level, visited, q_out, front_out, counters, next_level, n)
+                     w.totals, tlist, w.tbin, degree, level, visited, q_out, front_out,       \
+                     counters, next_level, n)
   if (mode == OUT_QUEUE) {
     if (rows) SCAPH_EXPAND(OUT_QUEUE, 1);
     else SCAPH_EXPAND(OUT_QUEUE, 0);
@@ -1353,6 +1472,9 @@ PushWs carve_runs_ws(void* base, uint32_t num_segs, unsigned long long max_entri
   w.gx = reinterpret_cast<uint64_t*>(take(m * 8));
   w.gep = reinterpret_cast<uint64_t*>(take(m * 8));
   w.tstart = reinterpret_cast<uint32_t*>(take(tiles * 4));
+  w.tslice = reinterpret_cast<uint8_t*>(take(tiles));
+  w.tlist = reinterpret_cast<uint32_t*>(take(tiles * 4));
+  w.tbin = reinterpret_cast<uint32_t*>(take(32 * 4));
   w.totals = reinterpret_cast<unsigned long long*>(take(64));
   carve_parts(w, nb, take);
   w.bytes = off;

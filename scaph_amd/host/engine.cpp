@@ -68,6 +68,8 @@ Config& global_flags() {
           "out-edges below which a push level runs as one launch (0 = 8192, 1 = never)");
     r.add("push_scan", FlagType::kInt32, &c.push_scan,
           "prep blocks scanned by one block before the scan takes three launches (0 = 1024)");
+    r.add("push_xcd", FlagType::kInt32, &c.push_xcd,
+          "1024-edge tiles from which a push level expands XCD-sliced (default 8192; 0 = off)");
     r.add("page_hint", FlagType::kBool, &c.page_hint,
           "paged: size page pushes by their page counts and read full pages through the frontier bits");
     r.add("page_groups", FlagType::kInt32, &c.page_groups,
@@ -142,6 +144,7 @@ Engine::Engine(const Config& cfg)
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
   push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);
   push_tune_.scan_single = (uint32_t)std::max(0, cfg.push_scan);
+  push_tune_.xcd_slices = (uint32_t)std::max(0, cfg.push_xcd);
 }
 
 Engine::~Engine() = default;

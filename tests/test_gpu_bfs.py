@@ -239,7 +239,8 @@ def test_paged_out_of_core_parity(page_words, max_mib, threshold, multi, streams
                                    ["-push_small=1"], ["-push_small=1000000"],
                                    ["-push_small=1", "-push_rows=false"], ["-page_groups=1"],
                                    ["-page_groups=64", "-multi=2"],
-                                   ["-push_small=1", "-push_scan=1"]])
+                                   ["-push_small=1", "-push_scan=1"],
+                                   ["-push_small=1", "-push_xcd=1"]])
 def test_paged_page_push_variants(flags):
     """Page pushes sized by the page counts: grouped launches over several
     resident pages, one-launch small pages and groups, full pages read
@@ -287,7 +288,9 @@ def test_pull_heavy_in_lists():
 @pytest.mark.parametrize("flags", [["-push_rows=false"], ["-push_rows=true", "-push_grid=64"],
                                    ["-pull_grid=16"], ["-push_small=1"], ["-push_small=1000000"],
                                    ["-pull_prefetch=false"], ["-push_small=1", "-push_scan=1"],
-                                   ["-push_scan=1000000000"]])
+                                   ["-push_scan=1000000000"], ["-push_small=1", "-push_xcd=1"],
+                                   ["-push_small=1", "-push_xcd=1", "-push_rows=false",
+                                    "-push_grid=24"]])
 def test_launch_shape_variants(flags):
     """The push/pull launch-shape switches change timing only."""
     off, ngh = O.rmat_csr(16, 16, seed=4)
