@@ -167,6 +167,7 @@ def hip():
         _sig(lib, "scaph_page_counts", I32, [P, U32, P, P, I32, P, P, P, P])
         _sig(lib, "scaph_bfs_push_page", I32,
              [P, U32, ctypes.c_ulonglong, U32, P, P, P, P, P, P, U32, P, SZ, P, P, P])
+        _sig(lib, "scaph_xcc_ids", I32, [P, U32, P])
         _sig(lib, "scaph_bfs_validate", I32,
              [ctypes.POINTER(GraphDesc), ctypes.POINTER(GraphDesc), P, P, U32, P, P])
         _sig(lib, "scaph_gen_rmat_keys", I32, [P, U64, U64, U32, U32, U32, U32, U64, P])

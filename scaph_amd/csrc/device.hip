@@ -156,7 +156,20 @@ k_publish(uint32_t* src, uint32_t words, uint32_t clear_words, uint32_t* dst, ui
     __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// XCC (XCD) that runs each workgroup, from the hardware register.
+__global__ void k_xcc_ids(uint32_t* out, uint32_t blocks) {
+  uint32_t x;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+  if (threadIdx.x == 0 && blockIdx.x < blocks) out[blockIdx.x] = x & 0xFu;
+}
+
 extern "C" {
+
+int scaph_xcc_ids(uint32_t* out, uint32_t blocks, void* stream) {
+  if (!out || blocks == 0) return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_xcc_ids, dim3(blocks), dim3(64), 0, as_stream(stream), out, blocks);
+  return (int)hipGetLastError();
+}
 
 int scaph_host_mapped_alloc(void** host_ptr, void** device_ptr, size_t bytes) {
   if (!host_ptr || !device_ptr || bytes == 0) return SCAPH_STATUS_INVALID;

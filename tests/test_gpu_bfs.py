@@ -304,3 +304,26 @@ def test_launch_shape_variants(flags):
                 e.set_schedule(sched)
                 e.run(src)
                 np.testing.assert_array_equal(e.levels(), want, err_msg=f"{flags} {src} {sched}")
+
+
+def test_xcd_round_robin_dispatch():
+    """The sliced push expand gives target eighth x to blocks b with b mod 8
+    == x, which keeps that eighth in one XCD's L2 when workgroups go to the
+    eight XCDs round-robin.  This checks that dispatch order on the device
+    (it affects speed only; every tile has one owner either way)."""
+    hip = _lib.hip()
+    blocks = 4096
+    out = np.full(blocks, 99, np.uint32)
+    p = ctypes.c_void_p()
+    _lib.check(hip.scaph_dev_alloc(ctypes.byref(p), out.nbytes))
+    try:
+        _lib.check(hip.scaph_xcc_ids(p, blocks, None))
+        _lib.check(hip.scaph_copy_d2h_async(out.ctypes.data, p, out.nbytes, None))
+        _lib.check(hip.scaph_device_sync())
+    finally:
+        hip.scaph_dev_free(p)
+    say("xcc of blocks 0..15", out[:16].tolist())
+    assert out.max() < 16
+    xcds = int(out.max()) + 1
+    if xcds == 8:
+        np.testing.assert_array_equal(out, np.arange(blocks, dtype=np.uint32) % 8)
