@@ -72,7 +72,8 @@ def test_rmat_parity_with_oracle(scale, ef, seed, page_words):
             assert e.check(src) == 0
 
 
-@pytest.mark.parametrize("head", [["-pull_head=true"], ["-pull_head=false"],
+@pytest.mark.parametrize("head", [["-pull_head=true", "-pull_head_outdeg=false"],
+                                  ["-pull_head=false"],
                                   ["-pull_head=true", "-pull_head_outdeg=true"]])
 def test_pull_head_records(head):
     """The pull expand with and without the 16-byte in-list head records,
