@@ -22,6 +22,7 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kHubTag = 0x80000000u;
 constexpr uint32_t kPushChunk = 2048;  // out-edges one wave takes per frontier vertex
 constexpr uint32_t kPullSerial = 128;  // in-edges one lane probes before the wave helps
+constexpr uint32_t kMsProbe = 4;       // in-edges a lane probes per step
 constexpr uint32_t kMaxWaves = 16;     // 1024-thread blocks
 
 struct W4 {
@@ -319,17 +320,32 @@ k_ms_pull(scaph_graph_desc g, const uint32_t* hub_list, scaph_ms_state s, uint32
       edges = nd.edges;
       len = nd.len;
       const uint32_t lim = len < kPullSerial ? len : kPullSerial;
-      for (uint32_t i = 0; i < lim; ++i) {
-        const uint32_t u = resolve(edges[i], hub_list);
-        if ((act[u >> 6] >> (u & 63)) & 1ull) {
-          const W4 f = ld4(fr + (size_t)u * 4);
+      // kMsProbe in-edges per step, each stage's loads independent of one
+      // another (entries, hub ids, membership bits, frontier words), so a
+      // lane keeps several misses in flight; the test for "every missing
+      // query found" runs once per step
+      for (uint32_t i = 0; i < lim; i += kMsProbe) {
+        uint32_t u[kMsProbe];
+        bool a[kMsProbe];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) acc.w[k] |= f.w[k] & need.w[k];
-          if (acc.w[0] == need.w[0] && acc.w[1] == need.w[1] && acc.w[2] == need.w[2] &&
-              acc.w[3] == need.w[3]) {
-            done = true;
-            break;
+        for (uint32_t j = 0; j < kMsProbe; ++j) u[j] = i + j < lim ? edges[i + j] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < kMsProbe; ++j) u[j] = i + j < lim ? resolve(u[j], hub_list) : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < kMsProbe; ++j)
+          a[j] = i + j < lim && ((act[u[j] >> 6] >> (u[j] & 63)) & 1ull);
+#pragma unroll
+        for (uint32_t j = 0; j < kMsProbe; ++j) {
+          if (a[j]) {
+            const W4 f = ld4(fr + (size_t)u[j] * 4);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc.w[k] |= f.w[k] & need.w[k];
           }
+        }
+        if (acc.w[0] == need.w[0] && acc.w[1] == need.w[1] && acc.w[2] == need.w[2] &&
+            acc.w[3] == need.w[3]) {
+          done = true;
+          break;
         }
       }
     }
