@@ -75,8 +75,9 @@ int scaph_device_count(int* count);
 int scaph_get_device_props(int device, scaph_device_props* out);
 /* out[b] = the XCC (XCD) that ran workgroup b of a `blocks`-workgroup
  * launch, read from HW_REG_XCC_ID.  The sliced push expand assumes the
- * round-robin dispatch this shows (workgroup b on XCD b mod 8); the
- * assumption affects speed only. */
+ * round-robin dispatch this shows (workgroup b on XCD (s + b) mod 8 for a
+ * start s that varies between launches); the assumption affects speed
+ * only. */
 int scaph_xcc_ids(uint32_t* out, uint32_t blocks, void* stream);
 int scaph_stream_create(void** stream);
 int scaph_stream_destroy(void* stream);

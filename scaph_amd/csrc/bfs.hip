@@ -734,8 +734,9 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
   unsigned long long acc_n = 0, acc_m = 0;
   // Sliced (tlist != nullptr): block b = 8j + x takes entries j, j + G/8,
   // ... of slice x's part of tlist, then entries b, b + G, ... of the mixed
-  // part.  Workgroups are dispatched to the eight XCDs round-robin, so slice
-  // x's visited words stay in one XCD's L2.  Unsliced: tiles b, b + G, ...
+  // part.  Workgroups are dispatched to the eight XCDs round-robin (block b
+  // on XCD (s + b) mod 8 for a per-launch start s), so slice x's visited
+  // words stay in one XCD's L2.  Unsliced: tiles b, b + G, ...
   uint32_t sl_beg = 0, sl_cnt = 0, mx_beg = 0, mx_cnt = 0;
   if (tlist) {
     const uint32_t xs = blockIdx.x % kSlices;
