@@ -56,7 +56,7 @@ def pick_sources(eng, n, count, seed, min_reach_frac):
 
 
 def alg_bytes_step(rec, n):
-    """Algorithmic bytes of one step's expand kernels (see DESIGN.md):
+    """Algorithmic bytes of one step's expand kernels:
     pull: Node record 12 B per examined vertex, 4 B per scanned in-edge,
           4 B level write per vertex found, and the visited-read,
           visited-write and next-frontier-write bitmaps (3 * n/8 B);
@@ -178,8 +178,8 @@ def main():
                     help="run one device-to-device copy of this size first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the BFS would shard)", file=sys.stderr)
+        print("bench.py measures one GPU: the BFS here has no multi-GPU path; the per-GPU "
+              "work at full shape is what --gpus 1 runs", file=sys.stderr)
         sys.exit(2)
 
     import scaph_amd
