@@ -385,6 +385,19 @@ int scaph_mlp_decide(const uint16_t* params_bf16, const float* norm, const uint3
                      const scaph_level_counters* counters, unsigned long long m_u,
                      uint32_t level, float* out, void* stream);
 
+/* The decision of a model whose W1 columns for the 12 histogram features
+ * are zero (it reads n_f, m_f, m_u and the level only), fused with
+ * scaph_publish_words: one launch per level computes the features from
+ * counters (= the counters words) and m_u, runs the forward pass, and
+ * publishes `words` words with the 32-float record of scaph_mlp_decide at
+ * word decide_word (histogram and page count zero), clearing the first
+ * clear_words.  Requires decide_word >= the counters' size in words,
+ * decide_word + 32 <= words and clear_words <= decide_word. */
+int scaph_mlp_decide_publish(const uint16_t* params_bf16, const float* norm, uint32_t* counters,
+                             uint32_t decide_word, unsigned long long m_u, uint32_t level,
+                             uint32_t words, uint32_t clear_words, uint32_t* dst_mapped,
+                             uint32_t* flag_mapped, uint32_t seq, void* stream);
+
 /* scaph_frontier_page_counts and scaph_mlp_decide as the engine uses them
  * once per level (two launches, no host work between): the frontier is a
  * bitmap, or (bitmap NULL) a list whose length is counters->q_len.

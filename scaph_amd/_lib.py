@@ -180,6 +180,11 @@ def hip():
         _sig(lib, "scaph_mlp_sgd", I32, [P, P, P, U32, ctypes.c_float, ctypes.c_float, P, P])
         _sig(lib, "scaph_frontier_page_counts", I32, [P, P, P, P, U32, U32, P, P])
         _sig(lib, "scaph_mlp_decide", I32, [P, P, P, P, U32, P, ctypes.c_ulonglong, U32, P, P])
+        _sig(lib, "scaph_mlp_decide_publish", I32,
+             [P, P, P, U32, ctypes.c_ulonglong, U32, U32, U32, P, P, U32, P])
+        _sig(lib, "scaph_host_mapped_alloc", I32,
+             [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), SZ])
+        _sig(lib, "scaph_publish_words", I32, [P, U32, U32, P, P, U32, P])
         _sig(lib, "scaph_mlp_decide_frontier", I32,
              [P, P, P, P, P, U32, U32, P, ctypes.c_ulonglong, U32, P, P, P])
         _sig(lib, "scaph_graph_capture_end", I32, [P, ctypes.POINTER(P)])

@@ -362,6 +362,67 @@ __device__ void decide_block(const uint16_t* params, const float* norm, const ui
   }
 }
 
+// The decision of a model that reads only the four scalar features (its
+// W1 columns for the histogram are zero), fused with the level hand-off:
+// one wave forms the features from the counters, runs the forward pass and
+// copies `words` words of src to the host-mapped dst with the DecideOut
+// record at word decide_word, clears the first clear_words of src, then
+// releases the flag as scaph_publish_words does.  No frontier pass and no
+// extra launch per level.
+__global__ void __launch_bounds__(SCAPH_WAVE)
+k_decide_publish(const uint16_t* params, const float* norm, uint32_t* src, uint32_t decide_word,
+                 unsigned long long m_u, uint32_t level, uint32_t words, uint32_t clear_words,
+                 uint32_t* dst, uint32_t* flag, uint32_t seq) {
+  __shared__ float rec[sizeof(DecideOut) / 4];
+  RowRegs rows;
+  load_rows(params, rows);
+  const float4* n4 = reinterpret_cast<const float4*>(norm);
+  float nrm[2 * kIn];
+#pragma unroll
+  for (int k = 0; k < 2 * kIn / 4; ++k) {
+    const float4 t = n4[k];
+    nrm[4 * k] = t.x;
+    nrm[4 * k + 1] = t.y;
+    nrm[4 * k + 2] = t.z;
+    nrm[4 * k + 3] = t.w;
+  }
+  const scaph_level_counters* cnt = reinterpret_cast<const scaph_level_counters*>(src);
+  const unsigned long long n_next = cnt->n_next, m_next = cnt->m_next;
+  float x[kIn];
+  x[0] = __log2f(1.f + (float)n_next);
+  x[1] = __log2f(1.f + (float)m_next);
+  x[2] = __log2f(1.f + (float)(m_u > m_next ? m_u - m_next : 0ull));
+  x[3] = (float)level;
+#pragma unroll
+  for (int i = 4; i < kIn; ++i) x[i] = 0.f;
+  float xn[kIn];
+#pragma unroll
+  for (int i = 0; i < kIn; ++i) {
+    const float sc = nrm[kIn + i];
+    xn[i] = (x[i] - nrm[i]) / (sc != 0.f ? sc : 1.f);
+  }
+  float y[kOut];
+  forward_rows(rows, xn, y);
+  const uint32_t lane = threadIdx.x;
+  constexpr uint32_t kRec = sizeof(DecideOut) / 4;
+  if (lane < kRec) {
+    float v = 0.f;
+    if (lane == 0) v = y[0];
+    else if (lane == 1) v = y[1];
+    else if (lane >= 16 && lane < 16 + kIn) v = x[lane - 16];
+    rec[lane] = v;
+  }
+  __syncthreads();
+  for (uint32_t i = lane; i < words; i += SCAPH_WAVE) {
+    const bool in_rec = i >= decide_word && i < decide_word + kRec;
+    dst[i] = in_rec ? __float_as_uint(rec[i - decide_word]) : src[i];
+    if (i < clear_words) src[i] = 0u;
+  }
+  __threadfence_system();
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_decide(const uint16_t* params, const float* norm /* mean[16], scale[16] */,
          const uint32_t* page_cnt, const scaph_page* pages, uint32_t num_pages,
@@ -530,6 +591,22 @@ int scaph_mlp_decide(const uint16_t* params_bf16, const float* norm, const uint3
   hipLaunchKernelGGL(k_decide, dim3(1), dim3(kBlock), 0, as_stream(stream), params_bf16, norm,
                      page_cnt, pages, num_pages, counters, m_u, level,
                      reinterpret_cast<DecideOut*>(out));
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_mlp_decide_publish(const uint16_t* params_bf16, const float* norm, uint32_t* counters,
+                             uint32_t decide_word, unsigned long long m_u, uint32_t level,
+                             uint32_t words, uint32_t clear_words, uint32_t* dst_mapped,
+                             uint32_t* flag_mapped, uint32_t seq, void* stream) {
+  constexpr uint32_t kRec = sizeof(DecideOut) / 4;
+  if (!params_bf16 || !norm || !counters || !dst_mapped || !flag_mapped ||
+      decide_word < sizeof(scaph_level_counters) / 4 || decide_word + kRec > words ||
+      clear_words > decide_word)
+    return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_decide_publish, dim3(1), dim3(SCAPH_WAVE), 0, as_stream(stream),
+                     params_bf16, norm, counters, decide_word, m_u, level, words, clear_words,
+                     dst_mapped, flag_mapped, seq);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

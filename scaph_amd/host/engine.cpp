@@ -58,6 +58,8 @@ Config& global_flags() {
     r.add("trace", FlagType::kString, &c.trace, "per-iteration JSONL trace file");
     r.add("build_pull", FlagType::kBool, &c.build_pull, "keep transposed pages for pull");
     r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
+    r.add("learned_fused", FlagType::kBool, &c.learned_fused,
+          "a model reading only scalar features decides inside the level hand-off");
     r.add("pull_prefetch", FlagType::kBool, &c.pull_prefetch,
           "request the next batch's head records before probing the current one");
     r.add("pull_head_outdeg", FlagType::kBool, &c.pull_head_outdeg,

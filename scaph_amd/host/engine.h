@@ -49,6 +49,7 @@ struct Config {
   int32_t pull_grid = 0;         // cap on pull blocks (0: four per compute unit)
   bool pull_head_outdeg = true;  // head records {e0, e1, in-degree, out-degree}
   bool pull_prefetch = true;     // request the next batch's head records early
+  bool learned_fused = true;     // a scalar-feature model decides inside the level hand-off
   int32_t push_grid = 0;         // cap on push expand blocks (0: one per tile, at most 8192)
   bool push_rows = true;         // push expand reads coalesced 256-edge rows
   int32_t push_small = 0;        // out-edges below which a push level is one launch (0: 8192)

@@ -18,6 +18,9 @@ struct Engine::DeviceModel {
   PinnedBuf hout;   // host landing zone for out
   PinnedBuf hccnt;
   bool ready = false;
+  // the model's W1 columns for the 12 histogram features are zero: the
+  // decision needs no frontier pass and rides on the level hand-off
+  bool scalar_only = false;
 };
 
 }  // namespace scaph

@@ -136,6 +136,12 @@ void Engine::set_device_model(const uint16_t* params, const float* mean, const f
   H_ERR(scaph_copy_h2d_async(model_->norm.get(), norm.data(), norm.size() * 4, main_->get()));
   main_->sync();
   model_->ready = true;
+  // W1 is [64][16] row-major: unit j's weight for feature i at j * 16 + i
+  bool scalar = true;
+  for (int j = 0; j < 64 && scalar; ++j)
+    for (int i = 4; i < 16; ++i)
+      if ((params[j * 16 + i] & 0x7FFFu) != 0u) scalar = false;
+  model_->scalar_only = scalar && cfg_.learned_fused;
 }
 
 bool Engine::device_model_ready() const { return model_ && model_->ready; }
@@ -186,7 +192,8 @@ RunStats Engine::bfs_resident(uint32_t src) {
       ev_[eb + 1]->record(st);
       lev_ev.push_back(eb);
     }
-    if (learned) {
+    const bool fused = learned && model_->scalar_only;
+    if (learned && !fused) {
       // features of the new frontier and the model's choice, on the device;
       // the decision lands beside the counters so one copy brings both back
       DeviceModel& m = *model_;
@@ -200,10 +207,19 @@ RunStats Engine::bfs_resident(uint32_t src) {
     // the counters (and the decision) land in mapped host memory, the
     // counters are cleared for the next level, and a flag says so
     const uint32_t seq = ++hand_seq_;
-    H_ERR(scaph_publish_words(counters_.get<uint32_t>(),
-                              (uint32_t)(learned ? kDecideWord + kDecideFloats : kCounterWords),
-                              (uint32_t)kCounterWords, hand_.dev<uint32_t>(),
-                              hand_.dev<uint32_t>() + kHandFlagWord, seq, st));
+    if (fused) {
+      // scalar-feature model: the decision is formed inside the hand-off
+      H_ERR(scaph_mlp_decide_publish(model_->params.get<uint16_t>(), model_->norm.get<float>(),
+                                     counters_.get<uint32_t>(), (uint32_t)kDecideWord, m_u,
+                                     cur + 1, (uint32_t)(kDecideWord + kDecideFloats),
+                                     (uint32_t)kCounterWords, hand_.dev<uint32_t>(),
+                                     hand_.dev<uint32_t>() + kHandFlagWord, seq, st));
+    } else {
+      H_ERR(scaph_publish_words(counters_.get<uint32_t>(),
+                                (uint32_t)(learned ? kDecideWord + kDecideFloats : kCounterWords),
+                                (uint32_t)kCounterWords, hand_.dev<uint32_t>(),
+                                hand_.dev<uint32_t>() + kHandFlagWord, seq, st));
+    }
     if (timed) ev_[eb + 2]->record(st);
     wait_handoff(seq);
     LevelRecord rec;

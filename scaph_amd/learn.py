@@ -149,12 +149,20 @@ def bf16_to_f32(h):
     return (np.asarray(h, np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
-def init_params(seed=0):
+SCALAR_FEATURES = 4  # log2(1+n_f), log2(1+m_f), log2(1+m_u), level
+
+
+def init_params(seed=0, scalar=False):
+    """He-normal weights, zero biases.  scalar: W1's columns for the 12
+    histogram features start (and, with those inputs held at zero, stay)
+    zero, so the model reads the scalar features only."""
     rng = np.random.default_rng(seed)
     p = np.zeros(NPARAM, np.float32)
     o = 0
     for fan_in, fan_out in ((16, 64), (64, 64), (64, 2)):
         w = rng.normal(0, np.sqrt(2.0 / fan_in), (fan_out, fan_in)).astype(np.float32)
+        if scalar and fan_in == 16:
+            w[:, SCALAR_FEATURES:] = 0.0
         p[o:o + w.size] = w.ravel()
         o += w.size + fan_out  # biases stay 0
     return p
@@ -202,12 +210,15 @@ class DeviceMLP:
         return dy.get(np.float32, 2 * len(X)).reshape(-1, 2)
 
 
-def train(X, T, steps=4000, lr=0.05, mu=0.9, w1=1.0, seed=0):
+def train(X, T, steps=4000, lr=0.05, mu=0.9, w1=1.0, seed=0, scalar=False):
+    if scalar:
+        X = np.array(X, np.float32)
+        X[:, SCALAR_FEATURES:] = 0.0
     mean = X.mean(0)
     scale = X.std(0)
     scale[scale < 1e-6] = 1.0
     Xn = ((X - mean) / scale).astype(np.float32)
-    mlp = DeviceMLP(init_params(seed))
+    mlp = DeviceMLP(init_params(seed, scalar))
     dx = DevArray(Xn)
     dT = DevArray(np.asarray(T, np.float32))
     losses = []
@@ -268,20 +279,33 @@ def main():
     ap.add_argument("--sources", type=int, default=2)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--out", default=DEFAULT_MODEL)
-    ap.add_argument("--traces", default="")
+    ap.add_argument("--traces", default="", help="also save the training traces here (.npz)")
+    ap.add_argument("--load-traces", default="", help="train on saved traces instead of new ones")
+    ap.add_argument("--histogram", action="store_true",
+                    help="let the model read the 12 page-histogram features too (the decision "
+                         "then needs a frontier pass per level)")
     args = ap.parse_args()
-    specs = training_specs(args.graphs)
-    recs = collect(specs, args.sources)
-    X, T = features_labels(recs)
-    if args.traces:
+    scalar = not args.histogram
+    if args.load_traces:
+        z = np.load(args.load_traces, allow_pickle=False)
+        X, T = z["X"], z["T"]
+        recs = None
+    else:
+        specs = training_specs(args.graphs)
+        recs = collect(specs, args.sources)
+        X, T = features_labels(recs)
+    if args.traces and recs is not None:
         np.savez_compressed(args.traces, X=X, T=T,
                             t_push=np.array([r["t_push_ms"] for r in recs], np.float32),
                             t_pull=np.array([r["t_pull_ms"] for r in recs], np.float32),
                             graph=np.array([r["graph"] for r in recs], np.int32))
     log(f"{len(X)} training records; pull faster in {(T[:, 0] > 0.5).mean() * 100:.1f}%")
-    mlp, mean, scale, losses = train(X, T, steps=args.steps)
+    mlp, mean, scale, losses = train(X, T, steps=args.steps, scalar=scalar)
     log("loss", losses)
-    pred = mlp.predict(((X - mean) / scale).astype(np.float32))
+    Xs = np.array(X, np.float32)
+    if scalar:
+        Xs[:, SCALAR_FEATURES:] = 0.0
+    pred = mlp.predict(((Xs - mean) / scale).astype(np.float32))
     acc = ((pred[:, 0] > 0) == (T[:, 0] > 0.5)).mean()
     log(f"training accuracy of the direction choice: {acc * 100:.1f}%")
     save_model(args.out, mlp.params_bf16(), mean, scale)

@@ -215,3 +215,55 @@ def test_trained_model_on_held_out_graph():
             f"per-level best {t_best:.3f} ms")
         # the model picks (nearly) the faster direction at every level
         assert t_model <= 1.15 * t_best
+
+
+def test_scalar_model_decision_fused_into_hand_off():
+    """A model with zero W1 columns for the histogram decides inside the
+    level hand-off (scaph_mlp_decide_publish): its record equals the
+    two-kernel decision's logit and threshold and the oracle's forward pass
+    on the four scalar features, the counters arrive in mapped host memory
+    behind the flag, and the device counters are cleared."""
+    hip = _lib.hip()
+    rng = np.random.default_rng(11)
+    master = L.init_params(seed=5, scalar=True)
+    assert np.all(master[:64 * 16].reshape(64, 16)[:, 4:] == 0)
+    params = L.f32_to_bf16(master)
+    norm = np.concatenate([rng.normal(0, 1, 16), rng.uniform(0.5, 2, 16)]).astype(np.float32)
+    d_par, d_norm = L.DevArray(params), L.DevArray(norm)
+    words, decide_word, clear_words = 96, 64, 12
+    m_u = 123456789
+    pages = np.array([[0, 99, 100, 0], [100, 199, 100, 0]], np.uint32)
+    d_pages = L.DevArray(pages)
+    d_pc = L.DevArray(np.zeros(2, np.uint32))
+    hp, dp = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(hip.scaph_host_mapped_alloc(ctypes.byref(hp), ctypes.byref(dp), 4 * 512))
+    try:
+        host = np.ctypeslib.as_array(ctypes.cast(hp, ctypes.POINTER(ctypes.c_uint32)), (512,))
+        host[:] = 0
+        for seq, (n_next, m_next, level) in enumerate([(5, 40, 1), (700000, 9000000, 4),
+                                                       (0, 0, 9)], start=1):
+            src = np.zeros(words, np.uint32)
+            cnt = src[:12].view(np.uint64)
+            cnt[0], cnt[1], cnt[2] = n_next, m_next, 77
+            d_src = L.DevArray(src)
+            d_out = L.DevArray(nbytes=128)
+            _lib.check(hip.scaph_mlp_decide(d_par.ptr, d_norm.ptr, d_pc.ptr, d_pages.ptr, 2,
+                                            d_src.ptr, m_u, level, d_out.ptr, None))
+            _lib.check(hip.scaph_mlp_decide_publish(
+                d_par.ptr, d_norm.ptr, d_src.ptr, decide_word, m_u, level, words, clear_words,
+                dp, ctypes.c_void_p(dp.value + 4 * 256), seq, None))
+            _lib.check(hip.scaph_device_sync())
+            assert host[256] == seq
+            np.testing.assert_array_equal(host[:12], src[:12])
+            assert np.all(d_src.get(np.uint32, words)[:clear_words] == 0)
+            rec = host[decide_word:decide_word + 32].view(np.float32)
+            two = d_out.get(np.float32, 32)
+            np.testing.assert_array_equal(rec[:2], two[:2])
+            x = np.zeros((1, 16), np.float32)
+            x[0, :4] = [np.log2(1 + n_next), np.log2(1 + m_next), np.log2(1 + m_u - m_next),
+                        level]
+            np.testing.assert_allclose(rec[16:20], x[0, :4], rtol=1e-5)
+            y, _, _ = O.mlp_forward(L.bf16_to_f32(params), ((x - norm[:16]) / norm[16:]))
+            np.testing.assert_allclose(rec[:2], y[0], rtol=1e-4, atol=1e-4)
+    finally:
+        hip.scaph_host_pinned_free(hp)
