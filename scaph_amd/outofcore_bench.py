@@ -38,9 +38,10 @@ def main():
     ap.add_argument("--budgets", default="15258,8192",
                     help="-max_memory values in MiB (15258 MiB = 16e9 bytes)")
     ap.add_argument("--sources", type=int, default=4)
-    ap.add_argument("--threshold", type=int, default=8,
-                    help="-threshold of the paged runs (1/32 pages; 8 is the reference's "
-                         "FLAGS_threshold, app/bfs.cu forces 3)")
+    ap.add_argument("--threshold", type=int, default=16,
+                    help="-threshold of the paged runs in 1/32 pages: 16 compacts pages whose "
+                         "active data is at most half a page (the reference's FLAGS_threshold is "
+                         "8; app/bfs.cu forces 3)")
     ap.add_argument("--flags", default="", help="more engine flags for the paged runs")
     ap.add_argument("--oracle", action="store_true", help="also run the CPU oracle BFS once")
     args = ap.parse_args()
