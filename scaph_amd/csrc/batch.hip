@@ -18,6 +18,7 @@ namespace {
 
 constexpr uint32_t kMaxBatchVertices = 65536;  // per graph: the LDS visited bitmap
 constexpr uint32_t kMaxBlock = 1024;
+constexpr uint32_t kRun = 4;  // consecutive edges per thread per expansion round
 
 // Exclusive block-wide prefix of v (any multiple of 64 up to 1024 threads);
 // *total receives the sum.
@@ -89,29 +90,38 @@ k_batch_bfs(const uint64_t* offsets, const uint32_t* ngh, const scaph_batch_entr
         c_pre[tid] = i < tail ? pre : 0xFFFFFFFFu;
         __syncthreads();
         const uint32_t rows = tail - c < B ? tail - c : B;
-        for (uint32_t e0 = 0; e0 < T; e0 += B) {
-          const uint32_t e = e0 + tid;
-          bool claimed = false;
-          uint32_t v = 0;
-          if (e < T) {
-            // last row whose prefix <= e
-            uint32_t lo = 0, hi = rows;
+        // each thread takes kRun consecutive edges of the chunk: one binary
+        // search over the rows' prefixes for the first, then a forward walk
+        for (uint32_t e0 = 0; e0 < T; e0 += B * kRun) {
+          const uint32_t ef = e0 + tid * kRun;
+          uint32_t lo = 0;
+          if (ef < T) {
+            uint32_t hi = rows;  // last row whose prefix <= ef
             while (hi - lo > 1) {
               const uint32_t mid = (lo + hi) >> 1;
-              if (c_pre[mid] <= e) lo = mid; else hi = mid;
+              if (c_pre[mid] <= ef) lo = mid; else hi = mid;
             }
-            v = (uint32_t)(ngh[c_off[lo] + (e - c_pre[lo])] - vb);
-            const uint32_t bit = 1u << (v & 31);
-            if (!(vis[v >> 5] & bit)) claimed = !(atomicOr(&vis[v >> 5], bit) & bit);
           }
-          const uint64_t m = __ballot(claimed);
-          if (m) {
-            uint32_t base = 0;
-            if (lane_id() == 0) base = atomicAdd(&s_tail, (uint32_t)__popcll(m));
-            base = __shfl(base, 0, SCAPH_WAVE);
-            if (claimed) {
-              q[base + lanes_below(m)] = v;
-              lev[v] = L + 1;
+#pragma unroll
+          for (uint32_t j = 0; j < kRun; ++j) {
+            const uint32_t e = ef + j;
+            bool claimed = false;
+            uint32_t v = 0;
+            if (e < T) {
+              while (lo + 1 < rows && c_pre[lo + 1] <= e) ++lo;
+              v = (uint32_t)(ngh[c_off[lo] + (e - c_pre[lo])] - vb);
+              const uint32_t bit = 1u << (v & 31);
+              if (!(vis[v >> 5] & bit)) claimed = !(atomicOr(&vis[v >> 5], bit) & bit);
+            }
+            const uint64_t m = __ballot(claimed);
+            if (m) {
+              uint32_t base = 0;
+              if (lane_id() == 0) base = atomicAdd(&s_tail, (uint32_t)__popcll(m));
+              base = __shfl(base, 0, SCAPH_WAVE);
+              if (claimed) {
+                q[base + lanes_below(m)] = v;
+                lev[v] = L + 1;
+              }
             }
           }
         }
