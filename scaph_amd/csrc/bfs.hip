@@ -23,7 +23,7 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kTile = 1024;                   // edges per expand tile
 constexpr uint32_t kPerThread = kTile / kBlock;    // 4 consecutive edges
 constexpr unsigned long long kSmallEdges = 8192;  // push frontiers this small run in one launch
-constexpr uint32_t kPullSerial = 16;               // per-lane scan before wave help
+constexpr uint32_t kPullSerial = 8;                // per-lane scan before wave help
 constexpr uint32_t kHeavy = 512;                   // longer in-lists go to k_pull_heavy
 constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work item
 constexpr uint32_t kMaxHubs = 32768;               // hub frontier bits kept in LDS (4 KB)
@@ -938,7 +938,7 @@ __device__ __forceinline__ NodeRef load_node_hint(const PageTab& t, const scaph_
 
 constexpr uint32_t kPullWinWords = 32;
 constexpr uint32_t kPullWin = kPullWinWords * 32;
-constexpr uint32_t kPullProbe = 4;
+constexpr uint32_t kPullProbe = 2;
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
