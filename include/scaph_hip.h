@@ -147,13 +147,14 @@ size_t scaph_push_workspace_bytes(uint32_t max_entries, unsigned long long max_e
 
 /* Launch shape of the push expand (NULL or 0 fields = built-in choice). */
 typedef struct scaph_push_tuning {
-  uint32_t grid_blocks; /* cap on expand blocks (one 1024-edge tile per block per pass) */
+  uint32_t grid_blocks; /* cap on expand blocks (one tile per block per pass: 2048 edges, 1024 on
+                           XCD-sliced levels) */
   uint32_t edge_rows;   /* 1: a wave reads 256 consecutive edges per step (coalesced rows) */
   uint32_t small_edges; /* frontiers with at most this many out-edges run in one launch
                            (0: 8192; 1: effectively never) */
   uint32_t scan_single; /* at most this many prep blocks are scanned by one 1024-thread block;
                            more take three coalesced launches (0: 1024) */
-  uint32_t xcd_slices;  /* levels with at least this many 1024-edge tiles expand XCD-sliced:
+  uint32_t xcd_slices;  /* levels with at least 1024 x this many out-edges expand XCD-sliced:
                            tiles inside one out-list go to the XCD owning their targets'
                            eighth of the vertex ids (0: off) */
   uint32_t reserved[3];

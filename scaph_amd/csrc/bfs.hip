@@ -13,15 +13,16 @@
 //
 // Push load balance: the frontier's adjacency lists are laid end to end
 // (block-local compaction + scan of the out-degrees), cut into tiles of
-// kTile edges, and every 256-thread block expands whole tiles — a hub with a
-// million out-edges is spread over ~1000 blocks instead of one CTA.
+// 2048 edges (1024 on levels routed to XCDs by target), and every 256-thread
+// block expands whole tiles — a hub with a million out-edges is spread over
+// hundreds of blocks instead of one.
 #include "common.h"
 
 namespace {
 
 constexpr uint32_t kBlock = 256;
-constexpr uint32_t kTile = 1024;                   // edges per expand tile
-constexpr uint32_t kPerThread = kTile / kBlock;    // 4 consecutive edges
+constexpr uint32_t kTileSmall = 10;               // log2 edges per tile of XCD-sliced levels
+constexpr uint32_t kTileLarge = 11;               // log2 edges per tile of other levels
 constexpr unsigned long long kSmallEdges = 8192;  // push frontiers this small run in one launch
 constexpr uint32_t kPullSerial = 8;                // per-lane scan before wave help
 constexpr uint32_t kHeavy = 512;                   // longer in-lists go to k_pull_heavy
@@ -70,7 +71,7 @@ PushWs carve_push_ws(void* base, uint32_t max_entries, unsigned long long max_ed
   PushWs w;
   const size_t nb = (max_entries + kBlock - 1) / kBlock + 1;
   const size_t m = (size_t)max_entries + 1;
-  const size_t tiles = (size_t)((max_edges + kTile - 1) / kTile) + 2;
+  const size_t tiles = (size_t)(max_edges >> kTileSmall) + 2;
   size_t off = 0;
   char* p = static_cast<char*>(base);
   auto take = [&](size_t bytes) {
@@ -390,6 +391,7 @@ k_runs_mark(const scaph_page_run* runs, uint32_t num_runs, scaph_page_run one,
   __shared__ unsigned long long we[kBlock / SCAPH_WAVE];
   const unsigned long long n_ent = totals[0];
   const unsigned long long n_tiles = totals[2];
+  const uint32_t ts = (uint32_t)totals[4];
   if (n_ent == 0) return;
   const SegBits s = seg_bits(run_of(runs, num_runs, one, blockIdx.x), blockIdx.x, front);
   uint32_t cnt = 0;
@@ -410,8 +412,8 @@ k_runs_mark(const scaph_page_run* runs, uint32_t num_runs, scaph_page_run one,
     if (!len || gi >= n_ent) continue;
     gx[gi] = g;
     gep[gi] = reinterpret_cast<uint64_t>(eb + s.buf[3ull * i + 1]);
-    const unsigned long long t0 = (g + kTile - 1) / kTile;
-    unsigned long long t1 = (g + len + kTile - 1) / kTile;
+    const unsigned long long t0 = (g + (1ull << ts) - 1) >> ts;
+    unsigned long long t1 = (g + len + (1ull << ts) - 1) >> ts;
     if (t1 > n_tiles) t1 = n_tiles;
     for (unsigned long long t = t0; t < t1; ++t) tstart[t] = gi;
     ++gi;
@@ -506,7 +508,7 @@ k_push_compact_small(const uint32_t* buf, uint32_t nodenum, const uint32_t* degr
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, uint32_t* blk_coff,
               uint64_t* blk_eoff, unsigned long long* totals, unsigned long long tile_cap,
-              unsigned long long ent_cap) {
+              unsigned long long ent_cap, uint32_t tile_shift) {
   __shared__ uint32_t wc[kScanThreads / SCAPH_WAVE];
   __shared__ unsigned long long we[kScanThreads / SCAPH_WAVE];
   const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
@@ -548,13 +550,14 @@ k_scan_blocks(const uint32_t* blk_cnt, const uint64_t* blk_edges, uint32_t nb, u
   if (threadIdx.x == 0) {
     // A caller that under-reports the frontier's edges gets a truncated
     // (wrong, detectable) level instead of writes past the tile table.
-    if (etot > tile_cap * kTile) etot = tile_cap * kTile;
+    if (etot > (tile_cap << tile_shift)) etot = tile_cap << tile_shift;
     // More entries than the dense arrays hold: nothing is expanded.
     if (ctot > ent_cap) ctot = 0, etot = 0;
     totals[0] = ctot;
     totals[1] = etot;
-    totals[2] = (etot + kTile - 1) / kTile;
+    totals[2] = (etot + (1ull << tile_shift) - 1) >> tile_shift;
     totals[3] = tile_cap;
+    totals[4] = tile_shift;
   }
 }
 
@@ -638,8 +641,9 @@ k_mark_tiles(const uint64_t* ent_ep, const uint32_t* ent_len, const uint32_t* bl
     const unsigned long long g0 = blk_eoff[b] + incl - len;
     gx[gi] = g0;
     gep[gi] = ent_ep[src];
-    const unsigned long long t0 = (g0 + kTile - 1) / kTile;
-    unsigned long long t1 = (g0 + len + kTile - 1) / kTile;
+    const uint32_t ts = (uint32_t)totals[4];
+    const unsigned long long t0 = (g0 + (1ull << ts) - 1) >> ts;
+    unsigned long long t1 = (g0 + len + (1ull << ts) - 1) >> ts;
     if (t1 > totals[2]) t1 = totals[2];
     for (unsigned long long t = t0; t < t1; ++t) tstart[t] = gi;
   }
@@ -665,8 +669,8 @@ k_tile_slices(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
   const unsigned long long stride = (unsigned long long)gridDim.x * kBlock;
   for (unsigned long long t = (unsigned long long)blockIdx.x * kBlock + threadIdx.x; t < n_tiles;
        t += stride) {
-    const unsigned long long r0 = t * kTile;
-    const unsigned long long r1 = min(r0 + kTile, n_edges) - 1;
+    const unsigned long long r0 = t << totals[4];
+    const unsigned long long r1 = min(r0 + (1ull << totals[4]), n_edges) - 1;
     const uint32_t i = tstart[t];
     uint8_t sl = kMixed;
     if (i + 1ull >= n_ent || gx[i + 1] > r1) {
@@ -715,13 +719,14 @@ k_tile_scatter(const unsigned long long* totals, const uint8_t* tslice, uint32_t
 
 // LAYOUT 0: each thread takes kPerThread consecutive edges of the tile;
 // LAYOUT 1: edge j*kBlock + thread (one coalesced 256-edge row per step).
-template <int MODE, int LAYOUT>
+template <int MODE, int LAYOUT, uint32_t kTile>
 __global__ void __launch_bounds__(kBlock)
 k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
          const unsigned long long* totals, const uint32_t* tlist, const uint32_t* tbin,
          const uint32_t* degree,
          uint32_t* level, uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
          scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
+  constexpr uint32_t kPerThread = kTile / kBlock;  // edges per thread per tile
   __shared__ unsigned long long s_gx[kTile + 1];
   __shared__ unsigned long long s_ep[kTile + 1];
   __shared__ uint32_t s_claim[kTile];
@@ -1381,34 +1386,45 @@ inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
 
 // Offsets of the prep blocks' entries and edges, and the level totals.
 hipError_t launch_block_scan(const PushWs& w, uint32_t nb, unsigned long long tile_cap,
-                             unsigned long long ent_cap, const scaph_push_tuning* tune,
-                             hipStream_t st) {
+                             unsigned long long ent_cap, uint32_t tile_shift,
+                             const scaph_push_tuning* tune, hipStream_t st) {
   const uint32_t single = tune && tune->scan_single ? tune->scan_single : kScanThreads;
   if (nb <= single) {
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.blk_cnt, w.blk_edges,
-                       nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap, ent_cap);
+                       nb, w.blk_coff, w.blk_eoff, w.totals, tile_cap, ent_cap, tile_shift);
     return hipGetLastError();
   }
   const uint32_t parts = (nb + kScanThreads - 1) / kScanThreads;
   hipLaunchKernelGGL(k_scan_chunk_sums, dim3(parts), dim3(kScanThreads), 0, st, w.blk_cnt,
                      w.blk_edges, nb, w.part_cnt, w.part_edges);
   hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanThreads), 0, st, w.part_cnt, w.part_edges,
-                     parts, w.part_coff, w.part_eoff, w.totals, tile_cap, ent_cap);
+                     parts, w.part_coff, w.part_eoff, w.totals, tile_cap, ent_cap, tile_shift);
   hipLaunchKernelGGL(k_scan_chunk_down, dim3(parts), dim3(kScanThreads), 0, st, w.blk_cnt,
                      w.blk_edges, nb, w.part_coff, w.part_eoff, w.blk_coff, w.blk_eoff);
   return hipGetLastError();
 }
 
+// Levels routed to XCDs by target use 1024-edge tiles (a tile inside one
+// sorted list then spans fewer targets); the others 2048-edge tiles (fewer
+// per-tile stagings and appends).
+bool sliced_level(unsigned long long max_edges, const scaph_push_tuning* tune) {
+  return tune && tune->xcd_slices && (max_edges >> kTileSmall) >= tune->xcd_slices;
+}
+
+uint32_t tile_shift_for(unsigned long long max_edges, const scaph_push_tuning* tune) {
+  return sliced_level(max_edges, tune) ? kTileSmall : kTileLarge;
+}
+
 int launch_expand(const PushWs& w, unsigned long long max_edges, const uint32_t* degree,
                   uint32_t* level, uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
                   scaph_level_counters* counters, uint32_t next_level, uint32_t n, int mode,
-                  const scaph_push_tuning* tune, hipStream_t st) {
-  const unsigned long long tiles = (max_edges + kTile - 1) / kTile;
+                  uint32_t tile_shift, const scaph_push_tuning* tune, hipStream_t st) {
+  const unsigned long long tiles = (max_edges + (1ull << tile_shift) - 1) >> tile_shift;
   unsigned grid = grid_for((size_t)tiles, 1, 8192);
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
   const bool rows = tune && tune->edge_rows;
   const uint32_t* tlist = nullptr;
-  if (tune && tune->xcd_slices && tiles >= tune->xcd_slices) {
+  if (tile_shift == kTileSmall && sliced_level(max_edges, tune)) {
     hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, w.tbin, 0u, 32u);
     SCAPH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_tile_slices, dim3(grid_for((size_t)tiles, kBlock, 4096)), dim3(kBlock), 0,
@@ -1420,17 +1436,21 @@ int launch_expand(const PushWs& w, unsigned long long max_edges, const uint32_t*
     tlist = w.tlist;
     grid = grid < kSlices ? kSlices : grid / kSlices * kSlices;
   }
-#define SCAPH_EXPAND(M, L)                                                                       \
-  hipLaunchKernelGGL((k_expand<M, L>), dim3(grid), dim3(kBlock), 0, st, w.gx, w.gep, w.tstart,  \
-                     w.totals, tlist, w.tbin, degree, level, visited, q_out, front_out,       \
-                     counters, next_level, n)
+#define SCAPH_EXPAND(M, L, T)                                                                    \
+  hipLaunchKernelGGL((k_expand<M, L, T>), dim3(grid), dim3(kBlock), 0, st, w.gx, w.gep,          \
+                     w.tstart, w.totals, tlist, w.tbin, degree, level, visited, q_out,          \
+                     front_out, counters, next_level, n)
+#define SCAPH_EXPAND_T(M, L)                                                                     \
+  if (tile_shift == kTileSmall) SCAPH_EXPAND(M, L, 1u << kTileSmall);                          \
+  else SCAPH_EXPAND(M, L, 1u << kTileLarge)
   if (mode == OUT_QUEUE) {
-    if (rows) SCAPH_EXPAND(OUT_QUEUE, 1);
-    else SCAPH_EXPAND(OUT_QUEUE, 0);
+    if (rows) { SCAPH_EXPAND_T(OUT_QUEUE, 1); }
+    else { SCAPH_EXPAND_T(OUT_QUEUE, 0); }
   } else {
-    if (rows) SCAPH_EXPAND(OUT_BITMAP, 1);
-    else SCAPH_EXPAND(OUT_BITMAP, 0);
+    if (rows) { SCAPH_EXPAND_T(OUT_BITMAP, 1); }
+    else { SCAPH_EXPAND_T(OUT_BITMAP, 0); }
   }
+#undef SCAPH_EXPAND_T
 #undef SCAPH_EXPAND
   SCAPH_LAUNCH_CHECK();
   return 0;
@@ -1442,13 +1462,14 @@ int launch_push_tail(const PushWs& w, uint32_t nb, unsigned long long max_edges,
                      const uint32_t* degree, uint32_t* level, uint32_t* visited, uint32_t* q_out,
                      uint32_t* front_out, scaph_level_counters* counters, uint32_t next_level,
                      uint32_t n, int mode, const scaph_push_tuning* tune, hipStream_t st) {
-  const unsigned long long tile_cap = (max_edges + kTile - 1) / kTile;
-  SCAPH_TRY(launch_block_scan(w, nb, tile_cap, ~0ull, tune, st));
+  const uint32_t ts = tile_shift_for(max_edges, tune);
+  const unsigned long long tile_cap = (max_edges + (1ull << ts) - 1) >> ts;
+  SCAPH_TRY(launch_block_scan(w, nb, tile_cap, ~0ull, ts, tune, st));
   hipLaunchKernelGGL(k_mark_tiles, dim3(nb), dim3(kBlock), 0, st, w.ent_ep, w.ent_len, w.blk_cnt,
                      w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
   SCAPH_LAUNCH_CHECK();
   return launch_expand(w, max_edges, degree, level, visited, q_out, front_out, counters,
-                       next_level, n, mode, tune, st);
+                       next_level, n, mode, ts, tune, st);
 }
 
 // Workspace of a run push: per-segment counts and offsets, the dense entry
@@ -1458,7 +1479,7 @@ PushWs carve_runs_ws(void* base, uint32_t num_segs, unsigned long long max_entri
   PushWs w{};
   const size_t nb = (size_t)num_segs + 1;
   const size_t m = (size_t)max_entries + 1;
-  const size_t tiles = (size_t)((max_edges + kTile - 1) / kTile) + 2;
+  const size_t tiles = (size_t)(max_edges >> kTileSmall) + 2;
   size_t off = 0;
   char* p = static_cast<char*>(base);
   auto take = [&](size_t bytes) {
@@ -1505,13 +1526,14 @@ int push_runs(const scaph_page_run* runs, uint32_t num_runs, const scaph_page_ru
   hipLaunchKernelGGL(k_runs_count, dim3(num_segs), dim3(kBlock), 0, st, runs, num_runs, one,
                      front_in, w.blk_cnt, w.blk_edges);
   SCAPH_LAUNCH_CHECK();
-  const unsigned long long tile_cap = (active_edges + kTile - 1) / kTile;
-  SCAPH_TRY(launch_block_scan(w, num_segs, tile_cap, active_nodes, tune, st));
+  const uint32_t ts = tile_shift_for(active_edges, tune);
+  const unsigned long long tile_cap = (active_edges + (1ull << ts) - 1) >> ts;
+  SCAPH_TRY(launch_block_scan(w, num_segs, tile_cap, active_nodes, ts, tune, st));
   hipLaunchKernelGGL(k_runs_mark, dim3(num_segs), dim3(kBlock), 0, st, runs, num_runs, one,
                      front_in, w.blk_coff, w.blk_eoff, w.gx, w.gep, w.tstart, w.totals);
   SCAPH_LAUNCH_CHECK();
   return launch_expand(w, active_edges, degree, level, visited, nullptr, front_out, counters,
-                       next_level, n, OUT_BITMAP, tune, st);
+                       next_level, n, OUT_BITMAP, ts, tune, st);
 }
 
 }  // namespace

@@ -71,7 +71,7 @@ Config& global_flags() {
     r.add("push_scan", FlagType::kInt32, &c.push_scan,
           "prep blocks scanned by one block before the scan takes three launches (0 = 1024)");
     r.add("push_xcd", FlagType::kInt32, &c.push_xcd,
-          "1024-edge tiles from which a push level expands XCD-sliced (default 65536; 0 = off)");
+          "out-edges / 1024 from which a push level expands XCD-sliced (default 65536; 0 = off)");
     r.add("page_hint", FlagType::kBool, &c.page_hint,
           "paged: size page pushes by their page counts and read full pages through the frontier bits");
     r.add("page_groups", FlagType::kInt32, &c.page_groups,

@@ -54,7 +54,7 @@ struct Config {
   bool push_rows = true;         // push expand reads coalesced 256-edge rows
   int32_t push_small = 0;        // out-edges below which a push level is one launch (0: 8192)
   int32_t push_scan = 0;         // prep blocks scanned by one block; more: 3 launches (0: 1024)
-  int32_t push_xcd = 65536;      // tiles from which a push level expands XCD-sliced (0: off)
+  int32_t push_xcd = 65536;      // out-edges/1024 from which a push level expands XCD-sliced
   bool page_hint = true;         // paged: size page pushes by their page counts, read full
                                  // pages through the frontier bits
   int32_t page_groups = 8;       // paged: cached active pages of a level go out in about this
