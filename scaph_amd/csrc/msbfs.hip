@@ -21,7 +21,7 @@ namespace {
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kHubTag = 0x80000000u;
 constexpr uint32_t kPushChunk = 2048;  // out-edges one wave takes per frontier vertex
-constexpr uint32_t kPullSerial = 128;  // in-edges one lane probes before the wave helps
+constexpr uint32_t kPullSerial = 8;    // in-edges one lane probes before the wave helps
 constexpr uint32_t kMsProbe = 4;       // in-edges a lane probes per step
 constexpr uint32_t kMaxWaves = 16;     // 1024-thread blocks
 
