@@ -78,6 +78,20 @@ int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* 
                      size_t errlen);
 int scaph_engine_set_schedule(scaph_engine* e, const char* name, double alpha, double beta,
                               char* err, size_t errlen);
+/* The reference's other App plugins on the resident graph (app/wcc.cu,
+ * app/sssp.cu): values until no update, read with scaph_engine_levels.
+ * WCC: value[v] = the least vertex id reaching v (v itself included), as
+ * app/wcc.cu's label push computes on directed edges.  SSSP: shortest-path
+ * length from src over the edge weights, 0x7FFFFFFF (MAXPATH) if
+ * unreached; weights must keep distances below 2^31.  stats: depth =
+ * rounds, m_reached = edges relaxed over all rounds, ms = wall time. */
+int scaph_engine_wcc(scaph_engine* e, scaph_bfs_stats* out, char* err, size_t errlen);
+int scaph_engine_sssp(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* err,
+                      size_t errlen);
+/* Edge weights in CSR edge order (_csr.wgh, src/graph.h:56-60); count = m.
+ * Loading a CSR that carries weights sets them already. */
+int scaph_engine_set_weights(scaph_engine* e, const uint32_t* wgh, uint64_t count, char* err,
+                             size_t errlen);
 /* Learned schedule weights: SCAPH_MLP_NPARAM bf16 values + feature scale. */
 int scaph_engine_set_model(scaph_engine* e, const uint16_t* params_bf16, const float* feat_mean,
                            const float* feat_scale, float prefetch_gain, char* err,

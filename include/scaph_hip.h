@@ -285,6 +285,34 @@ int scaph_bfs_push_runs(const scaph_page_run* runs, uint32_t num_runs, uint32_t 
                         scaph_level_counters* counters, uint32_t cur_level, void* workspace,
                         size_t workspace_bytes, const scaph_push_tuning* tune, void* stream);
 
+/* ----------------------------------------------------- other App plugins */
+/* Label-correcting push rounds for the reference's WCC and SSSP apps on the
+ * same paged graph (app/wcc.cu:14-24 with app/wcc_kernel.cuh: value[v] = v,
+ * every vertex active, dst = min(dst, src); app/sssp.cu:13-31 with
+ * app/sssp_kernel.cuh: value = 0x7fffffff except value[src] = 0,
+ * dst = min(dst, src + w)).  The fixpoint is the minimum over all paths,
+ * independent of the update order. */
+#define SCAPH_RELAX_WCC 0
+#define SCAPH_RELAX_SSSP 1
+#define SCAPH_RELAX_INF 0x7FFFFFFFu /* unreached distance (MAXPATH, app/sssp.cu:11) */
+/* value[v] = v (WCC), or SCAPH_RELAX_INF with value[src] = 0 (SSSP). */
+int scaph_relax_init(uint32_t* value, uint32_t n, int mode, uint32_t src, void* stream);
+/* Workspace for the chunks of long out-lists of one round (max_edges =
+ * the graph's edges). */
+size_t scaph_relax_workspace_bytes(unsigned long long max_edges);
+/* One round over the active vertices queue[0..n_queue): every out-edge
+ * (u, v) offers value[u] (+ its weight when `weights` is given) to v; a value
+ * lowered marks v in front_out (atomic OR; front_out must be clear) and
+ * counts in counters->n_next (newly marked vertices) and ->scanned.
+ * weights (optional): a device array laid out like the page pool whose
+ * base is pool_base — the weight of the out-edge at address a is at
+ * weights + (a - pool_base).  counters->reserved becomes 1 if the chunk
+ * workspace overflowed (the round is then incomplete). */
+int scaph_relax_round(const scaph_graph_desc* graph, const uint32_t* queue, uint32_t n_queue,
+                      const uint32_t* weights, const uint32_t* pool_base, uint32_t* value,
+                      uint32_t* front_out, scaph_level_counters* counters, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
 /* Level-array check kernel: counts edges (u,v) violating
  * level[v] <= level[u] + 1 and reached vertices without a parent at
  * level-1 (BFS invariants; in_graph entries decoded through hub_list).

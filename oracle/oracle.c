@@ -13,6 +13,17 @@
  *                         shortest paths, closed forms (path, star, grid,
  *                         binary tree, cycle), brute-force relaxation and the
  *                         depth-63 cap on a long path.
+ *   oracle_wcc            app/wcc.cu:60-85 (check()): value[v] = v, then
+ *                         sweeps of the out-edges in vertex order lowering
+ *                         value[ng] to value[vid] until nothing changes —
+ *                         pinned by scipy connected components on symmetric
+ *                         graphs and by brute-force reachability (least id
+ *                         among the vertices reaching v) on small digraphs.
+ *   oracle_sssp           app/sssp.cu / app/sssp_kernel.cuh (the relax
+ *                         dst = min(dst, src + w), value 0x7FFFFFFF for
+ *                         unreached): Dijkstra with a binary heap — pinned by
+ *                         scipy's dijkstra and, with unit weights, by
+ *                         oracle_bfs.
  *   oracle_page_cuts,     tools/bin2pagecsr.cpp:106-163, 197-252 — pinned
  *   oracle_fill_page      byte-for-byte against the reference tool compiled
  *                         from its own source (oracle/_ref) and the
@@ -57,6 +68,119 @@
  * relaxed from u when value[u] + 1 < value[v].  `inf` is the initial
  * value: 63 reproduces MAXDEPTH (app/bfs.cu:18), 0xFFFFFFFF gives exact
  * depths. */
+/* app/wcc.cu:60-85: label sweeps until a sweep changes nothing. */
+ORACLE_API int oracle_wcc(uint32_t n, const uint64_t* offsets, const uint32_t* ngh,
+                          uint32_t* value) {
+  for (uint32_t v = 0; v < n; ++v) value[v] = v;
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    for (uint32_t vid = 0; vid < n; ++vid)
+      for (uint64_t k = offsets[vid]; k < offsets[(uint64_t)vid + 1]; ++k) {
+        const uint32_t ng = ngh[k];
+        if (ng < n && value[vid] < value[ng]) {
+          value[ng] = value[vid];
+          changed = 1;
+        }
+      }
+  }
+  return 0;
+}
+
+/* Shortest paths from src over u32 weights (Dijkstra, binary heap of
+ * (distance, vertex) pairs with lazy deletion); 0x7FFFFFFF for unreached
+ * (MAXPATH, app/sssp.cu:11).  Returns 3 if a distance reaches 2^31. */
+ORACLE_API int oracle_sssp(uint32_t n, const uint64_t* offsets, const uint32_t* ngh,
+                           const uint32_t* wgh, uint32_t src, uint32_t* value) {
+  if (src >= n) return 1;
+  const uint64_t m = offsets[n];
+  uint64_t cap = m + 2, size = 0;
+  uint64_t* hd = (uint64_t*)malloc(sizeof(uint64_t) * cap);
+  uint32_t* hv = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+  uint64_t* dist = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n : 1));
+  if (!hd || !hv || !dist) {
+    free(hd);
+    free(hv);
+    free(dist);
+    return 2;
+  }
+  for (uint32_t v = 0; v < n; ++v) dist[v] = UINT64_MAX;
+  dist[src] = 0;
+  hd[0] = 0;
+  hv[0] = src;
+  size = 1;
+  int rc = 0;
+  while (size) {
+    const uint64_t d = hd[0];
+    const uint32_t u = hv[0];
+    /* pop: move the last element to the root and sift down */
+    --size;
+    hd[0] = hd[size];
+    hv[0] = hv[size];
+    for (uint64_t i = 0;;) {
+      uint64_t l = 2 * i + 1, r = l + 1, s = i;
+      if (l < size && hd[l] < hd[s]) s = l;
+      if (r < size && hd[r] < hd[s]) s = r;
+      if (s == i) break;
+      const uint64_t td = hd[i];
+      const uint32_t tv = hv[i];
+      hd[i] = hd[s];
+      hv[i] = hv[s];
+      hd[s] = td;
+      hv[s] = tv;
+      i = s;
+    }
+    if (d != dist[u]) continue; /* stale entry */
+    for (uint64_t k = offsets[u]; k < offsets[(uint64_t)u + 1]; ++k) {
+      const uint32_t v = ngh[k];
+      if (v >= n) continue;
+      const uint64_t nd = d + wgh[k];
+      if (nd < dist[v]) {
+        dist[v] = nd;
+        if (size == cap) { /* lazy entries beyond m + 2: grow */
+          cap *= 2;
+          uint64_t* nhd = (uint64_t*)realloc(hd, sizeof(uint64_t) * cap);
+          uint32_t* nhv = (uint32_t*)realloc(hv, sizeof(uint32_t) * cap);
+          if (!nhd || !nhv) {
+            free(nhd ? nhd : hd);
+            free(nhv ? nhv : hv);
+            free(dist);
+            return 2;
+          }
+          hd = nhd;
+          hv = nhv;
+        }
+        uint64_t i = size++;
+        hd[i] = nd;
+        hv[i] = v;
+        while (i > 0) { /* sift up */
+          const uint64_t p = (i - 1) / 2;
+          if (hd[p] <= hd[i]) break;
+          const uint64_t td = hd[i];
+          const uint32_t tv = hv[i];
+          hd[i] = hd[p];
+          hv[i] = hv[p];
+          hd[p] = td;
+          hv[p] = tv;
+          i = p;
+        }
+      }
+    }
+  }
+  for (uint32_t v = 0; v < n; ++v) {
+    if (dist[v] == UINT64_MAX) {
+      value[v] = 0x7FFFFFFFu;
+    } else {
+      if (dist[v] >= 0x7FFFFFFFull) rc = 3;
+      value[v] = (uint32_t)dist[v];
+    }
+  }
+  free(hd);
+  free(hv);
+  free(dist);
+  return rc;
+}
+
 ORACLE_API int oracle_bfs(uint32_t n, const uint64_t* offsets, const uint32_t* ngh, uint32_t src,
                           uint32_t inf, uint32_t* value) {
   if (src >= n) return 1;

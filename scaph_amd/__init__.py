@@ -107,6 +107,30 @@ class BFS:
                    e.value.decode())
         return {k: getattr(st, k) for k, _ in _lib.BfsStats._fields_}
 
+    # -- the reference's other App plugins (app/wcc.cu, app/sssp.cu) --------
+    def set_weights(self, wgh):
+        """Edge weights in CSR edge order (the _csr.wgh file), for sssp()."""
+        w = np.ascontiguousarray(wgh, dtype=np.uint32)
+        e = _err()
+        _lib.check(self._lib.scaph_engine_set_weights(self._h, _ptr(w), w.size, e, 1024),
+                   e.value.decode())
+
+    def wcc(self):
+        """Least vertex id reaching each vertex (read with levels())."""
+        st = _lib.BfsStats()
+        e = _err()
+        _lib.check(self._lib.scaph_engine_wcc(self._h, ctypes.byref(st), e, 1024),
+                   e.value.decode())
+        return {k: getattr(st, k) for k, _ in _lib.BfsStats._fields_}
+
+    def sssp(self, src):
+        """Shortest-path lengths over the weights (0x7FFFFFFF unreached; levels())."""
+        st = _lib.BfsStats()
+        e = _err()
+        _lib.check(self._lib.scaph_engine_sssp(self._h, src, ctypes.byref(st), e, 1024),
+                   e.value.decode())
+        return {k: getattr(st, k) for k, _ in _lib.BfsStats._fields_}
+
     def set_schedule(self, name, alpha=0.0, beta=0.0):
         e = _err()
         _lib.check(self._lib.scaph_engine_set_schedule(self._h, name.encode(), alpha, beta, e,

@@ -215,6 +215,9 @@ def host():
              [E, ctypes.POINTER(U32), ctypes.POINTER(U64), ctypes.POINTER(U32),
               ctypes.POINTER(I32), ctypes.POINTER(I32)])
         _sig(lib, "scaph_engine_bfs", I32, [E, U32, ctypes.POINTER(BfsStats), CHARP, SZ])
+        _sig(lib, "scaph_engine_wcc", I32, [E, ctypes.POINTER(BfsStats), CHARP, SZ])
+        _sig(lib, "scaph_engine_sssp", I32, [E, U32, ctypes.POINTER(BfsStats), CHARP, SZ])
+        _sig(lib, "scaph_engine_set_weights", I32, [E, P, U64, CHARP, SZ])
         _sig(lib, "scaph_engine_set_schedule", I32, [E, CHARP, DBL, DBL, CHARP, SZ])
         _sig(lib, "scaph_engine_set_model", I32, [E, P, P, P, ctypes.c_float, CHARP, SZ])
         _sig(lib, "scaph_engine_level_records", I32,

@@ -1,0 +1,208 @@
+// relax.hip — label-correcting push rounds for the reference's other App
+// plugins on the same paged graph:
+//   WCC  (app/wcc.cu:14-24, app/wcc_kernel.cuh): value[v] = v, every vertex
+//        active, an out-edge (u, v) offers value[u] to v;
+//   SSSP (app/sssp.cu:13-31, app/sssp_kernel.cuh): value = 0x7fffffff,
+//        value[src] = 0, an out-edge (u, v, w) offers value[u] + w.
+// A round takes the active vertices as a list.  One wave takes a vertex and
+// its lanes walk the out-list; a value lowered by atomicMin marks the target
+// active for the next round.  Lists longer than kRelaxChunk continue as
+// chunks in a second, grid-wide pass.  The fixpoint — the minimum over all
+// paths — does not depend on the order of the updates, so the values equal
+// the reference's asynchronous push for every schedule.
+#include "common.h"
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kRelaxChunk = 4096;  // out-edges per wave item; longer lists are chunked
+
+struct RelaxChunk {
+  uint32_t u;
+  uint32_t len;
+  uint64_t ptr;  // address of the chunk's first out-edge
+};
+
+struct RelaxWs {
+  unsigned long long* count;  // chunks appended this round
+  RelaxChunk* chunks;
+  unsigned long long cap;
+  size_t bytes;
+};
+
+RelaxWs carve_relax_ws(void* base, unsigned long long max_edges) {
+  RelaxWs w;
+  w.cap = max_edges / kRelaxChunk + 64;
+  w.bytes = 256 + w.cap * sizeof(RelaxChunk);
+  char* p = static_cast<char*>(base);
+  w.count = reinterpret_cast<unsigned long long*>(p);
+  w.chunks = p ? reinterpret_cast<RelaxChunk*>(p + 256) : nullptr;
+  return w;
+}
+
+// The lane's share of out-edges e[0..len) of a vertex whose value is `val`;
+// the weight of edge e[k] sits `wdelta` bytes after it (weights laid out
+// like the page pool), or every weight is 0 when wdelta == 0 and !WEIGHTED.
+template <int WEIGHTED>
+__device__ __forceinline__ void relax_list(const uint32_t* e, uint32_t len, uint32_t val,
+                                           long long wdelta, uint32_t n, uint32_t* value,
+                                           uint32_t* front_out, unsigned long long& acc_n,
+                                           unsigned long long& acc_s, uint32_t lane,
+                                           uint32_t stride) {
+  const uint32_t* w = WEIGHTED ? reinterpret_cast<const uint32_t*>(
+                                     reinterpret_cast<const char*>(e) + wdelta)
+                               : nullptr;
+  for (uint32_t k = lane; k < len; k += stride) {
+    const uint32_t v = e[k];
+    ++acc_s;
+    if (v >= n) continue;
+    const uint32_t cand = WEIGHTED ? val + w[k] : val;
+    if (cand < value[v]) {
+      const uint32_t old = atomicMin(&value[v], cand);
+      if (cand < old) {
+        const uint32_t bit = 1u << (v & 31);
+        if (!(atomicOr(&front_out[v >> 5], bit) & bit)) ++acc_n;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void relax_flush(scaph_level_counters* counters,
+                                            unsigned long long acc_n, unsigned long long acc_s) {
+  acc_n = wave_sum(acc_n);
+  acc_s = wave_sum(acc_s);
+  if (lane_id() == 0) {
+    if (acc_n) atomicAdd(&counters->n_next, acc_n);
+    if (acc_s) atomicAdd(&counters->scanned, acc_s);
+  }
+}
+
+template <int WEIGHTED>
+__global__ void __launch_bounds__(kBlock)
+k_relax(scaph_graph_desc g, const uint32_t* q, uint32_t n_q, long long wdelta, uint32_t* value,
+        uint32_t* front_out, scaph_level_counters* counters, RelaxWs ws) {
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  const PageTab t = stage_pages(g, dyn_lds);
+  __syncthreads();
+  const uint32_t n = g.num_vertices;
+  const uint32_t lane = lane_id();
+  const uint32_t nw = kBlock / SCAPH_WAVE;
+  unsigned long long acc_n = 0, acc_s = 0;
+  for (uint32_t i = blockIdx.x * nw + threadIdx.x / SCAPH_WAVE; i < n_q; i += gridDim.x * nw) {
+    const uint32_t u = q[i];
+    if (u >= n) continue;
+    const NodeRef nr = load_node(t, g, u);
+    const uint32_t val = value[u];
+    const uint32_t first = nr.len < kRelaxChunk ? nr.len : kRelaxChunk;
+    relax_list<WEIGHTED>(nr.edges, first, val, wdelta, n, value, front_out, acc_n, acc_s, lane,
+                         SCAPH_WAVE);
+    if (nr.len > kRelaxChunk && lane == 0) {
+      const uint32_t rest = nr.len - kRelaxChunk;
+      const uint32_t nch = (rest + kRelaxChunk - 1) / kRelaxChunk;
+      const unsigned long long at = atomicAdd(ws.count, (unsigned long long)nch);
+      if (at + nch <= ws.cap) {
+        for (uint32_t c = 0; c < nch; ++c) {
+          RelaxChunk ch;
+          ch.u = u;
+          ch.len = min(kRelaxChunk, rest - c * kRelaxChunk);
+          ch.ptr = reinterpret_cast<uint64_t>(nr.edges + kRelaxChunk + (uint64_t)c * kRelaxChunk);
+          ws.chunks[at + c] = ch;
+        }
+      } else {
+        counters->reserved = 1;  // overflow: the host reports an error
+      }
+    }
+  }
+  relax_flush(counters, acc_n, acc_s);
+}
+
+// The chunks of long lists: one block per chunk, grid-stride.
+template <int WEIGHTED>
+__global__ void __launch_bounds__(kBlock)
+k_relax_chunks(RelaxWs ws, uint32_t n, long long wdelta, uint32_t* value, uint32_t* front_out,
+               scaph_level_counters* counters) {
+  const unsigned long long cnt = *ws.count;
+  const unsigned long long total = cnt < ws.cap ? cnt : ws.cap;
+  unsigned long long acc_n = 0, acc_s = 0;
+  for (unsigned long long c = blockIdx.x; c < total; c += gridDim.x) {
+    const RelaxChunk ch = ws.chunks[c];
+    relax_list<WEIGHTED>(reinterpret_cast<const uint32_t*>(ch.ptr), ch.len, value[ch.u], wdelta,
+                         n, value, front_out, acc_n, acc_s, threadIdx.x, kBlock);
+  }
+  relax_flush(counters, acc_n, acc_s);
+}
+
+__global__ void k_relax_zero(unsigned long long* p) { *p = 0; }
+
+__global__ void __launch_bounds__(kBlock)
+k_relax_init(uint32_t* value, uint32_t n, uint32_t mode, uint32_t src) {
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += stride)
+    value[v] = mode == SCAPH_RELAX_WCC ? v : (v == src ? 0u : SCAPH_RELAX_INF);
+}
+
+inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
+  size_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t scaph_relax_workspace_bytes(unsigned long long max_edges) {
+  return carve_relax_ws(nullptr, max_edges).bytes;
+}
+
+int scaph_relax_init(uint32_t* value, uint32_t n, int mode, uint32_t src, void* stream) {
+  if (!value || n == 0 || (mode != SCAPH_RELAX_WCC && mode != SCAPH_RELAX_SSSP) ||
+      (mode == SCAPH_RELAX_SSSP && src >= n))
+    return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_relax_init, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0,
+                     as_stream(stream), value, n, (uint32_t)mode, src);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_relax_round(const scaph_graph_desc* graph, const uint32_t* queue, uint32_t n_queue,
+                      const uint32_t* weights, const uint32_t* pool_base, uint32_t* value,
+                      uint32_t* front_out, scaph_level_counters* counters, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+  if (!graph || !value || !front_out || !counters || !workspace || (weights && !pool_base))
+    return SCAPH_STATUS_INVALID;
+  if (n_queue == 0) return 0;
+  if (!queue) return SCAPH_STATUS_INVALID;
+  RelaxWs ws = carve_relax_ws(workspace, 0);
+  // chunk capacity from the workspace the caller sized with
+  // scaph_relax_workspace_bytes(graph edges): every chunk of one round fits
+  ws.cap = workspace_bytes > 256 ? (workspace_bytes - 256) / sizeof(RelaxChunk) : 0;
+  if (ws.cap == 0) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  const long long wdelta =
+      weights ? (long long)(reinterpret_cast<uintptr_t>(weights) -
+                            reinterpret_cast<uintptr_t>(pool_base))
+              : 0ll;
+  hipLaunchKernelGGL(k_relax_zero, dim3(1), dim3(1), 0, st, ws.count);
+  SCAPH_LAUNCH_CHECK();
+  const unsigned grid = grid_for(n_queue, kBlock / SCAPH_WAVE, 16384);
+  const size_t lds = page_lds_bytes(graph->num_pages);
+  if (weights) {
+    hipLaunchKernelGGL(k_relax<1>, dim3(grid), dim3(kBlock), lds, st, *graph, queue, n_queue,
+                       wdelta, value, front_out, counters, ws);
+    SCAPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_relax_chunks<1>, dim3(2048), dim3(kBlock), 0, st, ws,
+                       graph->num_vertices, wdelta, value, front_out, counters);
+  } else {
+    hipLaunchKernelGGL(k_relax<0>, dim3(grid), dim3(kBlock), lds, st, *graph, queue, n_queue,
+                       wdelta, value, front_out, counters, ws);
+    SCAPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_relax_chunks<0>, dim3(2048), dim3(kBlock), 0, st, ws,
+                       graph->num_vertices, wdelta, value, front_out, counters);
+  }
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

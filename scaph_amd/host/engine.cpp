@@ -165,6 +165,7 @@ void Engine::reset_graph() {
   dptr_out_.reset();
   dpages_in_.reset();
   dptr_in_.reset();
+  wpool_.reset();
   deg_out_.reset();
   deg_in_.reset();
   static_visited_.reset();
@@ -199,6 +200,40 @@ void Engine::plan_residency() {
   resident_ = pool_size_ >= list;
   const int64_t need_in = (int64_t)pages_in_.size();
   has_in_ = cfg_.build_pull && resident_ && need_in > 0 && pool - list >= need_in;
+}
+
+// Weights in the pages' own layout: page p's out-edge k sits at word
+// offset(p) + 3 * nodenum + k of the packed pool, and its weight at the same
+// offset of wpool_, so a kernel holding an out-edge's address finds its
+// weight at a fixed distance.  Pages are consecutive vertex ranges, so page
+// p's edges are CSR edges [sum of earlier edgenum, + edgenum).
+void Engine::set_weights(const uint32_t* wgh, uint64_t count) {
+  if (!resident_) throw Error(SCAPH_STATUS_INVALID, "edge weights need the graph resident");
+  if (count != m_ || (m_ && !wgh)) throw Error(SCAPH_STATUS_INVALID, "one weight per edge expected");
+  uint64_t total = 0;
+  std::vector<uint64_t> offs(pages_out_.size());
+  for (size_t p = 0; p < pages_out_.size(); ++p) {
+    offs[p] = total;
+    total += align_words(page_used_words(pages_out_[p], false));
+  }
+  if (total * 4 != pool_out_.bytes() && !(total == 0 && pool_out_.bytes() == 4))
+    throw Error(SCAPH_STATUS_INVALID, "page pool layout mismatch");
+  wpool_.alloc(std::max<uint64_t>(total, 1) * 4);
+  PinnedBuf stage;
+  const uint64_t stage_words = std::max<uint64_t>(P_, 1ull << 22);
+  stage.alloc(stage_words * 4);
+  uint32_t* host = stage.get<uint32_t>();
+  void* st = main_->get();
+  uint64_t e0 = 0;
+  for (size_t p = 0; p < pages_out_.size(); ++p) {
+    const scaph_page& pg = pages_out_[p];
+    const uint64_t words = page_used_words(pg, false);
+    std::fill(host, host + 3ull * pg.nodenum, 0u);
+    std::copy(wgh + e0, wgh + e0 + pg.edgenum, host + 3ull * pg.nodenum);
+    H_ERR(scaph_copy_h2d_async(wpool_.get<uint32_t>() + offs[p], host, words * 4, st));
+    main_->sync();
+    e0 += pg.edgenum;
+  }
 }
 
 void Engine::upload_pages(const HostCsr& g, bool weighted, std::vector<scaph_page>* pages,
@@ -327,6 +362,8 @@ void Engine::load_csr(HostCsr g, bool keep_host) {
       }
     }
     finish_setup(&in_deg);
+    // edge weights (the _csr.wgh file) for the SSSP app
+    if (g.wgh.size() == m_ && m_ > 0) set_weights(g.wgh.data(), g.wgh.size());
     if (keep_host) host_ = std::move(g);
   } else {
     host_ = std::move(g);

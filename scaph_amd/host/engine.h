@@ -128,6 +128,17 @@ class Engine {
 
   // One BFS from src.  Levels stay on the device (levels_to_host to read).
   RunStats bfs(uint32_t src);
+  // The reference's other App plugins on the resident graph
+  // (app/wcc.cu, app/sssp.cu): label-correcting push rounds until no value
+  // changes; the values land where the levels do (levels_to_host).  WCC:
+  // value[v] = the least vertex id that reaches v.  SSSP: shortest path
+  // lengths over the edge weights, 0x7FFFFFFF for unreached.
+  RunStats wcc();
+  RunStats sssp(uint32_t src);
+  // Edge weights in CSR edge order (the _csr.wgh file), for SSSP; set
+  // automatically when a loaded CSR carries them.
+  void set_weights(const uint32_t* wgh, uint64_t count);
+  bool has_weights() const { return wpool_.get() != nullptr; }
   void levels_to_host(uint32_t* out) const;
   // Device-side BFS invariant check; returns {edge violations, orphans}.
   std::array<uint64_t, 2> validate(uint32_t src);
@@ -212,6 +223,9 @@ class Engine {
   HostCsr host_;                  // kept when available (check(), paged mode)
   std::vector<scaph_page> pages_out_, pages_in_;
   DevBuf pool_out_, pool_in_;     // packed resident pages
+  DevBuf wpool_;                  // edge weights laid out like pool_out_ (SSSP)
+  DevBuf relax_ws_;               // chunk list of the WCC/SSSP rounds
+  RunStats relax(int mode, uint32_t src);
   DevBuf dpages_out_, dptr_out_, dpages_in_, dptr_in_;
   scaph_graph_desc g_out_{}, g_in_{};
   DevBuf deg_out_, deg_in_, static_visited_;

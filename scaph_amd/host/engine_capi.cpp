@@ -187,6 +187,40 @@ int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* 
   });
 }
 
+namespace {
+void fill_stats(const scaph::RunStats& r, scaph_bfs_stats* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->src = r.src;
+  out->depth = r.depth;
+  out->ms = r.ms;
+  out->n_reached = r.n_reached;
+  out->m_reached = r.m_reached;
+}
+}  // namespace
+
+int scaph_engine_set_weights(scaph_engine* e, const uint32_t* wgh, uint64_t count, char* err,
+                             size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] { e->eng->set_weights(wgh, count); });
+}
+
+int scaph_engine_wcc(scaph_engine* e, scaph_bfs_stats* out, char* err, size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] {
+    e->last = e->eng->wcc();
+    if (out) fill_stats(e->last, out);
+  });
+}
+
+int scaph_engine_sssp(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* err,
+                      size_t errlen) {
+  if (!e) return 1;
+  return guarded(err, errlen, [&] {
+    e->last = e->eng->sssp(src);
+    if (out) fill_stats(e->last, out);
+  });
+}
+
 int scaph_engine_set_schedule(scaph_engine* e, const char* name, double alpha, double beta,
                               char* err, size_t errlen) {
   if (!e || !name) return 1;

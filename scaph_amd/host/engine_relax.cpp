@@ -1,0 +1,88 @@
+// engine_relax.cpp — the reference's other App plugins on the resident
+// graph: WCC (app/wcc.cu) and SSSP (app/sssp.cu).  Each round turns the
+// active set (a bitmap) into a list and runs scaph_relax_round over it; the
+// vertices whose value it lowered form the next active set.  The rounds
+// stop when a round lowers nothing, as the reference's run() stops when no
+// vertex is active (src/app.cuh:220-265).
+#include <chrono>
+
+#include "engine_internal.h"
+
+namespace scaph {
+
+namespace {
+
+size_t bitmap_words(uint32_t n) { return (((size_t)n + 63) / 64) * 2; }
+
+}  // namespace
+
+RunStats Engine::wcc() { return relax(SCAPH_RELAX_WCC, 0); }
+
+RunStats Engine::sssp(uint32_t src) {
+  if (!has_weights()) throw Error(SCAPH_STATUS_INVALID, "SSSP needs edge weights (_csr.wgh)");
+  return relax(SCAPH_RELAX_SSSP, src);
+}
+
+RunStats Engine::relax(int mode, uint32_t src) {
+  if (n_ == 0) throw Error(SCAPH_STATUS_INVALID, "no graph loaded");
+  if (!resident_) throw Error(SCAPH_STATUS_INVALID, "WCC and SSSP need the graph resident");
+  if (mode == SCAPH_RELAX_SSSP && src >= n_)
+    throw Error(SCAPH_STATUS_INVALID, "source vertex out of range");
+  const size_t ws_bytes = scaph_relax_workspace_bytes(m_);
+  if (relax_ws_.bytes() < ws_bytes) relax_ws_.alloc(ws_bytes);
+  void* st = main_->get();
+  auto* cnt = counters_.get<scaph_level_counters>();
+  auto* hc = hcount_.get<scaph_level_counters>();
+  uint32_t* value = level_.get<uint32_t>();
+  const size_t words = bitmap_words(n_);
+  RunStats rs;
+  rs.src = src;
+  const auto t0 = std::chrono::steady_clock::now();
+  H_ERR(scaph_relax_init(value, n_, mode, src, st));
+  int fa = 0;
+  uint64_t n_act;
+  if (mode == SCAPH_RELAX_WCC) {
+    // every vertex starts active (app/wcc.cu:18-19); bits past n are ignored
+    H_ERR(scaph_fill_u32(front_[0].get<uint32_t>(), 0xFFFFFFFFu, words, st));
+    n_act = n_;
+  } else {
+    *hcount_.get<uint32_t>() = src;
+    H_ERR(scaph_copy_h2d_async(queue_[1].get(), hcount_.get(), 4, st));
+    H_ERR(scaph_fill_u32(front_[0].get<uint32_t>(), 0u, words, st));
+    H_ERR(scaph_frontier_queue_to_bitmap(queue_[1].get<uint32_t>(), 1, front_[0].get<uint32_t>(),
+                                         n_, st));
+    n_act = 1;
+  }
+  const uint32_t* weights = mode == SCAPH_RELAX_SSSP ? wpool_.get<uint32_t>() : nullptr;
+  uint32_t round = 0;
+  while (n_act > 0) {
+    H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, sizeof(scaph_level_counters) / 4,
+                         st));
+    H_ERR(scaph_fill_u32(front_[fa ^ 1].get<uint32_t>(), 0u, words, st));
+    H_ERR(scaph_frontier_bitmap_to_queue(front_[fa].get<uint32_t>(), n_, queue_[0].get<uint32_t>(),
+                                         cnt, st));
+    H_ERR(scaph_relax_round(&g_out_, queue_[0].get<uint32_t>(), (uint32_t)n_act, weights,
+                            pool_out_.get<uint32_t>(), value, front_[fa ^ 1].get<uint32_t>(), cnt,
+                            relax_ws_.get(), relax_ws_.bytes(), st));
+    H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
+    main_->sync();
+    if (hc->reserved) throw Error(SCAPH_STATUS_INVALID, "relax chunk list overflow");
+    if (hc->q_len != n_act) throw Error(SCAPH_STATUS_INVALID, "active set size mismatch");
+    LevelRecord rec;
+    rec.level = round;
+    rec.n_f = n_act;
+    rec.n_next = hc->n_next;
+    rec.scanned = hc->scanned;
+    rs.levels.push_back(rec);
+    rs.m_reached += hc->scanned;  // edges relaxed over all rounds
+    n_act = hc->n_next;
+    fa ^= 1;
+    ++round;
+  }
+  rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  rs.depth = round;
+  rs.n_reached = n_;
+  return rs;
+}
+
+}  // namespace scaph

@@ -22,6 +22,10 @@ def lib():
         U32, U64, I32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
         L.oracle_bfs.argtypes = [U32, P, P, U32, U32, P]
         L.oracle_bfs.restype = I32
+        L.oracle_wcc.argtypes = [U32, P, P, P]
+        L.oracle_wcc.restype = I32
+        L.oracle_sssp.argtypes = [U32, P, P, P, U32, P]
+        L.oracle_sssp.restype = I32
         L.oracle_page_cuts.argtypes = [P, U32, U64, I32, P, U32, ctypes.POINTER(U32)]
         L.oracle_page_cuts.restype = I32
         L.oracle_fill_page.argtypes = [P, P, P, U32, U32, I32, P]
@@ -67,6 +71,39 @@ def lib():
 
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def wcc(offsets, ngh):
+    """app/wcc.cu check(): least vertex id reaching each vertex."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    ngh = np.ascontiguousarray(ngh, np.uint32)
+    n = len(offsets) - 1
+    out = np.empty(max(n, 1), np.uint32)
+    if ngh.size == 0:
+        ngh = np.zeros(1, np.uint32)
+    rc = lib().oracle_wcc(n, _p(offsets), _p(ngh), _p(out))
+    if rc:
+        raise ValueError(f"oracle_wcc rc={rc}")
+    return out[:n]
+
+
+SSSP_INF = 0x7FFFFFFF
+
+
+def sssp(offsets, ngh, wgh, src):
+    """Shortest paths over u32 weights, 0x7FFFFFFF for unreached."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    ngh = np.ascontiguousarray(ngh, np.uint32)
+    wgh = np.ascontiguousarray(wgh, np.uint32)
+    n = len(offsets) - 1
+    out = np.empty(max(n, 1), np.uint32)
+    if ngh.size == 0:
+        ngh = np.zeros(1, np.uint32)
+        wgh = np.zeros(1, np.uint32)
+    rc = lib().oracle_sssp(n, _p(offsets), _p(ngh), _p(wgh), src, _p(out))
+    if rc:
+        raise ValueError(f"oracle_sssp rc={rc}")
+    return out[:n]
 
 
 def bfs(offsets, ngh, src, inf=UNREACHED):

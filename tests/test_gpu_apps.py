@@ -1,0 +1,178 @@
+"""GPU parity tests of the engine's other App plugins — WCC (app/wcc.cu)
+and SSSP (app/sssp.cu) — through the C-ABI, against the oracle bit for bit
+on the same graphs: the committed BFS fixtures, R-MAT graphs (directed and
+symmetric), out-lists long enough for the chunked pass, several pages, and
+the error paths."""
+import os
+
+import numpy as np
+import pytest
+
+import scaph_amd
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+INF = O.SSSP_INF
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def engine(*flags):
+    return scaph_amd.BFS("-max_memory=65536", *flags)
+
+
+def fixtures():
+    z = np.load(os.path.join(O.ROOT, "tests", "golden", "bfs_cases.npz"), allow_pickle=False)
+    return [(str(n), z[f"{n}__offsets"], z[f"{n}__ngh"], int(z[f"{n}__src"][0]))
+            for n in z["names"]]
+
+
+def weighted_csr(n, src, dst, wgh):
+    off, ngh = O.csr_from_edges(n, src, dst)
+    order = np.argsort(np.asarray(src, np.int64), kind="stable")
+    return off, ngh, np.asarray(wgh, np.uint32)[order]
+
+
+def symmetric_rmat(scale, ef, seed):
+    off, ngh = O.rmat_csr(scale, ef, seed=seed)
+    n = len(off) - 1
+    s = np.repeat(np.arange(n), np.diff(off.astype(np.int64)))
+    return O.csr_from_edges(n, np.concatenate([s, ngh]), np.concatenate([ngh, s]))
+
+
+@pytest.mark.parametrize("page_words", [0, 4096])
+def test_wcc_fixtures(page_words):
+    flags = [f"-page_words={page_words}"] if page_words else []
+    for name, off, ngh, _ in fixtures():
+        say("wcc fixture", name, "page_words", page_words)
+        with engine(*flags) as e:
+            e.load_csr(off, ngh)
+            st = e.wcc()
+            np.testing.assert_array_equal(e.levels(), O.wcc(off, ngh), err_msg=name)
+            assert st["depth"] >= 1
+
+
+@pytest.mark.parametrize("scale,ef,seed,sym", [(12, 8, 3, False), (14, 16, 5, True),
+                                               (16, 16, 1, False)])
+def test_wcc_rmat(scale, ef, seed, sym):
+    off, ngh = symmetric_rmat(scale, ef, seed) if sym else O.rmat_csr(scale, ef, seed=seed)
+    say("wcc rmat", scale, ef, seed, "symmetric" if sym else "directed")
+    want = O.wcc(off, ngh)
+    with engine() as e:
+        e.load_csr(off, ngh)
+        st = e.wcc()
+        np.testing.assert_array_equal(e.levels(), want)
+        # every round relaxes the out-lists of its active vertices; the first
+        # round has every vertex active
+        assert st["m_reached"] >= len(ngh)
+
+
+def test_wcc_long_lists_chunked():
+    """Out-lists longer than one wave item (4096 edges) continue as chunks:
+    a hub with 20000 out-edges whose label must reach all of them, behind a
+    chain that lowers the hub's label only in later rounds."""
+    k = 20000
+    n = k + 10
+    hub = n - 1
+    src = [hub] * k + [0, 1, 2, 3]
+    dst = list(range(5, 5 + k)) + [1, 2, 3, hub]
+    off, ngh = O.csr_from_edges(n, src, dst)
+    want = O.wcc(off, ngh)
+    assert want[5] == 0 and want[5 + k - 1] == 0
+    with engine() as e:
+        e.load_csr(off, ngh)
+        e.wcc()
+        np.testing.assert_array_equal(e.levels(), want)
+
+
+@pytest.mark.parametrize("page_words", [0, 4096])
+def test_sssp_fixtures_unit_and_random_weights(page_words):
+    flags = [f"-page_words={page_words}"] if page_words else []
+    rng = np.random.default_rng(21)
+    for name, off, ngh, src in fixtures():
+        say("sssp fixture", name, "page_words", page_words)
+        for wgh in (np.ones(len(ngh), np.uint32), rng.integers(0, 50, len(ngh)).astype(np.uint32)):
+            with engine(*flags) as e:
+                e.load_csr(off, ngh)
+                e.set_weights(wgh)
+                e.sssp(src)
+                np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, src),
+                                              err_msg=name)
+
+
+@pytest.mark.parametrize("scale,ef,seed,wmax,page_words", [(12, 8, 2, 1000, 0),
+                                                           (14, 16, 4, 64, 65536),
+                                                           (16, 16, 6, 255, 0)])
+def test_sssp_rmat(scale, ef, seed, wmax, page_words):
+    off, ngh = O.rmat_csr(scale, ef, seed=seed)
+    rng = np.random.default_rng(seed)
+    wgh = rng.integers(1, wmax + 1, len(ngh)).astype(np.uint32)
+    deg = np.diff(off.astype(np.int64))
+    flags = [f"-page_words={page_words}"] if page_words else []
+    with engine(*flags) as e:
+        e.load_csr(off, ngh)
+        e.set_weights(wgh)
+        for src in (int(np.argmax(deg)), int(np.nonzero(deg)[0][3])):
+            say("sssp rmat", scale, ef, seed, "src", src)
+            e.sssp(src)
+            np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, src))
+
+
+def test_sssp_long_lists_and_chain():
+    """A 20000-edge out-list (chunked) whose cheap edges improve distances
+    already set through a long chain."""
+    k = 20000
+    chain = 300
+    n = k + chain + 2
+    hub = n - 1
+    src, dst, w = [], [], []
+    for i in range(chain):        # 0 -> 1 -> ... -> chain, weight 1
+        src.append(i), dst.append(i + 1), w.append(1)
+    src.append(0), dst.append(hub), w.append(7)
+    for j in range(k):
+        src.append(hub), dst.append(chain + 1 + j % (k - 1)), w.append(1 + j % 13)
+    for i in range(chain + 1, chain + 200):
+        src.append(i), dst.append(i - chain), w.append(0)  # shortcut back into the chain
+    off, ngh, wgh = weighted_csr(n, src, dst, w)
+    want = O.sssp(off, ngh, wgh, 0)
+    with engine() as e:
+        e.load_csr(off, ngh)
+        e.set_weights(wgh)
+        st = e.sssp(0)
+        np.testing.assert_array_equal(e.levels(), want)
+        assert st["depth"] >= 2
+
+
+def test_apps_then_bfs_and_errors():
+    off, ngh = O.rmat_csr(12, 8, seed=9)
+    deg = np.diff(off.astype(np.int64))
+    s = int(np.argmax(deg))
+    with engine() as e:
+        e.load_csr(off, ngh)
+        with pytest.raises(RuntimeError):
+            e.sssp(s)                       # no weights loaded
+        with pytest.raises(RuntimeError):
+            e.set_weights(np.ones(len(ngh) + 1, np.uint32))   # wrong count
+        e.set_weights(np.full(len(ngh), 2, np.uint32))
+        with pytest.raises(RuntimeError):
+            e.sssp(len(off))                # source out of range
+        e.wcc()
+        e.sssp(s)
+        lv = O.bfs(off, ngh, s)
+        np.testing.assert_array_equal(e.levels(), np.where(lv == O.UNREACHED, INF, 2 * lv))
+        e.run(s)                            # BFS on the same engine afterwards
+        np.testing.assert_array_equal(e.levels(), lv)
+        e.wcc()
+        np.testing.assert_array_equal(e.levels(), O.wcc(off, ngh))
+
+
+def test_apps_need_resident_graph():
+    off, ngh = O.rmat_csr(16, 16, seed=8)
+    with scaph_amd.BFS("-page_words=262144", "-max_memory=3") as e:
+        e.load_csr(off, ngh)
+        assert not e.info()["resident"]
+        with pytest.raises(RuntimeError):
+            e.wcc()

@@ -404,3 +404,95 @@ def test_oracle_zipf_csr_is_the_degree_stream():
     for v in range(0, n, 97):
         row = ngh[off[v]:off[v + 1]]
         assert np.all(np.diff(row.astype(np.int64)) > 0) and v not in row
+
+
+def weighted_csr(n, src, dst, wgh):
+    """CSR with weights carried in the same stable order as csr_from_edges."""
+    off, ngh = O.csr_from_edges(n, src, dst)
+    order = np.argsort(np.asarray(src, np.int64), kind="stable")
+    return off, ngh, np.asarray(wgh, np.uint32)[order]
+
+
+def test_oracle_wcc_matches_scipy_components():
+    # symmetric graphs: the least vertex id of each connected component
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    rng = np.random.default_rng(11)
+    for n, m in ((1, 0), (50, 20), (400, 300), (2000, 1500), (3000, 9000)):
+        s = rng.integers(0, n, m)
+        d = rng.integers(0, n, m)
+        ss, dd = np.concatenate([s, d]), np.concatenate([d, s])
+        off, ngh = O.csr_from_edges(n, ss, dd)
+        ncomp, lab = connected_components(
+            csr_matrix((np.ones(len(ss)), (ss, dd)), shape=(n, n)), directed=False)
+        least = np.full(ncomp, n, np.int64)
+        np.minimum.at(least, lab, np.arange(n))
+        np.testing.assert_array_equal(O.wcc(off, ngh), least[lab].astype(np.uint32))
+
+
+def test_oracle_wcc_brute_force_directed():
+    # directed graphs: the least id among the vertices that reach v (v included)
+    rng = np.random.default_rng(12)
+    for trial in range(150):
+        n = int(rng.integers(1, 12))
+        m = int(rng.integers(0, 3 * n + 1))
+        s = rng.integers(0, n, m)
+        d = rng.integers(0, n, m)
+        reach = np.eye(n, dtype=bool)
+        reach[s, d] = True
+        for k in range(n):  # Warshall closure
+            reach |= reach[:, [k]] & reach[[k], :]
+        want = np.array([np.flatnonzero(reach[:, v]).min() for v in range(n)], np.uint32)
+        off, ngh = O.csr_from_edges(n, s, d)
+        np.testing.assert_array_equal(O.wcc(off, ngh), want)
+
+
+def test_oracle_sssp_matches_scipy_dijkstra():
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import dijkstra
+    rng = np.random.default_rng(13)
+    for n, m, wmax in ((30, 60, 5), (500, 2500, 100), (3000, 20000, 1000), (2000, 800, 7)):
+        s = rng.integers(0, n, m)
+        d = rng.integers(0, n, m)
+        w = rng.integers(1, wmax + 1, m)
+        # duplicate edges: scipy would sum them, keep the lightest instead
+        key = s * n + d
+        order = np.lexsort((w, key))
+        first = np.ones(m, bool)
+        first[1:] = key[order][1:] != key[order][:-1]
+        keep = order[first]
+        mat = csr_matrix((w[keep].astype(np.float64), (s[keep], d[keep])), shape=(n, n))
+        off, ngh, wg = weighted_csr(n, s, d, w)
+        for src in (0, int(rng.integers(0, n))):
+            dist = dijkstra(mat, directed=True, indices=src)
+            want = np.where(np.isinf(dist), O.SSSP_INF, dist).astype(np.uint32)
+            np.testing.assert_array_equal(O.sssp(off, ngh, wg, src), want)
+
+
+def test_oracle_sssp_brute_force_zero_weights():
+    # Bellman-Ford on tiny graphs, zero weights and self-loops included
+    rng = np.random.default_rng(14)
+    for trial in range(200):
+        n = int(rng.integers(1, 12))
+        m = int(rng.integers(0, 3 * n + 1))
+        s = rng.integers(0, n, m)
+        d = rng.integers(0, n, m)
+        w = rng.integers(0, 4, m)
+        src = int(rng.integers(0, n))
+        dist = [None] * n
+        dist[src] = 0
+        for _ in range(n):
+            for u, v, x in zip(s, d, w):
+                if dist[u] is not None and (dist[v] is None or dist[u] + x < dist[v]):
+                    dist[v] = dist[u] + int(x)
+        want = np.array([O.SSSP_INF if x is None else x for x in dist], np.uint32)
+        off, ngh, wg = weighted_csr(n, s, d, w)
+        np.testing.assert_array_equal(O.sssp(off, ngh, wg, src), want)
+
+
+def test_oracle_sssp_unit_weights_is_bfs():
+    for case in load_cases():
+        name, off, ngh, src, levels = case
+        got = O.sssp(off, ngh, np.ones(len(ngh), np.uint32), src)
+        want = np.where(levels == U, O.SSSP_INF, levels).astype(np.uint32)
+        np.testing.assert_array_equal(got, want, err_msg=name)
