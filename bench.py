@@ -154,6 +154,60 @@ def bench_small_batch(count, seed):
     return out
 
 
+SSSP_INF = 0x7FFFFFFF  # unreached distance (MAXPATH)
+
+
+def bench_apps(scale, ef, seed, wmax=255):
+    """The WCC and SSSP apps on the resident R-MAT scale/ef graph (weights
+    uniform on [1, wmax] for SSSP), timed; full-size results checked by the
+    properties that fix them: the SSSP distances satisfy d[v] <= d[u] + w on
+    every edge and every reached v != src has a tight in-edge (together they
+    characterise shortest paths for positive weights); the WCC labels are a
+    fixpoint (label[v] <= label[u] on every edge u->v, label[v] <= v)."""
+    import scaph_amd
+    eng = scaph_amd.BFS("-max_memory=262144", "-pagesize=10")
+    eng.gen_rmat_device(scale, ef, 0.57, 0.19, 0.19, seed=seed, keep_host=True)
+    off, ngh = eng.host_csr()
+    n, m = len(off) - 1, len(ngh)
+    rng = np.random.default_rng(seed)
+    wgh = rng.integers(1, wmax + 1, m, dtype=np.uint32)
+    eng.set_weights(wgh)
+    src_of = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off.astype(np.int64)))
+    out = {"workload": f"R-MAT scale {scale} edge factor {ef} seed {seed}, resident; "
+                       f"SSSP weights uniform on [1, {wmax}]", "vertices": n, "edges": m}
+    wcc = [eng.wcc() for _ in range(3)]
+    lab = eng.levels()
+    ms = statistics.median(r["ms"] for r in wcc)
+    out["wcc"] = {"ms": ms, "gteps": m / (ms / 1e3) / 1e9, "rounds": wcc[-1]["depth"],
+                  "edges_relaxed": wcc[-1]["m_reached"],
+                  "components": int(np.count_nonzero(lab == np.arange(n, dtype=np.uint32))),
+                  "fixpoint": bool((lab[ngh] <= lab[src_of]).all()
+                                   and (lab <= np.arange(n, dtype=np.uint32)).all())}
+    del lab
+    srcs, _ = pick_sources(eng, n, 3, seed=seed, min_reach_frac=0.01)
+    runs, exact = [], True
+    for s_ in srcs:
+        runs.append(eng.sssp(s_))
+        d = eng.levels().astype(np.int64)
+        du = d[src_of]
+        reach_u = du < SSSP_INF
+        cand = du + wgh
+        ok = bool((d[ngh][reach_u] <= cand[reach_u]).all())
+        tight = np.zeros(n, bool)
+        tight[ngh[reach_u & (cand == d[ngh])]] = True
+        reached = d < SSSP_INF
+        tight[s_] = True
+        ok &= bool(d[s_] == 0 and tight[reached].all())
+        exact &= ok
+    ms = statistics.median(r["ms"] for r in runs)
+    out["sssp"] = {"ms": ms, "gteps": m / (ms / 1e3) / 1e9, "sources": srcs,
+                   "rounds": [r["depth"] for r in runs],
+                   "edges_relaxed": [r["m_reached"] for r in runs],
+                   "shortest_path_conditions_hold": exact}
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +223,8 @@ def main():
                     help="extra engine flag, e.g. --flag=-pull_summary=false")
     ap.add_argument("--no-batched", dest="batched", action="store_false",
                     help="skip the batched BFS measurements")
+    ap.add_argument("--no-apps", dest="apps", action="store_false",
+                    help="skip the WCC and SSSP measurements")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip the learned-schedule pass")
     ap.add_argument("--learned-check", type=int, default=8,
@@ -330,6 +386,14 @@ def main():
         batched["small_graphs"] = bench_small_batch(4096, seed=4)
         log("batched:", batched["small_graphs"])
 
+    apps = None
+    if args.apps:
+        if not args.batched:
+            eng.close()
+        log("apps: WCC and SSSP")
+        apps = bench_apps(24, 16, seed=5)
+        log("apps:", apps)
+
     med = statistics.median(gteps)
     out = {
         "metric": "BFS GTEPS, R-MAT s26 ef32, learned vs fixed-alpha schedule; % HBM peak",
@@ -352,6 +416,7 @@ def main():
         "cpu_baseline": cpu,
         "learned": learned,
         "batched": batched,
+        "apps": apps,
         "bfs_hbm_frac_median": statistics.median(frac),
         "t_bfs_ms_median": statistics.median(r["ms"] for r in runs),
         "gteps_ref_formula_median": statistics.median(ref_gteps),

@@ -79,14 +79,14 @@ class App {
                 (unsigned long long)engine->num_edges());
   }
 
-  // src/app.cuh:220-265: one traversal, timed, with the reference banner.
+  // src/app.cuh:220-265: one run of the app, timed, with the reference banner.
   void run() {
     gpu_init();
-    last = engine->bfs(source);
+    last = execute();
     const double secs = last.ms / 1e3;
     const double gteps_ref = secs > 0 ? (double)engine->num_edges() / secs / 1e9 : 0.0;
     const double gteps = secs > 0 ? (double)last.m_reached / secs / 1e9 : 0.0;
-    std::printf("|                         BFS costs     : %.8f seconds        |\n", secs);
+    std::printf("|                        %4s costs     : %.8f seconds        |\n", name(), secs);
     std::printf("|                         Throughput    : %.8f GTEPS          |\n", gteps_ref);
     std::printf("|                 Traversed-edge rate   : %.8f GTEPS          |\n", gteps);
     Total_throughput += gteps_ref;
@@ -99,6 +99,10 @@ class App {
   void iter_init() {}  // per-level bookkeeping lives inside Engine::bfs
   virtual void gpu_init() = 0;
   virtual bool check() { return true; }
+  // The app's traversal on the engine (the reference's cond/update functors
+  // select it; here each app names its engine entry point).
+  virtual RunStats execute() { return engine->bfs(source); }
+  virtual const char* name() const { return "BFS"; }
 
   Config cfg;
   std::vector<std::string> positional;
@@ -159,6 +163,73 @@ class bfs : public App<vertex_t, index_t, value_t> {
       if (exact[vid] != this->h_value[vid]) ++errors;
     std::cout << "total errors: " << errors << std::endl;
     return errors == 0;
+  }
+};
+
+// Shared by the WCC and SSSP checks: GPU values against a CPU result.
+inline bool compare_values(const std::vector<uint32_t>& gpu, const std::vector<uint32_t>& cpu) {
+  uint64_t errors = 0;
+  for (size_t vid = 0; vid < cpu.size(); ++vid) {
+    if (gpu[vid] != cpu[vid]) {
+      if (errors < 10)
+        std::cout << "vid: " << vid << " gpu: " << gpu[vid] << " cpu: " << cpu[vid] << std::endl;
+      ++errors;
+    }
+  }
+  std::cout << "total error count: " << errors << std::endl;
+  return errors == 0;
+}
+
+// The WCC plugin (app/wcc.cu:25-97): value[v] = least vertex id reaching v,
+// every vertex active at the start.
+template <typename vertex_t, typename index_t, typename value_t>
+class wcc : public App<vertex_t, index_t, value_t> {
+ public:
+  using Base = App<vertex_t, index_t, value_t>;
+  wcc(int argc, char** argv) : Base(argc, argv) {}
+  explicit wcc(const Config& c) : Base(c) {}
+  void gpu_init() override {}  // value[v] = v and all-active are set by Engine::wcc
+  RunStats execute() override { return this->engine->wcc(); }
+  const char* name() const override { return "WCC"; }
+  bool check() override {
+    const HostCsr* g = this->engine->host_csr();
+    if (!g) {
+      std::cout << "check skipped: no host copy of the graph" << std::endl;
+      return false;
+    }
+    std::vector<uint32_t> gpu(this->vert_count), cpu;
+    this->engine->levels_to_host(gpu.data());
+    cpu_wcc(*g, &cpu);
+    return compare_values(gpu, cpu);
+  }
+};
+
+// The SSSP plugin (app/sssp.cu:33-61): shortest paths from -src over the
+// _csr.wgh weights (or a generator's wmax=), 0x7FFFFFFF for unreached.
+template <typename vertex_t, typename index_t, typename value_t>
+class sssp : public App<vertex_t, index_t, value_t> {
+ public:
+  using Base = App<vertex_t, index_t, value_t>;
+  sssp(int argc, char** argv) : Base(argc, argv) {}
+  explicit sssp(const Config& c) : Base(c) {}
+  void gpu_init() override {
+    if (this->cfg.src < 0 || (uint64_t)this->cfg.src >= this->engine->num_vertices())
+      throw Error(SCAPH_STATUS_INVALID, "-src is out of range");
+    this->source = (uint32_t)this->cfg.src;
+  }
+  RunStats execute() override { return this->engine->sssp(this->source); }
+  const char* name() const override { return "SSSP"; }
+  // The reference's check is a stub returning true; this one runs Dijkstra.
+  bool check() override {
+    const HostCsr* g = this->engine->host_csr();
+    if (!g || g->wgh.size() != g->ngh.size()) {
+      std::cout << "check skipped: no host copy of the weighted graph" << std::endl;
+      return false;
+    }
+    std::vector<uint32_t> gpu(this->vert_count), cpu;
+    this->engine->levels_to_host(gpu.data());
+    cpu_sssp(*g, this->source, &cpu);
+    return compare_values(gpu, cpu);
   }
 };
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <deque>
+#include <queue>
 #include <thread>
 
 #include "engine_internal.h"
@@ -673,6 +674,48 @@ void cpu_bfs(const HostCsr& g, uint32_t src, uint32_t inf, std::vector<uint32_t>
     cur.swap(next);
     next.clear();
   }
+}
+
+void cpu_wcc(const HostCsr& g, std::vector<uint32_t>* out) {
+  std::vector<uint32_t>& val = *out;
+  val.resize(g.n);
+  for (uint32_t v = 0; v < g.n; ++v) val[v] = v;
+  for (bool changed = true; changed;) {
+    changed = false;
+    for (uint32_t u = 0; u < g.n; ++u)
+      for (uint64_t k = g.offsets[u]; k < g.offsets[(size_t)u + 1]; ++k) {
+        const uint32_t v = g.ngh[k];
+        if (val[u] < val[v]) {
+          val[v] = val[u];
+          changed = true;
+        }
+      }
+  }
+}
+
+void cpu_sssp(const HostCsr& g, uint32_t src, std::vector<uint32_t>* out) {
+  std::vector<uint64_t> dist(g.n, UINT64_MAX);
+  using Item = std::pair<uint64_t, uint32_t>;
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+  if (src < g.n && g.wgh.size() == g.ngh.size()) {
+    dist[src] = 0;
+    heap.push({0, src});
+  }
+  while (!heap.empty()) {
+    const Item it = heap.top();
+    heap.pop();
+    if (it.first != dist[it.second]) continue;
+    for (uint64_t k = g.offsets[it.second]; k < g.offsets[(size_t)it.second + 1]; ++k) {
+      const uint64_t nd = it.first + g.wgh[k];
+      if (nd < dist[g.ngh[k]]) {
+        dist[g.ngh[k]] = nd;
+        heap.push({nd, g.ngh[k]});
+      }
+    }
+  }
+  out->resize(g.n);
+  for (uint32_t v = 0; v < g.n; ++v)
+    (*out)[v] = dist[v] == UINT64_MAX ? SCAPH_RELAX_INF : (uint32_t)dist[v];
 }
 
 }  // namespace scaph

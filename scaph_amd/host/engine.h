@@ -258,6 +258,11 @@ class Engine {
 // level(u) + 1 < level(v).  With inf = 63 it reproduces the reference's
 // capped levels; with inf = SCAPH_UNREACHED it gives exact BFS depths.
 void cpu_bfs(const HostCsr& g, uint32_t src, uint32_t inf, std::vector<uint32_t>* out);
+// The reference apps' CPU checks: WCC label sweeps (app/wcc.cu:60-85) and
+// Dijkstra over g.wgh (0x7FFFFFFF unreached) for SSSP, whose reference check
+// is a stub (app/sssp.cu:57-61).
+void cpu_wcc(const HostCsr& g, std::vector<uint32_t>* out);
+void cpu_sssp(const HostCsr& g, uint32_t src, std::vector<uint32_t>* out);
 
 // Reference cap: depth >= 63 (or unreached) reads as 63 (app/bfs.cu:18).
 inline uint32_t compat63(uint32_t lvl) { return lvl <= 62 ? lvl : 63u; }

@@ -143,6 +143,17 @@ uint32_t max_degree_vertex(const HostCsr& g) {
   return best;
 }
 
+void fill_uniform_weights(HostCsr* g, uint32_t wmax, uint64_t seed) {
+  constexpr uint64_t kWeightSalt = 0x7765696768747321ull;  // "weights!"
+  const uint64_t key = scaph_spec::fmix64(seed ^ kWeightSalt);
+  const uint64_t m = g->ngh.size();
+  g->wgh.resize(m);
+  uint32_t* w = g->wgh.data();
+  parallel_ranges(m, m < (1u << 20) ? 1 : pick_threads(0), [=](uint64_t b, uint64_t e) {
+    for (uint64_t k = b; k < e; ++k) w[k] = 1u + (uint32_t)(scaph_spec::word(key, k) % wmax);
+  });
+}
+
 bool generate_from_spec(const std::string& spec, HostCsr* out, std::string* err) {
   const size_t colon = spec.find(':');
   const std::string kind = spec.substr(0, colon);
@@ -163,6 +174,11 @@ bool generate_from_spec(const std::string& spec, HostCsr* out, std::string* err)
   auto u64 = [&](const char* k, uint64_t d) {
     return kv.count(k) ? (uint64_t)std::strtoull(kv[k].c_str(), nullptr, 0) : d;
   };
+  const uint64_t wmax = u64("wmax", 0);
+  if (wmax > 0xFFFFFFFFull) {
+    if (err) *err = "wmax must fit in 32 bits";
+    return false;
+  }
   if (kind == "rmat") {
     RmatParams p;
     p.scale = (uint32_t)u64("scale", 20);
@@ -176,6 +192,7 @@ bool generate_from_spec(const std::string& spec, HostCsr* out, std::string* err)
       return false;
     }
     *out = generate_rmat(p);
+    if (wmax) fill_uniform_weights(out, (uint32_t)wmax, p.seed);
     return true;
   }
   if (kind == "er") {
@@ -187,6 +204,7 @@ bool generate_from_spec(const std::string& spec, HostCsr* out, std::string* err)
       return false;
     }
     *out = generate_er(n, (uint64_t)std::llround(deg * n), seed);
+    if (wmax) fill_uniform_weights(out, (uint32_t)wmax, seed);
     return true;
   }
   if (err) *err = "unknown generator '" + kind + "'";

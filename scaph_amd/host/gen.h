@@ -39,8 +39,13 @@ HostCsr generate_er(uint32_t n, uint64_t m, uint64_t seed, int threads = 0);
 // Raw sorted-unique edge keys (u<<32|v) -> CSR on n vertices.
 HostCsr csr_from_sorted_keys(const std::vector<uint64_t>& keys, uint32_t n);
 
+// Edge weights for the SSSP app: weight of CSR edge k = 1 + word(seed ^
+// kWeightSalt, k) mod wmax (uniform on [1, wmax]).
+void fill_uniform_weights(HostCsr* g, uint32_t wmax, uint64_t seed);
+
 // Parses "rmat:scale=20,ef=16,a=0.57,b=0.19,c=0.19,seed=1" or
-// "er:n=1000,deg=8,seed=3".  Returns false on a malformed spec.
+// "er:n=1000,deg=8,seed=3"; either takes "wmax=K" for uniform edge weights
+// on [1, K] (fill_uniform_weights).  Returns false on a malformed spec.
 bool generate_from_spec(const std::string& spec, HostCsr* out, std::string* err);
 
 // Source selection used by the benchmarks: highest out-degree (lowest id

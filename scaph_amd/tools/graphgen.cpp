@@ -1,6 +1,8 @@
 // graphgen — writes a synthetic graph in the reference's flat CSR layout
 // (<base>_csr.{idx,ngh,deg[,wgh]}) and, with -pagesize, its page files.
-//   graphgen -gen rmat:scale=20,ef=16,seed=1 -out /data/g [-idx64] [-wgh] [-pagesize 10]
+//   graphgen -gen rmat:scale=20,ef=16,seed=1[,wmax=255] -out /data/g [-idx64] [-wgh]
+//            [-pagesize 10]
+// With -wgh the weights file holds the spec's wmax= weights, or zeros.
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -16,7 +18,7 @@ int main(int argc, char** argv) {
   r.add("gen", scaph::FlagType::kString, &spec, "generator spec");
   r.add("out", scaph::FlagType::kString, &out, "output base path");
   r.add("idx64", scaph::FlagType::kBool, &idx64, "u64 row offsets (else u32)");
-  r.add("wgh", scaph::FlagType::kBool, &wgh, "also write a zero _csr.wgh");
+  r.add("wgh", scaph::FlagType::kBool, &wgh, "also write _csr.wgh (wmax= weights, else zeros)");
   r.add("pagesize", scaph::FlagType::kInt32, &pagesize, "also write pages of this many Mi words");
   std::string err;
   if (!scaph::parse_command_line_flags(&argc, &argv, &err) || out.empty()) {

@@ -176,3 +176,17 @@ def test_apps_need_resident_graph():
         assert not e.info()["resident"]
         with pytest.raises(RuntimeError):
             e.wcc()
+
+
+@pytest.mark.parametrize("tool,extra", [("wcc", []), ("sssp", ["-src=7"]),
+                                        ("sssp", ["-src=7", "-page_words=8192"])])
+def test_app_tools_check_passes(tool, extra):
+    """bin/wcc and bin/sssp on a generated weighted graph, with the app's own
+    CPU check (label sweeps / Dijkstra) and two rounds."""
+    import subprocess
+    exe = os.path.join(O.ROOT, "scaph_amd", "bin", tool)
+    say("tool", tool, extra)
+    out = subprocess.run([exe, "-gen=rmat:scale=14,ef=16,seed=2,wmax=100", "-check", "-rounds=2",
+                          *extra], capture_output=True, text=True, timeout=100)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "check passed!" in out.stdout and f"{tool} cost" in out.stdout

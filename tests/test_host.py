@@ -202,3 +202,25 @@ def test_committed_schedule_model_loads():
     assert np.all(np.isfinite(w)) and np.all(scale > 0) and np.isfinite(gain)
     # bf16 round trip of the stored parameters is exact
     np.testing.assert_array_equal(L.f32_to_bf16(w), p)
+
+
+def test_graphgen_weights_and_app_tools():
+    """wmax= adds uniform weights on [1, wmax] without changing the edges;
+    the wcc and sssp tools parse the shared flags."""
+    with tempfile.TemporaryDirectory() as td:
+        for tag, spec in (("w", "rmat:scale=11,ef=8,seed=3,wmax=50"),
+                          ("u", "rmat:scale=11,ef=8,seed=3")):
+            subprocess.run([os.path.join(BIN, "graphgen"), "-gen", spec, "-wgh", "-out",
+                            os.path.join(td, tag)], check=True, stdout=subprocess.DEVNULL)
+        for ext in ("_csr.idx", "_csr.ngh"):
+            assert sha(os.path.join(td, "w" + ext)) == sha(os.path.join(td, "u" + ext))
+        w = np.fromfile(os.path.join(td, "w_csr.wgh"), np.uint32)
+        assert len(w) == len(np.fromfile(os.path.join(td, "w_csr.ngh"), np.uint32))
+        assert w.min() == 1 and w.max() == 50
+        assert abs(w.mean() - 25.5) < 1.0
+        assert not np.fromfile(os.path.join(td, "u_csr.wgh"), np.uint32).any()
+    for tool in ("wcc", "sssp"):
+        out = subprocess.run([os.path.join(BIN, tool), "-src=3", "-check", "-rounds=2"],
+                             capture_output=True, text=True)
+        assert out.returncode != 0
+        assert ("no -graphfile and no -gen" in out.stdout or "device" in out.stdout), out.stdout
