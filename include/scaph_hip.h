@@ -79,7 +79,17 @@ int scaph_get_device_props(int device, scaph_device_props* out);
  * start s that varies between launches); the assumption affects speed
  * only. */
 int scaph_xcc_ids(uint32_t* out, uint32_t blocks, void* stream);
+/* out[2b] = XCC and out[2b+1] = HW_REG_HW_ID (CU id bits 11:8, shader
+ * array bit 12, shader engine bits 15:13) of the wave running workgroup b;
+ * shows which CUs a stream's launches reach. */
+int scaph_hw_ids(uint32_t* out, uint32_t blocks, void* stream);
 int scaph_stream_create(void** stream);
+/* A stream whose kernels run on all but `reserved_cus` CUs (0 = all; must
+ * be below the device's CU count).  Replaces the reference's trick kernel
+ * (src/app.cuh:210-217, src/gpu_engine.cuh:464-475), which pins
+ * FLAGS_tricknum SMs with a spinning block so that BFS runs on fewer; a CU
+ * mask gives the same reduced machine without a spinning kernel. */
+int scaph_stream_create_cu_limited(void** stream, uint32_t reserved_cus);
 int scaph_stream_destroy(void* stream);
 int scaph_stream_query(void* stream); /* 0 = idle, SCAPH_STATUS_NOT_READY = busy */
 int scaph_stream_sync(void* stream);

@@ -141,6 +141,8 @@ def hip():
                   "scaph_graph_capture_begin"):
             _sig(lib, n, I32, [P])
         _sig(lib, "scaph_stream_create", I32, [ctypes.POINTER(P)])
+        _sig(lib, "scaph_stream_create_cu_limited", I32, [ctypes.POINTER(P), U32])
+        _sig(lib, "scaph_hw_ids", I32, [P, U32, P])
         _sig(lib, "scaph_event_create", I32, [ctypes.POINTER(P)])
         _sig(lib, "scaph_event_record", I32, [P, P])
         _sig(lib, "scaph_stream_wait_event", I32, [P, P])

@@ -6,6 +6,8 @@
 // priority.
 #include <string.h>
 
+#include <vector>
+
 #include "common.h"
 
 extern "C" {
@@ -163,7 +165,43 @@ __global__ void k_xcc_ids(uint32_t* out, uint32_t blocks) {
   if (threadIdx.x == 0 && blockIdx.x < blocks) out[blockIdx.x] = x & 0xFu;
 }
 
+// XCC and the HW_ID register (CU, shader array, shader engine) of the wave
+// that runs each workgroup.
+__global__ void k_hw_ids(uint32_t* out, uint32_t blocks) {
+  uint32_t x, h;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(h));
+  if (threadIdx.x == 0 && blockIdx.x < blocks) {
+    out[2ull * blockIdx.x] = x & 0xFu;
+    out[2ull * blockIdx.x + 1] = h;
+  }
+}
+
 extern "C" {
+
+int scaph_hw_ids(uint32_t* out, uint32_t blocks, void* stream) {
+  if (!out || blocks == 0) return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_hw_ids, dim3(blocks), dim3(64), 0, as_stream(stream), out, blocks);
+  return (int)hipGetLastError();
+}
+
+int scaph_stream_create_cu_limited(void** stream, uint32_t reserved_cus) {
+  if (!stream) return SCAPH_STATUS_INVALID;
+  if (reserved_cus == 0) return scaph_stream_create(stream);
+  int dev = 0, cus = 0;
+  SCAPH_TRY(hipGetDevice(&dev));
+  SCAPH_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (cus <= 0 || reserved_cus >= (uint32_t)cus) return SCAPH_STATUS_INVALID;
+  // keep CUs [0, cus - reserved) of the mask; the runtime maps mask bits to
+  // CUs across the XCDs
+  const uint32_t keep = (uint32_t)cus - reserved_cus;
+  std::vector<uint32_t> mask(((uint32_t)cus + 31) / 32, 0u);
+  for (uint32_t i = 0; i < keep; ++i) mask[i / 32] |= 1u << (i % 32);
+  hipStream_t s;
+  SCAPH_TRY(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
+  *stream = s;
+  return 0;
+}
 
 int scaph_xcc_ids(uint32_t* out, uint32_t blocks, void* stream) {
   if (!out || blocks == 0) return SCAPH_STATUS_INVALID;

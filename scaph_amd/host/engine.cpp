@@ -3,6 +3,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <queue>
@@ -44,7 +45,7 @@ Config& global_flags() {
     r.add("num_thds", FlagType::kInt32, &c.num_thds, "kernel mapping threads");
     r.add("threshold", FlagType::kInt32, &c.threshold, "threshold of compress point");
     r.add("check", FlagType::kBool, &c.check, "check result");
-    r.add("tricknum", FlagType::kInt32, &c.tricknum, "trick number");
+    r.add("tricknum", FlagType::kInt32, &c.tricknum, "CUs withheld from the traversal kernels");
     r.add("src", FlagType::kInt64, &c.src, "traversal start point");
     r.add("graphfile", FlagType::kString, &c.graphfile, "graphfile path");
     r.add("gen", FlagType::kString, &c.gen, "generator spec used when graphfile is empty");
@@ -109,6 +110,11 @@ bool parse_config(const std::vector<std::string>& args, Config* out,
     if (err) *err = "ERROR: -num_stream and -multi must be in 1..64";
     return false;
   }
+  if (g.tricknum < 0 || g.tricknum > 4095) {
+    // the upper bound is the device's CU count, checked when streams open
+    if (err) *err = "ERROR: -tricknum must be in 0..(CUs - 1)";
+    return false;
+  }
   if (g.page_groups < 0 || g.page_groups > 4096) {
     if (err) *err = "ERROR: -page_groups must be in 0..4096";
     return false;
@@ -136,7 +142,7 @@ bool parse_config(const std::vector<std::string>& args, Config* out,
 Engine::Engine(const Config& cfg)
     : cfg_(cfg), P_(cfg.page_words > 0 ? (uint64_t)cfg.page_words : (uint64_t)cfg.pagesize << 20) {
   H_ERR(scaph_init(cfg.device));
-  main_.reset(new Stream());
+  main_.reset(new Stream((uint32_t)cfg_.tricknum));
   hcount_.alloc(4096);
   hand_.alloc(4096);
   std::memset(hand_.host(), 0, hand_.bytes());
@@ -314,7 +320,7 @@ void Engine::alloc_state() {
   totals_.alloc(64);
 }
 
-void Engine::load_csr(HostCsr g, bool keep_host) {
+void Engine::load_csr(HostCsr g, bool keep_host, HostCsr* in_edges) {
   reset_graph();
   if (g.offsets.size() != (size_t)g.n + 1) throw Error(SCAPH_STATUS_INVALID, "malformed CSR");
   n_ = g.n;
@@ -328,7 +334,8 @@ void Engine::load_csr(HostCsr g, bool keep_host) {
   std::vector<uint32_t> hubs;
   if (cfg_.build_pull) {
     if (n_ > 0x7FFFFFFFu) throw Error(SCAPH_STATUS_INVALID, "pull needs fewer than 2^31 vertices");
-    t = transpose(g);
+    if (in_edges) t = std::move(*in_edges);
+    else t = transpose(g);
     hubs = select_hubs(h_deg_out_, kMaxHubs);
     encode_hub_in_lists(&t, hubs);
     if (!page_cuts(t, P_, false, &pages_in_, &err)) throw Error(SCAPH_STATUS_INVALID, err);
@@ -393,7 +400,21 @@ void Engine::load_files(const std::string& base) {
         std::memcmp(cuts.data(), listed.data(), cuts.size() * sizeof(scaph_page)) != 0)
       throw Error(SCAPH_STATUS_INVALID, "page.info does not match the page cut of the CSR");
   }
-  load_csr(std::move(g), true);
+  // the transposed-page generator's output (pagegencsr ... t), used instead
+  // of transposing here once checked to hold exactly the reversed edges
+  HostCsr t;
+  bool have_t = false;
+  if (cfg_.build_pull) {
+    if (std::FILE* f = std::fopen((base + "_in_csr.idx").c_str(), "rb")) {
+      std::fclose(f);
+      if (!read_csr_files(base + "_in", &t, &err)) throw Error(SCAPH_STATUS_INVALID, err);
+      if (t.n != g.n || t.num_edges() != g.num_edges() ||
+          edge_multiset_digest(t, true) != edge_multiset_digest(g, false))
+        throw Error(SCAPH_STATUS_INVALID, base + "_in_csr.* is not the transpose of the graph");
+      have_t = true;
+    }
+  }
+  load_csr(std::move(g), true, have_t ? &t : nullptr);
 }
 
 void Engine::load_from_config() {

@@ -30,7 +30,7 @@ struct Config {
   int32_t num_thds = 256;     // accepted; blocks are 256 threads (4 wave64)
   int32_t threshold = 3;      // cold-page compaction threshold, in 1/32 pages
   bool check = false;
-  int32_t tricknum = 0;       // accepted; no effect (no CU-occupying spin kernel)
+  int32_t tricknum = 0;       // CUs withheld from every engine stream (a CU mask)
   int64_t src = 0;
   std::string graphfile;
   // additions
@@ -117,7 +117,9 @@ class Engine {
   // Graph loading (replaces fetch_graph + load_graph, src/graph.h:24-108,
   // src/app.cuh:267-345).
   void load_files(const std::string& base);
-  void load_csr(HostCsr g, bool keep_host = true);
+  // in_edges: the graph's transpose when already at hand (read from the
+  // transposed-page generator's files); otherwise it is computed here.
+  void load_csr(HostCsr g, bool keep_host = true, HostCsr* in_edges = nullptr);
   void gen_rmat_device(const RmatParams& p, bool keep_host_csr);
   void gen_er_device(uint32_t n, uint64_t m, uint64_t seed, bool keep_host_csr);
   // Power-law digraph (zipf_degree out-degrees, uniform destinations):

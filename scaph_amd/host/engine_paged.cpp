@@ -77,7 +77,7 @@ void Engine::setup_paged() {
     if (ps.thres) ps.h_tail[p].alloc(ps.thres * 4);
   }
   ps.pool.alloc((size_t)pool_size_ * P_ * 4);
-  ps.copy.reset(new Stream());
+  ps.copy.reset(new Stream((uint32_t)cfg_.tricknum));
   ps.copy_beg.reset(new Event());
   ps.copy_end.reset(new Event());
   const int ns = cfg_.num_stream;
@@ -85,7 +85,7 @@ void Engine::setup_paged() {
   for (const auto& pg : pages_out_) max_nodes = std::max(max_nodes, pg.nodenum);
   ps.ws_bytes = scaph_push_workspace_bytes(max_nodes, P_);
   for (int s = 0; s < ns; ++s) {
-    ps.exec.emplace_back(new Stream());
+    ps.exec.emplace_back(new Stream((uint32_t)cfg_.tricknum));
     ps.exec_done.emplace_back(new Event());
     ps.ws.emplace_back(ps.ws_bytes);
   }
@@ -98,7 +98,7 @@ void Engine::setup_paged() {
     ps.group_ws_bytes = scaph_push_runs_workspace_bytes((uint32_t)segs, ps.group_nodes_cap,
                                                         std::max<uint64_t>(m_, 1));
     ps.group_ws.alloc(ps.group_ws_bytes);
-    ps.group.reset(new Stream());
+    ps.group.reset(new Stream((uint32_t)cfg_.tricknum));
     ps.group_done.reset(new Event());
     ps.h_runs.alloc(np * sizeof(scaph_page_run) + 64);
     ps.d_runs.alloc(np * sizeof(scaph_page_run) + 64);

@@ -5,9 +5,11 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <string>
 
 namespace scaph {
 
@@ -265,6 +267,132 @@ HostCsr transpose(const HostCsr& g) {
   for (uint32_t u = 0; u < g.n; ++u)
     for (uint64_t k = g.offsets[u]; k < g.offsets[(size_t)u + 1]; ++k) t.ngh[cur[g.ngh[k]]++] = u;
   return t;
+}
+
+namespace {
+
+// Buffered decimal text output (one large fwrite per 8 MiB).
+class TextOut {
+ public:
+  explicit TextOut(const std::string& path) : f_(std::fopen(path.c_str(), "wb")) {
+    buf_.reserve(kCap + 64);
+  }
+  ~TextOut() {
+    if (f_) std::fclose(f_);
+  }
+  bool ok() const { return f_ != nullptr && good_; }
+  void text(const char* s) {
+    buf_.append(s);
+    spill();
+  }
+  void num(uint64_t x, char end) {
+    char tmp[24];
+    const auto r = std::to_chars(tmp, tmp + sizeof tmp, x);
+    buf_.append(tmp, r.ptr);
+    buf_.push_back(end);
+    spill();
+  }
+  bool close() {
+    flush();
+    const bool fine = f_ && good_ && std::fclose(f_) == 0;
+    f_ = nullptr;
+    return fine;
+  }
+
+ private:
+  static constexpr size_t kCap = 8u << 20;
+  void spill() {
+    if (buf_.size() >= kCap) flush();
+  }
+  void flush() {
+    if (f_ && !buf_.empty() && std::fwrite(buf_.data(), 1, buf_.size(), f_) != buf_.size())
+      good_ = false;
+    buf_.clear();
+  }
+  std::FILE* f_;
+  std::string buf_;
+  bool good_ = true;
+};
+
+bool export_check(const HostCsr& g, bool weighted, const std::string& path, TextOut& out,
+                  std::string* err) {
+  if (weighted && g.wgh.size() != g.ngh.size()) {
+    if (err) *err = "weighted export needs the _csr.wgh file";
+    return false;
+  }
+  if (!out.ok()) {
+    if (err) *err = "cannot write " + path;
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+bool write_ligra(const std::string& base, const HostCsr& g, bool weighted, std::string* err) {
+  const std::string path = base + (weighted ? ".ligra.weight" : ".ligra");
+  TextOut out(path);
+  if (!export_check(g, weighted, path, out, err)) return false;
+  out.text(weighted ? "WeightedAdjacencyGraph\n" : "AdjacencyGraph\n");
+  out.num(g.n, '\n');
+  out.num(g.ngh.size(), '\n');
+  for (uint32_t v = 0; v < g.n; ++v) out.num(g.offsets[v], '\n');
+  for (uint32_t x : g.ngh) out.num(x, '\n');
+  if (weighted)
+    for (uint32_t w : g.wgh) out.num(w, '\n');
+  if (!out.close()) {
+    if (err) *err = "write failed: " + path;
+    return false;
+  }
+  return true;
+}
+
+bool write_totem(const std::string& base, const HostCsr& g, bool weighted, std::string* err) {
+  const std::string path = base + (weighted ? "weight.totem" : ".totem");
+  TextOut out(path);
+  if (!export_check(g, weighted, path, out, err)) return false;
+  out.text("#Nodes: ");
+  out.num(g.n, '\n');
+  out.text("#Edges: ");
+  out.num(g.ngh.size(), '\n');
+  out.text("#Undirected\n");
+  for (uint32_t u = 0; u < g.n; ++u)
+    for (uint64_t k = g.offsets[u]; k < g.offsets[(size_t)u + 1]; ++k) {
+      out.num(u, '\t');
+      if (weighted) {
+        out.num(g.ngh[k], '\t');
+        out.num(g.wgh[k], '\n');
+      } else {
+        out.num(g.ngh[k], '\n');
+      }
+    }
+  if (!out.close()) {
+    if (err) *err = "write failed: " + path;
+    return false;
+  }
+  return true;
+}
+
+uint64_t edge_multiset_digest(const HostCsr& g, bool transposed) {
+  auto mix = [](uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  uint64_t sum = 0;
+  for (uint32_t u = 0; u < g.n; ++u)
+    for (uint64_t k = g.offsets[u]; k < g.offsets[(size_t)u + 1]; ++k) {
+      const uint64_t a = transposed ? g.ngh[k] : u, b = transposed ? u : g.ngh[k];
+      sum += mix((a << 32 | b) + 0x9E3779B97F4A7C15ull);
+    }
+  return sum;
+}
+
+bool write_transposed_files(const std::string& base, const HostCsr& g, uint64_t pagesize_words,
+                            std::vector<scaph_page>* pages_out, std::string* err) {
+  const HostCsr t = transpose(g);
+  return write_csr_files(base + "_in", t, true, false, err) &&
+         write_page_files(base + "_in", t, pagesize_words, false, pages_out, err);
 }
 
 std::vector<uint32_t> select_hubs(const std::vector<uint32_t>& deg, uint32_t max_hubs) {

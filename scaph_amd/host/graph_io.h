@@ -83,8 +83,31 @@ bool read_graph_info(const std::string& base, GraphInfo* info, std::string* err)
 bool read_page_file(const std::string& base, int index, uint64_t pagesize_words, uint32_t* buf,
                     std::string* err);
 
+// Text exporters for CPU baselines, in the formats of tools/bin2ligra.cpp,
+// tools/bin2ligraweight.cpp, tools/bin2totem.cpp and
+// tools/bin2totemweight.cpp:
+//   <base>.ligra (or <base>.ligra.weight): "AdjacencyGraph" (or
+//     "WeightedAdjacencyGraph"), n, m, then the n row offsets, the m
+//     neighbours (and the m weights), one number per line;
+//   <base>.totem (or <base>weight.totem): "#Nodes: n", "#Edges: m",
+//     "#Undirected", then one "u\tv" (or "u\tv\tw") line per edge.
+bool write_ligra(const std::string& base, const HostCsr& g, bool weighted, std::string* err);
+bool write_totem(const std::string& base, const HostCsr& g, bool weighted, std::string* err);
+
 // In-edge CSR (counting sort; neighbour lists come out sorted by source).
 HostCsr transpose(const HostCsr& g);
+
+// Order-free 64-bit digest of the edge multiset {(u, v)} of g, or of
+// {(v, u)} when `transposed` (a sum of mixed per-edge words): equal for a
+// graph and its transpose read as transposed, so a stored transpose can be
+// checked against its graph in two sequential passes.
+uint64_t edge_multiset_digest(const HostCsr& g, bool transposed);
+
+// The transposed-page generator for the pull step: <base>_in_csr.{idx,ngh,
+// deg} (u64 offsets) and <base>_in page files of the in-edge graph, cut with
+// the same page size.
+bool write_transposed_files(const std::string& base, const HostCsr& g, uint64_t pagesize_words,
+                            std::vector<scaph_page>* pages_out, std::string* err);
 
 // Hubs: the (at most) max_hubs vertices of largest out-degree (> 0), by
 // degree descending then id.  Entry i of the result has hub index i.

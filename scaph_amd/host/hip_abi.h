@@ -147,6 +147,10 @@ class MappedBuf {
 class Stream {
  public:
   Stream() { H_ERR(scaph_stream_create(&s_)); }
+  // kernels on this stream leave `reserved_cus` CUs unused (-tricknum)
+  explicit Stream(uint32_t reserved_cus) {
+    H_ERR(scaph_stream_create_cu_limited(&s_, reserved_cus));
+  }
   ~Stream() {
     if (s_) scaph_stream_destroy(s_);
   }

@@ -1,7 +1,10 @@
 // pagegencsr — offline page generator with the command line and output of
-// tools/bin2pagecsr.cpp:254-312:  pagegencsr <base> <pagesize MiB words> w|u
+// tools/bin2pagecsr.cpp:254-312:  pagegencsr <base> <pagesize MiB words> w|u [t]
 // Reads <base>_csr.{ngh,deg,idx} (and _csr.wgh when weighted) and writes
-// <base>page<N>, <base>page.info and <base>graph.info.
+// <base>page<N>, <base>page.info and <base>graph.info.  With a fourth
+// argument `t` it also writes the in-edge graph for the pull step:
+// <base>_in_csr.{idx,ngh,deg} and its pages <base>_inpage<N>, which the
+// engine loads instead of transposing.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -10,7 +13,7 @@
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    std::printf("usage : ./pagegencsr filename pagesize(MB Vertex) w/u(weighted)\n");
+    std::printf("usage : ./pagegencsr filename pagesize(MB Vertex) w/u(weighted) [t]\n");
     return 0;
   }
   const std::string base = argv[1];
@@ -30,6 +33,14 @@ int main(int argc, char** argv) {
   if (!scaph::write_page_files(base, g, P, weighted, &pages, &err)) {
     std::fprintf(stderr, "%s\n", err.c_str());
     return 1;
+  }
+  if (argc > 4 && argv[4][0] == 't') {
+    std::vector<scaph_page> in_pages;
+    if (!scaph::write_transposed_files(base, g, P, &in_pages, &err)) {
+      std::fprintf(stderr, "%s\n", err.c_str());
+      return 1;
+    }
+    std::printf("in-edge pages: %zu\n", in_pages.size());
   }
   for (const auto& p : pages)
     std::printf("left: %u right: %u\nedgenum: %u\nnodenum: %u\n\n", p.left, p.right, p.edgenum,

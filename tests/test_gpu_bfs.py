@@ -330,3 +330,88 @@ def test_xcd_round_robin_dispatch():
     if xcds == 8:
         want = (int(out[0]) + np.arange(blocks, dtype=np.uint32)) % 8
         np.testing.assert_array_equal(out, want)
+
+
+def hw_id_tuples(hip, stream, blocks=65536):
+    out = np.zeros(2 * blocks, np.uint32)
+    p = ctypes.c_void_p()
+    _lib.check(hip.scaph_dev_alloc(ctypes.byref(p), out.nbytes))
+    try:
+        _lib.check(hip.scaph_hw_ids(p, blocks, stream))
+        _lib.check(hip.scaph_stream_sync(stream))
+        _lib.check(hip.scaph_copy_d2h_async(out.ctypes.data, p, out.nbytes, None))
+        _lib.check(hip.scaph_device_sync())
+    finally:
+        hip.scaph_dev_free(p)
+    xcc, hw = out[0::2], out[1::2]
+    # (XCC, shader engine, shader array, CU) of each workgroup
+    return set(zip(xcc.tolist(), ((hw >> 8) & 0xFF).tolist()))
+
+
+def test_cu_limited_stream_reaches_fewer_cus():
+    """-tricknum's CU mask: launches on a stream that withholds CUs reach at
+    most the CUs left (the reference's trick kernel occupies that many SMs
+    with a spinning block instead)."""
+    hip = _lib.hip()
+    props = _lib.DeviceProps()
+    _lib.check(hip.scaph_get_device_props(0, ctypes.byref(props)))
+    cus = props.compute_units
+    counts = {}
+    for reserved in (0, cus // 2, cus - 8):
+        s = ctypes.c_void_p()
+        _lib.check(hip.scaph_stream_create_cu_limited(ctypes.byref(s), reserved))
+        try:
+            counts[reserved] = len(hw_id_tuples(hip, s))
+        finally:
+            hip.scaph_stream_destroy(s)
+    say("CUs", cus, "distinct CUs reached by reserved count", counts)
+    assert counts[0] <= cus
+    assert counts[cus // 2] <= cus - cus // 2
+    assert counts[cus - 8] <= 8
+    assert counts[cus - 8] < counts[cus // 2] < counts[0]
+    s = ctypes.c_void_p()
+    assert hip.scaph_stream_create_cu_limited(ctypes.byref(s), cus) != 0
+
+
+@pytest.mark.parametrize("flags", [["-tricknum=200"],
+                                   ["-tricknum=128", "-page_words=262144", "-max_memory=3",
+                                    "-threshold=3", "-num_stream=3"]])
+def test_tricknum_bfs_parity(flags):
+    off, ngh = O.rmat_csr(16, 16, seed=8)
+    deg = np.diff(off.astype(np.int64))
+    src = int(np.argmax(deg))
+    say("tricknum", flags)
+    with engine(*flags) as e:
+        e.load_csr(off, ngh)
+        e.run(src)
+        np.testing.assert_array_equal(e.levels(), O.bfs(off, ngh, src))
+
+
+def test_graphfile_with_transposed_pages():
+    """-graphfile with the transposed-page generator's files: the engine takes
+    the stored in-edge graph (after checking it is the transpose) and the
+    levels stay exact; a stored transpose that differs is refused."""
+    import subprocess
+    import tempfile
+    off, ngh = O.rmat_csr(14, 16, seed=12)
+    deg = np.diff(off.astype(np.int64))
+    src = int(np.argmax(deg))
+    want = O.bfs(off, ngh, src)
+    pagegen = os.path.join(ROOT, "scaph_amd", "bin", "pagegencsr")
+    with tempfile.TemporaryDirectory() as td:
+        base = os.path.join(td, "g")
+        scaph_amd.write_graph(base, off, ngh, pagesize_words=0, idx64=True, write_wgh=False)
+        subprocess.run([pagegen, base, "1", "u", "t"], check=True, stdout=subprocess.DEVNULL)
+        assert os.path.exists(base + "_in_csr.ngh")
+        for sched in ("fixed", "pull"):
+            say("graphfile with transposed pages", sched)
+            with engine(f"-graphfile={base}", f"-schedule={sched}") as e:
+                e.load()
+                e.run(src)
+                np.testing.assert_array_equal(e.levels(), want)
+        t_ngh = np.fromfile(base + "_in_csr.ngh", np.uint32)
+        t_ngh[len(t_ngh) // 2] ^= 1
+        t_ngh.tofile(base + "_in_csr.ngh")
+        with engine(f"-graphfile={base}") as e:
+            with pytest.raises(RuntimeError, match="not the transpose"):
+                e.load()

@@ -224,3 +224,66 @@ def test_graphgen_weights_and_app_tools():
                              capture_output=True, text=True)
         assert out.returncode != 0
         assert ("no -graphfile and no -gen" in out.stdout or "device" in out.stdout), out.stdout
+
+
+def test_pagegencsr_transposed_pages():
+    """pagegencsr ... t writes the in-edge graph (sources ascending per
+    target) and its page files, cut like any other graph's."""
+    off, ngh = O.rmat_csr(13, 16, seed=6)
+    n = len(off) - 1
+    src = np.repeat(np.arange(n), np.diff(off.astype(np.int64)))
+    t_off, t_ngh = O.csr_from_edges(n, ngh, src)   # stable: sources stay ascending
+    with tempfile.TemporaryDirectory() as td:
+        base = os.path.join(td, "g")
+        off.astype(np.uint64).tofile(base + "_csr.idx")
+        ngh.tofile(base + "_csr.ngh")
+        np.diff(off).astype(np.uint32).tofile(base + "_csr.deg")
+        out = subprocess.run([os.path.join(BIN, "pagegencsr"), base, "1", "u", "t"],
+                             capture_output=True, text=True, check=True)
+        assert "in-edge pages:" in out.stdout
+        np.testing.assert_array_equal(np.fromfile(base + "_in_csr.idx", np.uint64), t_off)
+        np.testing.assert_array_equal(np.fromfile(base + "_in_csr.ngh", np.uint32), t_ngh)
+        np.testing.assert_array_equal(np.fromfile(base + "_in_csr.deg", np.uint32),
+                                      np.diff(t_off).astype(np.uint32))
+        cuts = O.page_cuts(t_off, 1 << 20)
+        info = np.fromfile(base + "_inpage.info", np.uint32).reshape(-1, 4)
+        np.testing.assert_array_equal(info, cuts)
+        assert len([f for f in os.listdir(td) if re.fullmatch(r"g_inpage\d+", f)]) == len(cuts)
+
+
+def test_exporters_match_reference_tools():
+    """bin2ligra[weight] / bin2totem[weight] (product) against the reference's
+    exporters compiled from source (u32 offsets, the reference's only read
+    width).  bin2totemweight of the reference overruns its weight array, so
+    the weighted Totem file is checked as the unweighted reference file with
+    the weights appended per line."""
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    if not os.path.exists(os.path.join(ref, "bin2totem")):
+        pytest.skip("reference exporters not built here")
+    off, ngh = O.rmat_csr(12, 8, seed=14)
+    rng = np.random.default_rng(3)
+    wgh = rng.integers(0, 1 << 32, len(ngh), dtype=np.uint64).astype(np.uint32)
+    with tempfile.TemporaryDirectory() as td:
+        for who in ("ref", "mine"):
+            base = os.path.join(td, who)
+            off.astype(np.uint32).tofile(base + "_csr.idx")
+            ngh.tofile(base + "_csr.ngh")
+            np.diff(off).astype(np.uint32).tofile(base + "_csr.deg")
+            wgh.tofile(base + "_csr.wgh")
+        for tool in ("bin2ligra", "bin2ligraweight", "bin2totem"):
+            subprocess.run([os.path.join(ref, tool), os.path.join(td, "ref")], check=True)
+        for tool in ("bin2ligra", "bin2ligraweight", "bin2totem", "bin2totemweight"):
+            subprocess.run([os.path.join(BIN, tool), os.path.join(td, "mine")], check=True)
+        for suffix in (".ligra", ".ligra.weight", ".totem"):
+            assert sha(os.path.join(td, "ref" + suffix)) == \
+                sha(os.path.join(td, "mine" + suffix)), suffix
+        ref_lines = open(os.path.join(td, "ref.totem")).read().splitlines()
+        mine_lines = open(os.path.join(td, "mineweight.totem")).read().splitlines()
+        assert mine_lines[:3] == ref_lines[:3]
+        want = [f"{a}\t{w}" for a, w in zip(ref_lines[3:], wgh.tolist())]
+        assert mine_lines[3:] == want
+        # "w" as a second argument selects the weighted format as well
+        os.remove(os.path.join(td, "mine.ligra.weight"))
+        subprocess.run([os.path.join(BIN, "bin2ligra"), os.path.join(td, "mine"), "w"], check=True)
+        assert sha(os.path.join(td, "ref.ligra.weight")) == \
+            sha(os.path.join(td, "mine.ligra.weight"))
