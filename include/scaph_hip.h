@@ -322,6 +322,17 @@ int scaph_relax_round(const scaph_graph_desc* graph, const uint32_t* queue, uint
                       const uint32_t* weights, const uint32_t* pool_base, uint32_t* value,
                       uint32_t* front_out, scaph_level_counters* counters, void* workspace,
                       size_t workspace_bytes, void* stream);
+/* The same round over one page buffer in device memory (paged mode; the
+ * reference's per-page gpu_push_csr launch, src/gpu_engine.cuh:41-169):
+ * the active Nodes of [first_vertex, first_vertex + nodenum) are those set
+ * in front_in.  weights/pool_base as above, with pool_base the base of the
+ * page slot pool holding buf.  Workspace from scaph_relax_workspace_bytes
+ * (page edges). */
+int scaph_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_vertex,
+                     const uint32_t* front_in, uint32_t n, const uint32_t* weights,
+                     const uint32_t* pool_base, uint32_t* value, uint32_t* front_out,
+                     scaph_level_counters* counters, void* workspace, size_t workspace_bytes,
+                     void* stream);
 
 /* Level-array check kernel: counts edges (u,v) violating
  * level[v] <= level[u] + 1 and reached vertices without a parent at

@@ -77,6 +77,36 @@ __device__ __forceinline__ void relax_flush(scaph_level_counters* counters,
   }
 }
 
+// One wave relaxes vertex u's out-list: the first kRelaxChunk edges here,
+// the rest appended as chunks for k_relax_chunks.
+template <int WEIGHTED>
+__device__ __forceinline__ void relax_node(uint32_t u, NodeRef nr, long long wdelta, uint32_t n,
+                                           uint32_t* value, uint32_t* front_out,
+                                           scaph_level_counters* counters, const RelaxWs& ws,
+                                           unsigned long long& acc_n, unsigned long long& acc_s,
+                                           uint32_t lane) {
+  const uint32_t val = value[u];
+  const uint32_t first = nr.len < kRelaxChunk ? nr.len : kRelaxChunk;
+  relax_list<WEIGHTED>(nr.edges, first, val, wdelta, n, value, front_out, acc_n, acc_s, lane,
+                       SCAPH_WAVE);
+  if (nr.len > kRelaxChunk && lane == 0) {
+    const uint32_t rest = nr.len - kRelaxChunk;
+    const uint32_t nch = (rest + kRelaxChunk - 1) / kRelaxChunk;
+    const unsigned long long at = atomicAdd(ws.count, (unsigned long long)nch);
+    if (at + nch <= ws.cap) {
+      for (uint32_t c = 0; c < nch; ++c) {
+        RelaxChunk ch;
+        ch.u = u;
+        ch.len = min(kRelaxChunk, rest - c * kRelaxChunk);
+        ch.ptr = reinterpret_cast<uint64_t>(nr.edges + kRelaxChunk + (uint64_t)c * kRelaxChunk);
+        ws.chunks[at + c] = ch;
+      }
+    } else {
+      counters->reserved = 1;  // overflow: the host reports an error
+    }
+  }
+}
+
 template <int WEIGHTED>
 __global__ void __launch_bounds__(kBlock)
 k_relax(scaph_graph_desc g, const uint32_t* q, uint32_t n_q, long long wdelta, uint32_t* value,
@@ -91,26 +121,43 @@ k_relax(scaph_graph_desc g, const uint32_t* q, uint32_t n_q, long long wdelta, u
   for (uint32_t i = blockIdx.x * nw + threadIdx.x / SCAPH_WAVE; i < n_q; i += gridDim.x * nw) {
     const uint32_t u = q[i];
     if (u >= n) continue;
-    const NodeRef nr = load_node(t, g, u);
-    const uint32_t val = value[u];
-    const uint32_t first = nr.len < kRelaxChunk ? nr.len : kRelaxChunk;
-    relax_list<WEIGHTED>(nr.edges, first, val, wdelta, n, value, front_out, acc_n, acc_s, lane,
-                         SCAPH_WAVE);
-    if (nr.len > kRelaxChunk && lane == 0) {
-      const uint32_t rest = nr.len - kRelaxChunk;
-      const uint32_t nch = (rest + kRelaxChunk - 1) / kRelaxChunk;
-      const unsigned long long at = atomicAdd(ws.count, (unsigned long long)nch);
-      if (at + nch <= ws.cap) {
-        for (uint32_t c = 0; c < nch; ++c) {
-          RelaxChunk ch;
-          ch.u = u;
-          ch.len = min(kRelaxChunk, rest - c * kRelaxChunk);
-          ch.ptr = reinterpret_cast<uint64_t>(nr.edges + kRelaxChunk + (uint64_t)c * kRelaxChunk);
-          ws.chunks[at + c] = ch;
-        }
-      } else {
-        counters->reserved = 1;  // overflow: the host reports an error
-      }
+    relax_node<WEIGHTED>(u, load_node(t, g, u), wdelta, n, value, front_out, counters, ws, acc_n,
+                         acc_s, lane);
+  }
+  relax_flush(counters, acc_n, acc_s);
+}
+
+// One page buffer (Node records {id, edge offset, degree} then the
+// out-lists, the layout of tools/bin2pagecsr.cpp) of Nodes
+// [first_vertex, first_vertex + nodenum): a wave takes 64 consecutive Nodes,
+// finds the active ones from front_in and relaxes each in turn.
+template <int WEIGHTED>
+__global__ void __launch_bounds__(kBlock)
+k_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_vertex, const uint32_t* front_in,
+             uint32_t n, long long wdelta, uint32_t* value, uint32_t* front_out,
+             scaph_level_counters* counters, RelaxWs ws) {
+  const uint32_t lane = lane_id();
+  const uint32_t nw = kBlock / SCAPH_WAVE;
+  const uint32_t* edges = buf + 3ull * nodenum;
+  unsigned long long acc_n = 0, acc_s = 0;
+  for (uint32_t i0 = (blockIdx.x * nw + threadIdx.x / SCAPH_WAVE) * SCAPH_WAVE; i0 < nodenum;
+       i0 += gridDim.x * nw * SCAPH_WAVE) {
+    const uint32_t i = i0 + lane;
+    bool act = false;
+    if (i < nodenum) {
+      const uint32_t v = first_vertex + i;
+      act = (front_in[v >> 5] >> (v & 31)) & 1u;
+    }
+    uint64_t mask = __ballot(act);
+    while (mask) {
+      const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+      mask &= mask - 1;
+      const uint32_t* node = buf + 3ull * (i0 + j);
+      NodeRef nr;
+      nr.edges = edges + node[1];
+      nr.len = node[2];
+      relax_node<WEIGHTED>(first_vertex + i0 + j, nr, wdelta, n, value, front_out, counters, ws,
+                           acc_n, acc_s, lane);
     }
   }
   relax_flush(counters, acc_n, acc_s);
@@ -200,6 +247,43 @@ int scaph_relax_round(const scaph_graph_desc* graph, const uint32_t* queue, uint
     SCAPH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_relax_chunks<0>, dim3(2048), dim3(kBlock), 0, st, ws,
                        graph->num_vertices, wdelta, value, front_out, counters);
+  }
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_vertex,
+                     const uint32_t* front_in, uint32_t n, const uint32_t* weights,
+                     const uint32_t* pool_base, uint32_t* value, uint32_t* front_out,
+                     scaph_level_counters* counters, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+  if (!buf || !front_in || !value || !front_out || !counters || !workspace ||
+      (weights && !pool_base) || (uint64_t)first_vertex + nodenum > n)
+    return SCAPH_STATUS_INVALID;
+  if (nodenum == 0) return 0;
+  RelaxWs ws = carve_relax_ws(workspace, 0);
+  ws.cap = workspace_bytes > 256 ? (workspace_bytes - 256) / sizeof(RelaxChunk) : 0;
+  if (ws.cap == 0) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  const long long wdelta =
+      weights ? (long long)(reinterpret_cast<uintptr_t>(weights) -
+                            reinterpret_cast<uintptr_t>(pool_base))
+              : 0ll;
+  hipLaunchKernelGGL(k_relax_zero, dim3(1), dim3(1), 0, st, ws.count);
+  SCAPH_LAUNCH_CHECK();
+  const unsigned grid = grid_for(nodenum, kBlock, 8192);
+  if (weights) {
+    hipLaunchKernelGGL(k_relax_page<1>, dim3(grid), dim3(kBlock), 0, st, buf, nodenum,
+                       first_vertex, front_in, n, wdelta, value, front_out, counters, ws);
+    SCAPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_relax_chunks<1>, dim3(1024), dim3(kBlock), 0, st, ws, n, wdelta, value,
+                       front_out, counters);
+  } else {
+    hipLaunchKernelGGL(k_relax_page<0>, dim3(grid), dim3(kBlock), 0, st, buf, nodenum,
+                       first_vertex, front_in, n, wdelta, value, front_out, counters, ws);
+    SCAPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_relax_chunks<0>, dim3(1024), dim3(kBlock), 0, st, ws, n, wdelta, value,
+                       front_out, counters);
   }
   SCAPH_LAUNCH_CHECK();
   return 0;

@@ -173,6 +173,7 @@ void Engine::reset_graph() {
   dpages_in_.reset();
   dptr_in_.reset();
   wpool_.reset();
+  weights_loaded_ = false;
   deg_out_.reset();
   deg_in_.reset();
   static_visited_.reset();
@@ -215,8 +216,23 @@ void Engine::plan_residency() {
 // weight at a fixed distance.  Pages are consecutive vertex ranges, so page
 // p's edges are CSR edges [sum of earlier edgenum, + edgenum).
 void Engine::set_weights(const uint32_t* wgh, uint64_t count) {
-  if (!resident_) throw Error(SCAPH_STATUS_INVALID, "edge weights need the graph resident");
   if (count != m_ || (m_ && !wgh)) throw Error(SCAPH_STATUS_INVALID, "one weight per edge expected");
+  if (!resident_) {
+    // paged: each page's weights in pinned memory, copied with the page
+    PagedState& ps = *paged_;
+    ps.h_wpage.clear();
+    ps.h_wpage.resize(pages_out_.size());
+    uint64_t e0 = 0;
+    for (size_t p = 0; p < pages_out_.size(); ++p) {
+      const uint64_t ne = pages_out_[p].edgenum;
+      ps.h_wpage[p].alloc(std::max<uint64_t>(ne, 1) * 4);
+      std::copy(wgh + e0, wgh + e0 + ne, ps.h_wpage[p].get<uint32_t>());
+      e0 += ne;
+    }
+    ps.wpool.reset();  // slots are filled on the next SSSP run
+    weights_loaded_ = true;
+    return;
+  }
   uint64_t total = 0;
   std::vector<uint64_t> offs(pages_out_.size());
   for (size_t p = 0; p < pages_out_.size(); ++p) {
@@ -241,6 +257,7 @@ void Engine::set_weights(const uint32_t* wgh, uint64_t count) {
     main_->sync();
     e0 += pg.edgenum;
   }
+  weights_loaded_ = true;
 }
 
 void Engine::upload_pages(const HostCsr& g, bool weighted, std::vector<scaph_page>* pages,
@@ -377,6 +394,7 @@ void Engine::load_csr(HostCsr g, bool keep_host, HostCsr* in_edges) {
     host_ = std::move(g);
     finish_setup(nullptr);
     setup_paged();
+    if (host_.wgh.size() == m_ && m_ > 0) set_weights(host_.wgh.data(), host_.wgh.size());
   }
 }
 

@@ -140,7 +140,7 @@ class Engine {
   // Edge weights in CSR edge order (the _csr.wgh file), for SSSP; set
   // automatically when a loaded CSR carries them.
   void set_weights(const uint32_t* wgh, uint64_t count);
-  bool has_weights() const { return wpool_.get() != nullptr; }
+  bool has_weights() const { return weights_loaded_; }
   void levels_to_host(uint32_t* out) const;
   // Device-side BFS invariant check; returns {edge violations, orphans}.
   std::array<uint64_t, 2> validate(uint32_t src);
@@ -214,6 +214,14 @@ class Engine {
   void ensure_model_buffers();
   RunStats bfs_paged(uint32_t src);
   void page_histogram(const uint32_t* nodes, LevelRecord* rec);
+  // WCC / SSSP rounds in paged mode: the value array, and the weight pool
+  // (same slots as the page pool) for SSSP
+  struct RelaxRound {
+    uint32_t* value;
+    const uint32_t* weights;
+  };
+  void schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunStats* rs);
+  RunStats relax_paged(int mode, uint32_t src);
 
   Config cfg_;
   uint64_t P_;  // page size in words
@@ -226,6 +234,7 @@ class Engine {
   std::vector<scaph_page> pages_out_, pages_in_;
   DevBuf pool_out_, pool_in_;     // packed resident pages
   DevBuf wpool_;                  // edge weights laid out like pool_out_ (SSSP)
+  bool weights_loaded_ = false;   // resident: wpool_; paged: PagedState::h_wpage
   DevBuf relax_ws_;               // chunk list of the WCC/SSSP rounds
   RunStats relax(int mode, uint32_t src);
   DevBuf dpages_out_, dptr_out_, dpages_in_, dptr_in_;

@@ -195,6 +195,12 @@ void fill_stats(const scaph::RunStats& r, scaph_bfs_stats* out) {
   out->ms = r.ms;
   out->n_reached = r.n_reached;
   out->m_reached = r.m_reached;
+  out->page_execs = r.page_execs;
+  out->page_execs_cached = r.page_execs_cached;
+  out->pages_full = r.pages_full;
+  out->pages_compacted = r.pages_compacted;
+  out->bytes_h2d = r.bytes_h2d;
+  out->h2d_ms = r.h2d_ms;
 }
 }  // namespace
 

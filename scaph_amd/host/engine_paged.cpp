@@ -148,7 +148,6 @@ RunStats Engine::bfs_paged(uint32_t src) {
   rs.n_reached = 1;
   rs.m_reached = m_f;
   uint32_t cur = 0;
-  const int ns = (int)ps.exec.size();
   while (n_f > 0) {
     const auto tl = std::chrono::steady_clock::now();
     // get_page_info / get_total_info + D2H (src/app.cuh:188-207)
@@ -162,7 +161,6 @@ RunStats Engine::bfs_paged(uint32_t src) {
     H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, sizeof(scaph_level_counters) / 4, st));
     main_->sync();
     const uint32_t* nodes = ps.h_nodes.get<uint32_t>();
-    const uint32_t* datas = ps.h_datas.get<uint32_t>();
     if (ps.h_totals.get<unsigned long long>()[1] == 0) break;
     LevelRecord rec;
     rec.level = cur;
@@ -172,181 +170,7 @@ RunStats Engine::bfs_paged(uint32_t src) {
     rec.m_u = m_u;
     page_histogram(nodes, &rec);
 
-    // cold pages need the frontier on the host for compaction
-    std::vector<int> cold;
-    for (size_t p = 0; p < np; ++p)
-      if (!ps.handle.vp[p].cached && datas[p] > 0 && datas[p] <= ps.thres) cold.push_back((int)p);
-    if (!cold.empty()) {
-      H_ERR(scaph_copy_d2h_async(ps.h_front.get(), front_[fa].get(), words * 4, st));
-      main_->sync();
-    }
-    ps.resident_at_reset.assign(np, 0);
-    for (size_t p = 0; p < np; ++p) ps.resident_at_reset[p] = ps.handle.vp[p].cached ? 1 : 0;
-    const bool grouping = cfg_.page_groups > 0;
-    ps.handle.queue_reset(datas, grouping);
-    rs.pages_full += ps.handle.num_full;
-    // grouped launches: about page_groups of them over the cached active
-    // pages; the first is smaller when copies wait for slots
-    uint64_t group_target = 0;
-    bool first_group = true;
-    std::vector<int> group_pages;
-    const bool copies_pending = ps.handle.num_full > 0 || !cold.empty();
-    if (grouping) {
-      uint64_t e = 0;
-      for (int p : ps.handle.group_queue()) e += datas[p] - 3ull * nodes[p];
-      group_target = std::max<uint64_t>(e / (uint64_t)cfg_.page_groups, 1);
-    }
-
-    // compaction workers (parallel_compress_stat_async, src/compress.h:42-56)
-    std::atomic<size_t> next{0};
-    std::atomic<bool> failed{false};
-    Workers workers;
-    const int nw = (int)std::min<size_t>(cold.size(), 8);
-    for (int w = 0; w < nw; ++w) {
-      workers.t.emplace_back([&] {
-        for (size_t k = next++; k < cold.size(); k = next++) {
-          const int p = cold[k];
-          try {
-            const uint64_t used = compact_page(ps.h_page[p].get<uint32_t>(), pages_out_[p],
-                                               ps.h_front.get<uint32_t>(), nodes[p],
-                                               ps.h_tail[p].get<uint32_t>());
-            VPstat& v = ps.handle.vp[p];
-            v.nodenum = nodes[p];
-            v.datanum = used;
-            v.chunknum = (int)((used + ps.chunk - 1) / ps.chunk);
-            v.shared = true;
-            ps.handle.merge_write(p);
-          } catch (...) {
-            failed = true;
-          }
-        }
-      });
-    }
-    rs.pages_compacted += cold.size();
-
-    // scheduling loop (gpu_push_batch, src/app.cuh:414-459), event driven
-    std::vector<int> last(ns, -1);
-    int copy_last = -1;
-    float copy_ms = 0;
-    bool copy_timed = false;
-    while (!ps.handle.finished()) {
-      if (failed) break;
-      bool progress = false;
-      for (int s = 0; s < ns; ++s) {
-        if (!ps.exec_done[s]->done()) continue;
-        if (last[s] != -1) {
-          ps.handle.release_exec_finished(last[s]);
-          last[s] = -1;
-          progress = true;
-        }
-        int e;
-        if (ps.handle.get_execute(&e)) {
-          const VPstat& v = ps.handle.vp[e];
-          const uint32_t* buf = ps.pool.get<uint32_t>() + (uint64_t)v.dpid * P_ +
-                                (uint64_t)v.dpoff * ps.chunk;
-          const uint64_t edges_bound = v.shared ? v.datanum - 3ull * v.nodenum
-                                                : (uint64_t)pages_out_[e].edgenum;
-          // page counts of this frontier: the exact active Nodes and out-edges
-          scaph_page_hint hint;
-          hint.first_vertex = v.shared ? SCAPH_PAGE_COMPACTED : pages_out_[e].left;
-          hint.active_nodes = nodes[e];
-          hint.active_edges = datas[e] - 3ull * nodes[e];
-          H_ERR(scaph_bfs_push_page(buf, v.nodenum, std::max<uint64_t>(edges_bound, 1), n_, deg,
-                                    level, visited, front_[fa].get<uint32_t>(),
-                                    front_[fa ^ 1].get<uint32_t>(), cnt, cur, ps.ws[s].get(),
-                                    ps.ws_bytes, cfg_.page_hint ? &hint : nullptr, &push_tune_,
-                                    ps.exec[s]->get()));
-          ps.exec_done[s]->record(ps.exec[s]->get());
-          last[s] = e;
-          ++rs.page_execs;
-          if (ps.resident_at_reset[e]) ++rs.page_execs_cached;
-          progress = true;
-        }
-      }
-      if (grouping && ps.group_done->done()) {
-        if (!group_pages.empty()) {
-          for (int p : group_pages) ps.handle.release_exec_finished(p);
-          group_pages.clear();
-          progress = true;
-        }
-        auto& gq = ps.handle.group_queue();
-        if (!gq.empty()) {
-          uint64_t target = group_target;
-          if (first_group && copies_pending) target = std::max<uint64_t>(target / 4, 1);
-          first_group = false;
-          auto* tab = ps.h_runs.get<scaph_page_run>();
-          uint64_t g_edges = 0, g_nodes = 0;
-          uint32_t segs = 0;
-          while (!gq.empty()) {
-            const int p = gq.front();
-            if (!group_pages.empty() &&
-                (g_edges >= target || g_nodes + nodes[p] > ps.group_nodes_cap))
-              break;
-            ps.handle.take_group();
-            const VPstat& v = ps.handle.vp[p];
-            scaph_page_run& r = tab[group_pages.size()];
-            r.buf = reinterpret_cast<unsigned long long>(ps.pool.get<uint32_t>() +
-                                                         (uint64_t)v.dpid * P_);
-            r.first_vertex = pages_out_[p].left;
-            r.nodenum = pages_out_[p].nodenum;
-            r.seg0 = segs;
-            r.reserved = 0;
-            segs += (r.nodenum + SCAPH_RUN_SEGMENT - 1) / SCAPH_RUN_SEGMENT;
-            g_edges += datas[p] - 3ull * nodes[p];
-            g_nodes += nodes[p];
-            group_pages.push_back(p);
-            ++rs.page_execs;
-            if (ps.resident_at_reset[p]) ++rs.page_execs_cached;
-          }
-          void* gs = ps.group->get();
-          H_ERR(scaph_copy_h2d_async(ps.d_runs.get(), tab,
-                                     group_pages.size() * sizeof(scaph_page_run), gs));
-          H_ERR(scaph_bfs_push_runs(ps.d_runs.get<scaph_page_run>(), (uint32_t)group_pages.size(),
-                                    segs, g_nodes, g_edges, n_, deg, level, visited,
-                                    front_[fa].get<uint32_t>(), front_[fa ^ 1].get<uint32_t>(),
-                                    cnt, cur, ps.group_ws.get(), ps.group_ws_bytes, &push_tune_,
-                                    gs));
-          ps.group_done->record(gs);
-          progress = true;
-        }
-      }
-      if (ps.copy_end->done()) {
-        if (copy_last != -1) {
-          if (copy_timed) copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
-          ps.handle.release_copy_finished(copy_last);
-          copy_last = -1;
-          progress = true;
-        }
-        int s;
-        if (ps.handle.get_empty(&s)) {
-          // PageCopy (src/app.cuh:371-393): a compacted run into its chunks,
-          // or the whole page into its slot
-          const VPstat& v = ps.handle.vp[s];
-          uint32_t* dst = ps.pool.get<uint32_t>() + (uint64_t)v.dpid * P_ +
-                          (uint64_t)v.dpoff * ps.chunk;
-          const void* srcp = v.shared ? ps.h_tail[s].get() : ps.h_page[s].get();
-          const uint64_t w = v.shared ? v.datanum : page_used_words(pages_out_[s], false);
-          ps.copy_beg->record(ps.copy->get());
-          H_ERR(scaph_copy_h2d_async(dst, srcp, w * 4, ps.copy->get()));
-          ps.copy_end->record(ps.copy->get());
-          copy_timed = true;
-          rs.bytes_h2d += w * 4;
-          copy_last = s;
-          progress = true;
-        }
-      }
-      if (!progress) std::this_thread::yield();
-    }
-    workers.join();
-    if (failed) throw Error(SCAPH_STATUS_INVALID, "page compaction failed");
-    for (int s = 0; s < ns; ++s) ps.exec[s]->sync();
-    if (ps.group) ps.group->sync();
-    ps.copy->sync();
-    if (copy_last != -1) {
-      copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
-      ps.handle.release_copy_finished(copy_last);
-    }
-    rs.h2d_ms += copy_ms;
+    schedule_pages(fa, cur, nullptr, &rs);
     H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
     main_->sync();
     rec.n_next = hc->n_next;
@@ -368,6 +192,222 @@ RunStats Engine::bfs_paged(uint32_t src) {
   rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   rs.depth = cur;
   return rs;
+}
+
+// One level (or relax round) over the active pages of front_[fa], whose
+// counts are in ps.h_nodes / ps.h_datas: classification (src/pagehandle.h:
+// 326-379), compaction of the cold pages on worker threads, copies on the
+// copy stream and page launches on the exec (and group) streams
+// (gpu_push_batch, src/app.cuh:395-468).  `relax` selects the WCC / SSSP
+// relax kernel instead of the BFS push.
+void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunStats* rs) {
+  PagedState& ps = *paged_;
+  const size_t np = pages_out_.size();
+  const size_t words = bitmap_words(n_);
+  auto* cnt = counters_.get<scaph_level_counters>();
+  uint32_t* level = level_.get<uint32_t>();
+  uint32_t* visited = visited_.get<uint32_t>();
+  const uint32_t* deg = deg_out_.get<uint32_t>();
+  void* st = main_->get();
+  const int ns = (int)ps.exec.size();
+  const uint32_t* nodes = ps.h_nodes.get<uint32_t>();
+  const uint32_t* datas = ps.h_datas.get<uint32_t>();
+  // relax rounds copy every page whole (no compaction: weights would need
+  // a compacted copy too); cold pages need the frontier on the host
+  const uint64_t thres = relax ? 0 : ps.thres;
+  ps.handle.set_threshold(thres);
+  std::vector<int> cold;
+  for (size_t p = 0; p < np; ++p)
+    if (!ps.handle.vp[p].cached && datas[p] > 0 && datas[p] <= thres) cold.push_back((int)p);
+  if (!cold.empty()) {
+    H_ERR(scaph_copy_d2h_async(ps.h_front.get(), front_[fa].get(), words * 4, st));
+    main_->sync();
+  }
+  ps.resident_at_reset.assign(np, 0);
+  for (size_t p = 0; p < np; ++p) ps.resident_at_reset[p] = ps.handle.vp[p].cached ? 1 : 0;
+  const bool grouping = cfg_.page_groups > 0 && !relax;
+  ps.handle.queue_reset(datas, grouping);
+  rs->pages_full += ps.handle.num_full;
+  // grouped launches: about page_groups of them over the cached active
+  // pages; the first is smaller when copies wait for slots
+  uint64_t group_target = 0;
+  bool first_group = true;
+  std::vector<int> group_pages;
+  const bool copies_pending = ps.handle.num_full > 0 || !cold.empty();
+  if (grouping) {
+    uint64_t e = 0;
+    for (int p : ps.handle.group_queue()) e += datas[p] - 3ull * nodes[p];
+    group_target = std::max<uint64_t>(e / (uint64_t)cfg_.page_groups, 1);
+  }
+
+  // compaction workers (parallel_compress_stat_async, src/compress.h:42-56)
+  std::atomic<size_t> next{0};
+  std::atomic<bool> failed{false};
+  Workers workers;
+  const int nw = (int)std::min<size_t>(cold.size(), 8);
+  for (int w = 0; w < nw; ++w) {
+    workers.t.emplace_back([&] {
+      for (size_t k = next++; k < cold.size(); k = next++) {
+        const int p = cold[k];
+        try {
+          const uint64_t used = compact_page(ps.h_page[p].get<uint32_t>(), pages_out_[p],
+                                             ps.h_front.get<uint32_t>(), nodes[p],
+                                             ps.h_tail[p].get<uint32_t>());
+          VPstat& v = ps.handle.vp[p];
+          v.nodenum = nodes[p];
+          v.datanum = used;
+          v.chunknum = (int)((used + ps.chunk - 1) / ps.chunk);
+          v.shared = true;
+          ps.handle.merge_write(p);
+        } catch (...) {
+          failed = true;
+        }
+      }
+    });
+  }
+  rs->pages_compacted += cold.size();
+
+  // scheduling loop (gpu_push_batch, src/app.cuh:414-459), event driven
+  std::vector<int> last(ns, -1);
+  int copy_last = -1;
+  float copy_ms = 0;
+  bool copy_timed = false;
+  while (!ps.handle.finished()) {
+    if (failed) break;
+    bool progress = false;
+    for (int s = 0; s < ns; ++s) {
+      if (!ps.exec_done[s]->done()) continue;
+      if (last[s] != -1) {
+        ps.handle.release_exec_finished(last[s]);
+        last[s] = -1;
+        progress = true;
+      }
+      int e;
+      if (ps.handle.get_execute(&e)) {
+        const VPstat& v = ps.handle.vp[e];
+        const uint32_t* buf = ps.pool.get<uint32_t>() + (uint64_t)v.dpid * P_ +
+                              (uint64_t)v.dpoff * ps.chunk;
+        const uint64_t edges_bound = v.shared ? v.datanum - 3ull * v.nodenum
+                                              : (uint64_t)pages_out_[e].edgenum;
+        if (relax) {
+          if (v.shared) throw Error(SCAPH_STATUS_INVALID, "relax rounds copy pages whole");
+          H_ERR(scaph_relax_page(buf, pages_out_[e].nodenum, pages_out_[e].left,
+                                 front_[fa].get<uint32_t>(),
+                                 n_, relax->weights, relax->weights ? ps.pool.get<uint32_t>()
+                                                                    : nullptr,
+                                 relax->value, front_[fa ^ 1].get<uint32_t>(), cnt,
+                                 ps.relax_ws[s].get(), ps.relax_ws[s].bytes(),
+                                 ps.exec[s]->get()));
+        } else {
+          // page counts of this frontier: the exact active Nodes and out-edges
+          scaph_page_hint hint;
+          hint.first_vertex = v.shared ? SCAPH_PAGE_COMPACTED : pages_out_[e].left;
+          hint.active_nodes = nodes[e];
+          hint.active_edges = datas[e] - 3ull * nodes[e];
+          H_ERR(scaph_bfs_push_page(buf, v.nodenum, std::max<uint64_t>(edges_bound, 1), n_, deg,
+                                    level, visited, front_[fa].get<uint32_t>(),
+                                    front_[fa ^ 1].get<uint32_t>(), cnt, cur, ps.ws[s].get(),
+                                    ps.ws_bytes, cfg_.page_hint ? &hint : nullptr, &push_tune_,
+                                    ps.exec[s]->get()));
+        }
+        ps.exec_done[s]->record(ps.exec[s]->get());
+        last[s] = e;
+        ++rs->page_execs;
+        if (ps.resident_at_reset[e]) ++rs->page_execs_cached;
+        progress = true;
+      }
+    }
+    if (grouping && ps.group_done->done()) {
+      if (!group_pages.empty()) {
+        for (int p : group_pages) ps.handle.release_exec_finished(p);
+        group_pages.clear();
+        progress = true;
+      }
+      auto& gq = ps.handle.group_queue();
+      if (!gq.empty()) {
+        uint64_t target = group_target;
+        if (first_group && copies_pending) target = std::max<uint64_t>(target / 4, 1);
+        first_group = false;
+        auto* tab = ps.h_runs.get<scaph_page_run>();
+        uint64_t g_edges = 0, g_nodes = 0;
+        uint32_t segs = 0;
+        while (!gq.empty()) {
+          const int p = gq.front();
+          if (!group_pages.empty() &&
+              (g_edges >= target || g_nodes + nodes[p] > ps.group_nodes_cap))
+            break;
+          ps.handle.take_group();
+          const VPstat& v = ps.handle.vp[p];
+          scaph_page_run& r = tab[group_pages.size()];
+          r.buf = reinterpret_cast<unsigned long long>(ps.pool.get<uint32_t>() +
+                                                       (uint64_t)v.dpid * P_);
+          r.first_vertex = pages_out_[p].left;
+          r.nodenum = pages_out_[p].nodenum;
+          r.seg0 = segs;
+          r.reserved = 0;
+          segs += (r.nodenum + SCAPH_RUN_SEGMENT - 1) / SCAPH_RUN_SEGMENT;
+          g_edges += datas[p] - 3ull * nodes[p];
+          g_nodes += nodes[p];
+          group_pages.push_back(p);
+          ++rs->page_execs;
+          if (ps.resident_at_reset[p]) ++rs->page_execs_cached;
+        }
+        void* gs = ps.group->get();
+        H_ERR(scaph_copy_h2d_async(ps.d_runs.get(), tab,
+                                   group_pages.size() * sizeof(scaph_page_run), gs));
+        H_ERR(scaph_bfs_push_runs(ps.d_runs.get<scaph_page_run>(), (uint32_t)group_pages.size(),
+                                  segs, g_nodes, g_edges, n_, deg, level, visited,
+                                  front_[fa].get<uint32_t>(), front_[fa ^ 1].get<uint32_t>(),
+                                  cnt, cur, ps.group_ws.get(), ps.group_ws_bytes, &push_tune_,
+                                  gs));
+        ps.group_done->record(gs);
+        progress = true;
+      }
+    }
+    if (ps.copy_end->done()) {
+      if (copy_last != -1) {
+        if (copy_timed) copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
+        ps.handle.release_copy_finished(copy_last);
+        copy_last = -1;
+        progress = true;
+      }
+      int s;
+      if (ps.handle.get_empty(&s)) {
+        // PageCopy (src/app.cuh:371-393): a compacted run into its chunks,
+        // or the whole page into its slot
+        const VPstat& v = ps.handle.vp[s];
+        uint32_t* dst = ps.pool.get<uint32_t>() + (uint64_t)v.dpid * P_ +
+                        (uint64_t)v.dpoff * ps.chunk;
+        const void* srcp = v.shared ? ps.h_tail[s].get() : ps.h_page[s].get();
+        const uint64_t w = v.shared ? v.datanum : page_used_words(pages_out_[s], false);
+        ps.copy_beg->record(ps.copy->get());
+        H_ERR(scaph_copy_h2d_async(dst, srcp, w * 4, ps.copy->get()));
+        if (relax && relax->weights && !v.shared && pages_out_[s].edgenum) {
+          // the page's weights into the same slot of the weight pool
+          const uint64_t off = (uint64_t)v.dpid * P_ + 3ull * pages_out_[s].nodenum;
+          H_ERR(scaph_copy_h2d_async(ps.wpool.get<uint32_t>() + off, ps.h_wpage[s].get(),
+                                     (uint64_t)pages_out_[s].edgenum * 4, ps.copy->get()));
+          rs->bytes_h2d += (uint64_t)pages_out_[s].edgenum * 4;
+        }
+        ps.copy_end->record(ps.copy->get());
+        copy_timed = true;
+        rs->bytes_h2d += w * 4;
+        copy_last = s;
+        progress = true;
+      }
+    }
+    if (!progress) std::this_thread::yield();
+  }
+  workers.join();
+  if (failed) throw Error(SCAPH_STATUS_INVALID, "page compaction failed");
+  for (int s = 0; s < ns; ++s) ps.exec[s]->sync();
+  if (ps.group) ps.group->sync();
+  ps.copy->sync();
+  if (copy_last != -1) {
+    copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
+    ps.handle.release_copy_finished(copy_last);
+  }
+  rs->h2d_ms += copy_ms;
 }
 
 // 12-bin histogram of the per-page active-vertex share (fraction of pages

@@ -4,6 +4,7 @@
 // vertices whose value it lowered form the next active set.  The rounds
 // stop when a round lowers nothing, as the reference's run() stops when no
 // vertex is active (src/app.cuh:220-265).
+#include <algorithm>
 #include <chrono>
 
 #include "engine_internal.h"
@@ -25,9 +26,9 @@ RunStats Engine::sssp(uint32_t src) {
 
 RunStats Engine::relax(int mode, uint32_t src) {
   if (n_ == 0) throw Error(SCAPH_STATUS_INVALID, "no graph loaded");
-  if (!resident_) throw Error(SCAPH_STATUS_INVALID, "WCC and SSSP need the graph resident");
   if (mode == SCAPH_RELAX_SSSP && src >= n_)
     throw Error(SCAPH_STATUS_INVALID, "source vertex out of range");
+  if (!resident_) return relax_paged(mode, src);
   const size_t ws_bytes = scaph_relax_workspace_bytes(m_);
   if (relax_ws_.bytes() < ws_bytes) relax_ws_.alloc(ws_bytes);
   void* st = main_->get();
@@ -79,6 +80,99 @@ RunStats Engine::relax(int mode, uint32_t src) {
     fa ^= 1;
     ++round;
   }
+  rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  rs.depth = round;
+  rs.n_reached = n_;
+  return rs;
+}
+
+// Paged rounds: the active pages of each round go through the same page
+// scheduler as a paged BFS level (cached pages first, the others copied
+// whole into free slots, with their weights for SSSP), each page relaxed by
+// scaph_relax_page on an exec stream.
+RunStats Engine::relax_paged(int mode, uint32_t src) {
+  PagedState& ps = *paged_;
+  void* st = main_->get();
+  const size_t np = pages_out_.size();
+  const size_t words = bitmap_words(n_);
+  auto* cnt = counters_.get<scaph_level_counters>();
+  auto* hc = hcount_.get<scaph_level_counters>();
+  uint32_t* value = level_.get<uint32_t>();
+  const bool weighted = mode == SCAPH_RELAX_SSSP;
+  // chunk workspaces: every page's long lists fit
+  uint64_t max_edges = 1;
+  for (const auto& pg : pages_out_) max_edges = std::max<uint64_t>(max_edges, pg.edgenum);
+  const size_t ws_bytes = scaph_relax_workspace_bytes(max_edges);
+  if (ps.relax_ws.size() != ps.exec.size() ||
+      (!ps.relax_ws.empty() && ps.relax_ws[0].bytes() < ws_bytes)) {
+    ps.relax_ws.clear();
+    for (size_t s = 0; s < ps.exec.size(); ++s) ps.relax_ws.emplace_back(ws_bytes);
+  }
+  if (weighted) {
+    // weight slots for the pages cached now (BFS levels may have replaced
+    // slots since the last SSSP run); later copies bring their own
+    if (!ps.wpool.get()) ps.wpool.alloc(ps.pool.bytes());
+    for (size_t p = 0; p < np; ++p) {
+      const VPstat& v = ps.handle.vp[p];
+      if (!v.cached || v.shared || pages_out_[p].edgenum == 0) continue;
+      const uint64_t off = (uint64_t)v.dpid * P_ + 3ull * pages_out_[p].nodenum;
+      H_ERR(scaph_copy_h2d_async(ps.wpool.get<uint32_t>() + off, ps.h_wpage[p].get(),
+                                 (uint64_t)pages_out_[p].edgenum * 4, ps.copy->get()));
+    }
+    ps.copy->sync();
+  }
+  RunStats rs;
+  rs.src = src;
+  const auto t0 = std::chrono::steady_clock::now();
+  H_ERR(scaph_relax_init(value, n_, mode, src, st));
+  int fa = 0;
+  uint64_t n_act;
+  if (mode == SCAPH_RELAX_WCC) {
+    H_ERR(scaph_fill_u32(front_[0].get<uint32_t>(), 0xFFFFFFFFu, words, st));
+    n_act = n_;
+  } else {
+    *hcount_.get<uint32_t>() = src;
+    H_ERR(scaph_copy_h2d_async(queue_[1].get(), hcount_.get(), 4, st));
+    H_ERR(scaph_fill_u32(front_[0].get<uint32_t>(), 0u, words, st));
+    H_ERR(scaph_frontier_queue_to_bitmap(queue_[1].get<uint32_t>(), 1, front_[0].get<uint32_t>(),
+                                         n_, st));
+    n_act = 1;
+  }
+  RelaxRound rr;
+  rr.value = value;
+  rr.weights = weighted ? ps.wpool.get<uint32_t>() : nullptr;
+  uint32_t round = 0;
+  while (n_act > 0) {
+    const auto tl = std::chrono::steady_clock::now();
+    H_ERR(scaph_page_counts(dpages_out_.get<scaph_page>(), (uint32_t)np, front_[fa].get<uint32_t>(),
+                            deg_out_.get<uint32_t>(), 0, page_nodes_.get<uint32_t>(),
+                            page_datas_.get<uint32_t>(), totals_.get<unsigned long long>(), st));
+    H_ERR(scaph_copy_d2h_async(ps.h_nodes.get(), page_nodes_.get(), np * 4, st));
+    H_ERR(scaph_copy_d2h_async(ps.h_datas.get(), page_datas_.get(), np * 4, st));
+    H_ERR(scaph_copy_d2h_async(ps.h_totals.get(), totals_.get(), 16, st));
+    H_ERR(scaph_fill_u32(front_[fa ^ 1].get<uint32_t>(), 0u, words, st));
+    H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, sizeof(scaph_level_counters) / 4,
+                         st));
+    main_->sync();
+    if (ps.h_totals.get<unsigned long long>()[1] == 0) break;
+    schedule_pages(fa, round, &rr, &rs);
+    H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
+    main_->sync();
+    if (hc->reserved) throw Error(SCAPH_STATUS_INVALID, "relax chunk list overflow");
+    LevelRecord rec;
+    rec.level = round;
+    rec.n_f = n_act;
+    rec.n_next = hc->n_next;
+    rec.scanned = hc->scanned;
+    rec.ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tl)
+                 .count();
+    rs.levels.push_back(rec);
+    rs.m_reached += hc->scanned;
+    n_act = hc->n_next;
+    fa ^= 1;
+    ++round;
+  }
+  ps.handle.set_threshold(ps.thres);
   rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   rs.depth = round;
   rs.n_reached = n_;

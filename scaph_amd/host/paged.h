@@ -33,6 +33,12 @@ struct Engine::PagedState {
   PinnedBuf h_runs;
   size_t group_ws_bytes = 0;
   uint64_t group_nodes_cap = 0;  // active Nodes one group may hold (workspace size)
+  // WCC / SSSP: per-exec-stream chunk workspaces, the pages' weights in
+  // pinned memory (edgenum words each) and a weight pool with the page
+  // pool's slots (a page's weights sit at its slot's edge offsets)
+  std::vector<DevBuf> relax_ws;
+  std::vector<PinnedBuf> h_wpage;
+  DevBuf wpool;
 };
 
 }  // namespace scaph

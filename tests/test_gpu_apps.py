@@ -169,17 +169,38 @@ def test_apps_then_bfs_and_errors():
         np.testing.assert_array_equal(e.levels(), O.wcc(off, ngh))
 
 
-def test_apps_need_resident_graph():
+@pytest.mark.parametrize("flags", [["-num_stream=1"], ["-num_stream=3"],
+                                   ["-num_stream=3", "-multi=2"], ["-num_stream=5", "-multi=3"]])
+def test_apps_paged(flags):
+    """WCC and SSSP out of core: a pool of a few page slots, pages copied
+    whole with their weights each round, on one or several exec streams, and
+    a BFS run in between (which replaces slots without weights)."""
     off, ngh = O.rmat_csr(16, 16, seed=8)
-    with scaph_amd.BFS("-page_words=262144", "-max_memory=3") as e:
+    rng = np.random.default_rng(4)
+    wgh = rng.integers(1, 200, len(ngh)).astype(np.uint32)
+    deg = np.diff(off.astype(np.int64))
+    base = ["-page_words=262144", "-max_memory=3", "-threshold=3"]
+    with scaph_amd.BFS(*base, *flags) as e:
         e.load_csr(off, ngh)
-        assert not e.info()["resident"]
-        with pytest.raises(RuntimeError):
-            e.wcc()
+        assert not e.info()["resident"] and e.info()["pages"] >= 4
+        say("paged apps", flags)
+        st = e.wcc()
+        np.testing.assert_array_equal(e.levels(), O.wcc(off, ngh))
+        assert st["bytes_h2d"] > 0
+        e.set_weights(wgh)
+        srcs = (int(np.argmax(deg)), int(np.nonzero(deg)[0][7]))
+        e.sssp(srcs[0])
+        np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, srcs[0]))
+        e.run(srcs[1])
+        np.testing.assert_array_equal(e.levels(), O.bfs(off, ngh, srcs[1]))
+        e.sssp(srcs[1])
+        np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, srcs[1]))
 
 
 @pytest.mark.parametrize("tool,extra", [("wcc", []), ("sssp", ["-src=7"]),
-                                        ("sssp", ["-src=7", "-page_words=8192"])])
+                                        ("sssp", ["-src=7", "-page_words=8192"]),
+                                        ("sssp", ["-src=7", "-page_words=65536",
+                                                  "-max_memory=1"])])
 def test_app_tools_check_passes(tool, extra):
     """bin/wcc and bin/sssp on a generated weighted graph, with the app's own
     CPU check (label sweeps / Dijkstra) and two rounds."""
