@@ -186,7 +186,7 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree,
 /* Bottom-up (pull) expand of one level: every unvisited vertex scans its
  * in-neighbours (transposed paged CSR) for one in the frontier bitmap.
  * In-list entries with the top bit set are hub indices into hub_list
- * (num_hubs <= 32768; see scaph_csr_from_keys): their frontier bits are
+ * (num_hubs <= 131072; see scaph_csr_from_keys): their frontier bits are
  * gathered once per level and probed from LDS.  Writes level/visited for
  * the vertices found and the full next-frontier bitmap (all words, so it
  * needs no clearing).  In-lists still unmatched after 512 entries are

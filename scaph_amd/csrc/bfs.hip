@@ -27,7 +27,7 @@ constexpr unsigned long long kSmallEdges = 8192;  // push frontiers this small r
 constexpr uint32_t kPullSerial = 8;                // per-lane scan before wave help
 constexpr uint32_t kHeavy = 512;                   // longer in-lists go to k_pull_heavy
 constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work item
-constexpr uint32_t kMaxHubs = 32768;               // hub frontier bits kept in LDS (4 KB)
+constexpr uint32_t kMaxHubs = 131072;              // hub frontier bits kept in LDS (16 KB)
 constexpr uint32_t kHubTag = 0x80000000u;          // in-list entry = kHubTag | hub index
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kBitsNodes = SCAPH_RUN_SEGMENT;  // Nodes per run segment (32 per thread)

@@ -18,7 +18,7 @@ double Total_throughput = 0;
 
 namespace {
 
-constexpr uint32_t kMaxHubs = 32768;  // frontier bits the pull expand keeps in LDS
+constexpr uint32_t kMaxHubs = 131072;  // frontier bits the pull expand keeps in LDS
 constexpr uint64_t kAlignWords = 64;  // page buffers start on 256-byte boundaries
 constexpr size_t kMaxTimedLevels = 1024;
 constexpr size_t kCounterWords = sizeof(scaph_level_counters) / 4;
