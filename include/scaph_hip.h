@@ -325,7 +325,8 @@ int scaph_relax_round(const scaph_graph_desc* graph, const uint32_t* queue, uint
 /* The same round over one page buffer in device memory (paged mode; the
  * reference's per-page gpu_push_csr launch, src/gpu_engine.cuh:41-169):
  * the active Nodes of [first_vertex, first_vertex + nodenum) are those set
- * in front_in.  weights/pool_base as above, with pool_base the base of the
+ * in front_in; with first_vertex == SCAPH_PAGE_COMPACTED the buffer holds
+ * only active Nodes (the compacted layout), each naming its vertex.  weights/pool_base as above, with pool_base the base of the
  * page slot pool holding buf.  Workspace from scaph_relax_workspace_bytes
  * (page edges). */
 int scaph_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_vertex,

@@ -130,7 +130,9 @@ k_relax(scaph_graph_desc g, const uint32_t* q, uint32_t n_q, long long wdelta, u
 // One page buffer (Node records {id, edge offset, degree} then the
 // out-lists, the layout of tools/bin2pagecsr.cpp) of Nodes
 // [first_vertex, first_vertex + nodenum): a wave takes 64 consecutive Nodes,
-// finds the active ones from front_in and relaxes each in turn.
+// finds the active ones from front_in and relaxes each in turn.  A
+// compacted buffer (first_vertex == SCAPH_PAGE_COMPACTED, src/compress.h
+// layout) holds only active Nodes, each naming its vertex.
 template <int WEIGHTED>
 __global__ void __launch_bounds__(kBlock)
 k_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_vertex, const uint32_t* front_in,
@@ -139,6 +141,7 @@ k_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_vertex, const
   const uint32_t lane = lane_id();
   const uint32_t nw = kBlock / SCAPH_WAVE;
   const uint32_t* edges = buf + 3ull * nodenum;
+  const bool compacted = first_vertex == SCAPH_PAGE_COMPACTED;
   unsigned long long acc_n = 0, acc_s = 0;
   for (uint32_t i0 = (blockIdx.x * nw + threadIdx.x / SCAPH_WAVE) * SCAPH_WAVE; i0 < nodenum;
        i0 += gridDim.x * nw * SCAPH_WAVE) {
@@ -146,7 +149,7 @@ k_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_vertex, const
     bool act = false;
     if (i < nodenum) {
       const uint32_t v = first_vertex + i;
-      act = (front_in[v >> 5] >> (v & 31)) & 1u;
+      act = compacted || ((front_in[v >> 5] >> (v & 31)) & 1u);
     }
     uint64_t mask = __ballot(act);
     while (mask) {
@@ -156,8 +159,9 @@ k_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_vertex, const
       NodeRef nr;
       nr.edges = edges + node[1];
       nr.len = node[2];
-      relax_node<WEIGHTED>(first_vertex + i0 + j, nr, wdelta, n, value, front_out, counters, ws,
-                           acc_n, acc_s, lane);
+      const uint32_t u = compacted ? node[0] : first_vertex + i0 + j;
+      if (u >= n) continue;
+      relax_node<WEIGHTED>(u, nr, wdelta, n, value, front_out, counters, ws, acc_n, acc_s, lane);
     }
   }
   relax_flush(counters, acc_n, acc_s);
@@ -258,7 +262,8 @@ int scaph_relax_page(const uint32_t* buf, uint32_t nodenum, uint32_t first_verte
                      scaph_level_counters* counters, void* workspace, size_t workspace_bytes,
                      void* stream) {
   if (!buf || !front_in || !value || !front_out || !counters || !workspace ||
-      (weights && !pool_base) || (uint64_t)first_vertex + nodenum > n)
+      (weights && !pool_base) ||
+      (first_vertex != SCAPH_PAGE_COMPACTED && (uint64_t)first_vertex + nodenum > n))
     return SCAPH_STATUS_INVALID;
   if (nodenum == 0) return 0;
   RelaxWs ws = carve_relax_ws(workspace, 0);

@@ -222,10 +222,13 @@ void Engine::set_weights(const uint32_t* wgh, uint64_t count) {
     PagedState& ps = *paged_;
     ps.h_wpage.clear();
     ps.h_wpage.resize(pages_out_.size());
+    ps.h_wtail.clear();
+    ps.h_wtail.resize(pages_out_.size());
     uint64_t e0 = 0;
     for (size_t p = 0; p < pages_out_.size(); ++p) {
       const uint64_t ne = pages_out_[p].edgenum;
       ps.h_wpage[p].alloc(std::max<uint64_t>(ne, 1) * 4);
+      if (ps.thres) ps.h_wtail[p].alloc(ps.thres * 4);
       std::copy(wgh + e0, wgh + e0 + ne, ps.h_wpage[p].get<uint32_t>());
       e0 += ne;
     }

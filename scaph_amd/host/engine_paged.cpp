@@ -46,6 +46,21 @@ uint64_t compact_page(const uint32_t* page, const scaph_page& pg, const uint32_t
   return 3ull * active_nodes + off;
 }
 
+// The weights of compact_page's output at the same word offsets (the Node
+// header words stay unused): wpage holds the page's weights in edge order.
+void compact_weights(const uint32_t* page, const scaph_page& pg, const uint32_t* wpage,
+                     const uint32_t* front, uint32_t active_nodes, uint32_t* out) {
+  uint32_t* wout = out + 3ull * active_nodes;
+  uint64_t off = 0;
+  for (uint32_t i = 0; i < pg.nodenum; ++i) {
+    const uint32_t v = pg.left + i;
+    if (!((front[v >> 5] >> (v & 31)) & 1u)) continue;
+    const uint32_t idx = page[3ull * i + 1], len = page[3ull * i + 2];
+    std::memcpy(wout + off, wpage + idx, (size_t)len * 4);
+    off += len;
+  }
+}
+
 // The compaction workers of one level, joined on every exit from it: an
 // error raised by the scheduling loop then reaches the caller instead of
 // destroying joinable threads (std::terminate).
@@ -212,10 +227,9 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   const int ns = (int)ps.exec.size();
   const uint32_t* nodes = ps.h_nodes.get<uint32_t>();
   const uint32_t* datas = ps.h_datas.get<uint32_t>();
-  // relax rounds copy every page whole (no compaction: weights would need
-  // a compacted copy too); cold pages need the frontier on the host
-  const uint64_t thres = relax ? 0 : ps.thres;
-  ps.handle.set_threshold(thres);
+  // cold pages need the frontier on the host for compaction
+  const uint64_t thres = ps.thres;
+  const bool weighted = relax && relax->weights;
   std::vector<int> cold;
   for (size_t p = 0; p < np; ++p)
     if (!ps.handle.vp[p].cached && datas[p] > 0 && datas[p] <= thres) cold.push_back((int)p);
@@ -253,6 +267,10 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
           const uint64_t used = compact_page(ps.h_page[p].get<uint32_t>(), pages_out_[p],
                                              ps.h_front.get<uint32_t>(), nodes[p],
                                              ps.h_tail[p].get<uint32_t>());
+          if (weighted)
+            compact_weights(ps.h_page[p].get<uint32_t>(), pages_out_[p],
+                            ps.h_wpage[p].get<uint32_t>(), ps.h_front.get<uint32_t>(), nodes[p],
+                            ps.h_wtail[p].get<uint32_t>());
           VPstat& v = ps.handle.vp[p];
           v.nodenum = nodes[p];
           v.datanum = used;
@@ -290,8 +308,9 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
         const uint64_t edges_bound = v.shared ? v.datanum - 3ull * v.nodenum
                                               : (uint64_t)pages_out_[e].edgenum;
         if (relax) {
-          if (v.shared) throw Error(SCAPH_STATUS_INVALID, "relax rounds copy pages whole");
-          H_ERR(scaph_relax_page(buf, pages_out_[e].nodenum, pages_out_[e].left,
+          // a compacted page holds only active Nodes, each naming its vertex
+          H_ERR(scaph_relax_page(buf, v.shared ? v.nodenum : pages_out_[e].nodenum,
+                                 v.shared ? SCAPH_PAGE_COMPACTED : pages_out_[e].left,
                                  front_[fa].get<uint32_t>(),
                                  n_, relax->weights, relax->weights ? ps.pool.get<uint32_t>()
                                                                     : nullptr,
@@ -382,7 +401,12 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
         const uint64_t w = v.shared ? v.datanum : page_used_words(pages_out_[s], false);
         ps.copy_beg->record(ps.copy->get());
         H_ERR(scaph_copy_h2d_async(dst, srcp, w * 4, ps.copy->get()));
-        if (relax && relax->weights && !v.shared && pages_out_[s].edgenum) {
+        if (weighted && v.shared) {
+          // the compacted weights at the compacted page's offsets
+          H_ERR(scaph_copy_h2d_async(ps.wpool.get<uint32_t>() + (dst - ps.pool.get<uint32_t>()),
+                                     ps.h_wtail[s].get(), w * 4, ps.copy->get()));
+          rs->bytes_h2d += w * 4;
+        } else if (weighted && pages_out_[s].edgenum) {
           // the page's weights into the same slot of the weight pool
           const uint64_t off = (uint64_t)v.dpid * P_ + 3ull * pages_out_[s].nodenum;
           H_ERR(scaph_copy_h2d_async(ps.wpool.get<uint32_t>() + off, ps.h_wpage[s].get(),

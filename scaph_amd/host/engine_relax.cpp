@@ -88,7 +88,8 @@ RunStats Engine::relax(int mode, uint32_t src) {
 
 // Paged rounds: the active pages of each round go through the same page
 // scheduler as a paged BFS level (cached pages first, the others copied
-// whole into free slots, with their weights for SSSP), each page relaxed by
+// whole, or compacted to their active Nodes when at most -threshold of a
+// page is active, with their weights for SSSP), each page relaxed by
 // scaph_relax_page on an exec stream.
 RunStats Engine::relax_paged(int mode, uint32_t src) {
   PagedState& ps = *paged_;
@@ -172,7 +173,6 @@ RunStats Engine::relax_paged(int mode, uint32_t src) {
     fa ^= 1;
     ++round;
   }
-  ps.handle.set_threshold(ps.thres);
   rs.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   rs.depth = round;
   rs.n_reached = n_;

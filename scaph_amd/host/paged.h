@@ -38,6 +38,7 @@ struct Engine::PagedState {
   // pool's slots (a page's weights sit at its slot's edge offsets)
   std::vector<DevBuf> relax_ws;
   std::vector<PinnedBuf> h_wpage;
+  std::vector<PinnedBuf> h_wtail;  // compacted weights of a cold page (<= thres words)
   DevBuf wpool;
 };
 

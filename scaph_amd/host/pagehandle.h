@@ -63,8 +63,6 @@ class PageHandle {
   // group_cached, the cached active pages wait in group_queue() for grouped
   // launches instead of joining the rank-0 queue.
   void queue_reset(const uint32_t* datas, bool group_cached = false);
-  // compaction threshold (words) for the following queue_reset calls
-  void set_threshold(uint64_t thres) { thres_ = thres; }
   std::deque<int>& group_queue() { return groupq_; }
   // Moves the front of group_queue() into execution at rank 0.
   int take_group();

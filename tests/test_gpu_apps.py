@@ -170,11 +170,13 @@ def test_apps_then_bfs_and_errors():
 
 
 @pytest.mark.parametrize("flags", [["-num_stream=1"], ["-num_stream=3"],
-                                   ["-num_stream=3", "-multi=2"], ["-num_stream=5", "-multi=3"]])
+                                   ["-num_stream=3", "-multi=2"], ["-num_stream=5", "-multi=3"],
+                                   ["-num_stream=2", "-threshold=16"], ["-threshold=0"]])
 def test_apps_paged(flags):
     """WCC and SSSP out of core: a pool of a few page slots, pages copied
-    whole with their weights each round, on one or several exec streams, and
-    a BFS run in between (which replaces slots without weights)."""
+    whole or compacted to their active Nodes (with their weights), on one or
+    several exec streams, and a BFS run in between (which replaces slots
+    without weights)."""
     off, ngh = O.rmat_csr(16, 16, seed=8)
     rng = np.random.default_rng(4)
     wgh = rng.integers(1, 200, len(ngh)).astype(np.uint32)
@@ -193,8 +195,12 @@ def test_apps_paged(flags):
         np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, srcs[0]))
         e.run(srcs[1])
         np.testing.assert_array_equal(e.levels(), O.bfs(off, ngh, srcs[1]))
-        e.sssp(srcs[1])
+        st2 = e.sssp(srcs[1])
         np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, srcs[1]))
+        if "-threshold=0" in flags:
+            assert st["pages_compacted"] == 0 and st2["pages_compacted"] == 0
+        elif "-threshold=16" in flags:
+            assert st["pages_compacted"] + st2["pages_compacted"] > 0
 
 
 @pytest.mark.parametrize("tool,extra", [("wcc", []), ("sssp", ["-src=7"]),
