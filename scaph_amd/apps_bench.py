@@ -45,6 +45,7 @@ def main():
     ap.add_argument("--budgets", default="640", help="-max_memory values in MiB")
     ap.add_argument("--sources", type=int, default=2)
     ap.add_argument("--repeat", type=int, default=2)
+    ap.add_argument("--flags", default="", help="more engine flags for the paged runs")
     args = ap.parse_args()
 
     res = BFS("-max_memory=262144")
@@ -64,7 +65,7 @@ def main():
     log("graph", n, m, "sources", srcs)
     out = {"workload": f"R-MAT scale {args.scale} edge factor {args.ef} seed {args.seed}; SSSP "
                        f"weights uniform on [1, {args.wmax}]",
-           "vertices": n, "edges": m, "sources": srcs}
+           "vertices": n, "edges": m, "sources": srcs, "paged_flags": args.flags}
     want_wcc = None
     want_sssp = {}
     runs = [res.wcc() for _ in range(args.repeat)]
@@ -78,7 +79,7 @@ def main():
     log("resident", out["resident"])
     res.close()
     for b in [int(x) for x in args.budgets.split(",") if x]:
-        eng = BFS(f"-max_memory={b}", "-build_pull=false")
+        eng = BFS(f"-max_memory={b}", "-build_pull=false", *args.flags.split())
         eng.load_csr(off, ngh, keep_host=False)
         eng.set_weights(wgh)
         info = eng.info()
