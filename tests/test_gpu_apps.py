@@ -217,3 +217,30 @@ def test_app_tools_check_passes(tool, extra):
                           *extra], capture_output=True, text=True, timeout=100)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "check passed!" in out.stdout and f"{tool} cost" in out.stdout
+
+
+def test_apps_edge_cases():
+    """Single vertex, no edges, self-loops only, parallel edges with different
+    weights, zero weights, an isolated source, and weights near 2^24."""
+    cases = []
+    cases.append(("single", 1, [], [], []))
+    cases.append(("no edges", 50, [], [], []))
+    cases.append(("self loops", 20, list(range(20)), list(range(20)), [5] * 20))
+    cases.append(("parallel", 4, [0, 0, 0, 1, 1, 2], [1, 1, 1, 2, 2, 3], [9, 3, 7, 4, 1, 0]))
+    rng = np.random.default_rng(31)
+    s = rng.integers(0, 300, 2000)
+    d = rng.integers(0, 300, 2000)
+    cases.append(("big weights", 300, s, d, rng.integers(1 << 23, 1 << 24, 2000)))
+    cases.append(("zero weights", 300, s, d, np.zeros(2000, np.int64)))
+    for name, n, src, dst, w in cases:
+        say("edge case", name)
+        off, ngh, wgh = weighted_csr(n, src, dst, w)
+        with engine() as e:
+            e.load_csr(off, ngh)
+            e.wcc()
+            np.testing.assert_array_equal(e.levels(), O.wcc(off, ngh), err_msg=name)
+            e.set_weights(wgh)
+            for source in sorted({0, n - 1}):
+                e.sssp(source)
+                np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, source),
+                                              err_msg=f"{name} {source}")
