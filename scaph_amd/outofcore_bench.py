@@ -3,7 +3,13 @@
 10^6; uniform destinations; seed 3), device page pool capped by
 -max_memory, pages streamed from pinned host memory.
 
-    python -m scaph_amd.outofcore_bench [--budgets 15258,8192] [--sources 4]
+    python -m scaph_amd.outofcore_bench [--budgets 15258,8192] [--pools 16e9] [--sources 4]
+
+--budgets are -max_memory values: the whole device budget, from which the
+engine (as src/app.cuh:281-327) first takes the per-vertex and per-page
+arrays and gives the rest to the page pool.  --pools instead names the page
+pool's own size in bytes (the "16 GB page pool" reading): the -max_memory
+that leaves exactly that pool is derived from the vertex and page counts.
 
 Prints one JSON line: per budget the median GTEPS, page hit rate
 (executions of already-resident pages / all page executions), host-to-device
@@ -22,6 +28,7 @@ import numpy as np
 from . import BFS
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PAGE_WORDS = 10 << 20  # the default -pagesize (10 Mi u32 words)
 
 
 def log(*a):
@@ -37,6 +44,8 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--budgets", default="15258,8192",
                     help="-max_memory values in MiB (15258 MiB = 16e9 bytes)")
+    ap.add_argument("--pools", default="",
+                    help="page pool sizes in bytes (e.g. 16e9); -max_memory is derived")
     ap.add_argument("--sources", type=int, default=4)
     ap.add_argument("--threshold", type=int, default=16,
                     help="-threshold of the paged runs in 1/32 pages: 16 compacts pages whose "
@@ -83,7 +92,13 @@ def main():
                want[s][1]["m_reached"] / (want[s][1]["ms"] / 1e3) / 1e9 for s in sources),
            "resident_ms_median": statistics.median(res_ms),
            "levels_resident_vs_oracle": oracle_ok, "budgets": []}
-    for b in [int(x) for x in args.budgets.split(",") if x]:
+    slot = PAGE_WORDS * 4
+    budgets = [(int(x), None) for x in args.budgets.split(",") if x]
+    for x in [float(x) for x in args.pools.split(",") if x]:
+        # plan_residency: pool = (budget - 9 n - 24 pages) / slot, whole slots
+        need = int(x) // slot * slot + 9 * info["n"] + 24 * info["pages"]
+        budgets.append(((need + (1 << 20) - 1) >> 20, int(x)))
+    for b, pool_bytes in budgets:
         paged_flags = [f"-max_memory={b}", "-build_pull=false", f"-threshold={args.threshold}"]
         paged_flags += args.flags.split()
         eng = BFS(*paged_flags)
@@ -106,8 +121,11 @@ def main():
         h2d = sum(r["bytes_h2d"] for r in runs)
         h2d_ms = sum(r["h2d_ms"] for r in runs)
         tot_ms = sum(r["ms"] for r in runs)
+        slots = ((b << 20) - 9 * inf["n"] - 24 * inf["pages"]) // slot
         out["budgets"].append({
-            "max_memory_mib": b, "flags": paged_flags, "resident": inf["resident"],
+            "max_memory_mib": b, "pool_bytes_requested": pool_bytes,
+            "pool_slots": min(slots, inf["pages"]), "pool_bytes": min(slots, inf["pages"]) * slot,
+            "flags": paged_flags, "resident": inf["resident"],
             "pages": inf["pages"],
             "gteps_median": statistics.median(r["m_reached"] / (r["ms"] / 1e3) / 1e9
                                               for r in runs),
