@@ -328,8 +328,7 @@ def main():
     model_path = os.path.join(ROOT, "scaph_amd", "schedule_mlp.bin")
     if args.learned and os.path.exists(model_path):
         from scaph_amd import learn as L
-        p, mean, scale, _ = L.load_model(model_path)
-        eng.set_model(p, mean, scale)
+        L.model_for_graph(eng, model_path)
         fixed_levels = {}
         for src in srcs[:args.learned_check]:
             eng.set_schedule("fixed")

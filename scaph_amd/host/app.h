@@ -64,7 +64,8 @@ class App {
         throw Error(SCAPH_STATUS_INVALID, "-schedule=learned needs -model=<file>");
       LearnedSchedule ls;
       ls.load_file(cfg.model);
-      engine->set_device_model(ls.params(), ls.mean(), ls.scale());
+      const auto mean = ls.mean_for(engine->num_vertices(), engine->num_edges());
+      engine->set_device_model(ls.params(), mean.data(), ls.scale());
     }
     vert_count = engine->num_vertices();
     edge_count = (index_t)engine->num_edges();

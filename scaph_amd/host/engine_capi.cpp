@@ -59,6 +59,14 @@ int guarded(char* err, size_t len, F f) {
 // The learned schedule decides on the device (scaph_mlp_decide inside the
 // level loop); the host-side model only answers the first level and serves
 // the CPU-side checks.
+// A model read from a file whose features are shares of the graph takes
+// the loaded graph's offsets; called after every graph load.
+void bind_model(scaph_engine* e) {
+  if (!e->learned.ready() || e->cfg.model.empty()) return;
+  e->learned.set_graph(e->eng->num_vertices(), e->eng->num_edges());
+  e->eng->set_device_model(e->learned.params(), e->learned.graph_mean(), e->learned.scale());
+}
+
 void apply_schedule(scaph_engine* e) {
   if (e->cfg.schedule == "learned") {
     scaph::LearnedSchedule* ls = &e->learned;
@@ -105,7 +113,10 @@ void scaph_engine_destroy(scaph_engine* e) { delete e; }
 
 int scaph_engine_load(scaph_engine* e, char* err, size_t errlen) {
   if (!e) return 1;
-  return guarded(err, errlen, [&] { e->eng->load_from_config(); });
+  return guarded(err, errlen, [&] {
+    e->eng->load_from_config();
+    bind_model(e);
+  });
 }
 
 int scaph_engine_load_csr(scaph_engine* e, uint32_t n, const uint64_t* offsets, const uint32_t* ngh,
@@ -123,6 +134,7 @@ int scaph_engine_load_csr(scaph_engine* e, uint32_t n, const uint64_t* offsets, 
     for (uint32_t v = 0; v < n; ++v)
       if (g.offsets[v] > g.offsets[(size_t)v + 1]) throw scaph::Error(1, "offsets decrease");
     e->eng->load_csr(std::move(g), keep_host != 0);
+    bind_model(e);
   });
 }
 
@@ -139,19 +151,26 @@ int scaph_engine_gen_rmat_device(scaph_engine* e, uint32_t scale, uint32_t edge_
     p.c = c;
     p.seed = seed;
     e->eng->gen_rmat_device(p, keep_host != 0);
+    bind_model(e);
   });
 }
 
 int scaph_engine_gen_er_device(scaph_engine* e, uint32_t n, uint64_t m, uint64_t seed,
                                int keep_host, char* err, size_t errlen) {
   if (!e) return 1;
-  return guarded(err, errlen, [&] { e->eng->gen_er_device(n, m, seed, keep_host != 0); });
+  return guarded(err, errlen, [&] {
+    e->eng->gen_er_device(n, m, seed, keep_host != 0);
+    bind_model(e);
+  });
 }
 
 int scaph_engine_gen_zipf(scaph_engine* e, uint32_t n, double x_min, double alpha, uint32_t cap,
                           uint64_t seed, char* err, size_t errlen) {
   if (!e) return 1;
-  return guarded(err, errlen, [&] { e->eng->gen_zipf(n, x_min, alpha, cap, seed); });
+  return guarded(err, errlen, [&] {
+    e->eng->gen_zipf(n, x_min, alpha, cap, seed);
+    bind_model(e);
+  });
 }
 
 int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_pages, int* resident,

@@ -42,11 +42,14 @@ void LearnedSchedule::set(const uint16_t* params, const float* mean, const float
     scale_[i] = scale[i] == 0.f ? 1.f : scale[i];
   }
   prefetch_gain_ = prefetch_gain;
+  flags_ = 0;  // the caller's offsets are used as given
+  eff_ = mean_;
   ready_ = true;
 }
 
 void LearnedSchedule::load_file(const std::string& path) {
-  // raw little-endian: kMlpParams u16, 16 f32 mean, 16 f32 scale, 1 f32 gain
+  // raw little-endian: kMlpParams u16, 16 f32 mean, 16 f32 scale, 1 f32
+  // gain, then (newer files) 1 u32 of flags
   std::ifstream in(path, std::ios::binary);
   if (!in) throw Error(1, "cannot open model file " + path);
   std::vector<uint16_t> p(kMlpParams);
@@ -57,7 +60,20 @@ void LearnedSchedule::load_file(const std::string& path) {
   in.read(reinterpret_cast<char*>(s.data()), sizeof(s));
   in.read(reinterpret_cast<char*>(&g), 4);
   if (!in) throw Error(1, "model file " + path + " is truncated");
+  uint32_t flags = 0;
+  if (!in.read(reinterpret_cast<char*>(&flags), 4)) flags = 0;
   set(p.data(), m.data(), s.data(), g);
+  flags_ = flags;
+}
+
+std::array<float, kMlpIn> LearnedSchedule::mean_for(uint64_t n, uint64_t m) const {
+  std::array<float, kMlpIn> out = mean_;
+  if (flags_ & 1u) {
+    out[0] += (float)std::log2((double)std::max<uint64_t>(n, 1));
+    out[1] += (float)std::log2((double)std::max<uint64_t>(m, 1));
+    out[2] += (float)std::log2((double)std::max<uint64_t>(m, 1));
+  }
+  return out;
 }
 
 void LearnedSchedule::forward(const float* x, float* out) const {
@@ -88,7 +104,7 @@ void LearnedSchedule::forward(const float* x, float* out) const {
 Dir LearnedSchedule::decide(const Engine& eng, const SchedState& s) const {
   if (!ready_ || !eng.has_pull()) return eng.beamer(s);
   std::array<float, kMlpIn> x = level_features(s, s.last);
-  for (int i = 0; i < kMlpIn; ++i) x[i] = (x[i] - mean_[i]) / scale_[i];
+  for (int i = 0; i < kMlpIn; ++i) x[i] = (x[i] - eff_[i]) / scale_[i];
   float y[kMlpOut];
   forward(x.data(), y);
   return y[0] > 0.f ? Dir::kPull : Dir::kPush;

@@ -38,13 +38,20 @@ class LearnedSchedule {
   Dir decide(const Engine& eng, const SchedState& s) const;
   const uint16_t* params() const { return p_.data(); }
   const float* mean() const { return mean_.data(); }
+  // the offsets for a graph of n vertices and m edges: mean(), plus log2 n,
+  // log2 m, log2 m on features 0..2 when the model reads shares (flag bit 0)
+  std::array<float, kMlpIn> mean_for(uint64_t n, uint64_t m) const;
+  // binds the offsets of the loaded graph (decide() and graph_mean() use them)
+  void set_graph(uint64_t n, uint64_t m) { eff_ = mean_for(n, m); }
+  const float* graph_mean() const { return eff_.data(); }
   const float* scale() const { return scale_.data(); }
 
  private:
   bool ready_ = false;
   std::vector<uint16_t> p_;
-  std::array<float, kMlpIn> mean_{}, scale_{};
+  std::array<float, kMlpIn> mean_{}, scale_{}, eff_{};
   float prefetch_gain_ = 0.f;
+  uint32_t flags_ = 0;
 };
 
 }  // namespace scaph

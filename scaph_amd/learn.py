@@ -81,9 +81,11 @@ def collect(specs, sources=2, reps=2):
     t0 = time.time()
     for gi, spec in enumerate(specs):
         build(eng, spec)
+        inf = eng.info()
         for src in pick_sources(eng, sources, seed=spec["seed"]):
             for r in eng.explore(src, reps):
                 r["graph"] = gi
+                r["n"], r["m"] = inf["n"], inf["m"]
                 recs.append(r)
         if gi % 10 == 0:
             log(f"traces: graph {gi}/{len(specs)} ({spec['kind']}), {len(recs)} records, "
@@ -150,6 +152,32 @@ def bf16_to_f32(h):
 
 
 SCALAR_FEATURES = 4  # log2(1+n_f), log2(1+m_f), log2(1+m_u), level
+RELATIVE = 1  # model flag: features 0..2 are taken relative to log2 n, log2 m, log2 m
+
+
+def relative_features(X, n, m):
+    """Features 0..2 as log2 of the frontier's share of the graph: the
+    device kernels compute log2(1+n_f), log2(1+m_f), log2(1+m_u); a model
+    trained on these minus log2 n, log2 m, log2 m compares shares, as the
+    alpha/beta rule does, and so carries across graph sizes."""
+    X = np.array(X, np.float32, copy=True)
+    X[:, 0] -= np.log2(np.asarray(n, np.float64)).astype(np.float32)
+    X[:, 1] -= np.log2(np.asarray(m, np.float64)).astype(np.float32)
+    X[:, 2] -= np.log2(np.asarray(m, np.float64)).astype(np.float32)
+    return X
+
+
+def graph_mean(mean, flags, n, m):
+    """The feature offsets the device uses for a graph of n vertices and m
+    edges: (x - offset) / scale with offset = mean, plus log2 n / log2 m on
+    features 0..2 for a relative model (normalisation is affine, so the
+    kernels need no change)."""
+    mean = np.array(mean, np.float32, copy=True)
+    if flags & RELATIVE:
+        mean[0] += np.float32(np.log2(max(n, 1)))
+        mean[1] += np.float32(np.log2(max(m, 1)))
+        mean[2] += np.float32(np.log2(max(m, 1)))
+    return mean
 
 
 def init_params(seed=0, scalar=False):
@@ -231,14 +259,17 @@ def train(X, T, steps=4000, lr=0.05, mu=0.9, w1=1.0, seed=0, scalar=False):
 
 # -------------------------------------------------------------- model file
 
-def save_model(path, params_u16, mean, scale, gain=0.0):
+def save_model(path, params_u16, mean, scale, gain=0.0, flags=0):
     """Raw little-endian: NPARAM u16 parameters, 16 f32 means, 16 f32
-    scales, one f32 prefetch gain (the layout LearnedSchedule::load_file reads)."""
+    scales, one f32 prefetch gain, one u32 of flags (RELATIVE; files
+    without it have absolute features) -- the layout
+    LearnedSchedule::load_file reads."""
     with open(path, "wb") as f:
         f.write(np.asarray(params_u16, np.uint16).tobytes())
         f.write(np.asarray(mean, np.float32).tobytes())
         f.write(np.asarray(scale, np.float32).tobytes())
         f.write(struct.pack("<f", float(gain)))
+        f.write(struct.pack("<I", int(flags)))
 
 
 def load_model(path=DEFAULT_MODEL):
@@ -249,6 +280,19 @@ def load_model(path=DEFAULT_MODEL):
     scale = np.frombuffer(raw[o + 64:o + 128], np.float32)
     gain = struct.unpack("<f", raw[o + 128:o + 132])[0]
     return p.copy(), mean.copy(), scale.copy(), gain
+
+
+def load_model_flags(path=DEFAULT_MODEL):
+    raw = open(path, "rb").read()
+    o = NPARAM * 2 + 132
+    return struct.unpack("<I", raw[o:o + 4])[0] if len(raw) >= o + 4 else 0
+
+
+def model_for_graph(eng, path=DEFAULT_MODEL):
+    """Loads the model into engine `eng` with the offsets of its loaded graph."""
+    p, mean, scale, _ = load_model(path)
+    inf = eng.info()
+    eng.set_model(p, graph_mean(mean, load_model_flags(path), inf["n"], inf["m"]), scale)
 
 
 # ------------------------------------------------------------- evaluation
@@ -281,6 +325,8 @@ def main():
     ap.add_argument("--out", default=DEFAULT_MODEL)
     ap.add_argument("--traces", default="", help="also save the training traces here (.npz)")
     ap.add_argument("--load-traces", default="", help="train on saved traces instead of new ones")
+    ap.add_argument("--absolute", action="store_true",
+                    help="train on absolute log sizes (the older models) instead of shares")
     ap.add_argument("--histogram", action="store_true",
                     help="let the model read the 12 page-histogram features too (the decision "
                          "then needs a frontier pass per level)")
@@ -289,16 +335,35 @@ def main():
     if args.load_traces:
         z = np.load(args.load_traces, allow_pickle=False)
         X, T = z["X"], z["T"]
+        if "n" in z:
+            gn, gm = z["n"], z["m"]
+        else:
+            # traces saved without the sizes: n from the (seeded) specs, m
+            # from each graph's first record, m_u + m_f of the source
+            specs = training_specs(args.graphs)
+            g = z["graph"]
+            gn = np.array([2 ** specs[i]["scale"] if specs[i]["kind"] == "rmat"
+                           else specs[i]["n"] for i in g], np.float64)
+            first = {}
+            for k, i in enumerate(g):
+                first.setdefault(int(i), k)
+            mm = {i: (2.0 ** X[k, 2] - 1) + (2.0 ** X[k, 1] - 1) for i, k in first.items()}
+            gm = np.array([mm[int(i)] for i in g], np.float64)
         recs = None
     else:
         specs = training_specs(args.graphs)
         recs = collect(specs, args.sources)
         X, T = features_labels(recs)
+        gn = np.array([r["n"] for r in recs], np.float64)
+        gm = np.array([r["m"] for r in recs], np.float64)
     if args.traces and recs is not None:
-        np.savez_compressed(args.traces, X=X, T=T,
+        np.savez_compressed(args.traces, X=X, T=T, n=gn, m=gm,
                             t_push=np.array([r["t_push_ms"] for r in recs], np.float32),
                             t_pull=np.array([r["t_pull_ms"] for r in recs], np.float32),
                             graph=np.array([r["graph"] for r in recs], np.int32))
+    flags = 0 if args.absolute else RELATIVE
+    if flags & RELATIVE:
+        X = relative_features(X, gn, gm)
     log(f"{len(X)} training records; pull faster in {(T[:, 0] > 0.5).mean() * 100:.1f}%")
     mlp, mean, scale, losses = train(X, T, steps=args.steps, scalar=scalar)
     log("loss", losses)
@@ -308,15 +373,14 @@ def main():
     pred = mlp.predict(((Xs - mean) / scale).astype(np.float32))
     acc = ((pred[:, 0] > 0) == (T[:, 0] > 0.5)).mean()
     log(f"training accuracy of the direction choice: {acc * 100:.1f}%")
-    save_model(args.out, mlp.params_bf16(), mean, scale)
+    save_model(args.out, mlp.params_bf16(), mean, scale, flags=flags)
     # held-out check on fresh graphs (not in the training set)
     held = training_specs(6, seed=99)
     eng = BFS("-max_memory=262144")
-    p, m, s, _ = load_model(args.out)
-    eng.set_model(p, m, s)
     summary = []
     for spec in held:
         build(eng, spec)
+        model_for_graph(eng, args.out)
         res = compare(eng, pick_sources(eng, 3, seed=spec["seed"]))
         f = sum(r["fixed_ms"] for r in res)
         l = sum(r["learned_ms"] for r in res)

@@ -53,6 +53,9 @@ def main():
                          "8; app/bfs.cu forces 3)")
     ap.add_argument("--flags", default="", help="more engine flags for the paged runs")
     ap.add_argument("--oracle", action="store_true", help="also run the CPU oracle BFS once")
+    ap.add_argument("--schedules", action="store_true",
+                    help="also run the resident graph with the pull step: the fixed alpha/beta "
+                         "rule against the learned schedule (committed model) on the sources")
     args = ap.parse_args()
 
     # resident reference run first: builds the graph, picks the sources
@@ -84,6 +87,27 @@ def main():
         del off, ngh
     res_ms = [want[s][1]["ms"] for s in sources]
     res.close()
+    schedules = None
+    if args.schedules:
+        from . import learn as L
+        eng = BFS("-max_memory=262144")
+        eng.gen_zipf(args.n, args.x_min, args.alpha, args.cap, args.seed)
+        L.model_for_graph(eng, L.DEFAULT_MODEL)
+        cmp_ = L.compare(eng, sources, repeats=3)
+        exact = True
+        eng.set_schedule("fixed")
+        for s in sources:
+            eng.run(s)
+            exact &= bool(np.array_equal(eng.levels(), want[s][0]))
+        fx = statistics.median(c["fixed_ms"] for c in cmp_)
+        ln = statistics.median(c["learned_ms"] for c in cmp_)
+        schedules = {"per_source": cmp_, "fixed_ms_median": fx, "learned_ms_median": ln,
+                     "learned_speedup_vs_fixed": fx / ln,
+                     "push_only_ms_median": statistics.median(res_ms),
+                     "fixed_speedup_vs_push_only": statistics.median(res_ms) / fx,
+                     "levels_equal_push_only": exact, "model": "scaph_amd/schedule_mlp.bin"}
+        log("schedules", schedules)
+        eng.close()
 
     out = {"workload": f"power-law digraph n={args.n} x_min={args.x_min} alpha={args.alpha} "
                        f"cap={args.cap} seed={args.seed}",
@@ -91,7 +115,7 @@ def main():
            "resident_gteps_median": statistics.median(
                want[s][1]["m_reached"] / (want[s][1]["ms"] / 1e3) / 1e9 for s in sources),
            "resident_ms_median": statistics.median(res_ms),
-           "levels_resident_vs_oracle": oracle_ok, "budgets": []}
+           "levels_resident_vs_oracle": oracle_ok, "schedules": schedules, "budgets": []}
     slot = PAGE_WORDS * 4
     budgets = [(int(x), None) for x in args.budgets.split(",") if x]
     for x in [float(x) for x in args.pools.split(",") if x]:

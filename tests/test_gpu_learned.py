@@ -180,11 +180,12 @@ def test_trained_model_on_held_out_graph():
     step times of both directions (explore) price its choices against the
     per-level best and against the fixed alpha/beta rule; the learned
     schedule's levels equal the fixed schedule's."""
-    p, mean, scale, _ = L.load_model()
+    p, mean0, scale, _ = L.load_model()
     pf = L.bf16_to_f32(p)
     with scaph_amd.BFS("-max_memory=65536") as e:
         e.gen_rmat_device(19, 16, 0.6, 0.16, 0.16, seed=777)
         n = e.info()["n"]
+        mean = L.graph_mean(mean0, L.load_model_flags(), n, e.info()["m"])
         srcs = L.pick_sources(e, 2, seed=777)
         t_model = t_best = t_fixed = 0.0
         for s in srcs:

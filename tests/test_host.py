@@ -202,6 +202,11 @@ def test_committed_schedule_model_loads():
     assert np.all(np.isfinite(w)) and np.all(scale > 0) and np.isfinite(gain)
     # bf16 round trip of the stored parameters is exact
     np.testing.assert_array_equal(L.f32_to_bf16(w), p)
+    # the committed model reads the frontier's shares of the graph
+    assert L.load_model_flags() & L.RELATIVE
+    m = L.graph_mean(mean, L.RELATIVE, 1 << 20, 1 << 24)
+    np.testing.assert_allclose(m[:3] - mean[:3], [20, 24, 24], rtol=1e-6)
+    np.testing.assert_array_equal(m[3:], mean[3:])
 
 
 def test_graphgen_weights_and_app_tools():
