@@ -316,8 +316,14 @@ def main():
         # passes over this same command (separate runs; calibrated by a copy)
         with open(tfile) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
+    # the profiled bytes over this run's launch time: how hard the kernel
+    # drives the memory side, beside the algorithmic figure above (counters
+    # include Infinity Cache hits, so this bounds HBM bandwidth from above)
+    t_gbps = traffic / (avg_ms / 1e3) / 1e9 if traffic and avg_ms > 0 else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                 "frac": achieved / 8000.0, "traffic": traffic,
+                "traffic_achieved": t_gbps,
+                "traffic_frac": t_gbps / 8000.0 if t_gbps else None,
                 "kernel": "k_pull" if dom == 1 else "push (k_prep_queue+k_scan_blocks+k_mark_tiles+k_expand)",
                 "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_bytes, "launches": launches,
                 "share_of_kernel_time": ms_tot / max(agg[0][0] + agg[1][0], 1e-9)}
