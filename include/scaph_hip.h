@@ -301,7 +301,9 @@ int scaph_bfs_push_runs(const scaph_page_run* runs, uint32_t num_runs, uint32_t 
  * every vertex active, dst = min(dst, src); app/sssp.cu:13-31 with
  * app/sssp_kernel.cuh: value = 0x7fffffff except value[src] = 0,
  * dst = min(dst, src + w)).  The fixpoint is the minimum over all paths,
- * independent of the update order. */
+ * independent of the update order.  Sums are u32 as in the reference's
+ * value_t: shortest distances must stay below 0x7FFFFFFF (MAXPATH, which
+ * marks unreached vertices) and value + weight below 2^32. */
 #define SCAPH_RELAX_WCC 0
 #define SCAPH_RELAX_SSSP 1
 #define SCAPH_RELAX_INF 0x7FFFFFFFu /* unreached distance (MAXPATH, app/sssp.cu:11) */
