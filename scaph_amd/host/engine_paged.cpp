@@ -21,6 +21,24 @@ namespace {
 
 size_t bitmap_words(uint32_t n) { return (((size_t)n + 63) / 64) * 2; }
 
+// Calls f(i) for each Node i of page pg (vertex pg.left + i) whose frontier
+// bit is set, in vertex order, skipping zero bitmap words.
+template <typename F>
+void for_active(const scaph_page& pg, const uint32_t* front, F f) {
+  if (pg.nodenum == 0) return;
+  const uint64_t first = pg.left, last = (uint64_t)pg.left + pg.nodenum - 1;
+  for (uint64_t w = first >> 5; w <= last >> 5; ++w) {
+    uint32_t bits = front[w];
+    if (w == first >> 5) bits &= ~0u << (first & 31);
+    if (w == last >> 5 && (last & 31) != 31) bits &= (2u << (last & 31)) - 1u;
+    while (bits) {
+      const uint32_t b = (uint32_t)__builtin_ctz(bits);
+      bits &= bits - 1;
+      f((uint32_t)(w * 32 + b - first));
+    }
+  }
+}
+
 // Active Node records of one page plus their adjacency, re-indexed: the
 // layout compress_stat writes (src/compress.h:18-40).  Returns words used.
 uint64_t compact_page(const uint32_t* page, const scaph_page& pg, const uint32_t* front,
@@ -29,9 +47,8 @@ uint64_t compact_page(const uint32_t* page, const scaph_page& pg, const uint32_t
   uint32_t* eout = out + 3ull * active_nodes;
   uint64_t off = 0;
   uint32_t cnt = 0;
-  for (uint32_t i = 0; i < pg.nodenum; ++i) {
+  for_active(pg, front, [&](uint32_t i) {
     const uint32_t v = pg.left + i;
-    if (!((front[v >> 5] >> (v & 31)) & 1u)) continue;
     const uint32_t idx = page[3ull * i + 1], len = page[3ull * i + 2];
     if (cnt < active_nodes) {
       out[3ull * cnt] = v;
@@ -41,7 +58,7 @@ uint64_t compact_page(const uint32_t* page, const scaph_page& pg, const uint32_t
     }
     off += len;
     ++cnt;
-  }
+  });
   if (cnt != active_nodes) throw Error(SCAPH_STATUS_INVALID, "compaction count mismatch");
   return 3ull * active_nodes + off;
 }
@@ -52,13 +69,11 @@ void compact_weights(const uint32_t* page, const scaph_page& pg, const uint32_t*
                      const uint32_t* front, uint32_t active_nodes, uint32_t* out) {
   uint32_t* wout = out + 3ull * active_nodes;
   uint64_t off = 0;
-  for (uint32_t i = 0; i < pg.nodenum; ++i) {
-    const uint32_t v = pg.left + i;
-    if (!((front[v >> 5] >> (v & 31)) & 1u)) continue;
+  for_active(pg, front, [&](uint32_t i) {
     const uint32_t idx = page[3ull * i + 1], len = page[3ull * i + 2];
     std::memcpy(wout + off, wpage + idx, (size_t)len * 4);
     off += len;
-  }
+  });
 }
 
 // The compaction workers of one level, joined on every exit from it: an
@@ -234,7 +249,18 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   for (size_t p = 0; p < np; ++p)
     if (!ps.handle.vp[p].cached && datas[p] > 0 && datas[p] <= thres) cold.push_back((int)p);
   if (!cold.empty()) {
-    H_ERR(scaph_copy_d2h_async(ps.h_front.get(), front_[fa].get(), words * 4, st));
+    if (cold.size() * 4 > np) {
+      H_ERR(scaph_copy_d2h_async(ps.h_front.get(), front_[fa].get(), words * 4, st));
+    } else {
+      // few cold pages: only the frontier words of their vertex ranges (a
+      // small level otherwise pays for the whole n/8-byte bitmap)
+      for (int p : cold) {
+        const uint64_t w0 = pages_out_[p].left / 32;
+        const uint64_t w1 = ((uint64_t)pages_out_[p].left + pages_out_[p].nodenum + 31) / 32;
+        H_ERR(scaph_copy_d2h_async(ps.h_front.get<uint32_t>() + w0,
+                                   front_[fa].get<uint32_t>() + w0, (w1 - w0) * 4, st));
+      }
+    }
     main_->sync();
   }
   ps.resident_at_reset.assign(np, 0);
