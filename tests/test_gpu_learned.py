@@ -2,6 +2,7 @@
 device decision against the host restatement, and levels that stay
 bit-exact whatever the model decides."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -268,3 +269,33 @@ def test_scalar_model_decision_fused_into_hand_off():
             np.testing.assert_allclose(rec[:2], y[0], rtol=1e-4, atol=1e-4)
     finally:
         hip.scaph_host_pinned_free(hp)
+
+
+def test_model_flag_binds_graph_offsets():
+    """-model=<file> (the command-line path) and learn.model_for_graph (the
+    Python path) give the same per-level directions on the same graph: the
+    engine folds the loaded graph's log2 n / log2 m into the offsets of a
+    model whose features are shares.  bin/bfs with the model passes -check."""
+    import subprocess
+    with scaph_amd.BFS("-max_memory=65536", "-schedule=learned",
+                       f"-model={L.DEFAULT_MODEL}") as a, \
+            scaph_amd.BFS("-max_memory=65536") as b:
+        for e in (a, b):
+            e.gen_rmat_device(18, 16, seed=41)
+        L.model_for_graph(b)
+        b.set_schedule("learned")
+        srcs = L.pick_sources(b, 3, seed=41)
+        for s in srcs:
+            say("model flag vs python path", s)
+            a.run(s)
+            da = [r["dir"] for r in a.records()]
+            la = a.levels()
+            b.run(s)
+            assert da == [r["dir"] for r in b.records()]
+            np.testing.assert_array_equal(la, b.levels())
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "scaph_amd",
+                       "bin", "bfs")
+    out = subprocess.run([exe, "-gen=rmat-device:scale=16,ef=16,seed=3", "-schedule=learned",
+                          f"-model={L.DEFAULT_MODEL}", "-rounds=2", "-check", "-src=5"],
+                         capture_output=True, text=True, timeout=100)
+    assert out.returncode == 0 and "check passed!" in out.stdout, out.stdout + out.stderr
