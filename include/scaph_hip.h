@@ -324,6 +324,16 @@ int scaph_relax_round(const scaph_graph_desc* graph, const uint32_t* queue, uint
                       const uint32_t* weights, const uint32_t* pool_base, uint32_t* value,
                       uint32_t* front_out, scaph_level_counters* counters, void* workspace,
                       size_t workspace_bytes, void* stream);
+/* Near/far split of a round's active list (the bucketed rounds of SSSP):
+ * vertices of queue[0..n_queue) whose value is below `bound` are appended
+ * to near_queue (this round's list); the others are set in front_out (they
+ * stay active) and their least value is kept.  split (3 u64, device):
+ * [0] near count, [1] far count, [2] least far value (~0 if none).  The
+ * values reached are the same for any bound: only the number of times a
+ * vertex is relaxed changes (label-correcting rounds, app/sssp.cu). */
+int scaph_relax_split(const uint32_t* queue, uint32_t n_queue, const uint32_t* value,
+                      uint32_t bound, uint32_t n, uint32_t* near_queue, uint32_t* front_out,
+                      unsigned long long* split, void* stream);
 /* The same round over one page buffer in device memory (paged mode; the
  * reference's per-page gpu_push_csr launch, src/gpu_engine.cuh:41-169):
  * the active Nodes of [first_vertex, first_vertex + nodenum) are those set

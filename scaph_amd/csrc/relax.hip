@@ -185,6 +185,55 @@ k_relax_chunks(RelaxWs ws, uint32_t n, long long wdelta, uint32_t* value, uint32
 
 __global__ void k_relax_zero(unsigned long long* p) { *p = 0; }
 
+// Near/far split of a round's active list (SSSP): vertices whose value is
+// below `bound` go to `near` for this round; the others keep their bit in
+// front_out (the next round's active set) and their least value is kept
+// for the next bound.  split[0] = near count, [1] = far count, [2] = least
+// far value (split must start as {0, 0, ~0}).
+__global__ void __launch_bounds__(kBlock)
+k_relax_split(const uint32_t* q, uint32_t n_q, const uint32_t* value, uint32_t bound,
+              uint32_t n, uint32_t* near, uint32_t* front_out, unsigned long long* split) {
+  const uint32_t stride = gridDim.x * kBlock;
+  unsigned long long far_cnt = 0, least = ~0ull;
+  for (uint32_t i0 = blockIdx.x * kBlock; i0 < n_q; i0 += stride) {
+    const uint32_t i = i0 + threadIdx.x;
+    uint32_t u = 0, val = 0;
+    bool valid = i < n_q;
+    if (valid) {
+      u = q[i];
+      valid = u < n;
+    }
+    if (valid) val = value[u];
+    const bool is_near = valid && val < bound;
+    const uint64_t m = __ballot(is_near);
+    unsigned long long base = 0;
+    if (lane_id() == 0 && m) base = atomicAdd(&split[0], (unsigned long long)__popcll(m));
+    base = wave_bcast64(base, 0);
+    if (is_near) near[base + lanes_below(m)] = u;
+    if (valid && !is_near) {
+      atomicOr(&front_out[u >> 5], 1u << (u & 31));
+      ++far_cnt;
+      if (val < least) least = val;
+    }
+  }
+  far_cnt = wave_sum(far_cnt);
+#pragma unroll
+  for (int d = SCAPH_WAVE / 2; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(least, d, SCAPH_WAVE);
+    least = o < least ? o : least;
+  }
+  if (lane_id() == 0) {
+    if (far_cnt) atomicAdd(&split[1], far_cnt);
+    if (least != ~0ull) atomicMin(&split[2], least);
+  }
+}
+
+__global__ void k_relax_split_init(unsigned long long* split) {
+  split[0] = 0;
+  split[1] = 0;
+  split[2] = ~0ull;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_relax_init(uint32_t* value, uint32_t n, uint32_t mode, uint32_t src) {
   const uint32_t stride = gridDim.x * kBlock;
@@ -252,6 +301,21 @@ int scaph_relax_round(const scaph_graph_desc* graph, const uint32_t* queue, uint
     hipLaunchKernelGGL(k_relax_chunks<0>, dim3(2048), dim3(kBlock), 0, st, ws,
                        graph->num_vertices, wdelta, value, front_out, counters);
   }
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_relax_split(const uint32_t* queue, uint32_t n_queue, const uint32_t* value,
+                      uint32_t bound, uint32_t n, uint32_t* near_queue, uint32_t* front_out,
+                      unsigned long long* split, void* stream) {
+  if (!value || !near_queue || !front_out || !split || (n_queue && !queue))
+    return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_relax_split_init, dim3(1), dim3(1), 0, st, split);
+  SCAPH_LAUNCH_CHECK();
+  if (n_queue == 0) return 0;
+  hipLaunchKernelGGL(k_relax_split, dim3(grid_for(n_queue, kBlock, 4096)), dim3(kBlock), 0, st,
+                     queue, n_queue, value, bound, n, near_queue, front_out, split);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

@@ -62,6 +62,8 @@ Config& global_flags() {
     r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
     r.add("learned_fused", FlagType::kBool, &c.learned_fused,
           "a model reading only scalar features decides inside the level hand-off");
+    r.add("sssp_delta", FlagType::kInt32, &c.sssp_delta,
+          "SSSP: relax only vertices below a bound that grows by this much (0 = all)");
     r.add("pull_prefetch", FlagType::kBool, &c.pull_prefetch,
           "request the next batch's head records before probing the current one");
     r.add("pull_head_outdeg", FlagType::kBool, &c.pull_head_outdeg,
@@ -113,6 +115,10 @@ bool parse_config(const std::vector<std::string>& args, Config* out,
   if (g.tricknum < 0 || g.tricknum > 4095) {
     // the upper bound is the device's CU count, checked when streams open
     if (err) *err = "ERROR: -tricknum must be in 0..(CUs - 1)";
+    return false;
+  }
+  if (g.sssp_delta < 0) {
+    if (err) *err = "ERROR: -sssp_delta must be >= 0";
     return false;
   }
   if (g.page_groups < 0 || g.page_groups > 4096) {

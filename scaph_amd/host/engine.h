@@ -50,6 +50,7 @@ struct Config {
   bool pull_head_outdeg = true;  // head records {e0, e1, in-degree, out-degree}
   bool pull_prefetch = true;     // request the next batch's head records early
   bool learned_fused = true;     // a scalar-feature model decides inside the level hand-off
+  int32_t sssp_delta = 0;        // resident SSSP: bucket width of near/far rounds (0 = off)
   int32_t push_grid = 0;         // cap on push expand blocks (0: one per tile, at most 8192)
   bool push_rows = true;         // push expand reads coalesced 256-edge rows
   int32_t push_small = 0;        // out-edges below which a push level is one launch (0: 8192)
@@ -236,6 +237,7 @@ class Engine {
   DevBuf wpool_;                  // edge weights laid out like pool_out_ (SSSP)
   bool weights_loaded_ = false;   // resident: wpool_; paged: PagedState::h_wpage
   DevBuf relax_ws_;               // chunk list of the WCC/SSSP rounds
+  DevBuf split_;                  // near/far split counters (3 u64)
   RunStats relax(int mode, uint32_t src);
   DevBuf dpages_out_, dptr_out_, dpages_in_, dptr_in_;
   scaph_graph_desc g_out_{}, g_in_{};

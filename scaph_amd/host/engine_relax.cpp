@@ -55,6 +55,13 @@ RunStats Engine::relax(int mode, uint32_t src) {
     n_act = 1;
   }
   const uint32_t* weights = mode == SCAPH_RELAX_SSSP ? wpool_.get<uint32_t>() : nullptr;
+  // near/far rounds (SSSP, -sssp_delta): only active vertices below `bound`
+  // relax; the rest stay active, and the bound moves on when none is below
+  // it, which saves re-relaxing vertices whose distance is still falling
+  const uint64_t delta = mode == SCAPH_RELAX_SSSP ? (uint64_t)cfg_.sssp_delta : 0;
+  uint64_t bound = delta;
+  if (delta && !split_.get()) split_.alloc(64);
+  auto* hsplit = reinterpret_cast<unsigned long long*>(hcount_.get<char>() + 256);
   uint32_t round = 0;
   while (n_act > 0) {
     H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, sizeof(scaph_level_counters) / 4,
@@ -62,21 +69,42 @@ RunStats Engine::relax(int mode, uint32_t src) {
     H_ERR(scaph_fill_u32(front_[fa ^ 1].get<uint32_t>(), 0u, words, st));
     H_ERR(scaph_frontier_bitmap_to_queue(front_[fa].get<uint32_t>(), n_, queue_[0].get<uint32_t>(),
                                          cnt, st));
-    H_ERR(scaph_relax_round(&g_out_, queue_[0].get<uint32_t>(), (uint32_t)n_act, weights,
-                            pool_out_.get<uint32_t>(), value, front_[fa ^ 1].get<uint32_t>(), cnt,
-                            relax_ws_.get(), relax_ws_.bytes(), st));
+    const uint32_t* q = queue_[0].get<uint32_t>();
+    uint64_t n_q = n_act, n_far = 0;
+    if (delta) {
+      const uint32_t b = (uint32_t)std::min<uint64_t>(bound, 0xFFFFFFFFull);
+      H_ERR(scaph_relax_split(q, (uint32_t)n_act, value, b, n_, queue_[1].get<uint32_t>(),
+                              front_[fa ^ 1].get<uint32_t>(),
+                              split_.get<unsigned long long>(), st));
+      H_ERR(scaph_copy_d2h_async(hsplit, split_.get(), 24, st));
+      main_->sync();
+      n_q = hsplit[0];
+      n_far = hsplit[1];
+      if (n_q + n_far != n_act) throw Error(SCAPH_STATUS_INVALID, "active set size mismatch");
+      if (n_q == 0) {
+        // every active vertex is far: the bound moves past the least of them
+        bound = std::max(bound + delta, (uint64_t)hsplit[2] + delta);
+        fa ^= 1;
+        continue;
+      }
+      q = queue_[1].get<uint32_t>();
+    }
+    H_ERR(scaph_relax_round(&g_out_, q, (uint32_t)n_q, weights, pool_out_.get<uint32_t>(), value,
+                            front_[fa ^ 1].get<uint32_t>(), cnt, relax_ws_.get(),
+                            relax_ws_.bytes(), st));
     H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
     main_->sync();
     if (hc->reserved) throw Error(SCAPH_STATUS_INVALID, "relax chunk list overflow");
     if (hc->q_len != n_act) throw Error(SCAPH_STATUS_INVALID, "active set size mismatch");
     LevelRecord rec;
     rec.level = round;
-    rec.n_f = n_act;
+    rec.n_f = n_q;
     rec.n_next = hc->n_next;
     rec.scanned = hc->scanned;
     rs.levels.push_back(rec);
     rs.m_reached += hc->scanned;  // edges relaxed over all rounds
-    n_act = hc->n_next;
+    // far vertices kept their bits; newly lowered ones were counted once
+    n_act = hc->n_next + n_far;
     fa ^= 1;
     ++round;
   }

@@ -244,3 +244,33 @@ def test_apps_edge_cases():
                 e.sssp(source)
                 np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, source),
                                               err_msg=f"{name} {source}")
+
+
+@pytest.mark.parametrize("delta", [1, 37, 300, 1 << 20])
+def test_sssp_near_far_rounds(delta):
+    """-sssp_delta: only active vertices below a moving bound relax each
+    round; the distances equal the oracle's for any bucket width (fixtures,
+    zero weights, R-MAT, a long chain behind a hub)."""
+    rng = np.random.default_rng(delta)
+    graphs = []
+    for name, off, ngh, src in fixtures():
+        graphs.append((name, off, ngh, rng.integers(0, 60, len(ngh)).astype(np.uint32), src))
+    off, ngh = O.rmat_csr(15, 16, seed=3)
+    deg = np.diff(off.astype(np.int64))
+    graphs.append(("rmat15", off, ngh, rng.integers(1, 256, len(ngh)).astype(np.uint32),
+                   int(np.argmax(deg))))
+    n = 3000
+    s = list(range(n - 1)) + [0] * 500
+    d = list(range(1, n)) + list(rng.integers(1, n, 500))
+    w = [1] * (n - 1) + list(rng.integers(1000, 5000, 500))
+    off, ngh, wgh = weighted_csr(n, s, d, w)
+    graphs.append(("chain", off, ngh, wgh, 0))
+    for name, off, ngh, wgh, src in graphs:
+        say("near/far", delta, name)
+        with engine(f"-sssp_delta={delta}") as e:
+            e.load_csr(off, ngh)
+            e.set_weights(wgh)
+            st = e.sssp(src)
+            np.testing.assert_array_equal(e.levels(), O.sssp(off, ngh, wgh, src),
+                                          err_msg=f"{name} delta {delta}")
+            assert st["depth"] >= 1
