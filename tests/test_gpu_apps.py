@@ -171,7 +171,8 @@ def test_apps_then_bfs_and_errors():
 
 @pytest.mark.parametrize("flags", [["-num_stream=1"], ["-num_stream=3"],
                                    ["-num_stream=3", "-multi=2"], ["-num_stream=5", "-multi=3"],
-                                   ["-num_stream=2", "-threshold=16"], ["-threshold=0"]])
+                                   ["-num_stream=2", "-threshold=16"], ["-threshold=0"],
+                                   ["-threshold=16", "-multi=2", "-num_stream=3"]])
 def test_apps_paged(flags):
     """WCC and SSSP out of core: a pool of a few page slots, pages copied
     whole or compacted to their active Nodes (with their weights), on one or
