@@ -1,9 +1,11 @@
-// engine_relax.cpp — the reference's other App plugins on the resident
-// graph: WCC (app/wcc.cu) and SSSP (app/sssp.cu).  Each round turns the
-// active set (a bitmap) into a list and runs scaph_relax_round over it; the
-// vertices whose value it lowered form the next active set.  The rounds
-// stop when a round lowers nothing, as the reference's run() stops when no
-// vertex is active (src/app.cuh:220-265).
+// engine_relax.cpp — the reference's other App plugins on the same engine:
+// WCC (app/wcc.cu) and SSSP (app/sssp.cu).  Resident, each round turns the
+// active set (a bitmap) into a list and runs scaph_relax_round over it
+// (with -sssp_delta, only its vertices below a moving bound); paged, the
+// round's active pages go through the paged BFS scheduler and
+// scaph_relax_page.  The vertices whose value a round lowered form the next
+// active set, and the rounds stop when none is active, as the reference's
+// run() stops (src/app.cuh:220-265).
 #include <algorithm>
 #include <chrono>
 
