@@ -16,6 +16,7 @@ namespace {
 
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kRelaxChunk = 4096;  // out-edges per wave item; longer lists are chunked
+constexpr uint32_t kRelaxSerial = 16;   // lists this short are walked by one lane
 
 struct RelaxChunk {
   uint32_t u;
@@ -107,6 +108,9 @@ __device__ __forceinline__ void relax_node(uint32_t u, NodeRef nr, long long wde
   }
 }
 
+// A wave takes 64 active vertices, one per lane: a lane walks a list of at
+// most kRelaxSerial edges itself; longer lists go one at a time to the
+// whole wave (relax_node), so short lists do not leave 63 lanes idle.
 template <int WEIGHTED>
 __global__ void __launch_bounds__(kBlock)
 k_relax(scaph_graph_desc g, const uint32_t* q, uint32_t n_q, long long wdelta, uint32_t* value,
@@ -118,11 +122,35 @@ k_relax(scaph_graph_desc g, const uint32_t* q, uint32_t n_q, long long wdelta, u
   const uint32_t lane = lane_id();
   const uint32_t nw = kBlock / SCAPH_WAVE;
   unsigned long long acc_n = 0, acc_s = 0;
-  for (uint32_t i = blockIdx.x * nw + threadIdx.x / SCAPH_WAVE; i < n_q; i += gridDim.x * nw) {
-    const uint32_t u = q[i];
-    if (u >= n) continue;
-    relax_node<WEIGHTED>(u, load_node(t, g, u), wdelta, n, value, front_out, counters, ws, acc_n,
-                         acc_s, lane);
+  for (uint32_t base = (blockIdx.x * nw + threadIdx.x / SCAPH_WAVE) * SCAPH_WAVE; base < n_q;
+       base += gridDim.x * nw * SCAPH_WAVE) {
+    const uint32_t i = base + lane;
+    uint32_t u = 0;
+    NodeRef nr;
+    nr.edges = nullptr;
+    nr.len = 0;
+    bool act = false;
+    if (i < n_q) {
+      u = q[i];
+      if (u < n) {
+        nr = load_node(t, g, u);
+        act = true;
+      }
+    }
+    if (act && nr.len <= kRelaxSerial)
+      relax_list<WEIGHTED>(nr.edges, nr.len, value[u], wdelta, n, value, front_out, acc_n, acc_s,
+                           0u, 1u);
+    uint64_t ml = __ballot(act && nr.len > kRelaxSerial);
+    while (ml) {
+      const int j = __ffsll((unsigned long long)ml) - 1;
+      ml &= ml - 1;
+      NodeRef nj;
+      nj.edges = reinterpret_cast<const uint32_t*>(
+          wave_bcast64(reinterpret_cast<uint64_t>(nr.edges), j));
+      nj.len = wave_bcast(nr.len, j);
+      relax_node<WEIGHTED>(wave_bcast(u, j), nj, wdelta, n, value, front_out, counters, ws, acc_n,
+                           acc_s, lane);
+    }
   }
   relax_flush(counters, acc_n, acc_s);
 }
@@ -286,7 +314,7 @@ int scaph_relax_round(const scaph_graph_desc* graph, const uint32_t* queue, uint
               : 0ll;
   hipLaunchKernelGGL(k_relax_zero, dim3(1), dim3(1), 0, st, ws.count);
   SCAPH_LAUNCH_CHECK();
-  const unsigned grid = grid_for(n_queue, kBlock / SCAPH_WAVE, 16384);
+  const unsigned grid = grid_for(n_queue, kBlock, 16384);  // a lane per queue entry
   const size_t lds = page_lds_bytes(graph->num_pages);
   if (weights) {
     hipLaunchKernelGGL(k_relax<1>, dim3(grid), dim3(kBlock), lds, st, *graph, queue, n_queue,
