@@ -72,6 +72,10 @@ int scaph_engine_gen_zipf(scaph_engine* e, uint32_t n, double x_min, double alph
                           uint64_t seed, char* err, size_t errlen);
 int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_pages,
                       int* resident, int* has_pull);
+/* Page slots of the loaded graph's device pool (scaph_host_plan_pool of its
+ * -max_memory; equal to num_pages when every page is resident) and the
+ * device bytes the engine holds for it. */
+int scaph_engine_pool(scaph_engine* e, int64_t* slots, uint64_t* device_bytes);
 /* run(): one BFS from src.  schedule: NULL keeps the -schedule flag, else
  * "fixed" | "push" | "pull" | "learned". */
 int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* err,
@@ -170,6 +174,14 @@ int scaph_host_write_graph(const char* base, uint32_t n, const uint64_t* offsets
                            int write_wgh, char* err, size_t errlen);
 int scaph_host_cpu_bfs(uint32_t n, const uint64_t* offsets, const uint32_t* ngh, uint32_t src,
                        uint32_t inf, uint32_t* levels);
+/* Device page-pool slots for -max_memory=max_memory_mib (the budget of
+ * App::load_graph, src/app.cuh:281-286, 323-324): the budget less 9 bytes
+ * per vertex and 24 per page, in whole pagesize_words slots, at most one per
+ * page.  *slots = -1 when the vertex arrays leave no more than one slot,
+ * where the reference prints "vertex data out of gpu memory. exit!"
+ * (src/app.cuh:310-313).  The engine sizes its pool with this function. */
+int scaph_host_plan_pool(int64_t max_memory_mib, uint32_t n, uint32_t num_pages,
+                         uint64_t pagesize_words, int64_t* slots);
 
 /* Batch of small graphs (no reference counterpart): `count` R-MAT graphs
  * generated on the device (graph g: scale and edge factor uniform in the

@@ -13,6 +13,9 @@
  *                         shortest paths, closed forms (path, star, grid,
  *                         binary tree, cycle), brute-force relaxation and the
  *                         depth-63 cap on a long path.
+ *   oracle_bfs_mt         the same distances, level-synchronous on POSIX
+ *                         threads (full-size checks) — pinned by equality
+ *                         with oracle_bfs, including the depth-63 cap.
  *   oracle_wcc            app/wcc.cu:60-85 (check()): value[v] = v, then
  *                         sweeps of the out-edges in vertex order lowering
  *                         value[ng] to value[vid] until nothing changes —
@@ -56,6 +59,7 @@
  *                         by finite differences and a numpy matmul form.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -215,6 +219,97 @@ ORACLE_API int oracle_bfs(uint32_t n, const uint64_t* offsets, const uint32_t* n
   free(cur);
   free(next);
   return 0;
+}
+
+/* The same distances as oracle_bfs, level-synchronous on `threads` POSIX
+ * threads so that full-size graphs (billions of edges) check in seconds:
+ * at level L every frontier vertex u offers L + 1 to its out-neighbours and
+ * a neighbour still at `inf` is claimed by compare-and-swap when
+ * L + 1 < inf (the relax of app/bfs_kernel.cuh:6-10 under the cap).  BFS
+ * distances are unique, so the claim order cannot change the result.  It
+ * is pinned by equality with oracle_bfs (tests/test_oracle.py). */
+typedef struct {
+  const uint64_t* off;
+  const uint32_t* ngh;
+  uint32_t* value;
+  const uint32_t* cur;
+  uint64_t ncur;
+  uint32_t* next;
+  uint64_t* nnext;  /* shared append position of next */
+  uint64_t* take;   /* shared position in cur (chunks of 64 vertices) */
+  uint32_t lvl, inf;
+} bfs_mt_arg;
+
+static void* bfs_mt_worker(void* p) {
+  bfs_mt_arg* a = (bfs_mt_arg*)p;
+  uint32_t buf[4096];
+  uint32_t nb = 0;
+  const uint32_t nl = a->lvl + 1;
+  for (;;) {
+    const uint64_t i0 = __atomic_fetch_add(a->take, 64, __ATOMIC_RELAXED);
+    if (i0 >= a->ncur) break;
+    const uint64_t i1 = i0 + 64 < a->ncur ? i0 + 64 : a->ncur;
+    for (uint64_t i = i0; i < i1; ++i) {
+      const uint32_t u = a->cur[i];
+      for (uint64_t k = a->off[u]; k < a->off[(uint64_t)u + 1]; ++k) {
+        const uint32_t v = a->ngh[k];
+        uint32_t old = __atomic_load_n(&a->value[v], __ATOMIC_RELAXED);
+        if (old != a->inf) continue;
+        if (!__atomic_compare_exchange_n(&a->value[v], &old, nl, 0, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED))
+          continue;
+        buf[nb++] = v;
+        if (nb == 4096) {
+          const uint64_t at = __atomic_fetch_add(a->nnext, nb, __ATOMIC_RELAXED);
+          memcpy(a->next + at, buf, sizeof(uint32_t) * nb);
+          nb = 0;
+        }
+      }
+    }
+  }
+  if (nb) {
+    const uint64_t at = __atomic_fetch_add(a->nnext, nb, __ATOMIC_RELAXED);
+    memcpy(a->next + at, buf, sizeof(uint32_t) * nb);
+  }
+  return NULL;
+}
+
+ORACLE_API int oracle_bfs_mt(uint32_t n, const uint64_t* offsets, const uint32_t* ngh,
+                             uint32_t src, uint32_t inf, uint32_t* value, uint32_t threads) {
+  if (src >= n) return 1;
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  uint32_t* cur = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t* next = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+  if (!cur || !next || !th) {
+    free(cur);
+    free(next);
+    free(th);
+    return 2;
+  }
+  for (uint32_t v = 0; v < n; ++v) value[v] = inf;
+  value[src] = 0;
+  cur[0] = src;
+  uint64_t ncur = 1;
+  int rc = 0;
+  for (uint32_t lvl = 0; ncur && lvl + 1 < inf; ++lvl) {
+    uint64_t nnext = 0, take = 0;
+    bfs_mt_arg a = {offsets, ngh, value, cur, ncur, next, &nnext, &take, lvl, inf};
+    uint32_t started = 0;
+    for (; started < threads; ++started)
+      if (pthread_create(&th[started], NULL, bfs_mt_worker, &a) != 0) break;
+    if (started == 0) bfs_mt_worker(&a);
+    for (uint32_t t = 0; t < started; ++t) pthread_join(th[t], NULL);
+    uint32_t* t = cur;
+    cur = next;
+    next = t;
+    ncur = nnext;
+  }
+  free(cur);
+  free(next);
+  free(th);
+  return rc;
 }
 
 /* --------------------------------------------------------- page layout */

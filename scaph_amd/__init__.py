@@ -16,7 +16,8 @@ import numpy as np
 from . import _lib
 from ._lib import UNREACHED, LibraryMissing, ScaphError  # noqa: F401
 
-__all__ = ["BFS", "Batch", "generate_rmat", "generate_er", "page_cuts", "write_graph", "UNREACHED",
+__all__ = ["BFS", "Batch", "generate_rmat", "generate_er", "page_cuts", "plan_pool", "write_graph",
+           "UNREACHED",
            "LibraryMissing", "ScaphError", "device_props"]
 
 
@@ -98,6 +99,14 @@ class BFS:
                                                ctypes.byref(pull)))
         return {"n": n.value, "m": m.value, "pages": np_.value, "resident": bool(res.value),
                 "pull": bool(pull.value)}
+
+    def pool(self):
+        """Page slots of the device pool (every page when resident) and the
+        device bytes the engine holds."""
+        slots = ctypes.c_int64()
+        nbytes = ctypes.c_uint64()
+        _lib.check(self._lib.scaph_engine_pool(self._h, ctypes.byref(slots), ctypes.byref(nbytes)))
+        return {"slots": slots.value, "device_bytes": nbytes.value}
 
     # -- run -------------------------------------------------------------
     def run(self, src):
@@ -372,6 +381,15 @@ def page_cuts(offsets, pagesize_words, weighted=False):
     _lib.check(lib.scaph_host_page_cuts(_ptr(offsets), n, pagesize_words, int(weighted),
                                         _ptr(pages), cnt.value, ctypes.byref(cnt)))
     return pages
+
+
+def plan_pool(max_memory_mib, n, num_pages, pagesize_words):
+    """Device page slots for -max_memory (src/app.cuh:281-286, 323-324);
+    -1 where the reference exits with "vertex data out of gpu memory"."""
+    slots = ctypes.c_int64()
+    _lib.check(_lib.host().scaph_host_plan_pool(max_memory_mib, n, num_pages, pagesize_words,
+                                                ctypes.byref(slots)))
+    return slots.value
 
 
 def write_graph(base, offsets, ngh, pagesize_words=0, idx64=True, write_wgh=False):

@@ -260,6 +260,10 @@ def host():
         _sig(lib, "scaph_host_gen_er", I32, [U32, U64, U64, P, P, U64, ctypes.POINTER(U64)])
         _sig(lib, "scaph_host_write_graph", I32, [CHARP, U32, P, P, U64, I32, I32, CHARP, SZ])
         _sig(lib, "scaph_host_cpu_bfs", I32, [U32, P, P, U32, U32, P])
+        _sig(lib, "scaph_host_plan_pool", I32,
+             [ctypes.c_int64, U32, U32, U64, ctypes.POINTER(ctypes.c_int64)])
+        _sig(lib, "scaph_engine_pool", I32,
+             [E, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(U64)])
         _host = lib
     return _host
 

@@ -1184,12 +1184,16 @@ k_pull_heavy(const HeavyChunk* chunks, const unsigned long long* count, const ui
         break;
       }
       if ((k0 / kBlock) % 4 == 3) {
-        // another chunk of the same list may have won meanwhile
+        // another chunk of the same list may have won meanwhile; every wave
+        // reads the flag before any wave can leave the loop and overwrite
+        // it for the next chunk
         if (threadIdx.x == 0)
           s_skip = (int)((__hip_atomic_load(&visited[c.v >> 5], __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT) >> (c.v & 31)) & 1u);
         __syncthreads();
-        if (s_skip) break;
+        const int won = s_skip;
+        __syncthreads();
+        if (won) break;
       }
     }
     if (hit && threadIdx.x == 0) {

@@ -194,26 +194,48 @@ void Engine::reset_graph() {
 }
 
 uint64_t Engine::device_bytes() const {
-  return pool_out_.bytes() + pool_in_.bytes() + deg_out_.bytes() + deg_in_.bytes() +
-         static_visited_.bytes() + level_.bytes() + visited_.bytes() + front_[0].bytes() +
-         front_[1].bytes() + queue_[0].bytes() + queue_[1].bytes() + ws_.bytes();
+  uint64_t b = pool_out_.bytes() + pool_in_.bytes() + deg_out_.bytes() + deg_in_.bytes() +
+               static_visited_.bytes() + level_.bytes() + visited_.bytes() + front_[0].bytes() +
+               front_[1].bytes() + queue_[0].bytes() + queue_[1].bytes() + ws_.bytes() +
+               pws_.bytes() + vsnap_.bytes() + in_head_.bytes() + hub_list_.bytes() +
+               counters_.bytes() + dpages_out_.bytes() + dptr_out_.bytes() + dpages_in_.bytes() +
+               dptr_in_.bytes() + page_nodes_.bytes() + page_datas_.bytes() + totals_.bytes() +
+               wpool_.bytes();
+  if (paged_) {
+    const PagedState& ps = *paged_;
+    b += ps.pool.bytes() + ps.group_ws.bytes() + ps.d_runs.bytes() + ps.wpool.bytes();
+    for (const auto& w : ps.ws) b += w.bytes();
+    for (const auto& w : ps.relax_ws) b += w.bytes();
+  }
+  return b;
 }
 
 // Budget of src/app.cuh:281-286, 323-324: max_memory MiB minus the vertex
-// arrays (value, stat, degree) and the per-page lists, divided into P-word
-// page slots.  The pull expand's transposed pages are budgeted too.
+// arrays (value, stat, degree: 9 bytes per vertex) and the per-page lists
+// (24 bytes per page), divided into P-word page slots; -1 when that leaves
+// no more than one slot ("vertex data out of gpu memory").
+int64_t plan_pool_slots(int64_t max_memory_mib, uint32_t n, uint32_t list_size,
+                        uint64_t page_words) {
+  int64_t budget = max_memory_mib << 20;
+  budget -= (int64_t)n * (4 + 1 + 4);
+  budget -= (int64_t)list_size * (4 + 4 + 16);
+  const int64_t slot = (int64_t)page_words * 4;
+  if (budget <= slot) return -1;
+  return std::min<int64_t>(budget / slot, list_size);
+}
+
+// The pull expand's transposed pages are budgeted after the out-pages.
 void Engine::plan_residency() {
-  int64_t budget = (int64_t)cfg_.max_memory << 20;
   const int64_t list = (int64_t)pages_out_.size();
-  budget -= (int64_t)n_ * (4 + 1 + 4);
-  budget -= list * (4 + 4 + 16);
-  const int64_t slot = (int64_t)P_ * 4;
-  if (budget <= slot) throw Error(SCAPH_STATUS_INVALID, "vertex data out of gpu memory. exit!");
-  int64_t pool = budget / slot;
-  pool_size_ = (int)std::min<int64_t>(pool, list);
+  const int64_t pool = plan_pool_slots(cfg_.max_memory, n_, (uint32_t)list, P_);
+  if (pool < 0) throw Error(SCAPH_STATUS_INVALID, "vertex data out of gpu memory. exit!");
+  pool_size_ = (int)pool;
   resident_ = pool_size_ >= list;
+  // slots left over once every out-page has one
+  const int64_t spare = ((((int64_t)cfg_.max_memory << 20) - (int64_t)n_ * 9 - list * 24) /
+                         ((int64_t)P_ * 4)) - list;
   const int64_t need_in = (int64_t)pages_in_.size();
-  has_in_ = cfg_.build_pull && resident_ && need_in > 0 && pool - list >= need_in;
+  has_in_ = cfg_.build_pull && resident_ && need_in > 0 && spare >= need_in;
 }
 
 // Weights in the pages' own layout: page p's out-edge k sits at word

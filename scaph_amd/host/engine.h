@@ -266,6 +266,14 @@ class Engine {
   uint64_t graph_gen_ = 0;
 };
 
+// Page slots of the device pool for a -max_memory budget of max_memory_mib
+// (src/app.cuh:281-286, 323-324): the budget less 9 bytes per vertex and 24
+// per page, in whole P-word slots, at most one per page; -1 when the vertex
+// arrays leave no more than one slot (the reference's "vertex data out of
+// gpu memory" exit).
+int64_t plan_pool_slots(int64_t max_memory_mib, uint32_t n, uint32_t list_size,
+                        uint64_t page_words);
+
 // CPU BFS used by bfs::check() (app/bfs.cu:67-117): FIFO level BFS over the
 // out-edges with levels starting at `inf`; a vertex is relaxed when
 // level(u) + 1 < level(v).  With inf = 63 it reproduces the reference's

@@ -184,6 +184,13 @@ int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_p
   return 0;
 }
 
+int scaph_engine_pool(scaph_engine* e, int64_t* slots, uint64_t* device_bytes) {
+  if (!e) return 1;
+  if (slots) *slots = e->eng->pool_size();
+  if (device_bytes) *device_bytes = e->eng->device_bytes();
+  return 0;
+}
+
 int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* err,
                      size_t errlen) {
   if (!e) return 1;
@@ -457,6 +464,13 @@ int scaph_engine_graph_desc(scaph_engine* e, int transposed, void* desc_out) {
 int scaph_engine_device_degree(scaph_engine* e, const uint32_t** degree) {
   if (!e || !degree) return 1;
   *degree = e->eng->d_degree();
+  return 0;
+}
+
+int scaph_host_plan_pool(int64_t max_memory_mib, uint32_t n, uint32_t num_pages,
+                         uint64_t pagesize_words, int64_t* slots) {
+  if (!slots || pagesize_words == 0) return 1;
+  *slots = scaph::plan_pool_slots(max_memory_mib, n, num_pages, pagesize_words);
   return 0;
 }
 

@@ -22,6 +22,8 @@ def lib():
         U32, U64, I32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
         L.oracle_bfs.argtypes = [U32, P, P, U32, U32, P]
         L.oracle_bfs.restype = I32
+        L.oracle_bfs_mt.argtypes = [U32, P, P, U32, U32, P, U32]
+        L.oracle_bfs_mt.restype = I32
         L.oracle_wcc.argtypes = [U32, P, P, P]
         L.oracle_wcc.restype = I32
         L.oracle_sssp.argtypes = [U32, P, P, P, U32, P]
@@ -116,6 +118,23 @@ def bfs(offsets, ngh, src, inf=UNREACHED):
     rc = lib().oracle_bfs(n, _p(offsets), _p(ngh), src, inf, _p(out))
     if rc:
         raise ValueError(f"oracle_bfs rc={rc}")
+    return out[:n]
+
+
+def bfs_mt(offsets, ngh, src, inf=UNREACHED, threads=None):
+    """oracle_bfs's distances, level-synchronous on `threads` threads (for
+    full-size graphs; pinned by equality with bfs())."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    ngh = np.ascontiguousarray(ngh, np.uint32)
+    n = len(offsets) - 1
+    out = np.empty(max(n, 1), np.uint32)
+    if ngh.size == 0:
+        ngh = np.zeros(1, np.uint32)
+    if threads is None:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    rc = lib().oracle_bfs_mt(n, _p(offsets), _p(ngh), src, inf, _p(out), threads)
+    if rc:
+        raise ValueError(f"oracle_bfs_mt rc={rc}")
     return out[:n]
 
 

@@ -415,3 +415,32 @@ def test_graphfile_with_transposed_pages():
         with engine(f"-graphfile={base}") as e:
             with pytest.raises(RuntimeError, match="not the transpose"):
                 e.load()
+
+
+def test_memory_budget_pool_equals_oracle():
+    """a15 on the device: the pool the engine allocates for -max_memory is
+    the oracle's pool size (src/app.cuh:281-286, 323-324), the graph is
+    resident exactly when the pool holds every page, levels are right at
+    every pool size, and a budget the vertex arrays exhaust fails with the
+    reference's message (src/app.cuh:310-313)."""
+    off, ngh = O.rmat_csr(16, 16, seed=6)
+    n = len(off) - 1
+    P = 65536
+    pages = len(O.page_cuts(off, P))
+    assert pages > 8
+    deg = np.diff(off.astype(np.int64))
+    src = int(np.argmax(deg))
+    want = O.bfs(off, ngh, src)
+    for mib in (1, 2, 3, 5, 64):
+        expect = O.lib().oracle_pool_size(mib, n, pages, P)
+        with engine(f"-max_memory={mib}", f"-page_words={P}", "-build_pull=false") as e:
+            e.load_csr(off, ngh)
+            got = e.pool()
+            say("budget", mib, "MiB: pool", got, "oracle", expect, "pages", pages)
+            assert got["slots"] == expect
+            assert e.info()["resident"] == (expect == pages)
+            e.run(src)
+            np.testing.assert_array_equal(e.levels(), want)
+    with engine("-max_memory=0", f"-page_words={P}", "-build_pull=false") as e:
+        with pytest.raises(scaph_amd.ScaphError, match="vertex data out of gpu memory"):
+            e.load_csr(off, ngh)

@@ -292,3 +292,28 @@ def test_exporters_match_reference_tools():
         subprocess.run([os.path.join(BIN, "bin2ligra"), os.path.join(td, "mine"), "w"], check=True)
         assert sha(os.path.join(td, "ref.ligra.weight")) == \
             sha(os.path.join(td, "mine.ligra.weight"))
+
+
+def test_pool_plan_equals_oracle_pool_size():
+    """a15: the engine's page-pool planner (the -max_memory budget of
+    src/app.cuh:281-286, 323-324) against the oracle's closed form, over
+    budgets around every boundary: the vertex-data exit (-1), one slot, a
+    partial pool and a pool that holds every page."""
+    P = 10 << 20
+    cases = []
+    for n, pages in ((1 << 20, 2), (67_108_864, 220), (105_000_000, 389), (1000, 1), (0, 0),
+                     (3_000_000_000, 7)):
+        need = 9 * n + 24 * pages
+        base = need >> 20
+        for mib in (0, 1, base, base + 40, base + 41, base + 80, base + 81, base + 40 * pages,
+                    base + 40 * pages + 41, 8192, 16142, 15258, 262144):
+            cases.append((mib, n, pages, P))
+    cases += [(16, 4096, 12, 4096), (1, 10, 3, 64), (2, 100000, 3, 1 << 16)]
+    for mib, n, pages, P_ in cases:
+        want = O.lib().oracle_pool_size(mib, n, pages, P_)
+        got = scaph_amd.plan_pool(mib, n, pages, P_)
+        assert got == want, (mib, n, pages, P_, got, want)
+    # the configurations' budgets: C3's 16e9-byte pool and 8 GiB
+    assert scaph_amd.plan_pool(16142, 105_000_000, 389, P) == 381
+    assert scaph_amd.plan_pool(8192, 105_000_000, 389, P) == 182
+    assert scaph_amd.plan_pool(100, 105_000_000, 389, P) == -1

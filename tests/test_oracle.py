@@ -103,6 +103,32 @@ def test_oracle_bfs_depth_cap_63():
     np.testing.assert_array_equal(np.where(exact <= 62, exact, 63), capped)
 
 
+def test_oracle_bfs_mt_equals_oracle_bfs():
+    """The threaded full-size checker gives oracle_bfs's distances: on the
+    scipy fixtures, R-MAT and ER graphs with hub out-lists, random digraphs
+    with self-loops and duplicates, for 1..16 threads, and under the cap."""
+    for name, off, ngh, src, want in load_cases():
+        np.testing.assert_array_equal(O.bfs_mt(off, ngh, src, threads=3), want, err_msg=name)
+    graphs = [O.rmat_csr(16, 16, seed=4), O.er_csr(50000, 400000, 9)]
+    for off, ngh in graphs:
+        deg = np.diff(off.astype(np.int64))
+        for src in (int(np.argmax(deg)), int(np.nonzero(deg)[0][3]), 0):
+            want = O.bfs(off, ngh, src)
+            for t in (1, 2, 7, 16):
+                np.testing.assert_array_equal(O.bfs_mt(off, ngh, src, threads=t), want)
+    rng = np.random.default_rng(12)
+    for trial in range(100):
+        n = int(rng.integers(1, 40))
+        m = int(rng.integers(0, 4 * n + 1))
+        off, ngh = O.csr_from_edges(n, rng.integers(0, n, m), rng.integers(0, n, m))
+        src = int(rng.integers(0, n))
+        np.testing.assert_array_equal(O.bfs_mt(off, ngh, src, threads=4), O.bfs(off, ngh, src))
+    n = 100
+    off, ngh = O.csr_from_edges(n, np.arange(n - 1), np.arange(1, n))
+    np.testing.assert_array_equal(O.bfs_mt(off, ngh, 0, inf=63, threads=4),
+                                  O.bfs(off, ngh, 0, inf=63))
+
+
 # ------------------------------------------------------------ page layout
 
 def ref_tool_available():
