@@ -217,15 +217,6 @@ int scaph_host_max_zero_run(uint32_t mask);
 int scaph_host_fit_zero_run(uint32_t mask, int required);
 uint32_t scaph_host_set_run(uint32_t mask, int off, int number);
 uint32_t scaph_host_clear_run(uint32_t mask, int off, int number);
-/* Runs the page-slot scheduler (src/pagehandle.h semantics) against a
- * simulated executor and copy engine for `iterations` iterations of
- * per-page active data datas[it * list_size + p] (in chunk units), checking
- * its invariants at every step.  0 = all held, otherwise the code of the
- * first violated invariant.  out_stats: executions, copies, compactions,
- * executions of pages resident at the start of their iteration. */
-int scaph_host_pagehandle_simulate(uint32_t list_size, uint32_t pool_size, uint64_t thres_words,
-                                   int maxentry, uint32_t iterations, const uint32_t* datas,
-                                   uint64_t seed, uint64_t* out_stats);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@
 #include "msbfs.h"
 #include "batch.h"
 #include "learned.h"
+#include "pagehandle.h"
 
 struct scaph_engine {
   scaph::Config cfg;
@@ -633,3 +634,15 @@ int scaph_batch_autotune(scaph_batch* b, scaph_batch_tune* best, double* best_ms
 }
 
 }  // extern "C"
+
+// Chunk masks of the page-slot cache (src/tools.h:10-85).
+extern "C" int scaph_host_max_zero_run(uint32_t mask) { return scaph::max_zero_run(mask); }
+extern "C" int scaph_host_fit_zero_run(uint32_t mask, int required) {
+  return scaph::fit_zero_run(mask, required);
+}
+extern "C" uint32_t scaph_host_set_run(uint32_t mask, int off, int number) {
+  return scaph::set_run(mask, off, number);
+}
+extern "C" uint32_t scaph_host_clear_run(uint32_t mask, int off, int number) {
+  return scaph::clear_run(mask, off, number);
+}

@@ -1,13 +1,13 @@
-// pagehandle_sim.cpp — drives PageHandle with a simulated executor and copy
-// engine (random completion order) and checks the scheduling invariants
-// on every step.  Used by the CPU tests through the host C-ABI.
+// pagehandle_sim.cpp — TEST ONLY (built into tests/native/libpagehandle_sim.so,
+// never into the product libraries): drives the engine's PageHandle with a
+// simulated executor and copy engine (random completion order) and checks
+// the scheduling invariants on every step.
 #include <algorithm>
 #include <random>
 #include <string>
 #include <vector>
 
 #include "pagehandle.h"
-#include "scaph_engine.h"
 
 namespace {
 
@@ -120,15 +120,4 @@ extern "C" int scaph_host_pagehandle_simulate(uint32_t list_size, uint32_t pool_
   } catch (const SimError& e) {
     return e.code;
   }
-}
-
-extern "C" int scaph_host_max_zero_run(uint32_t mask) { return scaph::max_zero_run(mask); }
-extern "C" int scaph_host_fit_zero_run(uint32_t mask, int required) {
-  return scaph::fit_zero_run(mask, required);
-}
-extern "C" uint32_t scaph_host_set_run(uint32_t mask, int off, int number) {
-  return scaph::set_run(mask, off, number);
-}
-extern "C" uint32_t scaph_host_clear_run(uint32_t mask, int off, int number) {
-  return scaph::clear_run(mask, off, number);
 }

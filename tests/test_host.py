@@ -171,6 +171,15 @@ def test_chunk_masks_match_oracle():
                                                        0xFFFFFFFF)
 
 
+def _sim_lib():
+    """tests/native/libpagehandle_sim.so: the test-only scheduler driver."""
+    path = os.path.join(ROOT, "tests", "native", "libpagehandle_sim.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run __graft_entry__.build()")
+    _lib.host()
+    return ctypes.CDLL(path)
+
+
 @pytest.mark.parametrize("list_size,pool,thres,multi,seed", [
     (40, 8, 3, 1, 1), (40, 8, 3, 2, 2), (100, 10, 12, 1, 3), (16, 16, 4, 1, 4),
     (64, 3, 16, 3, 5), (12, 1, 8, 1, 6)])
@@ -178,7 +187,7 @@ def test_page_scheduler_invariants(list_size, pool, thres, multi, seed):
     """Every active page runs at least once per iteration; a page runs only
     when resident; chunk runs in shared slots never overlap and match the
     slot masks (simulated executor, random completion order)."""
-    L = _lib.host()
+    L = _sim_lib()
     L.scaph_host_pagehandle_simulate.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
                                                  ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
                                                  ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
