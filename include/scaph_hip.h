@@ -33,6 +33,7 @@ extern "C" {
 #define SCAPH_UNREACHED 0xFFFFFFFFu /* level of a vertex no path reaches */
 #define SCAPH_STATUS_NOT_READY 600  /* == hipErrorNotReady (stream/event busy) */
 #define SCAPH_STATUS_INVALID 1      /* == hipErrorInvalidValue */
+#define SCAPH_STATUS_SCRATCH_TOO_SMALL 0x5C500001 /* input beyond a no-alloc entry point's scratch */
 
 /* On-disk / in-memory page record: src/page.h:22-29 (Page<uint32>). */
 typedef struct scaph_page {
@@ -182,6 +183,24 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree,
                    unsigned long long m_in, uint32_t* q_out, scaph_level_counters* counters,
                    uint32_t cur_level, void* workspace, size_t workspace_bytes,
                    const scaph_push_tuning* tune, void* stream);
+
+/* scaph_bfs_push without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace is a fixed device array of scaph_bfs_push_noalloc_max_bytes()
+ * (64 MiB) that each device the library runs on holds until the process
+ * ends; the call itself allocates and frees nothing.  A frontier whose
+ * workspace (scaph_push_workspace_bytes(n_in, m_in)) exceeds it is refused
+ * with SCAPH_STATUS_SCRATCH_TOO_SMALL before anything is launched.  Every
+ * call on one device shares that array: calls that overlap on one device
+ * (other streams or host threads) race and must be serialised by the
+ * caller.  Outputs and semantics are those of scaph_bfs_push. */
+size_t scaph_bfs_push_noalloc_max_bytes(void);
+int scaph_bfs_push_noalloc(const scaph_graph_desc* out_graph, const uint32_t* degree,
+                           uint32_t* level, uint32_t* visited, const uint32_t* q_in, uint32_t n_in,
+                           unsigned long long m_in, uint32_t* q_out,
+                           scaph_level_counters* counters, uint32_t cur_level,
+                           const scaph_push_tuning* tune, void* stream);
 
 /* Bottom-up (pull) expand of one level: every unvisited vertex scans its
  * in-neighbours (transposed paged CSR) for one in the frontier bitmap.

@@ -160,6 +160,9 @@ def hip():
         _sig(lib, "scaph_bfs_push", I32,
              [ctypes.POINTER(GraphDesc), P, P, P, P, U32, ctypes.c_ulonglong, P, P, U32, P, SZ, P,
               P])
+        _sig(lib, "scaph_bfs_push_noalloc_max_bytes", SZ, [])
+        _sig(lib, "scaph_bfs_push_noalloc", I32,
+             [ctypes.POINTER(GraphDesc), P, P, P, P, U32, ctypes.c_ulonglong, P, P, U32, P, P])
         _sig(lib, "scaph_pull_workspace_bytes", SZ, [ctypes.c_ulonglong])
         _sig(lib, "scaph_bfs_pull", I32, [ctypes.POINTER(GraphDesc), P, P, U32, P, P, P, P, P, P,
                                           U32, ctypes.c_ulonglong, P, SZ, P, P])

@@ -16,6 +16,8 @@
 // 2048 edges (1024 on levels routed to XCDs by target), and every 256-thread
 // block expands whole tiles — a hub with a million out-edges is spread over
 // hundreds of blocks instead of one.
+#include <mutex>
+
 #include "common.h"
 
 namespace {
@@ -1542,6 +1544,73 @@ int push_runs(const scaph_page_run* runs, uint32_t num_runs, const scaph_page_ru
 
 }  // namespace
 
+// ------------------------------------------------- the no-alloc push level
+// scaph_bfs_push_noalloc takes its workspace from `scratch`, a file-scope
+// device array, instead of a caller buffer or an allocation.  Every device
+// the library runs on holds its own copy (kNoAllocScratchBytes) until the
+// process ends.  Host code never uses the array's name as an address: the
+// device address of each device's copy comes from hipGetSymbolAddress, once
+// per device with that device current (scaph_init, or the first call on a
+// device), and is kept per device.  Every call on one device shares that
+// copy, so calls that overlap on a device race; the push kernels initialise
+// the parts of the workspace they read before reading them.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// One push level from a frontier queue with the given workspace (the body of
+// scaph_bfs_push and scaph_bfs_push_noalloc).
+int push_queue(const scaph_graph_desc* out_graph, const uint32_t* degree, uint32_t* level,
+               uint32_t* visited, const uint32_t* q_in, uint32_t n_in, unsigned long long m_in,
+               uint32_t* q_out, scaph_level_counters* counters, uint32_t cur_level,
+               void* workspace, size_t workspace_bytes, const scaph_push_tuning* tune,
+               hipStream_t st) {
+  if (!out_graph || !degree || !level || !visited || !q_out || !counters || !workspace)
+    return SCAPH_STATUS_INVALID;
+  if (n_in == 0) return 0;
+  if (!q_in) return SCAPH_STATUS_INVALID;
+  const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
+  if (m_in <= small && n_in <= 65536) {
+    const unsigned grid = grid_for(n_in, kBlock / SCAPH_WAVE, 64);
+    hipLaunchKernelGGL(k_push_small, dim3(grid), dim3(kBlock), page_lds_bytes(out_graph->num_pages),
+                       st, *out_graph, q_in, n_in, degree, level, visited, q_out, counters,
+                       cur_level + 1);
+    SCAPH_LAUNCH_CHECK();
+    return 0;
+  }
+  PushWs w = carve_push_ws(workspace, n_in, m_in);
+  if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
+  const uint32_t nb = (n_in + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_prep_queue, dim3(nb), dim3(kBlock), page_lds_bytes(out_graph->num_pages), st,
+                     *out_graph, q_in, n_in,
+                     w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
+  SCAPH_LAUNCH_CHECK();
+  return launch_push_tail(w, nb, m_in, degree, level, visited, q_out, nullptr, counters,
+                          cur_level + 1, out_graph->num_vertices, OUT_QUEUE, tune, st);
+}
+
+}  // namespace
+
+// Device address of `scratch` on device `dev`, which must be current.
+int scaph_noalloc_scratch_setup(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return SCAPH_STATUS_INVALID;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!p) return SCAPH_STATUS_INVALID;
+    g_scratch_ptr[dev] = p;
+  }
+  if (out) *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 extern "C" {
 
 int scaph_bfs_init(uint32_t* level, uint32_t* visited, const uint32_t* static_visited, uint32_t n,
@@ -1577,29 +1646,32 @@ int scaph_bfs_push(const scaph_graph_desc* out_graph, const uint32_t* degree, ui
                    unsigned long long m_in, uint32_t* q_out, scaph_level_counters* counters,
                    uint32_t cur_level, void* workspace, size_t workspace_bytes,
                    const scaph_push_tuning* tune, void* stream) {
-  if (!out_graph || !degree || !level || !visited || !q_out || !counters || !workspace)
+  if (!workspace) return SCAPH_STATUS_INVALID;
+  return push_queue(out_graph, degree, level, visited, q_in, n_in, m_in, q_out, counters,
+                    cur_level, workspace, workspace_bytes, tune, as_stream(stream));
+}
+
+size_t scaph_bfs_push_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+int scaph_bfs_push_noalloc(const scaph_graph_desc* out_graph, const uint32_t* degree,
+                           uint32_t* level, uint32_t* visited, const uint32_t* q_in, uint32_t n_in,
+                           unsigned long long m_in, uint32_t* q_out,
+                           scaph_level_counters* counters, uint32_t cur_level,
+                           const scaph_push_tuning* tune, void* stream) {
+  if (!out_graph || !degree || !level || !visited || !q_out || !counters)
     return SCAPH_STATUS_INVALID;
   if (n_in == 0) return 0;
-  if (!q_in) return SCAPH_STATUS_INVALID;
-  hipStream_t st = as_stream(stream);
-  const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
-  if (m_in <= small && n_in <= 65536) {
-    const unsigned grid = grid_for(n_in, kBlock / SCAPH_WAVE, 64);
-    hipLaunchKernelGGL(k_push_small, dim3(grid), dim3(kBlock), page_lds_bytes(out_graph->num_pages),
-                       st, *out_graph, q_in, n_in, degree, level, visited, q_out, counters,
-                       cur_level + 1);
-    SCAPH_LAUNCH_CHECK();
-    return 0;
-  }
-  PushWs w = carve_push_ws(workspace, n_in, m_in);
-  if (w.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
-  const uint32_t nb = (n_in + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_prep_queue, dim3(nb), dim3(kBlock), page_lds_bytes(out_graph->num_pages), st,
-                     *out_graph, q_in, n_in,
-                     w.ent_ep, w.ent_len, w.blk_cnt, w.blk_edges);
-  SCAPH_LAUNCH_CHECK();
-  return launch_push_tail(w, nb, m_in, degree, level, visited, q_out, nullptr, counters,
-                          cur_level + 1, out_graph->num_vertices, OUT_QUEUE, tune, st);
+  // the frontier's workspace must fit the file-scope array: refused before
+  // anything is launched
+  if (carve_push_ws(nullptr, n_in, m_in).bytes > kNoAllocScratchBytes)
+    return SCAPH_STATUS_SCRATCH_TOO_SMALL;
+  int dev = 0;
+  SCAPH_TRY(hipGetDevice(&dev));
+  void* ws = nullptr;
+  const int rc = scaph_noalloc_scratch_setup(dev, &ws);
+  if (rc) return rc;
+  return push_queue(out_graph, degree, level, visited, q_in, n_in, m_in, q_out, counters,
+                    cur_level, ws, kNoAllocScratchBytes, tune, as_stream(stream));
 }
 
 int scaph_bfs_push_page(const uint32_t* page_buf, uint32_t nodenum, unsigned long long edgenum,

@@ -18,6 +18,10 @@
 
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Device address of the no-alloc push workspace on device `dev` (which must
+// be the current device), looked up once per device (bfs.hip).
+int scaph_noalloc_scratch_setup(int dev, void** out);
+
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // Number of set bits of `mask` below this lane (ballot prefix).

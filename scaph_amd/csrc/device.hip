@@ -14,13 +14,17 @@ extern "C" {
 
 const char* scaph_error_string(int status) {
   if (status == 0) return "success";
+  if (status == SCAPH_STATUS_SCRATCH_TOO_SMALL)
+    return "input too large for the no-alloc entry point's fixed scratch "
+           "(scaph_bfs_push_noalloc_max_bytes)";
   return hipGetErrorString(static_cast<hipError_t>(status));
 }
 
 int scaph_init(int device) {
   SCAPH_TRY(hipSetDevice(device));
   SCAPH_TRY(hipFree(nullptr));  // create the context now, not on first use
-  return 0;
+  // the no-alloc push's workspace address on this device
+  return scaph_noalloc_scratch_setup(device, nullptr);
 }
 
 int scaph_device_count(int* count) {
