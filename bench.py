@@ -10,13 +10,16 @@ Prints ONE JSON line:
   roofline     the dominant kernel (by device time): algorithmic bytes per
                launch / its average launch time (HIP events around it)
   bfs_hbm_frac B_alg / t_BFS / 8 TB/s with B_alg = 4 m_c + 12 n_c + 4 n
-  cpu_baseline the oracle's single-core BFS on this host, one source
+  cpu_baseline the oracle's single-core BFS on this host: config 1 in full
+               and the first --cpu-sources sources of this graph
 """
 import argparse
 import ctypes
+import hashlib
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -67,27 +70,99 @@ def alg_bytes_step(rec, n):
     return 16 * rec["n_f"] + 4 * rec["scanned"] + 8 * rec["n_next"]
 
 
-def cpu_baseline(eng, src, gpu_levels, budget_s):
-    """Oracle BFS (single core) on the same graph for one source."""
-    from tests import _oracle
-    off, ngh = eng.host_csr()
-    cpu = os.sched_getaffinity(0)
-    core = min(cpu)
-    os.sched_setaffinity(0, {core})
+KERNEL_SOURCES = ("scaph_amd/csrc/bfs.hip", "scaph_amd/csrc/common.h", "include/scaph_hip.h")
+
+
+def kernel_source_hash():
+    """SHA-256 over the sources of the BFS kernels: a traffic profile is used
+    only by the build it was measured on."""
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def device_name(props):
+    """The marketing name rocminfo reports for the GPU (the runtime's name
+    field is empty on some drivers), else the architecture."""
+    if props.get("name"):
+        return props["name"]
     try:
-        t0 = time.perf_counter()
-        lv = _oracle.bfs(off, ngh, src)
-        dt = time.perf_counter() - t0
+        out = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=60).stdout
+        for line in out.splitlines():
+            line = line.strip()
+            if line.startswith("Marketing Name:") and "Instinct" in line:
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return props.get("arch", "")
+
+
+def host_cpu():
+    """lscpu's model name and the core counts of this host."""
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "cores_allowed": len(os.sched_getaffinity(0))}
+
+
+def _timed_oracle_bfs(off, ngh, src, runs):
+    """Oracle BFS (the reference's check(), app/bfs.cu:67-117) on one pinned
+    core: median seconds of `runs` runs and the levels."""
+    from tests import _oracle
+    cpu = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {min(cpu)})
+    try:
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            lv = _oracle.bfs(off, ngh, src)
+            ts.append(time.perf_counter() - t0)
     finally:
         os.sched_setaffinity(0, cpu)
+    return statistics.median(ts), lv
+
+
+def cpu_baseline(eng, srcs, runs):
+    """The oracle's single-core BFS timed on this host:
+    config 1 (R-MAT s20 ef16 seed 1, highest out-degree source) in full, and
+    the first len(srcs) sources of this graph, each the median of `runs`
+    runs.  `value` is the GTEPS of this graph's sample; the levels of every
+    timed source are compared with the GPU's."""
+    from tests import _oracle
+    c1_off, c1_ngh = _oracle.rmat_csr(20, 16, seed=1)
+    c1_deg = np.diff(c1_off.astype(np.int64))
+    c1_src = int(np.argmax(c1_deg))
+    c1_t, c1_lv = _timed_oracle_bfs(c1_off, c1_ngh, c1_src, 5)
+    c1_m = int(c1_deg[c1_lv != 0xFFFFFFFF].sum())
+    del c1_off, c1_ngh
+    off, ngh = eng.host_csr()
     deg = np.diff(off.astype(np.int64))
-    reached = lv != 0xFFFFFFFF
-    m_c = int(deg[reached].sum())
-    parity = bool(np.array_equal(lv, gpu_levels))
-    return {"value": m_c / dt / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
-            "sample": f"1 BFS (source {src}) of the same graph; {dt:.1f} s; "
-                      f"levels bit-exact vs GPU: {parity}",
-            "seconds": dt}, parity
+    per, parity = [], True
+    for s in srcs:
+        t, lv = _timed_oracle_bfs(off, ngh, s, runs)
+        eng.run(s)
+        parity &= bool(np.array_equal(lv, eng.levels()))
+        per.append({"src": s, "seconds": t, "m_c": int(deg[lv != 0xFFFFFFFF].sum())})
+    m_tot = sum(x["m_c"] for x in per)
+    t_tot = sum(x["seconds"] for x in per)
+    return {"value": m_tot / t_tot / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"oracle BFS on 1 pinned core: the first {len(srcs)} of the 64 sources of "
+                      f"this graph, median of {runs} run(s) each ({t_tot:.1f} s); config 1 "
+                      f"(R-MAT s20 ef16 seed 1, source {c1_src}) in full, median of 5; levels "
+                      f"bit-exact vs GPU: {parity}",
+            "per_source": per,
+            "config1": {"src": c1_src, "seconds": c1_t, "m_c": c1_m,
+                        "gteps": c1_m / c1_t / 1e9, "runs": 5},
+            "host": host_cpu()}, parity
 
 
 def bench_msbfs(scale, ef, queries, seed):
@@ -229,7 +304,10 @@ def main():
                     help="skip the learned-schedule pass")
     ap.add_argument("--learned-check", type=int, default=8,
                     help="sources whose levels are compared between the schedules")
-    ap.add_argument("--cpu-budget", type=float, default=60.0)
+    ap.add_argument("--cpu-sources", type=int, default=1,
+                    help="sources of this graph the CPU baseline times (BASELINE.md: 4)")
+    ap.add_argument("--cpu-runs", type=int, default=1,
+                    help="runs per CPU-baseline source, median taken (BASELINE.md: 5)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -310,18 +388,23 @@ def main():
     avg_bytes = bytes_tot / max(launches, 1)
     achieved = avg_bytes / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
+    traffic_src = None
     tfile = os.path.join(ROOT, "profiles", "bfs_s26_ef32_k_pull_traffic.json")
     if dom == 1 and os.path.exists(tfile) and args.scale == 26 and args.ef == 32:
         # HBM-side bytes per k_pull launch from rocprofv3 FETCH_SIZE/WRITE_SIZE
-        # passes over this same command (separate runs; calibrated by a copy)
+        # passes over this workload, taken only when they were measured on
+        # the kernel sources of this build
         with open(tfile) as f:
-            traffic = json.load(f).get("traffic_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("kernel_source_hash") == kernel_source_hash():
+            traffic = tj.get("traffic_bytes_per_launch")
+            traffic_src = os.path.relpath(tfile, ROOT)
     # the profiled bytes over this run's launch time: how hard the kernel
     # drives the memory side, beside the algorithmic figure above (counters
     # include Infinity Cache hits, so this bounds HBM bandwidth from above)
     t_gbps = traffic / (avg_ms / 1e3) / 1e9 if traffic and avg_ms > 0 else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                "frac": achieved / 8000.0, "traffic": traffic,
+                "frac": achieved / 8000.0, "traffic": traffic, "traffic_profile": traffic_src,
                 "traffic_achieved": t_gbps,
                 "traffic_frac": t_gbps / 8000.0 if t_gbps else None,
                 "kernel": "k_pull" if dom == 1 else "push (k_prep_queue+k_scan_blocks+k_mark_tiles+k_expand)",
@@ -374,10 +457,7 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu:
-        src0 = srcs[0]
-        eng.run(src0)
-        gl = eng.levels()
-        cpu, parity = cpu_baseline(eng, src0, gl, args.cpu_budget)
+        cpu, parity = cpu_baseline(eng, srcs[:args.cpu_sources], args.cpu_runs)
         log("cpu baseline", cpu)
 
     batched = None
@@ -414,7 +494,10 @@ def main():
         "dtype": "u32",
         "data": "synthetic R-MAT built on the device (SplitMix64 stream, seeded relabelling)",
         "config": {"workload": f"BFS R-MAT scale {args.scale} edge factor {args.ef} "
-                               f"(a,b,c)=(.57,.19,.19) seed {args.seed}, {len(srcs)} sources",
+                               f"(a,b,c)=(.57,.19,.19) seed {args.seed}; {len(srcs)} "
+                               f"giant-component sources, {args.steps} timed BFS runs over "
+                               f"sources 0..{min(args.steps, len(srcs)) - 1} in order",
+                   "sources": len(srcs), "sources_timed": min(args.steps, len(srcs)),
                    "vertices": n, "edges": info["m"], "pages": info["pages"],
                    "schedule": args.schedule, "alpha": 14, "beta": 24},
         "roofline": roofline,
@@ -428,7 +511,7 @@ def main():
         "gteps_mean_of_runs": sum(r["m_reached"] for r in runs) / sum(r["ms"] for r in runs) / 1e6,
         "levels_exact_vs_oracle": parity,
         "invariant_violations": list(viol),
-        "device": props["name"],
+        "device": device_name(props),
     }
     print(json.dumps(out), flush=True)
 

@@ -35,6 +35,7 @@ typedef struct scaph_bfs_stats {
   uint64_t pages_compacted;   /* compacted (cold) page copies */
   uint64_t bytes_h2d;         /* host-to-device page bytes */
   double h2d_ms;              /* copy-stream busy time */
+  uint64_t pages_prefetched;  /* whole-page copies made ahead for the next level */
 } scaph_bfs_stats;
 
 typedef struct scaph_level_record {
@@ -45,6 +46,13 @@ typedef struct scaph_level_record {
   float ms;        /* step time including host turnaround */
   float kernel_ms; /* expand kernels of the step alone (events around them) */
   float page_hist[12];
+  /* paged mode (zero when resident): pages with active vertices, those
+   * already in the pool at the level's start, whole-page and compacted
+   * copies, pages copied ahead for the next level, host-to-device bytes and
+   * the copy stream's busy time in this level */
+  uint32_t pages_active, pages_cached, pages_full, pages_compacted, pages_prefetched, pad;
+  uint64_t bytes_h2d;
+  float h2d_ms, pad2;
 } scaph_level_record;
 
 /* App(argc, argv): flags in gflags syntax (argv[0] is the program name). */
@@ -82,6 +90,13 @@ int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* 
                      size_t errlen);
 int scaph_engine_set_schedule(scaph_engine* e, const char* name, double alpha, double beta,
                               char* err, size_t errlen);
+/* Paged mode with -page_predict=exact: the u32 levels (n of them) of an
+ * earlier BFS from the source about to run, so that the page cache knows
+ * every page's future activity (the bound for any page-activity
+ * predictor; the levels computed are unaffected).  No reference
+ * counterpart. */
+int scaph_engine_set_page_levels(scaph_engine* e, const uint32_t* levels, uint32_t n, char* err,
+                                 size_t errlen);
 /* The reference's other App plugins on the resident graph (app/wcc.cu,
  * app/sssp.cu): values until no update, read with scaph_engine_levels.
  * WCC: value[v] = the least vertex id reaching v (v itself included), as

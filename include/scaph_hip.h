@@ -114,6 +114,11 @@ int scaph_copy_d2h_async(void* dst, const void* src, size_t bytes, void* stream)
 int scaph_copy_d2d_async(void* dst, const void* src, size_t bytes, void* stream);
 /* Fill n u32 words with value (a kernel, safe to capture in a hipGraph). */
 int scaph_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* stream);
+/* *out = the sum of src[0..n) (u64), every word read once.  With
+ * scaph_fill_u32 it gives kernels of known read and write bytes, against
+ * which rocprof's FETCH_SIZE / WRITE_SIZE are scaled.  No reference
+ * counterpart. */
+int scaph_checksum_u32(const uint32_t* src, size_t n, unsigned long long* out, void* stream);
 /* Host memory that kernels write directly (pinned, mapped, coherent):
  * *host_ptr for the CPU, *device_ptr for kernels; free with
  * scaph_host_pinned_free(host_ptr). */
@@ -250,6 +255,14 @@ int scaph_frontier_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t*
 int scaph_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_t* front,
                       const uint32_t* degree, int weighted, uint32_t* nodes, uint32_t* datas,
                       unsigned long long* totals, void* stream);
+
+/* Per-page count of vertices still reachable by a later level: out[p] =
+ * #v in [left, right] set in live (NULL = every vertex) and clear in the
+ * visited bitmap.  Feeds the page-activity prediction of the paged
+ * scheduler (a page whose count is 0 is never active again).  No reference
+ * counterpart. */
+int scaph_page_unvisited(const scaph_page* pages, uint32_t num_pages, const uint32_t* visited,
+                         const uint32_t* live, uint32_t* out, void* stream);
 
 /* What scaph_page_counts already says about one page push (optional).
  * first_vertex: Node i of a full page is vertex first_vertex + i (the

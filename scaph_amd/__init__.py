@@ -145,6 +145,14 @@ class BFS:
         _lib.check(self._lib.scaph_engine_set_schedule(self._h, name.encode(), alpha, beta, e,
                                                        1024), e.value.decode())
 
+    def set_page_levels(self, levels):
+        """-page_predict=exact: the levels of an earlier BFS from the next
+        source, so the paged scheduler knows each page's future activity."""
+        lv = np.ascontiguousarray(levels, dtype=np.uint32)
+        e = _err()
+        _lib.check(self._lib.scaph_engine_set_page_levels(self._h, _ptr(lv), lv.size, e, 1024),
+                   e.value.decode())
+
     def set_model(self, params_bf16, mean, scale, prefetch_gain=0.0):
         p = np.ascontiguousarray(params_bf16, dtype=np.uint16)
         m = np.ascontiguousarray(mean, dtype=np.float32)
@@ -160,7 +168,8 @@ class BFS:
         self._lib.scaph_engine_level_records(self._h, arr, cnt.value, ctypes.byref(cnt))
         out = []
         for r in arr[:cnt.value]:
-            d = {k: getattr(r, k) for k, _ in _lib.LevelRecord._fields_ if k != "page_hist"}
+            d = {k: getattr(r, k) for k, _ in _lib.LevelRecord._fields_
+                 if k not in ("page_hist", "pad", "pad2")}
             d["page_hist"] = list(r.page_hist)
             out.append(d)
         return out

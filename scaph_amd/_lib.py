@@ -93,7 +93,8 @@ class BfsStats(ctypes.Structure):
                 ("n_reached", ctypes.c_uint64), ("m_reached", ctypes.c_uint64),
                 ("page_execs", ctypes.c_uint64), ("page_execs_cached", ctypes.c_uint64),
                 ("pages_full", ctypes.c_uint64), ("pages_compacted", ctypes.c_uint64),
-                ("bytes_h2d", ctypes.c_uint64), ("h2d_ms", ctypes.c_double)]
+                ("bytes_h2d", ctypes.c_uint64), ("h2d_ms", ctypes.c_double),
+                ("pages_prefetched", ctypes.c_uint64)]
 
 
 class LevelRecord(ctypes.Structure):
@@ -102,7 +103,12 @@ class LevelRecord(ctypes.Structure):
                 ("n_next", ctypes.c_uint64), ("m_next", ctypes.c_uint64),
                 ("scanned", ctypes.c_uint64), ("examined", ctypes.c_uint64),
                 ("ms", ctypes.c_float), ("kernel_ms", ctypes.c_float),
-                ("page_hist", ctypes.c_float * 12)]
+                ("page_hist", ctypes.c_float * 12),
+                ("pages_active", ctypes.c_uint32), ("pages_cached", ctypes.c_uint32),
+                ("pages_full", ctypes.c_uint32), ("pages_compacted", ctypes.c_uint32),
+                ("pages_prefetched", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("bytes_h2d", ctypes.c_uint64), ("h2d_ms", ctypes.c_float),
+                ("pad2", ctypes.c_float)]
 
 
 _hip = None
@@ -154,6 +160,7 @@ def hip():
         for n in ("scaph_copy_h2d_async", "scaph_copy_d2h_async", "scaph_copy_d2d_async"):
             _sig(lib, n, I32, [P, P, SZ, P])
         _sig(lib, "scaph_fill_u32", I32, [P, U32, SZ, P])
+        _sig(lib, "scaph_checksum_u32", I32, [P, SZ, P, P])
         _sig(lib, "scaph_bfs_init", I32, [P, P, P, U32, U32, P, P, P])
         _sig(lib, "scaph_bitmap_from_zero_degree", I32, [P, P, U32, P])
         _sig(lib, "scaph_push_workspace_bytes", SZ, [U32, ctypes.c_ulonglong])
@@ -170,6 +177,7 @@ def hip():
         _sig(lib, "scaph_frontier_queue_to_bitmap", I32, [P, U32, P, U32, P])
         _sig(lib, "scaph_frontier_bitmap_to_queue", I32, [P, U32, P, P, P])
         _sig(lib, "scaph_page_counts", I32, [P, U32, P, P, I32, P, P, P, P])
+        _sig(lib, "scaph_page_unvisited", I32, [P, U32, P, P, P, P])
         _sig(lib, "scaph_bfs_push_page", I32,
              [P, U32, ctypes.c_ulonglong, U32, P, P, P, P, P, P, U32, P, SZ, P, P, P])
         _sig(lib, "scaph_xcc_ids", I32, [P, U32, P])
@@ -224,6 +232,7 @@ def host():
         _sig(lib, "scaph_engine_sssp", I32, [E, U32, ctypes.POINTER(BfsStats), CHARP, SZ])
         _sig(lib, "scaph_engine_set_weights", I32, [E, P, U64, CHARP, SZ])
         _sig(lib, "scaph_engine_set_schedule", I32, [E, CHARP, DBL, DBL, CHARP, SZ])
+        _sig(lib, "scaph_engine_set_page_levels", I32, [E, P, U32, CHARP, SZ])
         _sig(lib, "scaph_engine_set_model", I32, [E, P, P, P, ctypes.c_float, CHARP, SZ])
         _sig(lib, "scaph_engine_level_records", I32,
              [E, ctypes.POINTER(LevelRecord), U32, ctypes.POINTER(U32)])

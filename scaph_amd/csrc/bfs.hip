@@ -1318,6 +1318,38 @@ k_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_t* front
 }
 
 
+// Per-page count of vertices that can still become active: set in `live`
+// (NULL: every vertex) and clear in `visited`.  A page whose count is 0
+// holds no vertex any later level can reach.
+__global__ void __launch_bounds__(kBlock)
+k_page_unvisited(const scaph_page* pages, uint32_t num_pages, const uint32_t* visited,
+                 const uint32_t* live, uint32_t* out) {
+  __shared__ uint32_t red[kBlock / SCAPH_WAVE];
+  const uint32_t wid = threadIdx.x / SCAPH_WAVE;
+  for (uint32_t p = blockIdx.x; p < num_pages; p += gridDim.x) {
+    const uint32_t left = pages[p].left, right = pages[p].right;
+    uint32_t c = 0;
+    if (pages[p].nodenum) {
+      const uint32_t w0 = left >> 5, w1 = right >> 5;
+      for (uint32_t w = w0 + threadIdx.x; w <= w1; w += kBlock) {
+        uint32_t bits = ~visited[w] & (live ? live[w] : ~0u);
+        if (w == w0) bits &= ~0u << (left & 31);
+        if (w == w1) bits &= (right & 31) == 31 ? ~0u : ((1u << ((right & 31) + 1)) - 1u);
+        c += __popc(bits);
+      }
+    }
+    c = wave_sum(c);
+    if (lane_id() == 0) red[wid] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (uint32_t w = 0; w < kBlock / SCAPH_WAVE; ++w) t += red[w];
+      out[p] = t;
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------ validation
 
 __global__ void __launch_bounds__(kBlock)
@@ -1834,6 +1866,16 @@ int scaph_page_counts(const scaph_page* pages, uint32_t num_pages, const uint32_
   if (num_pages == 0) return 0;
   hipLaunchKernelGGL(k_page_counts, dim3(grid_for(num_pages, 1, 4096)), dim3(kBlock), 0, st, pages,
                      num_pages, front, degree, weighted, nodes, datas, totals);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_page_unvisited(const scaph_page* pages, uint32_t num_pages, const uint32_t* visited,
+                         const uint32_t* live, uint32_t* out, void* stream) {
+  if (num_pages == 0) return 0;
+  if (!pages || !visited || !out) return SCAPH_STATUS_INVALID;
+  hipLaunchKernelGGL(k_page_unvisited, dim3(grid_for(num_pages, 1, 4096)), dim3(kBlock), 0,
+                     as_stream(stream), pages, num_pages, visited, live, out);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

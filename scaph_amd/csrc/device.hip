@@ -145,6 +145,41 @@ extern "C" int scaph_fill_u32(uint32_t* dst, uint32_t value, size_t n, void* str
   return 0;
 }
 
+// Sum of n u32 words read once each, in 16-byte loads: a kernel of known
+// read bytes (4 n) against which the memory-side read counters are scaled.
+__global__ void __launch_bounds__(256) k_checksum_u32(const uint32_t* src, size_t n,
+                                                      unsigned long long* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t n4 = ((reinterpret_cast<uintptr_t>(src) & 15) == 0) ? n / 4 : 0;
+  const uint4* s4 = reinterpret_cast<const uint4*>(src);
+  unsigned long long acc = 0;
+  for (size_t j = i; j < n4; j += stride) {
+    const uint4 v = s4[j];
+    acc += (unsigned long long)v.x + v.y + v.z + v.w;
+  }
+  for (size_t j = n4 * 4 + i; j < n; j += stride) acc += src[j];
+  acc = wave_sum(acc);
+  if (lane_id() == 0 && acc) atomicAdd(out, acc);
+}
+
+__global__ void k_zero_one_u64(unsigned long long* p) { *p = 0; }
+
+extern "C" int scaph_checksum_u32(const uint32_t* src, size_t n, unsigned long long* out,
+                                  void* stream) {
+  if (!out || (n && !src)) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_zero_one_u64, dim3(1), dim3(1), 0, st, out);
+  SCAPH_LAUNCH_CHECK();
+  if (n == 0) return 0;
+  size_t blocks = (n / 4 + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_checksum_u32, dim3((unsigned)blocks), dim3(256), 0, st, src, n, out);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
 // Copies `words` u32 from device memory into host-mapped memory, clears
 // the first `clear_words` of the source, then stores `seq` into the host
 // flag after a system-scope fence: the host sees the flag only once the data

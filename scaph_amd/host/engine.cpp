@@ -83,6 +83,8 @@ Config& global_flags() {
     r.add("pull_head", FlagType::kBool, &c.pull_head,
           "keep each in-list's first entries beside its in-degree for the pull probes");
     r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");
+    r.add("page_predict", FlagType::kString, &c.page_predict,
+          "paged: next-level page-activity prediction: off|learned|exact");
   }
   return c;
 }
@@ -131,6 +133,10 @@ bool parse_config(const std::vector<std::string>& args, Config* out,
   }
   if (g.alpha <= 0 || g.beta <= 0) {
     if (err) *err = "ERROR: -alpha and -beta must be positive";
+    return false;
+  }
+  if (g.page_predict != "off" && g.page_predict != "learned" && g.page_predict != "exact") {
+    if (err) *err = "ERROR: -page_predict must be off, learned or exact";
     return false;
   }
   if (g.schedule != "fixed" && g.schedule != "push" && g.schedule != "pull" &&

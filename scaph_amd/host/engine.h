@@ -61,6 +61,9 @@ struct Config {
   int32_t page_groups = 8;       // paged: cached active pages of a level go out in about this
                                  // many grouped launches (0: one launch per page)
   int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
+  std::string page_predict = "off";  // paged: next-level page-activity prediction for slot reuse,
+                                     // look-ahead copies and whole-page copies (off | learned |
+                                     // exact: from set_page_levels, the bound for any predictor)
 };
 
 // Registers the flags above (once) against a process-wide Config.
@@ -82,6 +85,13 @@ struct LevelRecord {
   float ms = 0;                         // step time incl. host turnaround
   float kernel_ms = 0;                  // expand kernels alone
   std::array<float, 12> page_hist{};    // histogram of per-page active counts
+  // paged mode: pages active / already cached at the level's start, whole
+  // and compacted copies, copies made ahead for the next level, H2D bytes
+  // and copy-stream busy time
+  uint32_t pages_active = 0, pages_cached = 0, pages_full = 0, pages_compacted = 0;
+  uint32_t pages_prefetched = 0;
+  uint64_t bytes_h2d = 0;
+  float h2d_ms = 0;
 };
 
 struct RunStats {
@@ -93,6 +103,7 @@ struct RunStats {
   std::vector<LevelRecord> levels;
   // paged (out-of-core) mode
   uint64_t page_execs = 0, page_execs_cached = 0, pages_full = 0, pages_compacted = 0;
+  uint64_t pages_prefetched = 0;  // copies made ahead for the next level
   uint64_t bytes_h2d = 0;
   double h2d_ms = 0;
 };
@@ -147,6 +158,10 @@ class Engine {
   std::array<uint64_t, 2> validate(uint32_t src);
 
   void set_schedule(SchedFn fn) { sched_ = std::move(fn); }
+  // Paged mode, -page_predict=exact: the levels of a previous BFS from the
+  // source about to run (u32, unreached = 0xFFFFFFFF).  The scheduler then
+  // knows every page's future activity: the bound no predictor can beat.
+  void set_page_levels(const uint32_t* levels);
   // Learned schedule on the device: bf16 MLP parameters and the feature
   // normalisation; used by the level loop when -schedule=learned.
   void set_device_model(const uint16_t* params, const float* mean, const float* scale);
@@ -214,6 +229,8 @@ class Engine {
   std::unique_ptr<DeviceModel> model_;
   void ensure_model_buffers();
   RunStats bfs_paged(uint32_t src);
+  bool predict_launch(uint32_t cur, uint64_t n_f, uint64_t m_f, uint64_t m_u);
+  void predict_apply(uint32_t cur, bool learned);
   void page_histogram(const uint32_t* nodes, LevelRecord* rec);
   // WCC / SSSP rounds in paged mode: the value array, and the weight pool
   // (same slots as the page pool) for SSSP

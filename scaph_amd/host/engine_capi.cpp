@@ -185,6 +185,16 @@ int scaph_engine_info(scaph_engine* e, uint32_t* n, uint64_t* m, uint32_t* num_p
   return 0;
 }
 
+int scaph_engine_set_page_levels(scaph_engine* e, const uint32_t* levels, uint32_t n, char* err,
+                                 size_t errlen) {
+  if (!e || !levels) return 1;
+  return guarded(err, errlen, [&] {
+    if (n != e->eng->num_vertices())
+      throw scaph::Error(SCAPH_STATUS_INVALID, "one level per vertex expected");
+    e->eng->set_page_levels(levels);
+  });
+}
+
 int scaph_engine_pool(scaph_engine* e, int64_t* slots, uint64_t* device_bytes) {
   if (!e) return 1;
   if (slots) *slots = e->eng->pool_size();
@@ -210,6 +220,7 @@ int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* 
       out->pages_compacted = e->last.pages_compacted;
       out->bytes_h2d = e->last.bytes_h2d;
       out->h2d_ms = e->last.h2d_ms;
+      out->pages_prefetched = e->last.pages_prefetched;
     }
   });
 }
@@ -228,6 +239,7 @@ void fill_stats(const scaph::RunStats& r, scaph_bfs_stats* out) {
   out->pages_compacted = r.pages_compacted;
   out->bytes_h2d = r.bytes_h2d;
   out->h2d_ms = r.h2d_ms;
+  out->pages_prefetched = r.pages_prefetched;
 }
 }  // namespace
 
@@ -299,6 +311,15 @@ int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_
     out[i].ms = lv[i].ms;
     out[i].kernel_ms = lv[i].kernel_ms;
     for (int j = 0; j < 12; ++j) out[i].page_hist[j] = lv[i].page_hist[j];
+    out[i].pages_active = lv[i].pages_active;
+    out[i].pages_cached = lv[i].pages_cached;
+    out[i].pages_full = lv[i].pages_full;
+    out[i].pages_compacted = lv[i].pages_compacted;
+    out[i].pages_prefetched = lv[i].pages_prefetched;
+    out[i].pad = 0;
+    out[i].bytes_h2d = lv[i].bytes_h2d;
+    out[i].h2d_ms = lv[i].h2d_ms;
+    out[i].pad2 = 0;
   }
   if (count) *count = (uint32_t)lv.size();
   return 0;

@@ -133,6 +133,32 @@ void Engine::setup_paged() {
     ps.h_runs.alloc(np * sizeof(scaph_page_run) + 64);
     ps.d_runs.alloc(np * sizeof(scaph_page_run) + 64);
   }
+  ps.d_unv.alloc(np * 4 + 64);
+  ps.h_unv.alloc(np * 4 + 64);
+  if (cfg_.page_predict == "learned") {
+    // vertices with in-edges: only they can become active after the source
+    const size_t words = bitmap_words(n_);
+    std::vector<std::atomic<uint32_t>> live(words);
+    for (auto& w : live) w.store(0, std::memory_order_relaxed);
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const uint64_t m = host_.num_edges();
+    for (unsigned t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        for (uint64_t k = m * t / nt; k < m * (t + 1) / nt; ++k) {
+          const uint32_t v = host_.ngh[k];
+          const uint32_t bit = 1u << (v & 31);
+          if (!(live[v >> 5].load(std::memory_order_relaxed) & bit))
+            live[v >> 5].fetch_or(bit, std::memory_order_relaxed);
+        }
+      });
+    for (auto& x : th) x.join();
+    std::vector<uint32_t> h(words);
+    for (size_t w = 0; w < words; ++w) h[w] = live[w].load(std::memory_order_relaxed);
+    ps.live.alloc(words * 4);
+    H_ERR(scaph_copy_h2d_async(ps.live.get(), h.data(), words * 4, main_->get()));
+    main_->sync();
+  }
   ps.h_nodes.alloc(np * 4 + 64);
   ps.h_datas.alloc(np * 4 + 64);
   ps.h_totals.alloc(64);
@@ -189,6 +215,7 @@ RunStats Engine::bfs_paged(uint32_t src) {
     H_ERR(scaph_copy_d2h_async(ps.h_totals.get(), totals_.get(), 16, st));
     H_ERR(scaph_fill_u32(front_[fa ^ 1].get<uint32_t>(), 0u, words, st));
     H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, sizeof(scaph_level_counters) / 4, st));
+    const bool learned_pages = predict_launch(cur, n_f, m_f, m_u);
     main_->sync();
     const uint32_t* nodes = ps.h_nodes.get<uint32_t>();
     if (ps.h_totals.get<unsigned long long>()[1] == 0) break;
@@ -200,7 +227,21 @@ RunStats Engine::bfs_paged(uint32_t src) {
     rec.m_u = m_u;
     page_histogram(nodes, &rec);
 
+    predict_apply(cur, learned_pages);
+    const uint64_t full0 = rs.pages_full, comp0 = rs.pages_compacted, pre0 = rs.pages_prefetched;
+    const uint64_t h2d0 = rs.bytes_h2d;
+    const double h2d_ms0 = rs.h2d_ms;
+    for (size_t p = 0; p < np; ++p) {
+      if (!nodes[p]) continue;
+      ++rec.pages_active;
+      if (ps.handle.vp[p].cached) ++rec.pages_cached;
+    }
     schedule_pages(fa, cur, nullptr, &rs);
+    rec.pages_full = (uint32_t)(rs.pages_full - full0);
+    rec.pages_compacted = (uint32_t)(rs.pages_compacted - comp0);
+    rec.pages_prefetched = (uint32_t)(rs.pages_prefetched - pre0);
+    rec.bytes_h2d = rs.bytes_h2d - h2d0;
+    rec.h2d_ms = (float)(rs.h2d_ms - h2d_ms0);
     H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
     main_->sync();
     rec.n_next = hc->n_next;
@@ -243,11 +284,12 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   const uint32_t* nodes = ps.h_nodes.get<uint32_t>();
   const uint32_t* datas = ps.h_datas.get<uint32_t>();
   // cold pages need the frontier on the host for compaction
-  const uint64_t thres = ps.thres;
   const bool weighted = relax && relax->weights;
+  if (relax) ps.handle.set_prediction({}, 0);
+  const std::vector<char> whole = ps.handle.whole_copies(datas);
   std::vector<int> cold;
   for (size_t p = 0; p < np; ++p)
-    if (!ps.handle.vp[p].cached && datas[p] > 0 && datas[p] <= thres) cold.push_back((int)p);
+    if (!ps.handle.vp[p].cached && datas[p] > 0 && !whole[p]) cold.push_back((int)p);
   if (!cold.empty()) {
     if (cold.size() * 4 > np) {
       H_ERR(scaph_copy_d2h_async(ps.h_front.get(), front_[fa].get(), words * 4, st));
@@ -281,7 +323,7 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   }
 
   // compaction workers (parallel_compress_stat_async, src/compress.h:42-56)
-  std::atomic<size_t> next{0};
+  std::atomic<size_t> next{0}, done{0};
   std::atomic<bool> failed{false};
   Workers workers;
   const int nw = (int)std::min<size_t>(cold.size(), 8);
@@ -306,6 +348,7 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
         } catch (...) {
           failed = true;
         }
+        ++done;
       }
     });
   }
@@ -314,6 +357,7 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   // scheduling loop (gpu_push_batch, src/app.cuh:414-459), event driven
   std::vector<int> last(ns, -1);
   int copy_last = -1;
+  bool copy_ahead = false;  // copy_last is a look-ahead copy for the next level
   float copy_ms = 0;
   bool copy_timed = false;
   while (!ps.handle.finished()) {
@@ -412,12 +456,31 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
     if (ps.copy_end->done()) {
       if (copy_last != -1) {
         if (copy_timed) copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
-        ps.handle.release_copy_finished(copy_last);
+        if (copy_ahead) ps.handle.release_lookahead_finished(copy_last);
+        else ps.handle.release_copy_finished(copy_last);
         copy_last = -1;
         progress = true;
       }
       int s;
-      if (ps.handle.get_empty(&s)) {
+      if (!relax && ps.handle.predicting() && done.load() == cold.size() &&
+          ps.handle.merge_pending() == 0 && ps.handle.launches_queued() &&
+          ps.handle.get_lookahead(datas, &s)) {
+        // the link is idle for this level while page launches still wait:
+        // copy a page the next level is expected to need into a free slot
+        // no later level needs
+        const uint64_t w = page_used_words(pages_out_[s], false);
+        ps.copy_beg->record(ps.copy->get());
+        H_ERR(scaph_copy_h2d_async(ps.pool.get<uint32_t>() + (uint64_t)ps.handle.vp[s].dpid * P_,
+                                   ps.h_page[s].get(), w * 4, ps.copy->get()));
+        ps.copy_end->record(ps.copy->get());
+        copy_timed = true;
+        rs->bytes_h2d += w * 4;
+        ++rs->pages_prefetched;
+        copy_last = s;
+        copy_ahead = true;
+        progress = true;
+      } else if (ps.handle.get_empty(&s)) {
+        copy_ahead = false;
         // PageCopy (src/app.cuh:371-393): a compacted run into its chunks,
         // or the whole page into its slot
         const VPstat& v = ps.handle.vp[s];
@@ -455,9 +518,110 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   ps.copy->sync();
   if (copy_last != -1) {
     copy_ms += Event::elapsed_ms(*ps.copy_beg, *ps.copy_end);
-    ps.handle.release_copy_finished(copy_last);
+    if (copy_ahead) ps.handle.release_lookahead_finished(copy_last);
+    else ps.handle.release_copy_finished(copy_last);
   }
   rs->h2d_ms += copy_ms;
+}
+
+// Page-activity prediction for level `cur` (whose frontier has n_f vertices
+// and m_f out-edges, m_u edges still unexplored), handed to the page cache
+// before its queue_reset:
+//  * learned: the model's second output (the fraction of pages active at
+//    the next level, scaph_mlp_decide on this frontier) picks that many
+//    pages, those with the most vertices a later level can still reach;
+//    pages with none are never needed again;
+//  * exact: every page's next active level from set_page_levels (Belady's
+//    order: the page needed furthest ahead is reused first) and its data at
+//    the next level, so a page that level will copy whole is copied whole
+//    now — what no predictor can improve on.
+// predict_launch queues the learned prediction's kernels and copies on the
+// main stream ahead of the level's one synchronisation (true when it did);
+// predict_apply turns the results into the page cache's prediction.
+bool Engine::predict_launch(uint32_t cur, uint64_t n_f, uint64_t m_f, uint64_t m_u) {
+  if (cfg_.page_predict != "learned" || !device_model_ready()) return false;
+  PagedState& ps = *paged_;
+  const size_t np = pages_out_.size();
+  ensure_model_buffers();
+  DeviceModel& m = *model_;
+  void* st = main_->get();
+  H_ERR(scaph_page_unvisited(dpages_out_.get<scaph_page>(), (uint32_t)np, visited_.get<uint32_t>(),
+                             ps.live.get<uint32_t>(), ps.d_unv.get<uint32_t>(), st));
+  H_ERR(scaph_copy_d2h_async(ps.h_unv.get(), ps.d_unv.get(), np * 4, st));
+  auto* hcc = m.hccnt.get<scaph_level_counters>();
+  std::memset(hcc, 0, sizeof(*hcc));
+  hcc->n_next = n_f;
+  hcc->m_next = m_f;
+  H_ERR(scaph_copy_h2d_async(m.ccnt.get(), hcc, sizeof(*hcc), st));
+  // the page counts of this frontier are in page_nodes_ (already queued for
+  // the host; the decision kernel reads and clears them)
+  H_ERR(scaph_mlp_decide(m.params.get<uint16_t>(), m.norm.get<float>(),
+                         page_nodes_.get<uint32_t>(), dpages_out_.get<scaph_page>(), (uint32_t)np,
+                         m.ccnt.get<scaph_level_counters>(), m_u + m_f, cur, m.out.get<float>(),
+                         st));
+  H_ERR(scaph_copy_d2h_async(m.hout.get(), m.out.get(), 32 * 4, st));
+  return true;
+}
+
+void Engine::predict_apply(uint32_t cur, bool learned) {
+  PagedState& ps = *paged_;
+  const size_t np = pages_out_.size();
+  if (cfg_.page_predict == "exact" && !ps.level_mask.empty()) {
+    std::vector<double> value(np, 0.0);
+    std::vector<char> keep(np, 0);
+    const bool more = cur + 1 < 64;
+    for (size_t p = 0; p < np; ++p) {
+      const uint64_t later = more ? ps.level_mask[p] & (~0ull << (cur + 1)) : 0;
+      if (!later) continue;
+      const int next = __builtin_ctzll(later);
+      value[p] = 1000.0 - next;
+      keep[p] = next == (int)cur + 1 && (uint64_t)next < ps.level_datas.size() &&
+                ps.level_datas[next][p] > ps.thres;
+    }
+    ps.handle.set_prediction(std::move(value), 1000.0 - (cur + 1), std::move(keep));
+    return;
+  }
+  if (!learned) {
+    ps.handle.set_prediction({}, 0);
+    return;
+  }
+  const uint32_t* unv = ps.h_unv.get<uint32_t>();
+  const float frac = std::min(1.f, std::max(0.f, model_->hout.get<float>()[1]));
+  const size_t want = (size_t)std::lround(frac * (double)np);
+  std::vector<int> alive;
+  uint32_t umax = 1;
+  for (size_t p = 0; p < np; ++p)
+    if (unv[p]) {
+      alive.push_back((int)p);
+      umax = std::max(umax, unv[p]);
+    }
+  std::stable_sort(alive.begin(), alive.end(), [&](int a, int b) { return unv[a] > unv[b]; });
+  std::vector<double> value(np, 0.0);
+  for (size_t i = 0; i < alive.size(); ++i) {
+    const int p = alive[i];
+    value[p] = (i < want ? 2.0 : 1.0) + (double)unv[p] / umax;
+  }
+  // no whole-page copies: the model predicts which pages the next level
+  // touches, not how much of each
+  ps.handle.set_prediction(std::move(value), 2.0);
+}
+
+void Engine::set_page_levels(const uint32_t* levels) {
+  if (resident_ || !paged_) throw Error(SCAPH_STATUS_INVALID, "page levels need the paged mode");
+  PagedState& ps = *paged_;
+  const size_t np = pages_out_.size();
+  ps.level_mask.assign(np, 0);
+  ps.level_datas.clear();
+  for (size_t p = 0; p < np; ++p) {
+    const scaph_page& pg = pages_out_[p];
+    for (uint64_t v = pg.left; v < (uint64_t)pg.left + pg.nodenum; ++v) {
+      const uint32_t l = levels[v];
+      if (l >= 64) continue;
+      ps.level_mask[p] |= 1ull << l;
+      if (ps.level_datas.size() <= l) ps.level_datas.resize(l + 1, std::vector<uint64_t>(np, 0));
+      ps.level_datas[l][p] += h_deg_out_[v] + 3ull;
+    }
+  }
 }
 
 // 12-bin histogram of the per-page active-vertex share (fraction of pages

@@ -40,6 +40,14 @@ struct Engine::PagedState {
   std::vector<PinnedBuf> h_wpage;
   std::vector<PinnedBuf> h_wtail;  // compacted weights of a cold page (<= thres words)
   DevBuf wpool;
+  // page-activity prediction (-page_predict): per-page count of vertices a
+  // later level can still reach, the bitmap of vertices with in-edges, the
+  // model's decision record; for the exact bound, per page the levels at
+  // which it is active and its data at each level
+  DevBuf d_unv, live;
+  PinnedBuf h_unv;
+  std::vector<uint64_t> level_mask;
+  std::vector<std::vector<uint64_t>> level_datas;  // [level][page]
 };
 
 }  // namespace scaph

@@ -128,9 +128,13 @@ void PageHandle::queue_reset(const uint32_t* datas, bool group_cached) {
       }
     }
   }
+  if (!value_.empty())
+    std::stable_sort(emptyq_.begin(), emptyq_.end(),
+                     [&](int a, int b) { return slot_value(a) < slot_value(b); });
   num_full = 0;
+  const std::vector<char> whole = whole_copies(datas);
   for (size_t i = 0; i < vp.size(); ++i) {
-    if (datas[i] > thres_ && !vp[i].cached) {
+    if (whole[i]) {
       vp[i].shared = false;
       vp[i].nodenum = page_nodes_[i];
       vp[i].chunknum = 32;
@@ -172,13 +176,13 @@ void PageHandle::release_exec_finished(int vpid) {
     auto it = hold_.find(vpid);
     const int rank = it == hold_.end() ? maxentry_ - 1 : it->second;
     if (rank < maxentry_ - 1) execq_[rank + 1].push_back(vpid);
-    else emptyq_.push_back(v.dpid);
+    else push_empty(v.dpid);
   } else {
     const int d = v.dpid;
     remove_shared(d, vpid);
     if (!dp[d].shared) {
       sharedq_.erase(d);
-      emptyq_.push_back(d);
+      push_empty(d);
     }
   }
   hold_.erase(vpid);
@@ -189,6 +193,94 @@ void PageHandle::release_copy_finished(int vpid) {
   vp[vpid].cached = true;
   execq_[0].push_back(vpid);
 }
+
+// Pages of this level copied whole: every uncached active page with more
+// data than the threshold (src/pagehandle.h:326-379), plus, with a
+// prediction, the keep_whole pages (expected active next level) while the
+// pool has slots beyond the cached active pages and those copies; the rest
+// of the uncached active pages are compacted.
+std::vector<char> PageHandle::whole_copies(const uint32_t* datas) const {
+  std::vector<char> w(vp.size(), 0);
+  int room = (int)dp.size();
+  for (size_t i = 0; i < vp.size(); ++i) {
+    if (datas[i] == 0) continue;
+    if (vp[i].cached && !vp[i].shared) --room;
+    else if (datas[i] > thres_) {
+      w[i] = 1;
+      --room;
+    }
+  }
+  if (keep_whole_.empty() || room <= 0) return w;
+  std::vector<int> cand;
+  for (size_t i = 0; i < vp.size(); ++i)
+    if (keep_whole_[i] && datas[i] > 0 && !w[i] && !(vp[i].cached && !vp[i].shared))
+      cand.push_back((int)i);
+  std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return datas[a] > datas[b]; });
+  for (int i : cand) {
+    if (room <= 0) break;
+    w[i] = 1;
+    --room;
+  }
+  return w;
+}
+
+void PageHandle::set_prediction(std::vector<double> value, double lookahead_min,
+                                std::vector<char> keep_whole) {
+  value_ = std::move(value);
+  lookahead_min_ = lookahead_min;
+  keep_whole_ = std::move(keep_whole);
+}
+
+// A free slot's worth: that of the page still bound to it; nothing for an
+// empty slot or one holding compacted chunks (they are not kept).
+double PageHandle::slot_value(int d) const {
+  if (dp[d].shared || dp[d].vpid < 0) return -1e300;
+  return value_[dp[d].vpid];
+}
+
+void PageHandle::push_empty(int d) {
+  if (value_.empty()) {
+    emptyq_.push_back(d);
+    return;
+  }
+  const double v = slot_value(d);
+  auto it = emptyq_.begin();
+  while (it != emptyq_.end() && slot_value(*it) <= v) ++it;
+  emptyq_.insert(it, d);
+}
+
+bool PageHandle::get_lookahead(const uint32_t* datas, int* vpid) {
+  if (value_.empty() || emptyq_.empty() || merge_pending() > 0) return false;
+  int best = -1;
+  for (size_t p = 0; p < vp.size(); ++p) {
+    if (vp[p].dpid >= 0 || datas[p] > 0 || value_[p] < lookahead_min_) continue;
+    if (best < 0 || value_[p] > value_[best]) best = (int)p;
+  }
+  if (best < 0) return false;
+  // only a slot no later level needs (empty, compacted chunks, or a page
+  // valued 0): evicting a page needed later would trade this copy for
+  // another one
+  const int d = emptyq_.front();
+  if (!(slot_value(d) <= 0.0)) return false;
+  emptyq_.pop_front();
+  unbound(d);
+  bound(d, best);
+  vp[best].shared = false;
+  vp[best].nodenum = page_nodes_[best];
+  vp[best].chunknum = 32;
+  *vpid = best;
+  return true;
+}
+
+bool PageHandle::launches_queued() const {
+  if (!groupq_.empty()) return true;
+  for (const auto& q : execq_)
+    if (!q.empty()) return true;
+  return false;
+}
+
+// The page stays bound to its slot; the next level finds it resident.
+void PageHandle::release_lookahead_finished(int vpid) { vp[vpid].cached = true; }
 
 bool PageHandle::get_execute(int* vpid) {
   for (int r = 0; r < maxentry_; ++r) {

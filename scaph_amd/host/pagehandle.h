@@ -74,6 +74,31 @@ class PageHandle {
   void merge_write(int vpid);
   size_t merge_pending() const;
 
+  // Page-activity prediction for the level about to start (call before
+  // queue_reset; an empty vector restores the plain policy).  value[p]
+  // ranks how soon page p is expected to be active after this level:
+  // higher = sooner, 0 = never again.  With values, free slots are reused least valuable
+  // first (a page expected never again goes before one expected next), and
+  // pages with value >= lookahead_min are worth copying ahead of the next
+  // level.  keep_whole pages (expected active next level with more data
+  // than the threshold) are copied whole instead of compacted, so they stay
+  // resident, while the pool has room for them.
+  void set_prediction(std::vector<double> value, double lookahead_min,
+                      std::vector<char> keep_whole = {});
+  bool predicting() const { return !value_.empty(); }
+  // A whole-page copy ahead of the next level: the most valuable page that
+  // is not resident, not active in this level and valued >= lookahead_min,
+  // into the least valuable free slot if no later level needs that slot's
+  // page (value 0, or an empty or compacted-chunk slot).
+  // Call only once every copy of this level has been issued.
+  bool get_lookahead(const uint32_t* datas, int* vpid);
+  // Which active pages of a level with per-page data `datas` queue_reset
+  // copies whole (the others not resident are compacted).
+  std::vector<char> whole_copies(const uint32_t* datas) const;
+  void release_lookahead_finished(int vpid);
+  // Pages of this level still waiting for their launch.
+  bool launches_queued() const;
+
   std::vector<VPstat> vp;
   std::vector<DPstat> dp;
   // statistics of the current iteration
@@ -84,6 +109,8 @@ class PageHandle {
   bool merge_read(int* v);
   bool shared_find(int required, int* dpid) const;
   bool exec_read_kick(int* vpid);
+  double slot_value(int dpid) const;
+  void push_empty(int dpid);
   void unbound(int dpid);
   void bound(int dpid, int vpid);
   void make_shared(int dpid);
@@ -103,6 +130,9 @@ class PageHandle {
   size_t merge_reader_ = 0;
   std::vector<uint32_t> page_nodes_;
   uint64_t page_words_full_ = 0;
+  std::vector<double> value_;
+  std::vector<char> keep_whole_;
+  double lookahead_min_ = 0;
 };
 
 }  // namespace scaph

@@ -180,27 +180,34 @@ def _sim_lib():
     return ctypes.CDLL(path)
 
 
+@pytest.mark.parametrize("predict", [0, 1])
 @pytest.mark.parametrize("list_size,pool,thres,multi,seed", [
     (40, 8, 3, 1, 1), (40, 8, 3, 2, 2), (100, 10, 12, 1, 3), (16, 16, 4, 1, 4),
     (64, 3, 16, 3, 5), (12, 1, 8, 1, 6)])
-def test_page_scheduler_invariants(list_size, pool, thres, multi, seed):
+def test_page_scheduler_invariants(list_size, pool, thres, multi, seed, predict):
     """Every active page runs at least once per iteration; a page runs only
     when resident; chunk runs in shared slots never overlap and match the
-    slot masks (simulated executor, random completion order)."""
+    slot masks (simulated executor, random completion order).  With a
+    page-activity prediction (random values and keep-whole marks) the same
+    holds, look-ahead copies never target a page of the running level, and
+    every page over the threshold is still copied whole."""
     L = _sim_lib()
     L.scaph_host_pagehandle_simulate.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
                                                  ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
-                                                 ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+                                                 ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                                 ctypes.c_int]
     L.scaph_host_pagehandle_simulate.restype = ctypes.c_int
     rng = np.random.default_rng(seed)
     iters = 12
     # mix of inactive, cold (<= thres) and hot (> thres) pages, in chunk units
     datas = rng.choice([0, 0, 1, 2, thres, thres + 1, 32], size=(iters, list_size)).astype(np.uint32)
-    stats = np.zeros(4, np.uint64)
+    stats = np.zeros(5, np.uint64)
     rc = L.scaph_host_pagehandle_simulate(list_size, pool, thres, multi, iters, datas.ctypes.data,
-                                          seed, stats.ctypes.data)
+                                          seed, stats.ctypes.data, predict)
     assert rc == 0, f"invariant {rc} violated"
     assert stats[0] >= (datas > 0).sum()
+    if predict and 3 <= pool < list_size:
+        assert stats[4] > 0  # look-ahead copies were made
 
 
 def test_committed_schedule_model_loads():
