@@ -70,15 +70,17 @@ def alg_bytes_step(rec, n):
     return 16 * rec["n_f"] + 4 * rec["scanned"] + 8 * rec["n_next"]
 
 
-KERNEL_SOURCES = ("scaph_amd/csrc/bfs.hip", "scaph_amd/csrc/common.h", "include/scaph_hip.h")
-
-
 def kernel_source_hash():
-    """SHA-256 over the sources of the BFS kernels: a traffic profile is used
-    only by the build it was measured on."""
+    """SHA-256 over the kernel library's sources (scaph_amd/csrc and its
+    C-ABI header): a traffic profile is used only by the build it was
+    measured on."""
     h = hashlib.sha256()
-    for rel in KERNEL_SOURCES:
-        with open(os.path.join(ROOT, rel), "rb") as f:
+    csrc = os.path.join(ROOT, "scaph_amd", "csrc")
+    files = sorted(os.path.join(csrc, f) for f in os.listdir(csrc)
+                   if f.endswith((".hip", ".h")))
+    for path in files + [os.path.join(ROOT, "include", "scaph_hip.h")]:
+        h.update(os.path.basename(path).encode())
+        with open(path, "rb") as f:
             h.update(f.read())
     return h.hexdigest()[:16]
 

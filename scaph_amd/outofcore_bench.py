@@ -4,6 +4,7 @@
 -max_memory, pages streamed from pinned host memory.
 
     python -m scaph_amd.outofcore_bench [--budgets 15258,8192] [--pools 16e9] [--sources 4]
+                                        [--predict off,learned,exact] [--oracle]
 
 --budgets are -max_memory values: the whole device budget, from which the
 engine (as src/app.cuh:281-327) first takes the per-vertex and per-page
@@ -14,7 +15,11 @@ that leaves exactly that pool is derived from the vertex and page counts.
 Prints one JSON line: per budget the median GTEPS, page hit rate
 (executions of already-resident pages / all page executions), host-to-device
 bytes and PCIe rates, and parity of the levels with a resident (push-only)
-run of the same graph and, for one source, with the oracle BFS.
+run of the same graph and (--oracle) with the oracle BFS for every source.
+--predict runs each pool once per page-activity prediction mode
+(-page_predict): off (first free slot), learned (the committed model's
+next-level page-activity output) and exact (the levels of a previous run:
+the bound for any predictor).
 """
 import argparse
 import json
@@ -52,7 +57,10 @@ def main():
                          "active data is at most half a page (the reference's FLAGS_threshold is "
                          "8; app/bfs.cu forces 3)")
     ap.add_argument("--flags", default="", help="more engine flags for the paged runs")
-    ap.add_argument("--oracle", action="store_true", help="also run the CPU oracle BFS once")
+    ap.add_argument("--oracle", action="store_true",
+                    help="compare every source's resident levels with the oracle BFS")
+    ap.add_argument("--predict", default="off",
+                    help="comma-separated -page_predict modes run on every pool")
     ap.add_argument("--schedules", action="store_true",
                     help="also run the resident graph with the pull step: the fixed alpha/beta "
                          "rule against the learned schedule (committed model) on the sources")
@@ -82,7 +90,7 @@ def main():
         from tests import _oracle as O
         off, ngh = res.host_csr()
         t1 = time.perf_counter()
-        oracle_ok = bool(np.array_equal(O.bfs(off, ngh, sources[0]), want[sources[0]][0]))
+        oracle_ok = all(bool(np.array_equal(O.bfs_mt(off, ngh, s), want[s][0])) for s in sources)
         log("oracle BFS", oracle_ok, f"{time.perf_counter() - t1:.1f}s")
         del off, ngh
     res_ms = [want[s][1]["ms"] for s in sources]
@@ -115,53 +123,77 @@ def main():
            "resident_gteps_median": statistics.median(
                want[s][1]["m_reached"] / (want[s][1]["ms"] / 1e3) / 1e9 for s in sources),
            "resident_ms_median": statistics.median(res_ms),
-           "levels_resident_vs_oracle": oracle_ok, "schedules": schedules, "budgets": []}
+           "levels_resident_vs_oracle": oracle_ok,
+           "oracle_sources": len(sources) if args.oracle else 0,
+           "schedules": schedules, "budgets": []}
     slot = PAGE_WORDS * 4
     budgets = [(int(x), None) for x in args.budgets.split(",") if x]
     for x in [float(x) for x in args.pools.split(",") if x]:
         # plan_residency: pool = (budget - 9 n - 24 pages) / slot, whole slots
         need = int(x) // slot * slot + 9 * info["n"] + 24 * info["pages"]
         budgets.append(((need + (1 << 20) - 1) >> 20, int(x)))
+    host = None
     for b, pool_bytes in budgets:
-        paged_flags = [f"-max_memory={b}", "-build_pull=false", f"-threshold={args.threshold}"]
-        paged_flags += args.flags.split()
-        eng = BFS(*paged_flags)
-        t0 = time.perf_counter()
-        eng.gen_zipf(args.n, args.x_min, args.alpha, args.cap, args.seed)
-        inf = eng.info()
-        log("budget", b, inf, f"{time.perf_counter() - t0:.1f}s")
-        runs, exact = [], True
-        eng.run(sources[0])  # warm: first touches of the pinned pages
-        for s in sources:
-            st = eng.run(s)
-            exact &= bool(np.array_equal(eng.levels(), want[s][0]))
-            runs.append(st)
-            log(" ", s, {k: st[k] for k in ("ms", "page_execs", "page_execs_cached",
-                                             "pages_full", "pages_compacted", "bytes_h2d",
-                                             "h2d_ms")})
-        ms = statistics.median(r["ms"] for r in runs)
-        execs = sum(r["page_execs"] for r in runs)
-        cached = sum(r["page_execs_cached"] for r in runs)
-        h2d = sum(r["bytes_h2d"] for r in runs)
-        h2d_ms = sum(r["h2d_ms"] for r in runs)
-        tot_ms = sum(r["ms"] for r in runs)
-        slots = ((b << 20) - 9 * inf["n"] - 24 * inf["pages"]) // slot
-        out["budgets"].append({
-            "max_memory_mib": b, "pool_bytes_requested": pool_bytes,
-            "pool_slots": min(slots, inf["pages"]), "pool_bytes": min(slots, inf["pages"]) * slot,
-            "flags": paged_flags, "resident": inf["resident"],
-            "pages": inf["pages"],
-            "gteps_median": statistics.median(r["m_reached"] / (r["ms"] / 1e3) / 1e9
-                                              for r in runs),
-            "t_bfs_ms_median": ms,
-            "page_hit_rate": cached / execs if execs else None,
-            "page_execs": execs, "pages_full_copies": sum(r["pages_full"] for r in runs),
-            "pages_compacted": sum(r["pages_compacted"] for r in runs),
-            "h2d_bytes_per_bfs": h2d / len(runs),
-            "pcie_gbps_over_bfs": h2d / (tot_ms / 1e3) / 1e9 if tot_ms else None,
-            "pcie_gbps_during_copies": h2d / (h2d_ms / 1e3) / 1e9 if h2d_ms else None,
-            "levels_equal_resident": exact})
-        eng.close()
+        for mode in [m for m in args.predict.split(",") if m]:
+            paged_flags = [f"-max_memory={b}", "-build_pull=false", f"-threshold={args.threshold}",
+                           f"-page_predict={mode}"]
+            paged_flags += args.flags.split()
+            eng = BFS(*paged_flags)
+            t0 = time.perf_counter()
+            if host is None:
+                eng.gen_zipf(args.n, args.x_min, args.alpha, args.cap, args.seed)
+                host = eng.host_csr()
+            else:
+                eng.load_csr(*host)
+            inf = eng.info()
+            log("budget", b, mode, inf, f"{time.perf_counter() - t0:.1f}s")
+            if mode == "learned":
+                from . import learn as L
+                L.model_for_graph(eng, L.DEFAULT_MODEL)
+            runs, exact = [], True
+            if mode == "exact":
+                eng.set_page_levels(want[sources[0]][0])
+            eng.run(sources[0])  # warm: first touches of the pinned pages
+            for s in sources:
+                if mode == "exact":
+                    eng.set_page_levels(want[s][0])
+                st = eng.run(s)
+                exact &= bool(np.array_equal(eng.levels(), want[s][0]))
+                runs.append(st)
+                log(" ", s, {k: st[k] for k in ("ms", "page_execs", "page_execs_cached",
+                                                 "pages_full", "pages_compacted", "pages_prefetched",
+                                                 "bytes_h2d", "h2d_ms")})
+            ms = statistics.median(r["ms"] for r in runs)
+            execs = sum(r["page_execs"] for r in runs)
+            cached = sum(r["page_execs_cached"] for r in runs)
+            h2d = sum(r["bytes_h2d"] for r in runs)
+            h2d_ms = sum(r["h2d_ms"] for r in runs)
+            tot_ms = sum(r["ms"] for r in runs)
+            slots = eng.pool()["slots"]
+            out["budgets"].append({
+                "max_memory_mib": b, "pool_bytes_requested": pool_bytes, "page_predict": mode,
+                "pool_slots": slots, "pool_bytes": slots * slot,
+                "flags": paged_flags, "resident": inf["resident"],
+                "pages": inf["pages"],
+                "gteps_median": statistics.median(r["m_reached"] / (r["ms"] / 1e3) / 1e9
+                                                  for r in runs),
+                "t_bfs_ms_median": ms,
+                "t_bfs_ms": [r["ms"] for r in runs],
+                "page_hit_rate": cached / execs if execs else None,
+                "page_execs": execs, "pages_full_copies": sum(r["pages_full"] for r in runs),
+                "pages_compacted": sum(r["pages_compacted"] for r in runs),
+                "pages_copied_ahead": sum(r["pages_prefetched"] for r in runs),
+                "h2d_bytes_per_bfs": h2d / len(runs),
+                "pcie_gbps_over_bfs": h2d / (tot_ms / 1e3) / 1e9 if tot_ms else None,
+                "pcie_gbps_during_copies": h2d / (h2d_ms / 1e3) / 1e9 if h2d_ms else None,
+                "levels_equal_resident": exact})
+            eng.close()
+    # speed of each prediction mode against off on the same pool
+    base = {(x["max_memory_mib"]): x["t_bfs_ms_median"] for x in out["budgets"]
+            if x["page_predict"] == "off"}
+    for x in out["budgets"]:
+        if x["max_memory_mib"] in base:
+            x["speedup_vs_off"] = base[x["max_memory_mib"]] / x["t_bfs_ms_median"]
     print(json.dumps(out), flush=True)
 
 

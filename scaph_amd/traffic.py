@@ -10,13 +10,18 @@ Two steps:
 
 `run` first reads and writes a buffer of known size with the library's own
 kernels (scaph_checksum_u32 reads 4n bytes, scaph_fill_u32 writes 4n bytes),
-then builds the benchmark graph (R-MAT scale 26, edge factor 32, seed 2 by
-default) and runs BFS from its giant-component sources, writing every run's
-per-level records.  `summarize` scales FETCH_SIZE and WRITE_SIZE (KiB per
-dispatch) by what they read for those two kernels, maps the k_pull
-dispatches of the timed runs onto their pull levels, and writes bytes per
-launch beside the algorithmic bytes of the same launches, stamped with the
-hash of the kernel sources (bench.py uses the figure only for that build).
+then runs one workload, writing every run's per-level records:
+  --workload rmat   BFS on R-MAT scale 26, edge factor 32, seed 2 (the
+                    benchmark graph), giant-component sources;
+  --workload zipf   BFS on the resident power-law digraph (105 M vertices,
+                    3.7 B edges, seed 3), push only;
+  --workload msbfs  256-query multi-source BFS on R-MAT scale 24, edge
+                    factor 16, seed 3, without graph replay.
+`summarize --kernel K` scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
+by what they read for those two kernels, maps K's dispatches of the timed
+runs onto their levels, and writes bytes per launch beside the algorithmic
+bytes of the same launches (models in alg_bytes), stamped with the hash of
+the kernel sources (bench.py uses a figure only for that build).
 """
 import argparse
 import csv
@@ -56,21 +61,43 @@ def run(args):
     import bench
     import scaph_amd
     _calibrate()
-    flags = ["-max_memory=262144", "-pagesize=10"] + args.flag
-    e = scaph_amd.BFS(*flags)
-    e.gen_rmat_device(args.scale, args.ef, 0.57, 0.19, 0.19, seed=args.seed)
-    info = e.info()
-    srcs, _ = bench.pick_sources(e, info["n"], args.sources, seed=args.seed, min_reach_frac=0.01)
-    runs = []
-    for s in srcs[:2] + srcs:  # two warm-up runs, then the timed ones
-        st = e.run(s)
-        runs.append({"src": s, "ms": st["ms"], "n_reached": st["n_reached"],
-                     "m_reached": st["m_reached"], "levels": e.records()})
+    if args.workload == "msbfs":
+        flags = ["-max_memory=262144", "-pagesize=10"]
+        e = scaph_amd.BFS(*flags)
+        e.gen_rmat_device(24, 16, 0.57, 0.19, 0.19, seed=3)
+        info = e.info()
+        srcs, _ = bench.pick_sources(e, info["n"], 256, seed=3, min_reach_frac=0.01)
+        tune = {"push_div": 16, "push_mdiv": 8, "pull_block": 256, "use_graph": 0,
+                "segment": 8, "pull_grid": 1024}
+        runs = []
+        for _ in range(4):  # one warm-up, three timed
+            st = e.msbfs(srcs, tune)
+            runs.append({"ms": st["ms"], "depth": st["depth"], "levels": e.msbfs_counters()})
+    else:
+        if args.workload == "zipf":
+            flags = ["-max_memory=262144", "-build_pull=false"] + args.flag
+            e = scaph_amd.BFS(*flags)
+            e.gen_zipf(105_000_000, 1.2, 1.9, 1_000_000, 3)
+            nsrc = min(args.sources, 4)
+        else:
+            flags = ["-max_memory=262144", "-pagesize=10"] + args.flag
+            e = scaph_amd.BFS(*flags)
+            e.gen_rmat_device(args.scale, args.ef, 0.57, 0.19, 0.19, seed=args.seed)
+            nsrc = args.sources
+        info = e.info()
+        srcs, _ = bench.pick_sources(e, info["n"], nsrc,
+                                     seed=3 if args.workload == "zipf" else args.seed,
+                                     min_reach_frac=0.01)
+        runs = []
+        for s in srcs[:1] + srcs:  # a warm-up run, then the timed ones
+            st = e.run(s)
+            runs.append({"src": s, "ms": st["ms"], "n_reached": st["n_reached"],
+                         "m_reached": st["m_reached"], "levels": e.records()})
     e.close()
     with open(args.records, "w") as f:
-        json.dump({"n": info["n"], "m": info["m"], "flags": flags, "timed_from": 2,
-                   "runs": runs}, f)
-    print(json.dumps({"runs": len(runs), "sources": len(srcs)}), flush=True)
+        json.dump({"workload": args.workload, "n": info["n"], "m": info["m"], "flags": flags,
+                   "timed_from": 1, "runs": runs}, f)
+    print(json.dumps({"runs": len(runs)}), flush=True)
 
 
 def _kernel(name):
@@ -93,9 +120,43 @@ def _dispatches(path, counter):
     return out
 
 
-def pull_alg_bytes(rec, n):
-    """Algorithmic bytes of one pull launch (as bench.py counts them)."""
-    return 12 * rec["examined"] + 4 * rec["scanned"] + 4 * rec["n_next"] + 3 * (n / 8)
+def alg_bytes(kernel, rec, n, nxt=None):
+    """Algorithmic bytes of one launch.
+    k_pull (one pull level): 12 B Node per examined vertex, 4 B per scanned
+      in-edge, 4 B level write per vertex found, and the visited read,
+      visited write and next-frontier write (3 n/8), as bench.py counts them;
+    k_expand (one push level past the one-launch size): 12 B Node + 4 B
+      queue read per frontier vertex, 4 B per scanned out-edge, 4 B level
+      + 4 B queue write per vertex reached;
+    k_ms_pull (one multi-source pull level): the 32-B seen read and 32-B
+      frontier write of every vertex and the next membership bitmap (n/8),
+      plus per newly reached vertex its 32-B seen write and 4-B degree."""
+    if kernel == "k_pull":
+        return 12 * rec["examined"] + 4 * rec["scanned"] + 4 * rec["n_next"] + 3 * (n / 8)
+    if kernel == "k_expand":
+        return 16 * rec["n_f"] + 4 * rec["scanned"] + 8 * rec["n_next"]
+    return 64 * n + n / 8 + 36 * nxt["n_active"]
+
+
+def _launches(rec, kernel):
+    """The timed levels that launch `kernel`, in launch order."""
+    out = []
+    timed = rec["runs"][rec["timed_from"]:]
+    if kernel == "k_ms_pull":
+        # one launch per level (it returns at once on push levels)
+        for r in timed:
+            lv = r["levels"]
+            for L in range(r["depth"]):
+                out.append((lv[L], lv[L + 1] if L + 1 < len(lv) else {"n_active": 0}))
+        return out
+    for r in timed:
+        for lv in r["levels"]:
+            if kernel == "k_pull" and lv["dir"] == 1:
+                out.append((lv, None))
+            elif (kernel == "k_expand" and lv["dir"] == 0 and
+                  (lv["m_f"] > 8192 or lv["n_f"] > 65536)):
+                out.append((lv, None))
+    return out
 
 
 def summarize(args):
@@ -113,27 +174,32 @@ def summarize(args):
         raise SystemExit("calibration dispatches missing")
     f_scale = calib_bytes / (rd[0]["kib"] * 1024)
     w_scale = calib_bytes / (max(wr, key=lambda x: x["kib"])["kib"] * 1024)
-    # pull levels of the timed runs, in launch order
-    timed = rec["runs"][rec["timed_from"]:]
-    pull_levels = [lv for r in timed for lv in r["levels"] if lv["dir"] == 1]
-    fk = [x for x in fetch if x["kernel"] == "k_pull"]
-    wk = [x for x in write if x["kernel"] == "k_pull"]
-    if len(fk) != len(wk) or len(fk) < len(pull_levels):
-        raise SystemExit(f"k_pull dispatches: {len(fk)} fetch, {len(wk)} write, "
-                         f"{len(pull_levels)} timed pull levels")
-    fk, wk = fk[-len(pull_levels):], wk[-len(pull_levels):]
+    kernel = args.kernel
+    launches = _launches(rec, kernel)
+    fk = [x for x in fetch if x["kernel"] == kernel]
+    wk = [x for x in write if x["kernel"] == kernel]
+    if len(fk) != len(wk) or len(fk) < len(launches):
+        raise SystemExit(f"{kernel} dispatches: {len(fk)} fetch, {len(wk)} write, "
+                         f"{len(launches)} timed launches")
+    fk, wk = fk[-len(launches):], wk[-len(launches):]
     per = []
-    for lv, a, b in zip(pull_levels, fk, wk):
+    for (lv, nxt), a, b in zip(launches, fk, wk):
         fb = a["kib"] * 1024 * f_scale
         wb = b["kib"] * 1024 * w_scale
-        per.append({"level": lv["level"], "examined": lv["examined"], "scanned": lv["scanned"],
-                    "n_next": lv["n_next"], "fetch_bytes": fb, "write_bytes": wb,
-                    "alg_bytes": pull_alg_bytes(lv, rec["n"]), "us_profiled": a["us"]})
+        d = {k: lv[k] for k in ("level", "n_f", "examined", "scanned", "n_next", "n_active")
+             if k in lv}
+        d.update({"fetch_bytes": fb, "write_bytes": wb,
+                  "alg_bytes": alg_bytes(kernel, lv, rec["n"], nxt), "us_profiled": a["us"]})
+        per.append(d)
+    if kernel == "k_ms_pull":
+        # launches that returned at once (push levels) are not pull launches
+        per = [p for p in per if p["us_profiled"] > 20.0]
     k = len(per)
     traffic = sum(p["fetch_bytes"] + p["write_bytes"] for p in per) / k
     alg = sum(p["alg_bytes"] for p in per) / k
-    out = {"command": args.command, "kernel": "k_pull", "launches": k,
-           "kernel_source_hash": bench.kernel_source_hash(),
+    avg_us = statistics.mean(p["us_profiled"] for p in per)
+    out = {"command": args.command, "workload": rec.get("workload", "rmat"), "kernel": kernel,
+           "launches": k, "kernel_source_hash": bench.kernel_source_hash(),
            "calibration": {"read_kernel": "k_checksum_u32", "write_kernel": "k_fill_u32",
                            "bytes": calib_bytes, "fetch_raw_bytes": rd[0]["kib"] * 1024,
                            "write_raw_bytes": max(wr, key=lambda x: x["kib"])["kib"] * 1024,
@@ -142,7 +208,10 @@ def summarize(args):
            "write_bytes_per_launch": sum(p["write_bytes"] for p in per) / k,
            "traffic_bytes_per_launch": traffic, "alg_bytes_per_launch": alg,
            "traffic_over_alg": traffic / alg if alg else None,
-           "avg_launch_us_profiled": statistics.mean(p["us_profiled"] for p in per),
+           "avg_launch_us_profiled": avg_us,
+           "alg_gbps": alg / (avg_us * 1e-6) / 1e9,
+           "alg_frac_of_8tbps": alg / (avg_us * 1e-6) / 8e12,
+           "traffic_gbps": traffic / (avg_us * 1e-6) / 1e9,
            "per_launch": per}
     # every kernel of the timed region, for the other rows
     kern = {}
@@ -158,9 +227,10 @@ def summarize(args):
     out["kernels_whole_process"] = dict(sorted(kern.items(), key=lambda kv: -kv[1]["us"])[:16])
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)
-    print(json.dumps({k_: out[k_] for k_ in ("launches", "fetch_bytes_per_launch",
+    print(json.dumps({k_: out[k_] for k_ in ("kernel", "launches", "fetch_bytes_per_launch",
                                              "write_bytes_per_launch", "alg_bytes_per_launch",
-                                             "traffic_over_alg")}))
+                                             "traffic_over_alg", "avg_launch_us_profiled",
+                                             "alg_frac_of_8tbps")}))
 
 
 def main():
@@ -168,6 +238,7 @@ def main():
     sub = ap.add_subparsers(dest="cmd", required=True)
     r = sub.add_parser("run")
     r.add_argument("--records", required=True)
+    r.add_argument("--workload", choices=["rmat", "zipf", "msbfs"], default="rmat")
     r.add_argument("--scale", type=int, default=26)
     r.add_argument("--ef", type=int, default=32)
     r.add_argument("--seed", type=int, default=2)
@@ -177,6 +248,7 @@ def main():
     s.add_argument("--dir", required=True)
     s.add_argument("--out", required=True)
     s.add_argument("--command", default="")
+    s.add_argument("--kernel", choices=["k_pull", "k_expand", "k_ms_pull"], default="k_pull")
     a = ap.parse_args()
     run(a) if a.cmd == "run" else summarize(a)
 
