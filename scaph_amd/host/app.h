@@ -67,11 +67,7 @@ class App {
       const auto mean = ls.mean_for(engine->num_vertices(), engine->num_edges());
       engine->set_device_model(ls.params(), mean.data(), ls.scale());
     }
-    vert_count = engine->num_vertices();
-    edge_count = (index_t)engine->num_edges();
-    list_size = (int)engine->pages().size();
-    pool_size = engine->pool_size();
-    pagesize = (vertex_t)engine->page_words();
+    bind_graph();
     std::printf("|========================Dataset Information========================|\n");
     std::printf("|                       graph name      : %-26s|\n",
                 cfg.graphfile.empty() ? cfg.gen.c_str() : cfg.graphfile.c_str());
@@ -97,6 +93,23 @@ class App {
     }
   }
 
+  // Takes the loaded graph's sizes and device state (load_graph() calls it;
+  // call it after loading through engine-> directly).
+  void bind_graph() {
+    vert_count = engine->num_vertices();
+    edge_count = (index_t)engine->num_edges();
+    list_size = (int)engine->pages().size();
+    pool_size = engine->pool_size();
+    pagesize = (vertex_t)engine->page_words();
+    const Engine::DeviceState d = engine->device_state();
+    d_value = reinterpret_cast<value_t*>(d.value);
+    d_active = d.active;
+    active_words = d.active_words;
+    d_degree = d.degree;
+    stream = d.stream;
+    copystream = d.copy_stream;
+  }
+
   void iter_init() {}  // per-level bookkeeping lives inside Engine::bfs
   virtual void gpu_init() = 0;
   virtual bool check() { return true; }
@@ -117,6 +130,37 @@ class App {
   int list_size = 0, pool_size = 0;
   int num_blks = 256, num_thds = 256, num_stream = 5;
   std::vector<value_t> h_value;
+  // Device state for the app's gpu_init (the reference's public d_value,
+  // d_stat and streams, src/app.cuh:72-135), set by load_graph(): the
+  // per-vertex values, the active bitmap (bit v of word v/32 for the
+  // reference's bool d_stat[v]; active_words u32), the out-degrees, and the
+  // engine's exec and page-copy streams (opaque hipStream_t; the copy
+  // stream is null when every page is resident).  An app initialises them
+  // through the kernel C-ABI (scaph_hip.h) or kernels of its own, on
+  // `stream`, before execute() runs.
+  value_t* d_value = nullptr;
+  uint32_t* d_active = nullptr;
+  size_t active_words = 0;
+  const uint32_t* d_degree = nullptr;
+  void* stream = nullptr;
+  void* copystream = nullptr;
+};
+
+// A plugin app on the engine's label-correcting rounds, for apps beyond the
+// three built in: gpu_init() writes the start values and the active set
+// into d_value / d_active; execute() runs rounds in which every out-edge of
+// an active vertex offers its value (relax_mode() SCAPH_RELAX_WCC) or its
+// value plus the edge weight (SCAPH_RELAX_SSSP) to the target, until no
+// value is lowered — the role of the reference's cond/update/active
+// functors (app/wcc_kernel.cuh, app/sssp_kernel.cuh) for min-relaxing apps.
+template <typename vertex_t, typename index_t, typename value_t>
+class relax_app : public App<vertex_t, index_t, value_t> {
+ public:
+  using Base = App<vertex_t, index_t, value_t>;
+  relax_app(int argc, char** argv) : Base(argc, argv) {}
+  explicit relax_app(const Config& c) : Base(c) {}
+  virtual int relax_mode() const = 0;
+  RunStats execute() override { return this->engine->relax_prepared(relax_mode()); }
 };
 
 // The BFS plugin (app/bfs.cu:39-117).

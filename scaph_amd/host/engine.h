@@ -149,6 +149,29 @@ class Engine {
   // lengths over the edge weights, 0x7FFFFFFF for unreached.
   RunStats wcc();
   RunStats sssp(uint32_t src);
+  // The device state a plugin app initialises in its gpu_init (the
+  // reference's public d_value / d_stat and streams, src/app.cuh:72-135):
+  // the per-vertex values (n u32), the active-vertex bitmap (active_words
+  // u32, bit v of word v/32; the reference's bool stat[]), the out-degrees,
+  // the engine's exec stream and, in paged mode, its page copy stream.
+  // Valid once a graph is loaded; buffers belong to the engine.
+  struct DeviceState {
+    uint32_t* value;
+    uint32_t* active;
+    size_t active_words;
+    const uint32_t* degree;
+    void* stream;
+    void* copy_stream;
+    uint32_t n;
+  };
+  DeviceState device_state();
+  // Label-correcting rounds from the values and active set the app wrote
+  // into device_state(): every out-edge (u, v) of an active u offers
+  // value[u] (mode SCAPH_RELAX_WCC) or value[u] + w (SCAPH_RELAX_SSSP, edge
+  // weights required) to v, a lowered v is active in the next round, and
+  // the rounds stop when none is (the cond/update/active functors of
+  // app/wcc_kernel.cuh and app/sssp_kernel.cuh run by src/app.cuh:220-265).
+  RunStats relax_prepared(int mode);
   // Edge weights in CSR edge order (the _csr.wgh file), for SSSP; set
   // automatically when a loaded CSR carries them.
   void set_weights(const uint32_t* wgh, uint64_t count);
@@ -239,7 +262,7 @@ class Engine {
     const uint32_t* weights;
   };
   void schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunStats* rs);
-  RunStats relax_paged(int mode, uint32_t src);
+  RunStats relax_paged(int mode, uint32_t src, bool prepared);
 
   Config cfg_;
   uint64_t P_;  // page size in words
@@ -255,7 +278,8 @@ class Engine {
   bool weights_loaded_ = false;   // resident: wpool_; paged: PagedState::h_wpage
   DevBuf relax_ws_;               // chunk list of the WCC/SSSP rounds
   DevBuf split_;                  // near/far split counters (3 u64)
-  RunStats relax(int mode, uint32_t src);
+  RunStats relax(int mode, uint32_t src, bool prepared = false);
+  uint64_t count_active(int fa);
   DevBuf dpages_out_, dptr_out_, dpages_in_, dptr_in_;
   scaph_graph_desc g_out_{}, g_in_{};
   DevBuf deg_out_, deg_in_, static_visited_;

@@ -26,11 +26,45 @@ RunStats Engine::sssp(uint32_t src) {
   return relax(SCAPH_RELAX_SSSP, src);
 }
 
-RunStats Engine::relax(int mode, uint32_t src) {
+Engine::DeviceState Engine::device_state() {
+  if (n_ == 0) throw Error(SCAPH_STATUS_INVALID, "no graph loaded");
+  DeviceState d;
+  d.value = level_.get<uint32_t>();
+  d.active = front_[0].get<uint32_t>();
+  d.active_words = bitmap_words(n_);
+  d.degree = deg_out_.get<uint32_t>();
+  d.stream = main_->get();
+  d.copy_stream = paged_ && paged_->copy ? paged_->copy->get() : nullptr;
+  d.n = n_;
+  return d;
+}
+
+RunStats Engine::relax_prepared(int mode) {
+  if (mode != SCAPH_RELAX_WCC && mode != SCAPH_RELAX_SSSP)
+    throw Error(SCAPH_STATUS_INVALID, "relax mode must be SCAPH_RELAX_WCC or SCAPH_RELAX_SSSP");
+  if (mode == SCAPH_RELAX_SSSP && !has_weights())
+    throw Error(SCAPH_STATUS_INVALID, "SSSP-mode rounds need edge weights (_csr.wgh)");
+  return relax(mode, 0, true);
+}
+
+// Active vertices of front_[fa] (the list it converts to is not used).
+uint64_t Engine::count_active(int fa) {
+  void* st = main_->get();
+  auto* cnt = counters_.get<scaph_level_counters>();
+  auto* hc = hcount_.get<scaph_level_counters>();
+  H_ERR(scaph_fill_u32(reinterpret_cast<uint32_t*>(cnt), 0u, sizeof(scaph_level_counters) / 4, st));
+  H_ERR(scaph_frontier_bitmap_to_queue(front_[fa].get<uint32_t>(), n_, queue_[1].get<uint32_t>(),
+                                       cnt, st));
+  H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
+  main_->sync();
+  return hc->q_len;
+}
+
+RunStats Engine::relax(int mode, uint32_t src, bool prepared) {
   if (n_ == 0) throw Error(SCAPH_STATUS_INVALID, "no graph loaded");
   if (mode == SCAPH_RELAX_SSSP && src >= n_)
     throw Error(SCAPH_STATUS_INVALID, "source vertex out of range");
-  if (!resident_) return relax_paged(mode, src);
+  if (!resident_) return relax_paged(mode, src, prepared);
   const size_t ws_bytes = scaph_relax_workspace_bytes(m_);
   if (relax_ws_.bytes() < ws_bytes) relax_ws_.alloc(ws_bytes);
   void* st = main_->get();
@@ -41,14 +75,18 @@ RunStats Engine::relax(int mode, uint32_t src) {
   RunStats rs;
   rs.src = src;
   const auto t0 = std::chrono::steady_clock::now();
-  H_ERR(scaph_relax_init(value, n_, mode, src, st));
   int fa = 0;
   uint64_t n_act;
-  if (mode == SCAPH_RELAX_WCC) {
+  if (prepared) {
+    // the app's gpu_init wrote the values and the active set
+    n_act = count_active(0);
+  } else if (mode == SCAPH_RELAX_WCC) {
+    H_ERR(scaph_relax_init(value, n_, mode, src, st));
     // every vertex starts active (app/wcc.cu:18-19); bits past n are ignored
     H_ERR(scaph_fill_u32(front_[0].get<uint32_t>(), 0xFFFFFFFFu, words, st));
     n_act = n_;
   } else {
+    H_ERR(scaph_relax_init(value, n_, mode, src, st));
     *hcount_.get<uint32_t>() = src;
     H_ERR(scaph_copy_h2d_async(queue_[1].get(), hcount_.get(), 4, st));
     H_ERR(scaph_fill_u32(front_[0].get<uint32_t>(), 0u, words, st));
@@ -121,7 +159,7 @@ RunStats Engine::relax(int mode, uint32_t src) {
 // whole, or compacted to their active Nodes when at most -threshold of a
 // page is active, with their weights for SSSP), each page relaxed by
 // scaph_relax_page on an exec stream.
-RunStats Engine::relax_paged(int mode, uint32_t src) {
+RunStats Engine::relax_paged(int mode, uint32_t src, bool prepared) {
   PagedState& ps = *paged_;
   void* st = main_->get();
   const size_t np = pages_out_.size();
@@ -155,13 +193,16 @@ RunStats Engine::relax_paged(int mode, uint32_t src) {
   RunStats rs;
   rs.src = src;
   const auto t0 = std::chrono::steady_clock::now();
-  H_ERR(scaph_relax_init(value, n_, mode, src, st));
   int fa = 0;
   uint64_t n_act;
-  if (mode == SCAPH_RELAX_WCC) {
+  if (prepared) {
+    n_act = count_active(0);
+  } else if (mode == SCAPH_RELAX_WCC) {
+    H_ERR(scaph_relax_init(value, n_, mode, src, st));
     H_ERR(scaph_fill_u32(front_[0].get<uint32_t>(), 0xFFFFFFFFu, words, st));
     n_act = n_;
   } else {
+    H_ERR(scaph_relax_init(value, n_, mode, src, st));
     *hcount_.get<uint32_t>() = src;
     H_ERR(scaph_copy_h2d_async(queue_[1].get(), hcount_.get(), 4, st));
     H_ERR(scaph_fill_u32(front_[0].get<uint32_t>(), 0u, words, st));
