@@ -86,19 +86,26 @@ def kernel_source_hash():
 
 
 def device_name(props):
-    """The marketing name rocminfo reports for the GPU (the runtime's name
-    field is empty on some drivers), else the architecture."""
+    """The GPU's name as the runtime or rocminfo reports it; where both
+    leave it empty (as on some drivers), its architecture, compute units and
+    memory."""
     if props.get("name"):
         return props["name"]
     try:
         out = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=60).stdout
+        agent = None
         for line in out.splitlines():
             line = line.strip()
-            if line.startswith("Marketing Name:") and "Instinct" in line:
-                return line.split(":", 1)[1].strip()
+            if line.startswith("Name:"):
+                agent = line.split(":", 1)[1].strip()
+            if line.startswith("Marketing Name:") and agent and agent.startswith("gfx"):
+                name = line.split(":", 1)[1].strip()
+                if name:
+                    return name
     except (OSError, subprocess.SubprocessError):
         pass
-    return props.get("arch", "")
+    return (f"{props.get('arch', '')}, {props.get('cu', 0)} CUs, "
+            f"{props.get('total_mem', 0) / 2**30:.0f} GiB")
 
 
 def host_cpu():
@@ -196,9 +203,31 @@ def bench_msbfs(scale, ef, queries, seed):
            "single_source_total_ms": sum(single),
            "speedup_vs_single_source": sum(single) / ms,
            "tune": best, "autotune_candidates": len(log_),
-           "levels_equal_single_source": exact}
+           "levels_equal_single_source": exact,
+           "k_ms_pull_roofline": profiled_roofline("msbfs_s24_ef16_256q_k_ms_pull_traffic.json")}
     eng.close()
     return out
+
+
+def profiled_roofline(name):
+    """Roofline of a kernel from its committed rocprofv3 profile (algorithmic
+    bytes per launch over the profiled average launch time, and the
+    calibrated FETCH_SIZE + WRITE_SIZE bytes), when the profile was taken on
+    this build's kernel sources; None otherwise."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("kernel_source_hash") != kernel_source_hash():
+        return None
+    us = t["avg_launch_us_profiled"]
+    alg = t["alg_bytes_per_launch"]
+    return {"kernel": t["kernel"], "bound": "hbm", "avg_launch_us": us,
+            "alg_bytes_per_launch": alg, "achieved": alg / (us * 1e-6) / 1e9, "peak": 8000.0,
+            "unit": "GB/s", "frac": alg / (us * 1e-6) / 8e12,
+            "traffic": t["traffic_bytes_per_launch"], "launches": t["launches"],
+            "profile": os.path.join("profiles", name)}
 
 
 def bench_small_batch(count, seed):
