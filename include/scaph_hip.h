@@ -215,11 +215,25 @@ int scaph_bfs_push_noalloc(const scaph_graph_desc* out_graph, const uint32_t* de
  * the vertices found and the full next-frontier bitmap (all words, so it
  * needs no clearing).  In-lists still unmatched after 512 entries are
  * finished by a second, grid-wide pass.  max_edges bounds the graph's
- * edges (workspace: scaph_pull_workspace_bytes).  in_head (optional,
- * u32[4n]) holds per vertex its first three in-list entries and its
- * in-degree (scaph_list_heads_from_csr): vertices found there skip their
- * page.  No reference counterpart (the reference engine is push-only). */
+ * edges (workspace: scaph_pull_workspace_bytes).  heads (optional,
+ * scaph_pull_heads) holds per vertex its first in-list entries and its
+ * in-degree (scaph_list_heads_from_csr / scaph_list_heads_live): vertices
+ * found there skip their page.  No reference counterpart (the reference
+ * engine is push-only). */
 size_t scaph_pull_workspace_bytes(unsigned long long max_edges);
+/* The pull expand's head records (optional): per vertex its first in-list
+ * entries, its in-degree and (head_outdeg) its out-degree, 16 bytes.  With
+ * no_in_edges and live_base set, records exist only for vertices with
+ * in-edges, in id order (vertex v's record is number live_base[v / 1024]
+ * plus the vertices with in-edges in [1024 * (v / 1024), v)): vertices
+ * without in-edges are never examined (they start visited), and leaving
+ * them out halves the record bytes a pull level reads on R-MAT graphs.
+ * Otherwise record v is at index v. */
+typedef struct scaph_pull_heads {
+  const uint32_t* head;         /* u32[4 * records] */
+  const uint32_t* no_in_edges;  /* bitmap of the vertices without in-edges (padding bits set) */
+  const uint32_t* live_base;    /* per 1024-id window: vertices with in-edges before it */
+} scaph_pull_heads;
 /* Launch shape of the pull expand (NULL or 0 fields = built-in choice). */
 typedef struct scaph_pull_tuning {
   uint32_t grid_blocks; /* cap on 256-thread blocks (each owns 4 x 1024-vertex windows per pass) */
@@ -227,7 +241,7 @@ typedef struct scaph_pull_tuning {
   uint32_t no_prefetch; /* do not request the next batch's head records early */
   uint32_t reserved;
 } scaph_pull_tuning;
-int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
+int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,
                    const uint32_t* hub_list, uint32_t num_hubs,
                    const uint32_t* degree, uint32_t* level, uint32_t* visited,
                    const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
@@ -411,6 +425,17 @@ int scaph_gen_er_keys(uint64_t* keys, uint64_t e_begin, uint64_t e_end, uint32_t
  * (scaph_pull_tuning.head_outdeg). */
 int scaph_list_heads_from_csr(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
                               const uint32_t* out_degree, uint32_t* head, void* stream);
+/* base[k] = vertices with in-edges among ids < 1024 k, for every 1024-id
+ * window k of the no_in_edges bitmap (scaph_bitmap_from_zero_degree);
+ * allocates its own scan storage and returns after the stream drains. */
+int scaph_live_window_base(const uint32_t* no_in_edges, uint32_t n, uint32_t* base,
+                           void* stream);
+/* The head records of scaph_list_heads_from_csr for the vertices with
+ * in-edges only, in id order, numbered from base (scaph_live_window_base);
+ * head must hold 4 u32 per such vertex. */
+int scaph_list_heads_live(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
+                          const uint32_t* out_degree, const uint32_t* no_in_edges,
+                          const uint32_t* base, uint32_t* head, void* stream);
 /* offsets[0..n] = exclusive prefix of degree[0..n) (offsets[n] = total);
  * allocates its own scan storage and returns after the stream drains. */
 int scaph_degrees_to_offsets(const uint32_t* degree, uint64_t* offsets, uint32_t n, void* stream);

@@ -955,8 +955,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 template <int PREFETCH>
 __global__ void __launch_bounds__(kBlock)
-k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32_t* degree,
-       uint32_t* level,
+k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint32_t* live_base,
+       uint32_t head_outdeg, const uint32_t* degree, uint32_t* level,
        uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
        scaph_level_counters* counters, uint32_t next_level, PullWs hw, uint32_t num_hubs) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
@@ -964,6 +964,10 @@ k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32
   __shared__ uint32_t s_fw[kBlock / SCAPH_WAVE][kPullWinWords];
   __shared__ uint32_t s_hub[kMaxHubs / 32];
   __shared__ unsigned long long red[4][kBlock / SCAPH_WAVE];
+  // head records by live rank: per window word, its live (in-edge) bits and
+  // the live vertices before it in the window
+  __shared__ uint32_t s_lw[kBlock / SCAPH_WAVE][kPullWinWords];
+  __shared__ uint32_t s_lb[kBlock / SCAPH_WAVE][kPullWinWords];
   stage_hubs(s_hub, hw.hub_front, num_hubs);
   const PageTab s = stage_pages(g, dyn_lds);
   __syncthreads();
@@ -1005,20 +1009,36 @@ k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32
       q[o++] = (lane << 5) | b;
     }
     if (lane < kPullWinWords) fw[lane] = 0u;
+    uint32_t wbase = 0;
+    if (live_base && tot) {
+      const uint32_t lw = own ? ~no_in[w] : 0u;
+      const uint32_t lc = __popc(lw);
+      const uint32_t li = wave_incl_scan(lc);
+      if (lane < kPullWinWords) {
+        s_lw[wid][lane] = lw;
+        s_lb[wid][lane] = li - lc;
+      }
+      wbase = live_base[w0 / kPullWinWords];
+    }
     wave_lds_sync();
+    // index of the head record of window vertex loc
+    auto hidx = [&](uint32_t loc) -> uint32_t {
+      if (!live_base) return (uint32_t)(w0 * 32) + loc;
+      const uint32_t j = loc >> 5;
+      return wbase + s_lb[wid][j] + __popc(s_lw[wid][j] & ((1u << (loc & 31)) - 1u));
+    };
     const uint32_t hint = tot ? find_page(s, g, (uint32_t)(w0 * 32)) : 0u;
     // the head records of the next batch are requested before this batch
     // is probed (one extra load in flight per lane)
     uint4 hd_next = make_uint4(0u, 0u, 0u, 0u);
-    if (PREFETCH && head && lane < tot) hd_next = head[(uint32_t)(w0 * 32) + q[lane]];
+    if (PREFETCH && head && lane < tot) hd_next = head[hidx(q[lane])];
     for (uint32_t i0 = 0; i0 < tot; i0 += SCAPH_WAVE) {
       const uint32_t i = i0 + lane;
       const bool act = i < tot;
       const uint32_t loc = act ? q[i] : 0u;
       const uint32_t v = (uint32_t)(w0 * 32) + loc;
       const uint4 hd_cur = hd_next;
-      if (PREFETCH && head && i + SCAPH_WAVE < tot)
-        hd_next = head[(uint32_t)(w0 * 32) + q[i + SCAPH_WAVE]];
+      if (PREFETCH && head && i + SCAPH_WAVE < tot) hd_next = head[hidx(q[i + SCAPH_WAVE])];
       bool found = false, longl = false;
       NodeRef nr;
       nr.edges = nullptr;
@@ -1033,7 +1053,7 @@ k_pull(scaph_graph_desc g, const uint4* head, uint32_t head_outdeg, const uint32
           // lists start with their hubs) never touches its page
           // (head_outdeg: {e0, e1, in-degree, out-degree} instead, so a
           // found vertex also needs no degree[] load)
-          const uint4 hd = PREFETCH ? hd_cur : head[v];
+          const uint4 hd = PREFETCH ? hd_cur : head[hidx(loc)];
           const uint32_t cap = head_outdeg ? 2u : 3u;
           const uint32_t len = head_outdeg ? hd.z : hd.w;
           if (head_outdeg) outdeg = hd.w;
@@ -1775,7 +1795,7 @@ size_t scaph_pull_workspace_bytes(unsigned long long max_edges) {
   return carve_pull_ws(nullptr, max_edges).bytes;
 }
 
-int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
+int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,
                    const uint32_t* hub_list, uint32_t num_hubs,
                    const uint32_t* degree, uint32_t* level, uint32_t* visited,
                    const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
@@ -1802,15 +1822,18 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const uint32_t* in_head,
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
   const uint32_t outdeg_heads = tune ? tune->head_outdeg : 0u;
   const size_t lds = page_lds_bytes(in_graph->num_pages);
-  const uint4* heads = reinterpret_cast<const uint4*>(in_head);
+  const uint4* head = heads ? reinterpret_cast<const uint4*>(heads->head) : nullptr;
+  const uint32_t* no_in = head ? heads->no_in_edges : nullptr;
+  const uint32_t* live_base = no_in ? heads->live_base : nullptr;
+  if (head && heads->live_base && !no_in) return SCAPH_STATUS_INVALID;
   if (!tune || !tune->no_prefetch)
-    hipLaunchKernelGGL(k_pull<1>, dim3(grid), dim3(kBlock), lds, st, *in_graph, heads, outdeg_heads,
-                       degree, level, visited, front_in, front_out, counters, cur_level + 1, hw,
-                       num_hubs);
+    hipLaunchKernelGGL(k_pull<1>, dim3(grid), dim3(kBlock), lds, st, *in_graph, head, no_in,
+                       live_base, outdeg_heads, degree, level, visited, front_in, front_out,
+                       counters, cur_level + 1, hw, num_hubs);
   else
-    hipLaunchKernelGGL(k_pull<0>, dim3(grid), dim3(kBlock), lds, st, *in_graph, heads, outdeg_heads,
-                       degree, level, visited, front_in, front_out, counters, cur_level + 1, hw,
-                       num_hubs);
+    hipLaunchKernelGGL(k_pull<0>, dim3(grid), dim3(kBlock), lds, st, *in_graph, head, no_in,
+                       live_base, outdeg_heads, degree, level, visited, front_in, front_out,
+                       counters, cur_level + 1, hw, num_hubs);
   SCAPH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), 0, st, hw.chunks, hw.count, degree,
                      level, visited, front_in, front_out, counters, cur_level + 1, hw.hub_front,

@@ -75,6 +75,54 @@ k_list_heads(const uint64_t* offsets, const uint32_t* ngh, uint32_t n, const uin
   }
 }
 
+// Vertices with in-edges per window of 1024 ids (32 bitmap words): the
+// complement of the no-in-edge bitmap (whose padding bits past n are set).
+__global__ void __launch_bounds__(kBlock)
+k_window_live(const uint32_t* no_in, uint64_t words, uint32_t windows, uint32_t* cnt) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < windows; k += stride) {
+    uint32_t c = 0;
+    for (uint64_t w = (uint64_t)k * 32; w < (uint64_t)k * 32 + 32 && w < words; ++w)
+      c += __popc(~no_in[w]);
+    cnt[k] = c;
+  }
+}
+
+// Head records of the vertices with in-edges only, in id order: a thread
+// per window writes its live vertices' records from the window's base on.
+__global__ void __launch_bounds__(kBlock)
+k_list_heads_live(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
+                  const uint32_t* out_degree, const uint32_t* no_in, const uint32_t* base,
+                  uint32_t windows, uint4* head) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < windows; k += stride) {
+    uint32_t r = base[k];
+    for (uint64_t w = (uint64_t)k * 32; w < (uint64_t)k * 32 + 32 && w < words; ++w) {
+      uint32_t bits = ~no_in[w];
+      while (bits) {
+        const uint32_t b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const uint32_t v = (uint32_t)(w * 32 + b);
+        if (v >= n) continue;
+        const uint64_t o = offsets[v];
+        const uint32_t d = (uint32_t)(offsets[v + 1] - o);
+        uint4 rec;
+        rec.x = d > 0 ? ngh[o] : 0xFFFFFFFFu;
+        rec.y = d > 1 ? ngh[o + 1] : 0xFFFFFFFFu;
+        if (out_degree) {
+          rec.z = d;
+          rec.w = out_degree[v];
+        } else {
+          rec.z = d > 2 ? ngh[o + 2] : 0xFFFFFFFFu;
+          rec.w = d;
+        }
+        head[r++] = rec;
+      }
+    }
+  }
+}
+
 // Power-law digraph edges: edge e belongs to the last vertex v with
 // offsets[v] <= e (offsets = exclusive prefix of the out-degrees, n + 1
 // entries); its destination is the multiply-high of SplitMix64 word e of the
@@ -232,6 +280,43 @@ int scaph_list_heads_from_csr(const uint64_t* offsets, const uint32_t* ngh, uint
   if (!offsets || !head || n == 0) return SCAPH_STATUS_INVALID;
   hipLaunchKernelGGL(k_list_heads, dim3(grid_for(n, kBlock, 16384)), dim3(kBlock), 0,
                      as_stream(stream), offsets, ngh, n, out_degree,
+                     reinterpret_cast<uint4*>(head));
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_live_window_base(const uint32_t* no_in_edges, uint32_t n, uint32_t* base,
+                           void* stream) {
+  if (!no_in_edges || !base || n == 0) return SCAPH_STATUS_INVALID;
+  hipStream_t st = as_stream(stream);
+  const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
+  const uint32_t windows = (uint32_t)((words + 31) / 32);
+  uint32_t* cnt = nullptr;
+  SCAPH_TRY(hipMalloc(&cnt, (size_t)windows * 4 + 16));
+  hipLaunchKernelGGL(k_window_live, dim3(grid_for(windows, kBlock, 4096)), dim3(kBlock), 0, st,
+                     no_in_edges, words, windows, cnt);
+  int rc = (int)hipGetLastError();
+  size_t bytes = 0;
+  void* tmp = nullptr;
+  if (!rc) rc = (int)rocprim::exclusive_scan(nullptr, bytes, cnt, base, 0u, (size_t)windows,
+                                             rocprim::plus<uint32_t>(), st);
+  if (!rc) rc = (int)hipMalloc(&tmp, bytes + 16);
+  if (!rc) rc = (int)rocprim::exclusive_scan(tmp, bytes, cnt, base, 0u, (size_t)windows,
+                                             rocprim::plus<uint32_t>(), st);
+  if (!rc) rc = (int)hipStreamSynchronize(st);
+  const int rf = (int)hipFree(cnt);
+  const int rt = tmp ? (int)hipFree(tmp) : 0;
+  return rc ? rc : (rf ? rf : rt);
+}
+
+int scaph_list_heads_live(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
+                          const uint32_t* out_degree, const uint32_t* no_in_edges,
+                          const uint32_t* base, uint32_t* head, void* stream) {
+  if (!offsets || !no_in_edges || !base || !head || n == 0) return SCAPH_STATUS_INVALID;
+  const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
+  const uint32_t windows = (uint32_t)((words + 31) / 32);
+  hipLaunchKernelGGL(k_list_heads_live, dim3(grid_for(windows, kBlock, 4096)), dim3(kBlock), 0,
+                     as_stream(stream), offsets, ngh, n, out_degree, no_in_edges, base, windows,
                      reinterpret_cast<uint4*>(head));
   SCAPH_LAUNCH_CHECK();
   return 0;

@@ -66,6 +66,8 @@ Config& global_flags() {
           "SSSP: relax only vertices below a bound that grows by this much (0 = all)");
     r.add("pull_prefetch", FlagType::kBool, &c.pull_prefetch,
           "request the next batch's head records before probing the current one");
+    r.add("pull_head_live", FlagType::kBool, &c.pull_head_live,
+          "head records only for vertices with in-edges, numbered by live rank");
     r.add("pull_head_outdeg", FlagType::kBool, &c.pull_head_outdeg,
           "head records carry the out-degree instead of a third in-list entry");
     r.add("push_grid", FlagType::kInt32, &c.push_grid, "cap on push expand blocks (0 = default)");
@@ -173,6 +175,7 @@ Engine::~Engine() = default;
 void Engine::reset_graph() {
   ++graph_gen_;
   in_head_.reset();
+  live_base_.reset();
   n_ = 0;
   m_ = 0;
   host_ = HostCsr();
@@ -203,7 +206,7 @@ uint64_t Engine::device_bytes() const {
   uint64_t b = pool_out_.bytes() + pool_in_.bytes() + deg_out_.bytes() + deg_in_.bytes() +
                static_visited_.bytes() + level_.bytes() + visited_.bytes() + front_[0].bytes() +
                front_[1].bytes() + queue_[0].bytes() + queue_[1].bytes() + ws_.bytes() +
-               pws_.bytes() + vsnap_.bytes() + in_head_.bytes() + hub_list_.bytes() +
+               pws_.bytes() + vsnap_.bytes() + in_head_.bytes() + live_base_.bytes() + hub_list_.bytes() +
                counters_.bytes() + dpages_out_.bytes() + dptr_out_.bytes() + dpages_in_.bytes() +
                dptr_in_.bytes() + page_nodes_.bytes() + page_datas_.bytes() + totals_.bytes() +
                wpool_.bytes();
@@ -407,19 +410,38 @@ void Engine::load_csr(HostCsr g, bool keep_host, HostCsr* in_edges) {
       in_deg.resize(n_);
       for (uint32_t v = 0; v < n_; ++v) in_deg[v] = t.degree(v);
       if (cfg_.pull_head) {
-        std::vector<uint32_t> head((size_t)n_ * 4);
+        // records of the vertices with in-edges only, numbered per 1024-id
+        // window (live_base_), as scaph_list_heads_live writes them
+        const size_t windows = (bitmap_words(n_) + 31) / 32;
+        std::vector<uint32_t> base(windows, 0);
+        std::vector<uint32_t> head;
+        head.reserve((size_t)n_ * 4);
+        uint32_t live = 0;
+        const bool live_only = cfg_.pull_head_live;
         for (uint32_t v = 0; v < n_; ++v) {
-          const uint64_t o = t.offsets[v];
+          if ((v & 1023u) == 0) base[v >> 10] = live;
           const uint32_t d = t.degree(v);
-          for (uint32_t j = 0; j < 3; ++j) head[4ull * v + j] = j < d ? t.ngh[o + j] : 0xFFFFFFFFu;
-          head[4ull * v + 3] = d;
+          if (d == 0 && live_only) continue;
+          ++live;
+          const uint64_t o = t.offsets[v];
+          uint32_t r[4];
+          for (uint32_t j = 0; j < 3; ++j) r[j] = j < d ? t.ngh[o + j] : 0xFFFFFFFFu;
+          r[3] = d;
           if (cfg_.pull_head_outdeg) {
-            head[4ull * v + 2] = d;
-            head[4ull * v + 3] = h_deg_out_[v];
+            r[2] = d;
+            r[3] = h_deg_out_[v];
           }
+          head.insert(head.end(), r, r + 4);
         }
-        in_head_.alloc(std::max<size_t>(head.size(), 4) * 4);
+        for (size_t k = (n_ + 1023u) >> 10; k < windows; ++k) base[k] = live;
+        // one spare record: nothing reads past the last live vertex's
+        head.insert(head.end(), 4, 0xFFFFFFFFu);
+        in_head_.alloc(head.size() * 4);
         H_ERR(scaph_copy_h2d_async(in_head_.get(), head.data(), head.size() * 4, main_->get()));
+        if (live_only) {
+          live_base_.alloc(windows * 4);
+          H_ERR(scaph_copy_h2d_async(live_base_.get(), base.data(), windows * 4, main_->get()));
+        }
         main_->sync();
       }
     }
@@ -640,11 +662,38 @@ void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
     H_ERR(scaph_pages_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(),
                                dpages.get<scaph_page>(), dptr.get<uint64_t>(),
                                (uint32_t)pages.size(), n, st));
-    if (dir == 1 && cfg_.pull_head) {
+    if (dir == 1 && cfg_.pull_head && !cfg_.pull_head_live) {
       in_head_.alloc((size_t)n * 16);
       H_ERR(scaph_list_heads_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
                                       cfg_.pull_head_outdeg ? deg_out_.get<uint32_t>() : nullptr,
                                       in_head_.get<uint32_t>(), st));
+    } else if (dir == 1 && cfg_.pull_head) {
+      // records of the vertices with in-edges only (deg holds the in-degrees)
+      static_visited_.alloc(bitmap_words(n) * 4);
+      H_ERR(scaph_bitmap_from_zero_degree(static_visited_.get<uint32_t>(), deg.get<uint32_t>(), n,
+                                          st));
+      const size_t windows = (bitmap_words(n) + 31) / 32;
+      live_base_.alloc(windows * 4);
+      H_ERR(scaph_live_window_base(static_visited_.get<uint32_t>(), n,
+                                   live_base_.get<uint32_t>(), st));
+      uint32_t live = 0, last = 0;
+      H_ERR(scaph_copy_d2h_async(&last, live_base_.get<uint32_t>() + windows - 1, 4, st));
+      main_->sync();
+      {
+        // live vertices = base of the last window + those in it
+        std::vector<uint32_t> tail(32, ~0u);
+        const size_t w0 = (windows - 1) * 32;
+        const size_t nw = std::min<size_t>(32, bitmap_words(n) - w0);
+        H_ERR(scaph_copy_d2h_async(tail.data(), static_visited_.get<uint32_t>() + w0, nw * 4, st));
+        main_->sync();
+        live = last;
+        for (size_t k = 0; k < nw; ++k) live += (uint32_t)__builtin_popcount(~tail[k]);
+      }
+      in_head_.alloc(((size_t)live + 1) * 16);
+      H_ERR(scaph_list_heads_live(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
+                                  cfg_.pull_head_outdeg ? deg_out_.get<uint32_t>() : nullptr,
+                                  static_visited_.get<uint32_t>(), live_base_.get<uint32_t>(),
+                                  in_head_.get<uint32_t>(), st));
     }
     main_->sync();
     desc.page_ptr = dptr.get<const uint64_t>();
@@ -662,6 +711,7 @@ void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
   }
   if (!has_in_) {
     in_head_.reset();
+    live_base_.reset();
     pool_in_.reset();
     dpages_in_.reset();
     dptr_in_.reset();

@@ -48,6 +48,7 @@ struct Config {
   bool pull_head = true;         // 16-byte in-list head records in front of the in-pages
   int32_t pull_grid = 0;         // cap on pull blocks (0: four per compute unit)
   bool pull_head_outdeg = true;  // head records {e0, e1, in-degree, out-degree}
+  bool pull_head_live = true;    // head records only for vertices with in-edges (live rank)
   bool pull_prefetch = true;     // request the next batch's head records early
   bool learned_fused = true;     // a scalar-feature model decides inside the level hand-off
   int32_t sssp_delta = 0;        // resident SSSP: bucket width of near/far rounds (0 = off)
@@ -284,7 +285,8 @@ class Engine {
   scaph_graph_desc g_out_{}, g_in_{};
   DevBuf deg_out_, deg_in_, static_visited_;
   DevBuf hub_list_;  // hub vertex ids (pull in-lists refer to them by index)
-  DevBuf in_head_;   // per vertex: first three in-list entries and in-degree
+  DevBuf in_head_;   // per vertex with in-edges: first in-list entries, in-degree, out-degree
+  DevBuf live_base_; // per 1024-id window: vertices with in-edges before it (in_head_ numbering)
   uint32_t num_hubs_ = 0;
   // BFS state
   DevBuf level_, visited_, front_[2], queue_[2], counters_, ws_, page_nodes_, page_datas_, totals_;

@@ -69,8 +69,11 @@ void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m
     f.front_ok = false;
     f.snap_ok = snap;
   } else {
-    H_ERR(scaph_bfs_pull(&g_in_, in_head_.get<uint32_t>(), hub_list_.get<uint32_t>(), num_hubs_,
-                         deg, level, visited,
+    scaph_pull_heads heads{in_head_.get<uint32_t>(),
+                           live_base_.get() ? static_visited_.get<uint32_t>() : nullptr,
+                           live_base_.get<uint32_t>()};
+    H_ERR(scaph_bfs_pull(&g_in_, in_head_.get() ? &heads : nullptr, hub_list_.get<uint32_t>(),
+                         num_hubs_, deg, level, visited,
                          front_[f.fa].get<uint32_t>(), front_[f.fa ^ 1].get<uint32_t>(), cnt, cur,
                          m_, pws_.get(), pws_bytes_, &pull_tune_, st));
     f.fa ^= 1;

@@ -74,7 +74,11 @@ def test_rmat_parity_with_oracle(scale, ef, seed, page_words):
 
 @pytest.mark.parametrize("head", [["-pull_head=true", "-pull_head_outdeg=false"],
                                   ["-pull_head=false"],
-                                  ["-pull_head=true", "-pull_head_outdeg=true"]])
+                                  ["-pull_head=true", "-pull_head_outdeg=true"],
+                                  ["-pull_head_live=false"],
+                                  ["-pull_head_live=false", "-pull_head_outdeg=false",
+                                   "-pull_prefetch=false"],
+                                  ["-pull_prefetch=false"]])
 def test_pull_head_records(head):
     """The pull expand with and without the 16-byte in-list head records,
     on the host-loaded and the device-built graph."""
