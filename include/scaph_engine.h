@@ -52,7 +52,8 @@ typedef struct scaph_level_record {
    * the copy stream's busy time in this level */
   uint32_t pages_active, pages_cached, pages_full, pages_compacted, pages_prefetched, pad;
   uint64_t bytes_h2d;
-  float h2d_ms, pad2;
+  float h2d_ms;
+  float logit; /* learned schedule: the model's pull logit behind this level's direction */
 } scaph_level_record;
 
 /* App(argc, argv): flags in gflags syntax (argv[0] is the program name). */

@@ -169,7 +169,7 @@ class BFS:
         out = []
         for r in arr[:cnt.value]:
             d = {k: getattr(r, k) for k, _ in _lib.LevelRecord._fields_
-                 if k not in ("page_hist", "pad", "pad2")}
+                 if k not in ("page_hist", "pad")}
             d["page_hist"] = list(r.page_hist)
             out.append(d)
         return out

@@ -108,7 +108,7 @@ class LevelRecord(ctypes.Structure):
                 ("pages_full", ctypes.c_uint32), ("pages_compacted", ctypes.c_uint32),
                 ("pages_prefetched", ctypes.c_uint32), ("pad", ctypes.c_uint32),
                 ("bytes_h2d", ctypes.c_uint64), ("h2d_ms", ctypes.c_float),
-                ("pad2", ctypes.c_float)]
+                ("logit", ctypes.c_float)]
 
 
 _hip = None

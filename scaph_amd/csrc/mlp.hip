@@ -93,7 +93,9 @@ __device__ __forceinline__ void load_rows(const uint16_t* p, RowRegs& r) {
   r.b31 = bf16_to_f32(p[kOffB3 + 1]);
 }
 
-// forward_wave's arithmetic (same order of accumulation) on loaded rows.
+// forward_wave's arithmetic (same order of accumulation) on loaded rows;
+// unit i's h1 reaches every lane by v_readlane (a scalar operand of the
+// FMA) instead of an LDS permute per unit.
 __device__ __forceinline__ void forward_rows(const RowRegs& r, const float* x, float* y) {
   float w1[kIn];
   unpack8(r.q1[0], w1);
@@ -108,7 +110,8 @@ __device__ __forceinline__ void forward_rows(const RowRegs& r, const float* x, f
     float w2[8];
     unpack8(r.q2[k], w2);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) b += w2[i] * __shfl(h1, 8 * k + i, SCAPH_WAVE);
+    for (int i = 0; i < 8; ++i)
+      b += w2[i] * __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(h1), 8 * k + i));
   }
   const float h2 = b > 0.f ? b : 0.f;
   y[0] = wave_sum(r.w30 * h2) + r.b30;

@@ -62,6 +62,8 @@ Config& global_flags() {
     r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
     r.add("learned_fused", FlagType::kBool, &c.learned_fused,
           "a model reading only scalar features decides inside the level hand-off");
+    r.add("learned_margin", FlagType::kDouble, &c.learned_margin,
+          "learned schedule: the model decides when |pull logit| >= this, else the fixed rule");
     r.add("sssp_delta", FlagType::kInt32, &c.sssp_delta,
           "SSSP: relax only vertices below a bound that grows by this much (0 = all)");
     r.add("pull_prefetch", FlagType::kBool, &c.pull_prefetch,

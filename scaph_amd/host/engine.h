@@ -51,6 +51,8 @@ struct Config {
   bool pull_head_live = true;    // head records only for vertices with in-edges (live rank)
   bool pull_prefetch = true;     // request the next batch's head records early
   bool learned_fused = true;     // a scalar-feature model decides inside the level hand-off
+  double learned_margin = 0.5;   // the model decides only when |pull logit| >= this; nearer
+                                 // the boundary the fixed rule does
   int32_t sssp_delta = 0;        // resident SSSP: bucket width of near/far rounds (0 = off)
   int32_t push_grid = 0;         // cap on push expand blocks (0: one per tile, at most 8192)
   bool push_rows = true;         // push expand reads coalesced 256-edge rows
@@ -93,6 +95,7 @@ struct LevelRecord {
   uint32_t pages_prefetched = 0;
   uint64_t bytes_h2d = 0;
   float h2d_ms = 0;
+  float logit = 0;  // learned schedule: the model's pull logit for this level (0 otherwise)
 };
 
 struct RunStats {

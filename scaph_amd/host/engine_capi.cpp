@@ -319,7 +319,7 @@ int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_
     out[i].pad = 0;
     out[i].bytes_h2d = lv[i].bytes_h2d;
     out[i].h2d_ms = lv[i].h2d_ms;
-    out[i].pad2 = 0;
+    out[i].logit = lv[i].logit;
   }
   if (count) *count = (uint32_t)lv.size();
   return 0;

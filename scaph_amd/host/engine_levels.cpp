@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <cmath>
+
 #include "engine_internal.h"
 
 namespace scaph {
@@ -14,8 +16,12 @@ constexpr size_t kMaxTimedLevels = 1024;
 constexpr size_t kCounterWords = sizeof(scaph_level_counters) / 4;
 constexpr size_t kDecideFloats = 32;  // logit, threshold, pages active, pad, hist[12], feat[16]
 // Word offset of the learned decision's output inside the counters buffer
-// (beside the level counters, so one copy returns both).
-constexpr size_t kDecideWord = 64;
+// (beside the level counters, so one copy returns both): after the level's
+// record (words 0..11) and the queue conversion's record (12..23), so that
+// the whole hand-off is 64 words, one store per lane of the publishing wave.
+constexpr size_t kDecideWord = 32;
+static_assert(kDecideWord >= 2 * kCounterWords && kDecideWord + kDecideFloats <= 64,
+              "hand-off layout");
 // Word of the hand-off flag in the mapped hand-off buffer.
 constexpr size_t kHandFlagWord = 256;
 // Push levels whose frontier has more out-edges than this snapshot visited
@@ -173,13 +179,17 @@ RunStats Engine::bfs_resident(uint32_t src) {
   rs.m_reached = deg_src;
   std::vector<size_t> lev_ev;
   bool have_decision = false;
-  Dir decided = Dir::kPush;
+  float logit = 0.f;
   while (n_f > 0) {
     SchedState ss{dir, cur, n_, m_, n_f, m_f, m_u, n_f_prev,
                   rs.levels.empty() ? nullptr : &rs.levels.back()};
     Dir d;
-    if (learned && have_decision) d = decided;
-    else d = sched_ ? sched_(ss) : beamer(ss);
+    if (learned && have_decision && std::fabs(logit) >= cfg_.learned_margin)
+      d = logit > 0.f ? Dir::kPull : Dir::kPush;
+    else if (learned)
+      d = beamer(ss);  // the fixed rule: the first level, and calls the model is unsure of
+    else
+      d = sched_ ? sched_(ss) : beamer(ss);
     if (d == Dir::kPull && !has_in_) d = Dir::kPush;
     ensure_forms(f, d == Dir::kPush, d == Dir::kPull, n_f);
     // the counters start cleared: by scaph_bfs_init for level 0, by the
@@ -228,6 +238,7 @@ RunStats Engine::bfs_resident(uint32_t src) {
     LevelRecord rec;
     rec.level = cur;
     rec.dir = (int)d;
+    rec.logit = learned && have_decision ? logit : 0.f;
     rec.n_f = n_f;
     rec.m_f = m_f;
     rec.m_u = m_u;
@@ -237,7 +248,7 @@ RunStats Engine::bfs_resident(uint32_t src) {
     rec.examined = hc->examined;
     if (hc->reserved) throw Error(SCAPH_STATUS_INVALID, "pull work list overflow");
     if (learned) {
-      decided = hout[0] > 0.f ? Dir::kPull : Dir::kPush;
+      logit = hout[0];
       have_decision = true;
       for (int i = 0; i < 12; ++i) rec.page_hist[i] = hout[4 + i];
     }

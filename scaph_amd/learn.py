@@ -4,7 +4,8 @@ file, and the A/B evaluation against the fixed alpha/beta rule.
 
 Training data are per-level traces of plain BFS runs over seeded graphs
 (R-MAT scales 18-24 with a in [0.45, 0.65], Erdos-Renyi with average degree
-8-64): at each level both directions run from the same state, the step times
+8-64; --powerlaw adds power-law digraphs with Pareto out-degrees and uniform
+destinations): at each level both directions run from the same state, the step times
 become a soft label (is pull faster?) and the fraction of pages active at the
 next level is the second target.  Levels never depend on the model; only the
 wall time does.
@@ -34,22 +35,32 @@ def log(*a):
 
 # ----------------------------------------------------------------- graphs
 
-def training_specs(count=200, seed=5):
+def training_specs(count=200, seed=5, powerlaw=False):
     """Seeded training graphs: 70% R-MAT (scale 18-24, a in [0.45, 0.65],
     b = c = 0.4(1-a), edge factor 8-32), 30% Erdos-Renyi (2^18-2^21
-    vertices, average degree 8-64)."""
+    vertices, average degree 8-64) -- the committed model's set.
+
+    powerlaw: 60% R-MAT, 20% Erdos-Renyi, 20% power-law digraphs (2^18-2^21
+    vertices, Pareto out-degrees with x_min in [1, 2] and alpha in
+    [1.7, 2.1] capped at n/100, uniform destinations)."""
     rng = np.random.default_rng(seed)
     specs = []
     for i in range(count):
-        if i % 10 < 7:
+        k = i % 10
+        if k < (6 if powerlaw else 7):
             a = float(rng.uniform(0.45, 0.65))
             specs.append({"kind": "rmat", "scale": int(rng.integers(18, 25)),
                           "ef": int(rng.integers(8, 33)), "a": a, "b": 0.4 * (1 - a),
                           "c": 0.4 * (1 - a), "seed": 1000 + i})
-        else:
+        elif not powerlaw or k < 8:
             n = 1 << int(rng.integers(18, 22))
             specs.append({"kind": "er", "n": n, "m": n * int(rng.integers(8, 65)),
                           "seed": 2000 + i})
+        else:
+            n = 1 << int(rng.integers(18, 22))
+            specs.append({"kind": "zipf", "n": n, "x_min": float(rng.uniform(1.0, 2.0)),
+                          "alpha": float(rng.uniform(1.7, 2.1)), "cap": max(1000, n // 100),
+                          "seed": 3000 + i})
     return specs
 
 
@@ -57,8 +68,10 @@ def build(eng, spec):
     if spec["kind"] == "rmat":
         eng.gen_rmat_device(spec["scale"], spec["ef"], spec["a"], spec["b"], spec["c"],
                             seed=spec["seed"])
-    else:
+    elif spec["kind"] == "er":
         eng.gen_er_device(spec["n"], spec["m"], seed=spec["seed"])
+    else:
+        eng.gen_zipf(spec["n"], spec["x_min"], spec["alpha"], spec["cap"], seed=spec["seed"])
 
 
 def pick_sources(eng, count, seed, min_frac=0.01):
@@ -327,6 +340,8 @@ def main():
     ap.add_argument("--load-traces", default="", help="train on saved traces instead of new ones")
     ap.add_argument("--absolute", action="store_true",
                     help="train on absolute log sizes (the older models) instead of shares")
+    ap.add_argument("--powerlaw", action="store_true",
+                    help="train on R-MAT, Erdos-Renyi and power-law graphs (60/20/20)")
     ap.add_argument("--histogram", action="store_true",
                     help="let the model read the 12 page-histogram features too (the decision "
                          "then needs a frontier pass per level)")
@@ -340,7 +355,7 @@ def main():
         else:
             # traces saved without the sizes: n from the (seeded) specs, m
             # from each graph's first record, m_u + m_f of the source
-            specs = training_specs(args.graphs)
+            specs = training_specs(args.graphs, powerlaw=args.powerlaw)
             g = z["graph"]
             gn = np.array([2 ** specs[i]["scale"] if specs[i]["kind"] == "rmat"
                            else specs[i]["n"] for i in g], np.float64)
@@ -351,7 +366,7 @@ def main():
             gm = np.array([mm[int(i)] for i in g], np.float64)
         recs = None
     else:
-        specs = training_specs(args.graphs)
+        specs = training_specs(args.graphs, powerlaw=args.powerlaw)
         recs = collect(specs, args.sources)
         X, T = features_labels(recs)
         gn = np.array([r["n"] for r in recs], np.float64)
@@ -375,7 +390,7 @@ def main():
     log(f"training accuracy of the direction choice: {acc * 100:.1f}%")
     save_model(args.out, mlp.params_bf16(), mean, scale, flags=flags)
     # held-out check on fresh graphs (not in the training set)
-    held = training_specs(6, seed=99)
+    held = training_specs(10, seed=99, powerlaw=True)
     eng = BFS("-max_memory=262144")
     summary = []
     for spec in held:

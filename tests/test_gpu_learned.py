@@ -177,18 +177,22 @@ def test_frontier_page_counts_bitmap_and_list(page_words):
 
 
 def test_trained_model_on_held_out_graph():
-    """The committed model, on a graph outside its training set: per-level
-    step times of both directions (explore) price its choices against the
-    per-level best and against the fixed alpha/beta rule; the learned
-    schedule's levels equal the fixed schedule's."""
+    """The committed model, on a graph outside its training set: at every
+    level both directions are timed from the same state (explore); where
+    one direction is clearly faster (by 2x or more) the model must pick it
+    on at least 80% of those levels — a count of decisions, not a bound on
+    wall time.  The per-level costs of the model's, the fixed rule's and the
+    best choices are printed.  The learned schedule's levels equal the fixed
+    schedule's."""
     p, mean0, scale, _ = L.load_model()
     pf = L.bf16_to_f32(p)
     with scaph_amd.BFS("-max_memory=65536") as e:
         e.gen_rmat_device(19, 16, 0.6, 0.16, 0.16, seed=777)
         n = e.info()["n"]
         mean = L.graph_mean(mean0, L.load_model_flags(), n, e.info()["m"])
-        srcs = L.pick_sources(e, 2, seed=777)
+        srcs = L.pick_sources(e, 3, seed=777)
         t_model = t_best = t_fixed = 0.0
+        clear = right = 0
         for s in srcs:
             say("held-out explore", s)
             recs = e.explore(s, 2)
@@ -197,6 +201,9 @@ def test_trained_model_on_held_out_graph():
             prev_pull, prev_nf = False, 0
             for r, yy in zip(recs, y):
                 tp, tq = r["t_push_ms"], r["t_pull_ms"]
+                if max(tp, tq) >= 2.0 * min(tp, tq) and max(tp, tq) > 0.02:
+                    clear += 1
+                    right += int((yy[0] > 0) == (tq < tp))
                 t_model += tq if yy[0] > 0 else tp
                 t_best += min(tp, tq)
                 m_u = 2.0 ** r["x"][2] - 1
@@ -214,9 +221,9 @@ def test_trained_model_on_held_out_graph():
             e.run(s)
             np.testing.assert_array_equal(e.levels(), want)
         say(f"step time: model {t_model:.3f} ms, fixed rule {t_fixed:.3f} ms, "
-            f"per-level best {t_best:.3f} ms")
-        # the model picks (nearly) the faster direction at every level
-        assert t_model <= 1.15 * t_best
+            f"per-level best {t_best:.3f} ms; clear levels {clear}, model right on {right}")
+        assert clear >= 3
+        assert right >= 0.8 * clear
 
 
 def test_scalar_model_decision_fused_into_hand_off():
@@ -269,6 +276,40 @@ def test_scalar_model_decision_fused_into_hand_off():
             np.testing.assert_allclose(rec[:2], y[0], rtol=1e-4, atol=1e-4)
     finally:
         hip.scaph_host_pinned_free(hp)
+
+
+def test_margin_falls_back_to_fixed_rule():
+    """-learned_margin: the model decides a level only when its |pull logit|
+    reaches the margin, otherwise the fixed alpha/beta rule does.  A margin
+    above every logit gives the fixed schedule's directions level for level;
+    margin 0 gives the sign of every recorded logit.  The first level has no
+    decision yet and is the fixed rule's in both."""
+    with scaph_amd.BFS("-max_memory=65536", "-learned_margin=1e9") as a, \
+            scaph_amd.BFS("-max_memory=65536", "-learned_margin=0") as b:
+        for e in (a, b):
+            e.gen_rmat_device(18, 16, seed=41)
+            L.model_for_graph(e)
+        srcs = L.pick_sources(a, 3, seed=41)
+        decided = 0
+        for s in srcs:
+            say("margin fallback", s)
+            a.set_schedule("fixed")
+            a.run(s)
+            fixed = [r["dir"] for r in a.records()]
+            want = a.levels()
+            a.set_schedule("learned")
+            a.run(s)
+            assert [r["dir"] for r in a.records()] == fixed
+            np.testing.assert_array_equal(a.levels(), want)
+            b.set_schedule("learned")
+            b.run(s)
+            recs = b.records()
+            assert recs[0]["dir"] == fixed[0] and recs[0]["logit"] == 0.0
+            for r in recs[1:]:
+                assert r["dir"] == int(r["logit"] > 0), (s, r["level"], r["logit"])
+                decided += 1
+            np.testing.assert_array_equal(b.levels(), want)
+        assert decided >= 6
 
 
 def test_model_flag_binds_graph_offsets():

@@ -225,6 +225,23 @@ def test_committed_schedule_model_loads():
     np.testing.assert_array_equal(m[3:], mean[3:])
 
 
+def test_training_graph_specs():
+    """The committed model's set: 200 seeded graphs, 70% R-MAT at scales
+    18-24 and 30% Erdos-Renyi; --powerlaw trades a third of the ER share and
+    a tenth of the R-MAT share for power-law digraphs.  The specs are a pure
+    function of the seed."""
+    from scaph_amd import learn as L
+    base = L.training_specs(200)
+    assert base == L.training_specs(200)
+    kinds = [s["kind"] for s in base]
+    assert kinds.count("rmat") == 140 and kinds.count("er") == 60
+    assert all(18 <= s["scale"] <= 24 and 8 <= s["ef"] <= 32 and 0.45 <= s["a"] <= 0.65
+               for s in base if s["kind"] == "rmat")
+    assert all(8 * s["n"] <= s["m"] <= 64 * s["n"] for s in base if s["kind"] == "er")
+    pl = [s["kind"] for s in L.training_specs(200, powerlaw=True)]
+    assert (pl.count("rmat"), pl.count("er"), pl.count("zipf")) == (120, 40, 40)
+
+
 def test_graphgen_weights_and_app_tools():
     """wmax= adds uniform weights on [1, wmax] without changing the edges;
     the wcc and sssp tools parse the shared flags."""
