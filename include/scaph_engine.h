@@ -36,6 +36,7 @@ typedef struct scaph_bfs_stats {
   uint64_t bytes_h2d;         /* host-to-device page bytes */
   double h2d_ms;              /* copy-stream busy time */
   uint64_t pages_prefetched;  /* whole-page copies made ahead for the next level */
+  uint64_t pages_direct;      /* cold pages pushed straight from pinned host memory */
 } scaph_bfs_stats;
 
 typedef struct scaph_level_record {
@@ -48,9 +49,10 @@ typedef struct scaph_level_record {
   float page_hist[12];
   /* paged mode (zero when resident): pages with active vertices, those
    * already in the pool at the level's start, whole-page and compacted
-   * copies, pages copied ahead for the next level, host-to-device bytes and
-   * the copy stream's busy time in this level */
-  uint32_t pages_active, pages_cached, pages_full, pages_compacted, pages_prefetched, pad;
+   * copies, pages copied ahead for the next level, cold pages pushed from
+   * pinned host memory, host-to-device bytes and the copy stream's busy
+   * time in this level */
+  uint32_t pages_active, pages_cached, pages_full, pages_compacted, pages_prefetched, pages_direct;
   uint64_t bytes_h2d;
   float h2d_ms;
   float logit; /* learned schedule: the model's pull logit behind this level's direction */

@@ -109,6 +109,10 @@ int scaph_dev_alloc(void** ptr, size_t bytes);
 int scaph_dev_free(void* ptr);
 int scaph_host_pinned_alloc(void** ptr, size_t bytes);
 int scaph_host_pinned_free(void* ptr);
+/* The address at which kernels read memory from scaph_host_pinned_alloc
+ * (each access crosses PCIe).  No reference counterpart: the reference only
+ * copies pages (src/app.cuh:371-393). */
+int scaph_host_device_pointer(const void* host_ptr, void** device_ptr);
 int scaph_copy_h2d_async(void* dst, const void* src, size_t bytes, void* stream);
 int scaph_copy_d2h_async(void* dst, const void* src, size_t bytes, void* stream);
 int scaph_copy_d2d_async(void* dst, const void* src, size_t bytes, void* stream);

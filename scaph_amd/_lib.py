@@ -94,7 +94,7 @@ class BfsStats(ctypes.Structure):
                 ("page_execs", ctypes.c_uint64), ("page_execs_cached", ctypes.c_uint64),
                 ("pages_full", ctypes.c_uint64), ("pages_compacted", ctypes.c_uint64),
                 ("bytes_h2d", ctypes.c_uint64), ("h2d_ms", ctypes.c_double),
-                ("pages_prefetched", ctypes.c_uint64)]
+                ("pages_prefetched", ctypes.c_uint64), ("pages_direct", ctypes.c_uint64)]
 
 
 class LevelRecord(ctypes.Structure):
@@ -106,7 +106,7 @@ class LevelRecord(ctypes.Structure):
                 ("page_hist", ctypes.c_float * 12),
                 ("pages_active", ctypes.c_uint32), ("pages_cached", ctypes.c_uint32),
                 ("pages_full", ctypes.c_uint32), ("pages_compacted", ctypes.c_uint32),
-                ("pages_prefetched", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("pages_prefetched", ctypes.c_uint32), ("pages_direct", ctypes.c_uint32),
                 ("bytes_h2d", ctypes.c_uint64), ("h2d_ms", ctypes.c_float),
                 ("logit", ctypes.c_float)]
 

@@ -106,6 +106,11 @@ int scaph_host_pinned_alloc(void** ptr, size_t bytes) {
   return (int)hipHostMalloc(ptr, bytes, hipHostMallocDefault);
 }
 int scaph_host_pinned_free(void* ptr) { return ptr ? (int)hipHostFree(ptr) : 0; }
+int scaph_host_device_pointer(const void* host_ptr, void** device_ptr) {
+  if (!host_ptr || !device_ptr) return SCAPH_STATUS_INVALID;
+  *device_ptr = nullptr;
+  return (int)hipHostGetDevicePointer(device_ptr, const_cast<void*>(host_ptr), 0);
+}
 
 int scaph_copy_h2d_async(void* dst, const void* src, size_t bytes, void* stream) {
   if (bytes == 0) return 0;

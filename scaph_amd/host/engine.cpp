@@ -83,7 +83,11 @@ Config& global_flags() {
     r.add("page_hint", FlagType::kBool, &c.page_hint,
           "paged: size page pushes by their page counts and read full pages through the frontier bits");
     r.add("page_groups", FlagType::kInt32, &c.page_groups,
-          "paged: grouped launches per level for the cached active pages (0 = one per page)");
+          "paged: grouped launches per level for the cached active pages while copies wait for "
+          "slots, else one (0 = one per page)");
+    r.add("page_direct", FlagType::kInt64, &c.page_direct,
+          "paged: uncached pages with at most this many active words are pushed straight from "
+          "pinned host memory over PCIe, without compaction or a copy (0 = off)");
     r.add("pull_head", FlagType::kBool, &c.pull_head,
           "keep each in-list's first entries beside its in-degree for the pull probes");
     r.add("page_words", FlagType::kInt64, &c.page_words, "exact page size in u32 words (0: -pagesize)");
@@ -129,6 +133,10 @@ bool parse_config(const std::vector<std::string>& args, Config* out,
   }
   if (g.page_groups < 0 || g.page_groups > 4096) {
     if (err) *err = "ERROR: -page_groups must be in 0..4096";
+    return false;
+  }
+  if (g.page_direct < 0) {
+    if (err) *err = "ERROR: -page_direct must be >= 0";
     return false;
   }
   if (g.page_words < 0 || (g.page_words > 0 && g.page_words < 64)) {

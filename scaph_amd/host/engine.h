@@ -62,7 +62,11 @@ struct Config {
   bool page_hint = true;         // paged: size page pushes by their page counts, read full
                                  // pages through the frontier bits
   int32_t page_groups = 8;       // paged: cached active pages of a level go out in about this
-                                 // many grouped launches (0: one launch per page)
+                                 // many grouped launches while copies wait for slots, else in
+                                 // one (0: one launch per page)
+  int64_t page_direct = 65536;   // paged: an uncached page with at most this many active words
+                                 // is pushed from pinned host memory, no compaction or copy
+                                 // (0 = off; needs -page_groups > 0)
   int64_t page_words = 0;        // exact page size in words (overrides -pagesize; tests)
   std::string page_predict = "off";  // paged: next-level page-activity prediction for slot reuse,
                                      // look-ahead copies and whole-page copies (off | learned |
@@ -93,6 +97,7 @@ struct LevelRecord {
   // and copy-stream busy time
   uint32_t pages_active = 0, pages_cached = 0, pages_full = 0, pages_compacted = 0;
   uint32_t pages_prefetched = 0;
+  uint32_t pages_direct = 0;  // cold pages pushed from pinned host memory (-page_direct)
   uint64_t bytes_h2d = 0;
   float h2d_ms = 0;
   float logit = 0;  // learned schedule: the model's pull logit for this level (0 otherwise)
@@ -108,6 +113,7 @@ struct RunStats {
   // paged (out-of-core) mode
   uint64_t page_execs = 0, page_execs_cached = 0, pages_full = 0, pages_compacted = 0;
   uint64_t pages_prefetched = 0;  // copies made ahead for the next level
+  uint64_t pages_direct = 0;      // cold pages pushed from pinned host memory
   uint64_t bytes_h2d = 0;
   double h2d_ms = 0;
 };

@@ -58,8 +58,9 @@ int guarded(char* err, size_t len, F f) {
 }
 
 // The learned schedule decides on the device (scaph_mlp_decide inside the
-// level loop); the host-side model only answers the first level and serves
-// the CPU-side checks.
+// level loop; the fixed rule takes the first level and uncertain calls);
+// the host-side model is the schedule of an engine without a device model
+// and serves the CPU-side checks.
 // A model read from a file whose features are shares of the graph takes
 // the loaded graph's offsets; called after every graph load.
 void bind_model(scaph_engine* e) {
@@ -221,6 +222,7 @@ int scaph_engine_bfs(scaph_engine* e, uint32_t src, scaph_bfs_stats* out, char* 
       out->bytes_h2d = e->last.bytes_h2d;
       out->h2d_ms = e->last.h2d_ms;
       out->pages_prefetched = e->last.pages_prefetched;
+      out->pages_direct = e->last.pages_direct;
     }
   });
 }
@@ -240,6 +242,7 @@ void fill_stats(const scaph::RunStats& r, scaph_bfs_stats* out) {
   out->bytes_h2d = r.bytes_h2d;
   out->h2d_ms = r.h2d_ms;
   out->pages_prefetched = r.pages_prefetched;
+  out->pages_direct = r.pages_direct;
 }
 }  // namespace
 
@@ -316,7 +319,7 @@ int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_
     out[i].pages_full = lv[i].pages_full;
     out[i].pages_compacted = lv[i].pages_compacted;
     out[i].pages_prefetched = lv[i].pages_prefetched;
-    out[i].pad = 0;
+    out[i].pages_direct = lv[i].pages_direct;
     out[i].bytes_h2d = lv[i].bytes_h2d;
     out[i].h2d_ms = lv[i].h2d_ms;
     out[i].logit = lv[i].logit;

@@ -21,6 +21,10 @@ namespace {
 
 size_t bitmap_words(uint32_t n) { return (((size_t)n + 63) / 64) * 2; }
 
+// Out-edges below which splitting a level's cached pages into more grouped
+// launches costs more in launches than the slots it frees early.
+constexpr uint64_t kMinGroupEdges = 1ull << 22;
+
 // Calls f(i) for each Node i of page pg (vertex pg.left + i) whose frontier
 // bit is set, in vertex order, skipping zero bitmap words.
 template <typename F>
@@ -106,6 +110,13 @@ void Engine::setup_paged() {
     fill_page(host_, pages_out_[p], false, ps.h_page[p].get<uint32_t>());
     if (ps.thres) ps.h_tail[p].alloc(ps.thres * 4);
   }
+  ps.d_hpage.assign(np, 0);
+  if (cfg_.page_direct > 0 && cfg_.page_groups > 0)
+    for (size_t p = 0; p < np; ++p) {
+      void* d = nullptr;
+      H_ERR(scaph_host_device_pointer(ps.h_page[p].get(), &d));
+      ps.d_hpage[p] = reinterpret_cast<unsigned long long>(d);
+    }
   ps.pool.alloc((size_t)pool_size_ * P_ * 4);
   ps.copy.reset(new Stream((uint32_t)cfg_.tricknum));
   ps.copy_beg.reset(new Event());
@@ -229,6 +240,7 @@ RunStats Engine::bfs_paged(uint32_t src) {
 
     predict_apply(cur, learned_pages);
     const uint64_t full0 = rs.pages_full, comp0 = rs.pages_compacted, pre0 = rs.pages_prefetched;
+    const uint64_t dir0 = rs.pages_direct;
     const uint64_t h2d0 = rs.bytes_h2d;
     const double h2d_ms0 = rs.h2d_ms;
     for (size_t p = 0; p < np; ++p) {
@@ -240,6 +252,7 @@ RunStats Engine::bfs_paged(uint32_t src) {
     rec.pages_full = (uint32_t)(rs.pages_full - full0);
     rec.pages_compacted = (uint32_t)(rs.pages_compacted - comp0);
     rec.pages_prefetched = (uint32_t)(rs.pages_prefetched - pre0);
+    rec.pages_direct = (uint32_t)(rs.pages_direct - dir0);
     rec.bytes_h2d = rs.bytes_h2d - h2d0;
     rec.h2d_ms = (float)(rs.h2d_ms - h2d_ms0);
     H_ERR(scaph_copy_d2h_async(hc, cnt, sizeof(scaph_level_counters), st));
@@ -287,9 +300,32 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   const bool weighted = relax && relax->weights;
   if (relax) ps.handle.set_prediction({}, 0);
   const std::vector<char> whole = ps.handle.whole_copies(datas);
+  // uncached pages with little active data (-page_direct) are pushed
+  // straight from pinned host memory: the kernels read their active Node
+  // records and out-lists over PCIe, with no host compaction, no slot and
+  // no copy.  The page cache does not see them (their data counts as zero
+  // in queue_reset); they go out in one grouped launch ahead of the cached
+  // pages' groups.
+  std::vector<int> direct;
+  std::vector<uint32_t> sched_datas;
+  const uint32_t* qdatas = datas;
+  if (!relax && cfg_.page_direct > 0 && ps.group && !ps.d_hpage.empty() && ps.d_hpage[0]) {
+    uint64_t dn = 0;
+    for (size_t p = 0; p < np; ++p)
+      if (!ps.handle.vp[p].cached && datas[p] > 0 && !whole[p] &&
+          datas[p] <= (uint64_t)cfg_.page_direct && dn + nodes[p] <= ps.group_nodes_cap) {
+        direct.push_back((int)p);
+        dn += nodes[p];
+      }
+    if (!direct.empty()) {
+      sched_datas.assign(datas, datas + np);
+      for (int p : direct) sched_datas[p] = 0;
+      qdatas = sched_datas.data();
+    }
+  }
   std::vector<int> cold;
   for (size_t p = 0; p < np; ++p)
-    if (!ps.handle.vp[p].cached && datas[p] > 0 && !whole[p]) cold.push_back((int)p);
+    if (!ps.handle.vp[p].cached && qdatas[p] > 0 && !whole[p]) cold.push_back((int)p);
   if (!cold.empty()) {
     if (cold.size() * 4 > np) {
       H_ERR(scaph_copy_d2h_async(ps.h_front.get(), front_[fa].get(), words * 4, st));
@@ -308,10 +344,12 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   ps.resident_at_reset.assign(np, 0);
   for (size_t p = 0; p < np; ++p) ps.resident_at_reset[p] = ps.handle.vp[p].cached ? 1 : 0;
   const bool grouping = cfg_.page_groups > 0 && !relax;
-  ps.handle.queue_reset(datas, grouping);
+  ps.handle.queue_reset(qdatas, grouping);
   rs->pages_full += ps.handle.num_full;
-  // grouped launches: about page_groups of them over the cached active
-  // pages; the first is smaller when copies wait for slots
+  // grouped launches over the cached active pages: when copies wait for
+  // slots, about page_groups of them (at least kMinGroupEdges out-edges
+  // each, the first a quarter of that), so slots free as groups finish;
+  // otherwise one launch sequence for the whole level
   uint64_t group_target = 0;
   bool first_group = true;
   std::vector<int> group_pages;
@@ -319,7 +357,38 @@ void Engine::schedule_pages(int fa, uint32_t cur, const RelaxRound* relax, RunSt
   if (grouping) {
     uint64_t e = 0;
     for (int p : ps.handle.group_queue()) e += datas[p] - 3ull * nodes[p];
-    group_target = std::max<uint64_t>(e / (uint64_t)cfg_.page_groups, 1);
+    group_target = copies_pending
+                       ? std::max<uint64_t>(e / (uint64_t)cfg_.page_groups, kMinGroupEdges)
+                       : std::max<uint64_t>(e, 1);
+  }
+
+  if (!direct.empty()) {
+    auto* tab = ps.h_runs.get<scaph_page_run>();
+    uint64_t g_edges = 0, g_nodes = 0;
+    uint32_t segs = 0;
+    for (size_t k = 0; k < direct.size(); ++k) {
+      const int p = direct[k];
+      scaph_page_run& r = tab[k];
+      r.buf = ps.d_hpage[p];
+      r.first_vertex = pages_out_[p].left;
+      r.nodenum = pages_out_[p].nodenum;
+      r.seg0 = segs;
+      r.reserved = 0;
+      segs += (r.nodenum + SCAPH_RUN_SEGMENT - 1) / SCAPH_RUN_SEGMENT;
+      g_edges += datas[p] - 3ull * nodes[p];
+      g_nodes += nodes[p];
+    }
+    // on the group stream, whose first group waits for group_done: the run
+    // table in h_runs is rewritten only after this launch has finished
+    void* gs = ps.group->get();
+    H_ERR(scaph_copy_h2d_async(ps.d_runs.get(), tab, direct.size() * sizeof(scaph_page_run), gs));
+    H_ERR(scaph_bfs_push_runs(ps.d_runs.get<scaph_page_run>(), (uint32_t)direct.size(), segs,
+                              g_nodes, g_edges, n_, deg, level, visited,
+                              front_[fa].get<uint32_t>(), front_[fa ^ 1].get<uint32_t>(), cnt,
+                              cur, ps.group_ws.get(), ps.group_ws_bytes, &push_tune_, gs));
+    ps.group_done->record(gs);
+    rs->page_execs += direct.size();
+    rs->pages_direct += direct.size();
   }
 
   // compaction workers (parallel_compress_stat_async, src/compress.h:42-56)

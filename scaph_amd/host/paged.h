@@ -12,6 +12,7 @@ namespace scaph {
 
 struct Engine::PagedState {
   std::vector<PinnedBuf> h_page;  // page content (Node records + neighbours), used words
+  std::vector<unsigned long long> d_hpage;  // the same pages as kernels address them (PCIe)
   std::vector<PinnedBuf> h_tail;  // compacted copy of a cold page (<= thres words)
   DevBuf pool;                    // pool_size slots of P words
   std::vector<std::unique_ptr<Stream>> exec;

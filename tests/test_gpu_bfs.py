@@ -221,11 +221,13 @@ def test_page_counts_kernel_vs_oracle():
     (8192, 1, 12, 1, 1)])
 def test_paged_out_of_core_parity(page_words, max_mib, threshold, multi, streams):
     """Out-of-core mode: device pool smaller than the page set, pages stream
-    from pinned memory (full copies and compacted cold pages)."""
+    from pinned memory (full copies and compacted cold pages; no direct
+    pushes from host memory here)."""
     off, ngh = O.rmat_csr(14, 16, seed=6)
     deg = np.diff(off.astype(np.int64))
     with engine(f"-page_words={page_words}", f"-max_memory={max_mib}",
-                f"-threshold={threshold}", f"-multi={multi}", f"-num_stream={streams}") as e:
+                f"-threshold={threshold}", f"-multi={multi}", f"-num_stream={streams}",
+                "-page_direct=0") as e:
         e.load_csr(off, ngh)
         inf = e.info()
         assert not inf["resident"], inf
@@ -234,7 +236,7 @@ def test_paged_out_of_core_parity(page_words, max_mib, threshold, multi, streams
             want = O.bfs(off, ngh, src)
             st = e.run(src)
             np.testing.assert_array_equal(e.levels(), want)
-            assert st["page_execs"] >= 1 and st["bytes_h2d"] > 0
+            assert st["page_execs"] >= 1 and st["bytes_h2d"] > 0 and st["pages_direct"] == 0
             if threshold == 0:
                 assert st["pages_compacted"] == 0
             assert e.check(src) == 0
@@ -245,17 +247,23 @@ def test_paged_out_of_core_parity(page_words, max_mib, threshold, multi, streams
                                    ["-push_small=1", "-push_rows=false"], ["-page_groups=1"],
                                    ["-page_groups=64", "-multi=2"],
                                    ["-push_small=1", "-push_scan=1"],
-                                   ["-push_small=1", "-push_xcd=1"]])
+                                   ["-push_small=1", "-push_xcd=1"],
+                                   ["-page_direct=4096"], ["-page_direct=1099511627776"],
+                                   ["-page_direct=1099511627776", "-push_small=1"]])
 def test_paged_page_push_variants(flags):
     """Page pushes sized by the page counts: grouped launches over several
     resident pages, one-launch small pages and groups, full pages read
     through their frontier bits (pages of ~14K Nodes, so two 8192-Node
     segments, at unaligned first vertices), compacted pages, and the
-    per-page and unhinted paths, all against the oracle."""
+    per-page and unhinted paths, and uncached pages pushed straight from
+    pinned host memory (-page_direct: some, every one, through the one-launch
+    small path too), all against the oracle."""
     off, ngh = O.rmat_csr(16, 16, seed=8)
     deg = np.diff(off.astype(np.int64))
+    direct = any(f.startswith("-page_direct") for f in flags)
+    base = [] if direct else ["-page_direct=0"]
     with engine("-page_words=262144", "-max_memory=3", "-threshold=3", "-num_stream=3",
-                *flags) as e:
+                *base, *flags) as e:
         e.load_csr(off, ngh)
         inf = e.info()
         assert not inf["resident"] and inf["pages"] >= 4, inf
@@ -264,7 +272,13 @@ def test_paged_page_push_variants(flags):
             want = O.bfs(off, ngh, src)
             st = e.run(src)
             np.testing.assert_array_equal(e.levels(), want, err_msg=f"{flags} {src}")
-            assert st["bytes_h2d"] > 0
+            say("  ", {k: st[k] for k in ("bytes_h2d", "pages_direct", "pages_compacted")})
+            if direct:
+                assert st["pages_direct"] > 0
+                if "1099511627776" in flags[0]:
+                    assert st["pages_compacted"] == 0
+            else:
+                assert st["bytes_h2d"] > 0 and st["pages_direct"] == 0
 
 
 def test_pull_heavy_in_lists():

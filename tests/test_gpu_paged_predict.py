@@ -70,7 +70,7 @@ def test_layered_graph_pages():
 
 def run_mode(off, ngh, src, want, mode, extra=()):
     flags = ["-max_memory=3", f"-page_words={P}", "-build_pull=false",
-             f"-page_predict={mode}", *extra]
+             f"-page_predict={mode}", "-page_direct=0", *extra]
     with scaph_amd.BFS(*flags) as e:
         e.load_csr(off, ngh)
         assert not e.info()["resident"] and e.pool()["slots"] == 9

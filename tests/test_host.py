@@ -127,6 +127,8 @@ def test_engine_config_parse_via_capi_rejects_bad_threshold():
     # threshold > 16 would overflow the half-page compaction tail
     with pytest.raises(scaph_amd.ScaphError, match="threshold"):
         scaph_amd.BFS("-threshold=17")
+    with pytest.raises(scaph_amd.ScaphError, match="page_direct"):
+        scaph_amd.BFS("-page_direct=-1")
 
 
 def test_host_cpu_bfs_equals_oracle():
