@@ -8,6 +8,11 @@ Two steps:
       python3 -m scaph_amd.traffic run --records <dir>/records.json
   python3 -m scaph_amd.traffic summarize --dir <dir> --out profiles/<name>.json
 
+and, from a kernel-trace run into <dir>/kt, a text summary:
+
+  python3 -m scaph_amd.traffic stats --dir <dir> --out profiles/<name>.txt \\
+      --title "..." --program "<the profiled command>"
+
 `run` first reads and writes a buffer of known size with the library's own
 kernels (scaph_checksum_u32 reads 4n bytes, scaph_fill_u32 writes 4n bytes),
 then runs one workload, writing every run's per-level records:
@@ -198,7 +203,13 @@ def summarize(args):
     traffic = sum(p["fetch_bytes"] + p["write_bytes"] for p in per) / k
     alg = sum(p["alg_bytes"] for p in per) / k
     avg_us = statistics.mean(p["us_profiled"] for p in per)
-    out = {"command": args.command, "workload": rec.get("workload", "rmat"), "kernel": kernel,
+    command = args.command or (
+        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/fetch -o fetch -- python3 -m "
+        f"scaph_amd.traffic run --workload {rec.get('workload', 'rmat')} --records "
+        "<dir>/records.json; rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/write -o "
+        "write -- (the same run); python3 -m scaph_amd.traffic summarize --dir <dir> --kernel "
+        f"{kernel} --out {args.out}")
+    out = {"command": command, "workload": rec.get("workload", "rmat"), "kernel": kernel,
            "launches": k, "kernel_source_hash": bench.kernel_source_hash(),
            "calibration": {"read_kernel": "k_checksum_u32", "write_kernel": "k_fill_u32",
                            "bytes": calib_bytes, "fetch_raw_bytes": rd[0]["kib"] * 1024,
@@ -233,6 +244,32 @@ def summarize(args):
                                              "alg_frac_of_8tbps")}))
 
 
+def stats(args):
+    """The kernel-trace summary as text: per kernel (short name) its calls,
+    total and average duration and share of the process's kernel time."""
+    with open(os.path.join(args.dir, "kt", "kt_kernel_stats.csv")) as f:
+        rows = list(csv.DictReader(f))
+    rows.sort(key=lambda r: -int(r["TotalDurationNs"]))
+    total = sum(int(r["TotalDurationNs"]) for r in rows) or 1
+    lines = [f"# {args.title}",
+             f"# command: rocprofv3 --kernel-trace --stats --output-format csv -d <dir>/kt -o kt "
+             f"-- {args.program}; python3 -m scaph_amd.traffic stats --dir <dir> --out {args.out}",
+             "# per-kernel totals of the whole process (graph build included)",
+             f"{'kernel':<28} {'calls':>7} {'total_ms':>10} {'avg_us':>10} {'pct':>6}"]
+    def short(name):
+        if "rocprim" in name:
+            return "rocprim " + ("radix sort" if "radix_sort" in name else
+                                 "partition" if "partition" in name else "primitive")
+        return _kernel(name)[:28]
+    for r in rows[:args.top]:
+        t = int(r["TotalDurationNs"])
+        lines.append(f"{short(r['Name']):<28} {int(r['Calls']):>7} {t / 1e6:>10.3f} "
+                     f"{float(r['AverageNs']) / 1e3:>10.2f} {100.0 * t / total:>6.2f}")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -249,8 +286,14 @@ def main():
     s.add_argument("--out", required=True)
     s.add_argument("--command", default="")
     s.add_argument("--kernel", choices=["k_pull", "k_expand", "k_ms_pull"], default="k_pull")
+    t = sub.add_parser("stats")
+    t.add_argument("--dir", required=True)
+    t.add_argument("--out", required=True)
+    t.add_argument("--title", required=True)
+    t.add_argument("--program", required=True, help="the profiled command line, for the header")
+    t.add_argument("--top", type=int, default=24)
     a = ap.parse_args()
-    run(a) if a.cmd == "run" else summarize(a)
+    {"run": run, "summarize": summarize, "stats": stats}[a.cmd](a)
 
 
 if __name__ == "__main__":
