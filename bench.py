@@ -442,8 +442,11 @@ def main():
                 "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_bytes, "launches": launches,
                 "share_of_kernel_time": ms_tot / max(agg[0][0] + agg[1][0], 1e-9)}
 
-    # the learned schedule (committed model) on the same sources: same step
-    # count, levels compared with the fixed schedule's for every source
+    # the learned schedule (committed model) on the same sources, paired
+    # with the fixed schedule source by source (fixed, then learned, on the
+    # same engine, one after the other) so that clock and cache drift between
+    # phases cannot enter the ratio; levels compared with the fixed
+    # schedule's for the first learned_check sources
     learned = None
     model_path = os.path.join(ROOT, "scaph_amd", "schedule_mlp.bin")
     if args.learned and os.path.exists(model_path):
@@ -454,30 +457,40 @@ def main():
             eng.set_schedule("fixed")
             eng.run(src)
             fixed_levels[src] = eng.levels()
-        eng.set_schedule("learned")
         for i in range(args.warmup):
-            eng.run(srcs[i % len(srcs)])
+            for sched in ("fixed", "learned"):
+                eng.set_schedule(sched)
+                eng.run(srcs[i % len(srcs)])
         _lib.check(hip.scaph_device_sync())
-        lruns = []
+        fruns, lruns = [], []
         for k in range(args.steps):
-            st = eng.run(srcs[k % len(srcs)])
-            st["dirs"] = [r["dir"] for r in eng.records()]
-            lruns.append(st)
+            for sched, out in (("fixed", fruns), ("learned", lruns)):
+                eng.set_schedule(sched)
+                st = eng.run(srcs[k % len(srcs)])
+                st["dirs"] = [r["dir"] for r in eng.records()]
+                out.append(st)
         _lib.check(hip.scaph_device_sync())
         equal = True
+        eng.set_schedule("learned")
         for src, want in fixed_levels.items():
             eng.run(src)
             equal &= bool(np.array_equal(eng.levels(), want))
         eng.set_schedule("fixed")
         lg = [r["m_reached"] / (r["ms"] / 1e3) / 1e9 for r in lruns]
         lt = statistics.median(r["ms"] for r in lruns)
-        ft = statistics.median(r["ms"] for r in runs)
-        same_dirs = sum(1 for a, b in zip(runs, lruns)
-                        if [x["dir"] for x in a["records"]] == b["dirs"])
+        ft = statistics.median(r["ms"] for r in fruns)
+        same_dirs = sum(1 for a, b in zip(fruns, lruns) if a["dirs"] == b["dirs"])
         learned = {"gteps_median": statistics.median(lg), "t_bfs_ms_median": lt,
-                   "speedup_vs_fixed": ft / lt, "levels_equal_fixed": equal,
-                   "sources_checked": len(fixed_levels),
-                   "runs_with_same_directions": same_dirs, "model": "scaph_amd/schedule_mlp.bin"}
+                   "fixed_t_bfs_ms_median_paired": ft,
+                   "speedup_vs_fixed": ft / lt,
+                   "speedup_vs_fixed_total": sum(r["ms"] for r in fruns) /
+                                             sum(r["ms"] for r in lruns),
+                   "levels_equal_fixed": equal, "sources_checked": len(fixed_levels),
+                   "runs_with_same_directions": same_dirs, "paired_runs": len(lruns),
+                   "learned_margin": next((float(f.split("=", 1)[1]) for f in args.flag
+                                           if f.lstrip("-").startswith("learned_margin=")),
+                                          0.5),
+                   "model": "scaph_amd/schedule_mlp.bin"}
         log("learned schedule", learned)
 
     # parity at full size: device invariant check on the last source

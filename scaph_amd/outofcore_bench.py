@@ -111,6 +111,10 @@ def main():
         ln = statistics.median(c["learned_ms"] for c in cmp_)
         schedules = {"per_source": cmp_, "fixed_ms_median": fx, "learned_ms_median": ln,
                      "learned_speedup_vs_fixed": fx / ln,
+                     "learned_speedup_vs_fixed_total":
+                         sum(c["fixed_ms"] for c in cmp_) / sum(c["learned_ms"] for c in cmp_),
+                     "slowest_source_learned_over_fixed":
+                         max(c["learned_ms"] / c["fixed_ms"] for c in cmp_),
                      "push_only_ms_median": statistics.median(res_ms),
                      "fixed_speedup_vs_push_only": statistics.median(res_ms) / fx,
                      "levels_equal_push_only": exact, "model": "scaph_amd/schedule_mlp.bin"}
