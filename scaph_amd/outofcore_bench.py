@@ -52,6 +52,9 @@ def main():
     ap.add_argument("--pools", default="",
                     help="page pool sizes in bytes (e.g. 16e9); -max_memory is derived")
     ap.add_argument("--sources", type=int, default=4)
+    ap.add_argument("--reps", type=int, default=3,
+                    help="paged runs per source, one after the other (the reference's bench "
+                         "repeats one source); the median per source is kept")
     ap.add_argument("--threshold", type=int, default=16,
                     help="-threshold of the paged runs in 1/32 pages: 16 compacts pages whose "
                          "active data is at most half a page (the reference's FLAGS_threshold is "
@@ -158,16 +161,22 @@ def main():
             if mode == "exact":
                 eng.set_page_levels(want[sources[0]][0])
             eng.run(sources[0])  # warm: first touches of the pinned pages
+            per_src = []
             for s in sources:
                 if mode == "exact":
                     eng.set_page_levels(want[s][0])
-                st = eng.run(s)
-                exact &= bool(np.array_equal(eng.levels(), want[s][0]))
-                runs.append(st)
-                log(" ", s, {k: st[k] for k in ("ms", "page_execs", "page_execs_cached",
-                                                 "pages_full", "pages_compacted", "pages_prefetched",
-                                                 "bytes_h2d", "h2d_ms")})
-            ms = statistics.median(r["ms"] for r in runs)
+                rep = []
+                for _ in range(args.reps):
+                    st = eng.run(s)
+                    exact &= bool(np.array_equal(eng.levels(), want[s][0]))
+                    runs.append(st)
+                    rep.append(st["ms"])
+                    log(" ", s, {k: st[k] for k in ("ms", "page_execs", "page_execs_cached",
+                                                     "pages_full", "pages_compacted",
+                                                     "pages_prefetched", "pages_direct",
+                                                     "bytes_h2d", "h2d_ms")})
+                per_src.append(statistics.median(rep))
+            ms = statistics.median(per_src)
             execs = sum(r["page_execs"] for r in runs)
             cached = sum(r["page_execs_cached"] for r in runs)
             h2d = sum(r["bytes_h2d"] for r in runs)
@@ -182,11 +191,13 @@ def main():
                 "gteps_median": statistics.median(r["m_reached"] / (r["ms"] / 1e3) / 1e9
                                                   for r in runs),
                 "t_bfs_ms_median": ms,
+                "t_bfs_ms_per_source": per_src,
                 "t_bfs_ms": [r["ms"] for r in runs],
                 "page_hit_rate": cached / execs if execs else None,
                 "page_execs": execs, "pages_full_copies": sum(r["pages_full"] for r in runs),
                 "pages_compacted": sum(r["pages_compacted"] for r in runs),
                 "pages_copied_ahead": sum(r["pages_prefetched"] for r in runs),
+                "pages_direct": sum(r["pages_direct"] for r in runs),
                 "h2d_bytes_per_bfs": h2d / len(runs),
                 "pcie_gbps_over_bfs": h2d / (tot_ms / 1e3) / 1e9 if tot_ms else None,
                 "pcie_gbps_during_copies": h2d / (h2d_ms / 1e3) / 1e9 if h2d_ms else None,
