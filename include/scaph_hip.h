@@ -243,7 +243,7 @@ typedef struct scaph_pull_tuning {
   uint32_t grid_blocks; /* cap on 256-thread blocks (each owns 4 x 1024-vertex windows per pass) */
   uint32_t head_outdeg; /* in_head records are {e0, e1, in-degree, out-degree} */
   uint32_t no_prefetch; /* do not request the next batch's head records early */
-  uint32_t reserved;
+  uint32_t head_indexed; /* heads from scaph_list_heads_indexed (implies head_outdeg) */
 } scaph_pull_tuning;
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,
                    const uint32_t* hub_list, uint32_t num_hubs,
@@ -440,6 +440,18 @@ int scaph_live_window_base(const uint32_t* no_in_edges, uint32_t n, uint32_t* ba
 int scaph_list_heads_live(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
                           const uint32_t* out_degree, const uint32_t* no_in_edges,
                           const uint32_t* base, uint32_t* head, void* stream);
+/* Indexed head records {first two entries, (offset << 8) | min(length, 255),
+ * out_degree[v]}: offset is v's Node idx, the word offset of its list in its
+ * page's edge array (pages: the page table of the same CSR, page_cuts; every
+ * page's edges must number below 2^24).  A pull that finds no parent in the
+ * first two entries then reads the list without the Node record; a length
+ * field of 255 means 255 or more (the Node is read).  With no_in_edges and
+ * base (scaph_live_window_base) only vertices with in-edges get records, as
+ * in scaph_list_heads_live; with both NULL record v is at index v. */
+int scaph_list_heads_indexed(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
+                             const uint32_t* out_degree, const scaph_page* pages,
+                             uint32_t num_pages, const uint32_t* no_in_edges, const uint32_t* base,
+                             uint32_t* head, void* stream);
 /* offsets[0..n] = exclusive prefix of degree[0..n) (offsets[n] = total);
  * allocates its own scan storage and returns after the stream drains. */
 int scaph_degrees_to_offsets(const uint32_t* degree, uint64_t* offsets, uint32_t n, void* stream);

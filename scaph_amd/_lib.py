@@ -176,6 +176,7 @@ def hip():
         _sig(lib, "scaph_list_heads_from_csr", I32, [P, P, U32, P, P, P])
         _sig(lib, "scaph_live_window_base", I32, [P, U32, P, P])
         _sig(lib, "scaph_list_heads_live", I32, [P, P, U32, P, P, P, P, P])
+        _sig(lib, "scaph_list_heads_indexed", I32, [P, P, U32, P, P, U32, P, P, P, P])
         _sig(lib, "scaph_frontier_queue_to_bitmap", I32, [P, U32, P, U32, P])
         _sig(lib, "scaph_frontier_bitmap_to_queue", I32, [P, U32, P, P, P])
         _sig(lib, "scaph_page_counts", I32, [P, U32, P, P, I32, P, P, P, P])

@@ -943,6 +943,18 @@ __device__ __forceinline__ NodeRef load_node_hint(const PageTab& t, const scaph_
   return r;
 }
 
+// The edge array of v's page (its list starts at the Node idx there),
+// from the page table alone: no Node record is read.
+__device__ __forceinline__ const uint32_t* page_edges_hint(const PageTab& t,
+                                                           const scaph_graph_desc& g, uint32_t v,
+                                                           uint32_t hint) {
+  uint32_t p = hint;
+  while (p + 1 < g.num_pages && page_left(t, g, p + 1) <= v) ++p;
+  if (t.in_lds)
+    return reinterpret_cast<const uint32_t*>(t.ptr[p]) + 3ull * t.nodenum[p];
+  return reinterpret_cast<const uint32_t*>(g.page_ptr[p]) + 3ull * g.pages[p].nodenum;
+}
+
 constexpr uint32_t kPullWinWords = 32;
 constexpr uint32_t kPullWin = kPullWinWords * 32;
 constexpr uint32_t kPullProbe = 2;
@@ -956,7 +968,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 template <int PREFETCH>
 __global__ void __launch_bounds__(kBlock)
 k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint32_t* live_base,
-       uint32_t head_outdeg, const uint32_t* degree, uint32_t* level,
+       uint32_t head_outdeg, uint32_t head_indexed, const uint32_t* degree, uint32_t* level,
        uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
        scaph_level_counters* counters, uint32_t next_level, PullWs hw, uint32_t num_hubs) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
@@ -1053,9 +1065,11 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
           // lists start with their hubs) never touches its page
           // (head_outdeg: {e0, e1, in-degree, out-degree} instead, so a
           // found vertex also needs no degree[] load)
+          // (head_indexed: {e0, e1, Node idx << 8 | min(in-degree, 255),
+          // out-degree}, so a list past its head needs no Node record)
           const uint4 hd = PREFETCH ? hd_cur : head[hidx(loc)];
           const uint32_t cap = head_outdeg ? 2u : 3u;
-          const uint32_t len = head_outdeg ? hd.z : hd.w;
+          const uint32_t len = head_indexed ? (hd.z & 0xFFu) : head_outdeg ? hd.z : hd.w;
           if (head_outdeg) outdeg = hd.w;
           nr.len = len;
           const uint32_t l0 = len < cap ? len : cap;
@@ -1065,7 +1079,10 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
           k = l0;
           found = h0 | h1 | h2;
           if (!found && len > cap) {
-            nr = load_node_hint(s, g, v, hint);
+            if (head_indexed && len < 255u)
+              nr.edges = page_edges_hint(s, g, v, hint) + (hd.z >> 8);
+            else
+              nr = load_node_hint(s, g, v, hint);
           }
         } else {
           nr = load_node_hint(s, g, v, hint);
@@ -1820,7 +1837,8 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
   // tail of late blocks; measured best on s26 among 512..4096 blocks)
   unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 4u * cu_count());
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
-  const uint32_t outdeg_heads = tune ? tune->head_outdeg : 0u;
+  const uint32_t indexed_heads = tune ? tune->head_indexed : 0u;
+  const uint32_t outdeg_heads = tune ? (tune->head_outdeg | indexed_heads) : 0u;
   const size_t lds = page_lds_bytes(in_graph->num_pages);
   const uint4* head = heads ? reinterpret_cast<const uint4*>(heads->head) : nullptr;
   const uint32_t* no_in = head ? heads->no_in_edges : nullptr;
@@ -1828,12 +1846,12 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
   if (head && heads->live_base && !no_in) return SCAPH_STATUS_INVALID;
   if (!tune || !tune->no_prefetch)
     hipLaunchKernelGGL(k_pull<1>, dim3(grid), dim3(kBlock), lds, st, *in_graph, head, no_in,
-                       live_base, outdeg_heads, degree, level, visited, front_in, front_out,
-                       counters, cur_level + 1, hw, num_hubs);
+                       live_base, outdeg_heads, indexed_heads, degree, level, visited, front_in,
+                       front_out, counters, cur_level + 1, hw, num_hubs);
   else
     hipLaunchKernelGGL(k_pull<0>, dim3(grid), dim3(kBlock), lds, st, *in_graph, head, no_in,
-                       live_base, outdeg_heads, degree, level, visited, front_in, front_out,
-                       counters, cur_level + 1, hw, num_hubs);
+                       live_base, outdeg_heads, indexed_heads, degree, level, visited, front_in,
+                       front_out, counters, cur_level + 1, hw, num_hubs);
   SCAPH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), 0, st, hw.chunks, hw.count, degree,
                      level, visited, front_in, front_out, counters, cur_level + 1, hw.hub_front,

@@ -123,6 +123,50 @@ k_list_heads_live(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
   }
 }
 
+// Indexed head records (scaph_list_heads_indexed): a thread per 1024-id
+// window finds the page of the window's first vertex once, then walks the
+// window's vertices (all of them, or those with in-edges when no_in is set)
+// with the page advancing as ids pass its end.
+__global__ void __launch_bounds__(kBlock)
+k_list_heads_idx(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
+                 const uint32_t* out_degree, const scaph_page* pages, uint32_t num_pages,
+                 const uint32_t* no_in, const uint32_t* base, uint32_t windows, uint4* head) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < windows; k += stride) {
+    const uint32_t v0 = k * 1024u;
+    uint32_t lo = 0, hi = num_pages;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pages[mid].left <= v0) lo = mid; else hi = mid;
+    }
+    uint32_t p = lo;
+    uint64_t page_e0 = offsets[pages[p].left];
+    uint32_t r = base ? base[k] : v0;
+    for (uint64_t w = (uint64_t)k * 32; w < (uint64_t)k * 32 + 32 && w < words; ++w) {
+      uint32_t bits = no_in ? ~no_in[w] : ~0u;
+      while (bits) {
+        const uint32_t b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const uint32_t v = (uint32_t)(w * 32 + b);
+        if (v >= n) continue;
+        while (p + 1 < num_pages && pages[p + 1].left <= v) {
+          ++p;
+          page_e0 = offsets[pages[p].left];
+        }
+        const uint64_t o = offsets[v];
+        const uint32_t d = (uint32_t)(offsets[v + 1] - o);
+        uint4 rec;
+        rec.x = d > 0 ? ngh[o] : 0xFFFFFFFFu;
+        rec.y = d > 1 ? ngh[o + 1] : 0xFFFFFFFFu;
+        rec.z = ((uint32_t)(o - page_e0) << 8) | (d < 255u ? d : 255u);
+        rec.w = out_degree[v];
+        head[r++] = rec;
+      }
+    }
+  }
+}
+
 // Power-law digraph edges: edge e belongs to the last vertex v with
 // offsets[v] <= e (offsets = exclusive prefix of the out-degrees, n + 1
 // entries); its destination is the multiply-high of SplitMix64 word e of the
@@ -318,6 +362,22 @@ int scaph_list_heads_live(const uint64_t* offsets, const uint32_t* ngh, uint32_t
   hipLaunchKernelGGL(k_list_heads_live, dim3(grid_for(windows, kBlock, 4096)), dim3(kBlock), 0,
                      as_stream(stream), offsets, ngh, n, out_degree, no_in_edges, base, windows,
                      reinterpret_cast<uint4*>(head));
+  SCAPH_LAUNCH_CHECK();
+  return 0;
+}
+
+int scaph_list_heads_indexed(const uint64_t* offsets, const uint32_t* ngh, uint32_t n,
+                             const uint32_t* out_degree, const scaph_page* pages,
+                             uint32_t num_pages, const uint32_t* no_in_edges, const uint32_t* base,
+                             uint32_t* head, void* stream) {
+  if (!offsets || !out_degree || !pages || num_pages == 0 || !head || n == 0 ||
+      (!no_in_edges) != (!base))
+    return SCAPH_STATUS_INVALID;
+  const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
+  const uint32_t windows = (uint32_t)((words + 31) / 32);
+  hipLaunchKernelGGL(k_list_heads_idx, dim3(grid_for(windows, kBlock, 4096)), dim3(kBlock), 0,
+                     as_stream(stream), offsets, ngh, n, out_degree, pages, num_pages,
+                     no_in_edges, base, windows, reinterpret_cast<uint4*>(head));
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

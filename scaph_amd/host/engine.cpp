@@ -66,6 +66,9 @@ Config& global_flags() {
           "learned schedule: the model decides when |pull logit| >= this, else the fixed rule");
     r.add("sssp_delta", FlagType::kInt32, &c.sssp_delta,
           "SSSP: relax only vertices below a bound that grows by this much (0 = all)");
+    r.add("pull_head_index", FlagType::kBool, &c.pull_head_index,
+          "pull head records carry the in-list's offset in its page, so a list past its head is "
+          "read without its Node record");
     r.add("pull_prefetch", FlagType::kBool, &c.pull_prefetch,
           "request the next batch's head records before probing the current one");
     r.add("pull_head_live", FlagType::kBool, &c.pull_head_live,
@@ -428,8 +431,11 @@ void Engine::load_csr(HostCsr g, bool keep_host, HostCsr* in_edges) {
         head.reserve((size_t)n_ * 4);
         uint32_t live = 0;
         const bool live_only = cfg_.pull_head_live;
+        const bool indexed = heads_indexable(pages_in_);
+        size_t pg = 0;
         for (uint32_t v = 0; v < n_; ++v) {
           if ((v & 1023u) == 0) base[v >> 10] = live;
+          while (pg + 1 < pages_in_.size() && pages_in_[pg + 1].left <= v) ++pg;
           const uint32_t d = t.degree(v);
           if (d == 0 && live_only) continue;
           ++live;
@@ -441,8 +447,14 @@ void Engine::load_csr(HostCsr g, bool keep_host, HostCsr* in_edges) {
             r[2] = d;
             r[3] = h_deg_out_[v];
           }
+          if (indexed) {
+            // the list's word offset in its page's edge array (its Node idx)
+            const uint64_t idx = o - t.offsets[pages_in_[pg].left];
+            r[2] = (uint32_t)(idx << 8) | std::min<uint32_t>(d, 255u);
+          }
           head.insert(head.end(), r, r + 4);
         }
+        pull_tune_.head_indexed = indexed ? 1u : 0u;
         for (size_t k = (n_ + 1023u) >> 10; k < windows; ++k) base[k] = live;
         // one spare record: nothing reads past the last live vertex's
         head.insert(head.end(), 4, 0xFFFFFFFFu);
@@ -672,11 +684,19 @@ void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
     H_ERR(scaph_pages_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(),
                                dpages.get<scaph_page>(), dptr.get<uint64_t>(),
                                (uint32_t)pages.size(), n, st));
+    const bool indexed = dir == 1 && cfg_.pull_head && heads_indexable(pages);
+    if (dir == 1) pull_tune_.head_indexed = indexed ? 1u : 0u;
     if (dir == 1 && cfg_.pull_head && !cfg_.pull_head_live) {
       in_head_.alloc((size_t)n * 16);
-      H_ERR(scaph_list_heads_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
-                                      cfg_.pull_head_outdeg ? deg_out_.get<uint32_t>() : nullptr,
-                                      in_head_.get<uint32_t>(), st));
+      if (indexed)
+        H_ERR(scaph_list_heads_indexed(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
+                                       deg_out_.get<uint32_t>(), dpages.get<scaph_page>(),
+                                       (uint32_t)pages.size(), nullptr, nullptr,
+                                       in_head_.get<uint32_t>(), st));
+      else
+        H_ERR(scaph_list_heads_from_csr(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
+                                        cfg_.pull_head_outdeg ? deg_out_.get<uint32_t>() : nullptr,
+                                        in_head_.get<uint32_t>(), st));
     } else if (dir == 1 && cfg_.pull_head) {
       // records of the vertices with in-edges only (deg holds the in-degrees)
       static_visited_.alloc(bitmap_words(n) * 4);
@@ -700,10 +720,16 @@ void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
         for (size_t k = 0; k < nw; ++k) live += (uint32_t)__builtin_popcount(~tail[k]);
       }
       in_head_.alloc(((size_t)live + 1) * 16);
-      H_ERR(scaph_list_heads_live(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
-                                  cfg_.pull_head_outdeg ? deg_out_.get<uint32_t>() : nullptr,
-                                  static_visited_.get<uint32_t>(), live_base_.get<uint32_t>(),
-                                  in_head_.get<uint32_t>(), st));
+      if (indexed)
+        H_ERR(scaph_list_heads_indexed(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
+                                       deg_out_.get<uint32_t>(), dpages.get<scaph_page>(),
+                                       (uint32_t)pages.size(), static_visited_.get<uint32_t>(),
+                                       live_base_.get<uint32_t>(), in_head_.get<uint32_t>(), st));
+      else
+        H_ERR(scaph_list_heads_live(offs.get<uint64_t>(), ngh.get<uint32_t>(), n,
+                                    cfg_.pull_head_outdeg ? deg_out_.get<uint32_t>() : nullptr,
+                                    static_visited_.get<uint32_t>(), live_base_.get<uint32_t>(),
+                                    in_head_.get<uint32_t>(), st));
     }
     main_->sync();
     desc.page_ptr = dptr.get<const uint64_t>();
@@ -752,6 +778,13 @@ uint32_t Engine::in_degree(uint32_t v) {
   H_ERR(scaph_copy_d2h_async(&d, deg_in_.get<uint32_t>() + v, 4, main_->get()));
   main_->sync();
   return d;
+}
+
+bool Engine::heads_indexable(const std::vector<scaph_page>& in_pages) const {
+  if (!cfg_.pull_head_index || !cfg_.pull_head_outdeg) return false;
+  for (const auto& p : in_pages)
+    if (p.edgenum >= (1u << 24)) return false;
+  return true;
 }
 
 // Beamer's direction rule (GAP's form): push -> pull when the frontier's

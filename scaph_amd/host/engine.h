@@ -50,6 +50,8 @@ struct Config {
   bool pull_head_outdeg = true;  // head records {e0, e1, in-degree, out-degree}
   bool pull_head_live = true;    // head records only for vertices with in-edges (live rank)
   bool pull_prefetch = true;     // request the next batch's head records early
+  bool pull_head_index = true;   // head records carry the list's page offset (no Node read past
+                                 // the head; needs -pull_head_outdeg and pages below 2^24 edges)
   bool learned_fused = true;     // a scalar-feature model decides inside the level hand-off
   double learned_margin = 0.5;   // the model decides only when |pull logit| >= this; nearer
                                  // the boundary the fixed rule does
@@ -199,6 +201,9 @@ class Engine {
   // normalisation; used by the level loop when -schedule=learned.
   void set_device_model(const uint16_t* params, const float* mean, const float* scale);
   bool device_model_ready() const;
+  // indexed pull heads (-pull_head_index): with outdeg records and every
+  // in-page holding fewer than 2^24 edges (the offset field's width)
+  bool heads_indexable(const std::vector<scaph_page>& in_pages) const;
   // Training traces: at every level both directions run from the same
   // state (min of `reps` timings each), the faster one continues.
   struct ExploreRecord {

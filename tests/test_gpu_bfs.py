@@ -78,10 +78,16 @@ def test_rmat_parity_with_oracle(scale, ef, seed, page_words):
                                   ["-pull_head_live=false"],
                                   ["-pull_head_live=false", "-pull_head_outdeg=false",
                                    "-pull_prefetch=false"],
-                                  ["-pull_prefetch=false"]])
+                                  ["-pull_prefetch=false"],
+                                  ["-pull_head_index=false"],
+                                  ["-pull_head_index=true", "-pull_head_live=false"],
+                                  ["-pull_head_index=true", "-pull_prefetch=false"],
+                                  ["-pull_head_index=true", "-pull_head_outdeg=false"]])
 def test_pull_head_records(head):
     """The pull expand with and without the 16-byte in-list head records,
-    on the host-loaded and the device-built graph."""
+    on the host-loaded and the device-built graph; indexed records (the
+    list's page offset beside its first two entries; in-degrees of 255 and
+    more read the Node) with and without the live-only numbering."""
     off, ngh = O.rmat_csr(15, 16, seed=8)
     deg = np.diff(off.astype(np.int64))
     srcs = [int(np.argmax(deg)), int(np.nonzero(deg)[0][3])]
@@ -98,6 +104,66 @@ def test_pull_head_records(head):
                 e.set_schedule(sched)
                 e.run(src)
                 np.testing.assert_array_equal(e.levels(), O.bfs(off, ngh, src))
+
+
+def test_indexed_head_records_builder():
+    """scaph_list_heads_indexed against a numpy restatement: {e0, e1,
+    (offset of the list in its page's edges) << 8 | min(length, 255),
+    out-degree}, for every vertex and for the vertices with a nonempty list
+    only (numbered per 1024-id window), on pages of 4096 words with lists of
+    0, 1, 2 and more than 255 entries."""
+    hip = _lib.hip()
+    off, ngh = O.rmat_csr(13, 16, seed=3)
+    n = len(off) - 1
+    deg = np.diff(off.astype(np.int64))
+    assert (deg == 0).any() and (deg == 1).any() and (deg >= 255).any()
+    pages = O.page_cuts(off, 4096)
+    assert len(pages) > 8
+    outdeg = np.random.default_rng(2).integers(0, 1 << 20, n).astype(np.uint32)
+    page_of = np.searchsorted(pages[:, 0].astype(np.int64), np.arange(n), side="right") - 1
+    idx = off[:-1].astype(np.int64) - off[pages[page_of, 0]].astype(np.int64)
+    ng = ngh.astype(np.int64)
+    o = off[:-1].astype(np.int64)
+    want = np.empty((n, 4), np.uint32)
+    want[:, 0] = np.where(deg > 0, ng[np.minimum(o, len(ng) - 1)], 0xFFFFFFFF)
+    want[:, 1] = np.where(deg > 1, ng[np.minimum(o + 1, len(ng) - 1)], 0xFFFFFFFF)
+    want[:, 2] = (idx << 8) | np.minimum(deg, 255)
+    want[:, 3] = outdeg
+    words = ((n + 63) // 64) * 2
+    bits = np.ones(words * 32, np.uint8)
+    bits[:n] = deg == 0
+    no_in = np.packbits(bits, bitorder="little").view(np.uint32)
+    live = deg > 0
+    windows = (words + 31) // 32
+    base = np.array([live[:1024 * k].sum() for k in range(windows)], np.uint32)
+    bufs = []
+
+    def dev(arr):
+        p = ctypes.c_void_p()
+        _lib.check(hip.scaph_dev_alloc(ctypes.byref(p), max(arr.nbytes, 8)))
+        _lib.check(hip.scaph_copy_h2d_async(p, arr.ctypes.data, arr.nbytes, None))
+        bufs.append(p)
+        return p
+
+    try:
+        d_off, d_ngh = dev(np.ascontiguousarray(off, np.uint64)), dev(ngh)
+        d_out, d_pages = dev(outdeg), dev(np.ascontiguousarray(pages))
+        d_no, d_base = dev(no_in), dev(base)
+        for live_only in (False, True):
+            rows = int(live.sum()) if live_only else n
+            got = np.zeros((rows, 4), np.uint32)
+            d_head = dev(got)
+            _lib.check(hip.scaph_list_heads_indexed(d_off, d_ngh, n, d_out, d_pages, len(pages),
+                                                    d_no if live_only else None,
+                                                    d_base if live_only else None, d_head, None))
+            _lib.check(hip.scaph_device_sync())
+            _lib.check(hip.scaph_copy_d2h_async(got.ctypes.data, d_head, got.nbytes, None))
+            _lib.check(hip.scaph_device_sync())
+            np.testing.assert_array_equal(got, want[live] if live_only else want,
+                                          err_msg=f"live_only={live_only}")
+    finally:
+        for p in bufs:
+            hip.scaph_dev_free(p)
 
 
 def test_config1_s20_ef16_seed1_from_max_degree():
