@@ -214,8 +214,9 @@ int scaph_bfs_push_noalloc(const scaph_graph_desc* out_graph, const uint32_t* de
 /* Bottom-up (pull) expand of one level: every unvisited vertex scans its
  * in-neighbours (transposed paged CSR) for one in the frontier bitmap.
  * In-list entries with the top bit set are hub indices into hub_list
- * (num_hubs <= 131072; see scaph_csr_from_keys): their frontier bits are
- * gathered once per level and probed from LDS.  Writes level/visited for
+ * (num_hubs <= 262144; see scaph_csr_from_keys): their frontier bits are
+ * gathered once per level and probed from LDS (num_hubs / 8 bytes per block,
+ * so the hub count sets how many pull blocks stay resident on a CU).  Writes level/visited for
  * the vertices found and the full next-frontier bitmap (all words, so it
  * needs no clearing).  In-lists still unmatched after 512 entries are
  * finished by a second, grid-wide pass.  max_edges bounds the graph's
@@ -240,10 +241,12 @@ typedef struct scaph_pull_heads {
 } scaph_pull_heads;
 /* Launch shape of the pull expand (NULL or 0 fields = built-in choice). */
 typedef struct scaph_pull_tuning {
-  uint32_t grid_blocks; /* cap on 256-thread blocks (each owns 4 x 1024-vertex windows per pass) */
+  uint32_t grid_blocks; /* cap on blocks (each owns block_waves x 1024-vertex windows per pass) */
   uint32_t head_outdeg; /* in_head records are {e0, e1, in-degree, out-degree} */
   uint32_t no_prefetch; /* do not request the next batch's head records early */
   uint32_t head_indexed; /* heads from scaph_list_heads_indexed (implies head_outdeg) */
+  uint32_t block_waves; /* waves per pull block: 4 or 8 (0: 8); the grid is what stays resident */
+  uint32_t waves_per_simd; /* 8: registers capped so 8 waves share a SIMD (0: compiler's choice) */
 } scaph_pull_tuning;
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,
                    const uint32_t* hub_list, uint32_t num_hubs,

@@ -16,7 +16,9 @@
 // 2048 edges (1024 on levels routed to XCDs by target), and every 256-thread
 // block expands whole tiles — a hub with a million out-edges is spread over
 // hundreds of blocks instead of one.
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "common.h"
 
@@ -29,7 +31,7 @@ constexpr unsigned long long kSmallEdges = 8192;  // push frontiers this small r
 constexpr uint32_t kPullSerial = 8;                // per-lane scan before wave help
 constexpr uint32_t kHeavy = 512;                   // longer in-lists go to k_pull_heavy
 constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work item
-constexpr uint32_t kMaxHubs = 131072;              // hub frontier bits kept in LDS (16 KB)
+constexpr uint32_t kMaxHubs = 262144;              // hub frontier bits kept in LDS (<= 32 KB)
 constexpr uint32_t kHubTag = 0x80000000u;          // in-list entry = kHubTag | hub index
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kBitsNodes = SCAPH_RUN_SEGMENT;  // Nodes per run segment (32 per thread)
@@ -899,6 +901,15 @@ k_hub_gather(const uint32_t* hub_list, uint32_t num_hubs, const uint32_t* front,
   if (lane == 32 && base + 1 < kMaxHubs / 32) hub_front[base + 1] = (uint32_t)(m >> 32);
 }
 
+// The hub bits sit in dynamic LDS behind the page table; a launch passes
+// page_lds_bytes(num_pages) + hub_lds_bytes(num_hubs) bytes.
+inline size_t hub_lds_bytes(uint32_t num_hubs) { return ((size_t)(num_hubs + 127) / 128) * 16; }
+
+__device__ __forceinline__ uint32_t* hub_lds(uint8_t* dyn_lds, uint32_t num_pages) {
+  const uint32_t np = num_pages < SCAPH_LDS_PAGES ? num_pages : SCAPH_LDS_PAGES;
+  return reinterpret_cast<uint32_t*>(dyn_lds + (size_t)np * 16);
+}
+
 __device__ __forceinline__ void stage_hubs(uint32_t* s_hub, const uint32_t* hub_front,
                                            uint32_t num_hubs) {
   for (uint32_t i = threadIdx.x; i < (num_hubs + 31) / 32; i += blockDim.x) s_hub[i] = hub_front[i];
@@ -958,6 +969,7 @@ __device__ __forceinline__ const uint32_t* page_edges_hint(const PageTab& t,
 constexpr uint32_t kPullWinWords = 32;
 constexpr uint32_t kPullWin = kPullWinWords * 32;
 constexpr uint32_t kPullProbe = 2;
+constexpr uint32_t kPullWaves = 8;  // waves per pull block by default
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -965,21 +977,24 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int PREFETCH>
-__global__ void __launch_bounds__(kBlock)
+// WAVES waves per block share one copy of the hub bits; the window queue
+// holds 10-bit window offsets as u16, so 8-wave blocks fit three to a CU.
+// MINW > 1 asks the compiler for registers that let MINW waves share a SIMD.
+template <int PREFETCH, int WAVES, int MINW>
+__global__ void __launch_bounds__(WAVES * SCAPH_WAVE, MINW)
 k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint32_t* live_base,
        uint32_t head_outdeg, uint32_t head_indexed, const uint32_t* degree, uint32_t* level,
        uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
        scaph_level_counters* counters, uint32_t next_level, PullWs hw, uint32_t num_hubs) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
-  __shared__ uint32_t s_q[kBlock / SCAPH_WAVE][kPullWin];
-  __shared__ uint32_t s_fw[kBlock / SCAPH_WAVE][kPullWinWords];
-  __shared__ uint32_t s_hub[kMaxHubs / 32];
-  __shared__ unsigned long long red[4][kBlock / SCAPH_WAVE];
+  __shared__ uint16_t s_q[WAVES][kPullWin];
+  __shared__ uint32_t s_fw[WAVES][kPullWinWords];
+  uint32_t* const s_hub = hub_lds(dyn_lds, g.num_pages);
+  __shared__ unsigned long long red[4][WAVES];
   // head records by live rank: per window word, its live (in-edge) bits and
   // the live vertices before it in the window
-  __shared__ uint32_t s_lw[kBlock / SCAPH_WAVE][kPullWinWords];
-  __shared__ uint32_t s_lb[kBlock / SCAPH_WAVE][kPullWinWords];
+  __shared__ uint32_t s_lw[WAVES][kPullWinWords];
+  __shared__ uint32_t s_lb[WAVES][kPullWinWords];
   stage_hubs(s_hub, hw.hub_front, num_hubs);
   const PageTab s = stage_pages(g, dyn_lds);
   __syncthreads();
@@ -987,10 +1002,10 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x / SCAPH_WAVE;
   const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
-  const uint64_t per_block = (uint64_t)kPullWinWords * (kBlock / SCAPH_WAVE);
+  const uint64_t per_block = (uint64_t)kPullWinWords * WAVES;
   const uint64_t iters = (words + per_block - 1) / per_block;
   unsigned long long acc_n = 0, acc_m = 0, acc_s = 0, acc_x = 0;
-  uint32_t* q = s_q[wid];
+  uint16_t* q = s_q[wid];
   uint32_t* fw = s_fw[wid];
   // visited word of this lane in the wave's window of iteration bi
   auto visited_word = [&](uint64_t bi) -> uint32_t {
@@ -1018,7 +1033,7 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
     while (unv) {
       const uint32_t b = __ffs(unv) - 1;
       unv &= unv - 1;
-      q[o++] = (lane << 5) | b;
+      q[o++] = (uint16_t)((lane << 5) | b);
     }
     if (lane < kPullWinWords) fw[lane] = 0u;
     uint32_t wbase = 0;
@@ -1173,7 +1188,7 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long a = 0, b = 0, c = 0, x = 0;
-    for (uint32_t w = 0; w < kBlock / SCAPH_WAVE; ++w) {
+    for (uint32_t w = 0; w < WAVES; ++w) {
       a += red[0][w];
       b += red[1][w];
       c += red[2][w];
@@ -1195,9 +1210,10 @@ k_pull_heavy(const HeavyChunk* chunks, const unsigned long long* count, const ui
              uint32_t* level, uint32_t* visited, const uint32_t* front_in, uint32_t* front_out,
              scaph_level_counters* counters, uint32_t next_level, const uint32_t* hub_front,
              uint32_t num_hubs) {
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
   __shared__ HeavyChunk s_c;
   __shared__ int s_skip;
-  __shared__ uint32_t s_hub[kMaxHubs / 32];
+  uint32_t* const s_hub = hub_lds(dyn_lds, 0);
   stage_hubs(s_hub, hub_front, num_hubs);
   __syncthreads();
   const unsigned long long total = *count;
@@ -1457,6 +1473,23 @@ inline unsigned grid_for(size_t items, size_t per_block, unsigned cap) {
   if (b < 1) b = 1;
   if (b > cap) b = cap;
   return (unsigned)b;
+}
+
+// Blocks of `kernel` that stay resident on one CU at this block size and
+// dynamic LDS (the runtime's occupancy calculation, cached per process).
+unsigned resident_blocks(const void* kernel, unsigned threads, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, unsigned, size_t>, unsigned> cache;
+  const auto key = std::make_tuple(kernel, threads, lds);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, (int)threads, lds) != hipSuccess ||
+      nb < 1)
+    nb = 1;
+  cache.emplace(key, (unsigned)nb);
+  return (unsigned)nb;
 }
 
 // Offsets of the prep blocks' entries and edges, and the level totals.
@@ -1832,28 +1865,50 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
     SCAPH_LAUNCH_CHECK();
   }
   const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
-  // persistent-style: four resident blocks per CU, each walking its share
-  // of the window groups (a launch larger than what is resident leaves a
-  // tail of late blocks; measured best on s26 among 512..4096 blocks)
-  unsigned grid = grid_for(words, (size_t)kPullWinWords * (kBlock / SCAPH_WAVE), 4u * cu_count());
-  if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
+  const uint32_t waves = tune && tune->block_waves ? tune->block_waves : kPullWaves;
+  if (waves != 4 && waves != 8) return SCAPH_STATUS_INVALID;
   const uint32_t indexed_heads = tune ? tune->head_indexed : 0u;
   const uint32_t outdeg_heads = tune ? (tune->head_outdeg | indexed_heads) : 0u;
-  const size_t lds = page_lds_bytes(in_graph->num_pages);
+  const size_t lds = page_lds_bytes(in_graph->num_pages) + hub_lds_bytes(num_hubs);
   const uint4* head = heads ? reinterpret_cast<const uint4*>(heads->head) : nullptr;
   const uint32_t* no_in = head ? heads->no_in_edges : nullptr;
   const uint32_t* live_base = no_in ? heads->live_base : nullptr;
   if (head && heads->live_base && !no_in) return SCAPH_STATUS_INVALID;
-  if (!tune || !tune->no_prefetch)
-    hipLaunchKernelGGL(k_pull<1>, dim3(grid), dim3(kBlock), lds, st, *in_graph, head, no_in,
-                       live_base, outdeg_heads, indexed_heads, degree, level, visited, front_in,
-                       front_out, counters, cur_level + 1, hw, num_hubs);
-  else
-    hipLaunchKernelGGL(k_pull<0>, dim3(grid), dim3(kBlock), lds, st, *in_graph, head, no_in,
-                       live_base, outdeg_heads, indexed_heads, degree, level, visited, front_in,
-                       front_out, counters, cur_level + 1, hw, num_hubs);
+  const bool prefetch = !tune || !tune->no_prefetch;
+  const uint32_t simd = tune ? tune->waves_per_simd : 0u;
+  if (simd != 0 && simd != 8) return SCAPH_STATUS_INVALID;
+  const int variant = (prefetch ? 1 : 0) | (waves == 8 ? 2 : 0) | (simd == 8 ? 4 : 0);
+  static const void* const kerns[8] = {
+      (const void*)k_pull<0, 4, 1>, (const void*)k_pull<1, 4, 1>, (const void*)k_pull<0, 8, 1>,
+      (const void*)k_pull<1, 8, 1>, (const void*)k_pull<0, 4, 8>, (const void*)k_pull<1, 4, 8>,
+      (const void*)k_pull<0, 8, 8>, (const void*)k_pull<1, 8, 8>};
+  const void* kern = kerns[variant];
+  // persistent-style: as many blocks as stay resident on every CU, each
+  // walking its share of the window groups (a launch larger than what is
+  // resident leaves a tail of late blocks; measured best on s26 among
+  // 512..4096 256-thread blocks)
+  const unsigned threads = waves * SCAPH_WAVE;
+  unsigned grid = grid_for(words, (size_t)kPullWinWords * waves,
+                           resident_blocks(kern, threads, lds) * cu_count());
+  if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
+#define SCAPH_PULL_LAUNCH(P, W, M)                                                        \
+  hipLaunchKernelGGL((k_pull<P, W, M>), dim3(grid), dim3(threads), lds, st, *in_graph, head, \
+                     no_in, live_base, outdeg_heads, indexed_heads, degree, level, visited,    \
+                     front_in, front_out, counters, cur_level + 1, hw, num_hubs)
+  switch (variant) {
+    case 0: SCAPH_PULL_LAUNCH(0, 4, 1); break;
+    case 1: SCAPH_PULL_LAUNCH(1, 4, 1); break;
+    case 2: SCAPH_PULL_LAUNCH(0, 8, 1); break;
+    case 3: SCAPH_PULL_LAUNCH(1, 8, 1); break;
+    case 4: SCAPH_PULL_LAUNCH(0, 4, 8); break;
+    case 5: SCAPH_PULL_LAUNCH(1, 4, 8); break;
+    case 6: SCAPH_PULL_LAUNCH(0, 8, 8); break;
+    default: SCAPH_PULL_LAUNCH(1, 8, 8); break;
+  }
+#undef SCAPH_PULL_LAUNCH
   SCAPH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), 0, st, hw.chunks, hw.count, degree,
+  hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), hub_lds_bytes(num_hubs), st,
+                     hw.chunks, hw.count, degree,
                      level, visited, front_in, front_out, counters, cur_level + 1, hw.hub_front,
                      num_hubs);
   SCAPH_LAUNCH_CHECK();

@@ -18,7 +18,7 @@ double Total_throughput = 0;
 
 namespace {
 
-constexpr uint32_t kMaxHubs = 131072;  // frontier bits the pull expand keeps in LDS
+constexpr uint32_t kMaxHubs = 262144;  // frontier bits the pull expand can keep in LDS
 constexpr uint64_t kAlignWords = 64;  // page buffers start on 256-byte boundaries
 constexpr size_t kMaxTimedLevels = 1024;
 constexpr size_t kCounterWords = sizeof(scaph_level_counters) / 4;
@@ -60,6 +60,12 @@ Config& global_flags() {
     r.add("trace", FlagType::kString, &c.trace, "per-iteration JSONL trace file");
     r.add("build_pull", FlagType::kBool, &c.build_pull, "keep transposed pages for pull");
     r.add("pull_grid", FlagType::kInt32, &c.pull_grid, "cap on pull expand blocks (0 = default)");
+    r.add("pull_waves", FlagType::kInt32, &c.pull_waves,
+          "waves per pull block, 4 or 8 (0 = default); the hub bits are staged once per block");
+    r.add("pull_hubs", FlagType::kInt32, &c.pull_hubs,
+          "highest-out-degree vertices whose frontier bits the pull keeps in LDS (<= 262144)");
+    r.add("pull_simd", FlagType::kInt32, &c.pull_simd,
+          "8: pull kernel registers capped so 8 waves share a SIMD (0 = compiler's choice)");
     r.add("learned_fused", FlagType::kBool, &c.learned_fused,
           "a model reading only scalar features decides inside the level hand-off");
     r.add("learned_margin", FlagType::kDouble, &c.learned_margin,
@@ -176,6 +182,8 @@ Engine::Engine(const Config& cfg)
   pull_tune_.grid_blocks = (uint32_t)std::max(0, cfg.pull_grid);
   pull_tune_.head_outdeg = cfg.pull_head_outdeg ? 1u : 0u;
   pull_tune_.no_prefetch = cfg.pull_prefetch ? 0u : 1u;
+  pull_tune_.block_waves = (uint32_t)std::max(0, cfg.pull_waves);
+  pull_tune_.waves_per_simd = (uint32_t)std::max(0, cfg.pull_simd);
   push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
   push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);
@@ -406,7 +414,7 @@ void Engine::load_csr(HostCsr g, bool keep_host, HostCsr* in_edges) {
     if (n_ > 0x7FFFFFFFu) throw Error(SCAPH_STATUS_INVALID, "pull needs fewer than 2^31 vertices");
     if (in_edges) t = std::move(*in_edges);
     else t = transpose(g);
-    hubs = select_hubs(h_deg_out_, kMaxHubs);
+    hubs = select_hubs(h_deg_out_, hub_count());
     encode_hub_in_lists(&t, hubs);
     if (!page_cuts(t, P_, false, &pages_in_, &err)) throw Error(SCAPH_STATUS_INVALID, err);
   }
@@ -625,7 +633,7 @@ void Engine::build_device(uint32_t n, uint64_t raw, bool keep_host_csr,
     DevBuf hub_idx;
     if (dir == 1) {
       // hubs from the out-degrees (h_deg_out_ holds them after the first pass)
-      const std::vector<uint32_t> hubs = select_hubs(h_deg_out_, kMaxHubs);
+      const std::vector<uint32_t> hubs = select_hubs(h_deg_out_, hub_count());
       std::vector<uint32_t> idx(n, 0xFFFFFFFFu);
       for (uint32_t i = 0; i < hubs.size(); ++i) idx[hubs[i]] = i;
       hub_idx.alloc((size_t)n * 4);

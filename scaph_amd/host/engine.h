@@ -5,6 +5,7 @@
 #ifndef SCAPH_HOST_ENGINE_H
 #define SCAPH_HOST_ENGINE_H
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <functional>
@@ -46,7 +47,10 @@ struct Config {
   std::string trace;             // per-iteration JSONL trace file
   bool build_pull = true;        // keep transposed pages for the pull expand
   bool pull_head = true;         // 16-byte in-list head records in front of the in-pages
-  int32_t pull_grid = 0;         // cap on pull blocks (0: four per compute unit)
+  int32_t pull_grid = 0;         // cap on pull blocks (0: as many as stay resident)
+  int32_t pull_waves = 0;        // waves per pull block, 4 or 8 (0: the kernel default, 8)
+  int32_t pull_simd = 0;         // 8: pull registers capped for 8 waves per SIMD (0: compiler)
+  int32_t pull_hubs = 131072;    // in-list sources tagged as hubs, frontier bits in LDS
   bool pull_head_outdeg = true;  // head records {e0, e1, in-degree, out-degree}
   bool pull_head_live = true;    // head records only for vertices with in-edges (live rank)
   bool pull_prefetch = true;     // request the next batch's head records early
@@ -302,6 +306,10 @@ class Engine {
   DevBuf in_head_;   // per vertex with in-edges: first in-list entries, in-degree, out-degree
   DevBuf live_base_; // per 1024-id window: vertices with in-edges before it (in_head_ numbering)
   uint32_t num_hubs_ = 0;
+  // -pull_hubs clamped to what the pull kernel's LDS holds
+  uint32_t hub_count() const {
+    return (uint32_t)std::min<int64_t>(std::max<int64_t>(cfg_.pull_hubs, 0), 262144);
+  }
   // BFS state
   DevBuf level_, visited_, front_[2], queue_[2], counters_, ws_, page_nodes_, page_datas_, totals_;
   size_t ws_bytes_ = 0;

@@ -372,7 +372,13 @@ def test_pull_heavy_in_lists():
 
 @pytest.mark.parametrize("flags", [["-push_rows=false"], ["-push_rows=true", "-push_grid=64"],
                                    ["-pull_grid=16"], ["-push_small=1"], ["-push_small=1000000"],
-                                   ["-pull_prefetch=false"], ["-push_small=1", "-push_scan=1"],
+                                   ["-pull_prefetch=false"], ["-pull_waves=4"],
+                                   ["-pull_waves=4", "-pull_prefetch=false", "-pull_grid=7"],
+                                   ["-pull_simd=8"], ["-pull_simd=8", "-pull_prefetch=false"],
+                                   ["-pull_simd=8", "-pull_waves=4"], ["-pull_hubs=0"],
+                                   ["-pull_hubs=1000"], ["-pull_hubs=229376", "-pull_waves=4"],
+                                   ["-pull_hubs=262144"],
+                                   ["-push_small=1", "-push_scan=1"],
                                    ["-push_scan=1000000000"], ["-push_small=1", "-push_xcd=1"],
                                    ["-push_small=1", "-push_xcd=1", "-push_rows=false",
                                     "-push_grid=24"]])
