@@ -731,7 +731,12 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
          uint32_t* level, uint32_t* visited, uint32_t* q_out, uint32_t* front_out,
          scaph_level_counters* counters, uint32_t next_level, uint32_t n) {
   constexpr uint32_t kPerThread = kTile / kBlock;  // edges per thread per tile
-  __shared__ unsigned long long s_gx[kTile + 1];
+  static_assert(kTile < 65536, "tile offsets are kept as u16");
+  // entry k of the tile: its first edge relative to the tile start (0 for
+  // the entry the tile starts inside, whose list address is advanced to
+  // match) and its list address; u16 offsets keep the block's LDS small
+  // enough for five blocks per CU with 2048-edge tiles
+  __shared__ uint16_t s_gx[kTile + 1];
   __shared__ unsigned long long s_ep[kTile + 1];
   __shared__ uint32_t s_claim[kTile];
   __shared__ uint32_t s_cnt;
@@ -778,13 +783,21 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
     uint32_t cnt = i1 - i0 + 1;
     if (cnt > kTile + 1) cnt = kTile + 1;
     for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
-      s_gx[k] = gx[i0 + k];
-      s_ep[k] = gep[i0 + k];
+      const unsigned long long x = gx[i0 + k], ep = gep[i0 + k];
+      // entry i0 holds edge s0, so only entry 0 starts at or before the
+      // tile start; the others start inside it (x - s0 in [1, kTile]).
+      // Keyed on k, not on x < s0: with "x < s0 ? 0 : min(x - s0, kTile)"
+      // hipcc -O3 (ROCm 7.2) emitted min(x - s0, kTile) with a wrapping
+      // subtract and no select, storing kTile for entry 0.
+      const unsigned long long d = x - s0;
+      s_gx[k] = k == 0 ? (uint16_t)0 : (uint16_t)(d > kTile ? kTile : d);
+      s_ep[k] = k == 0 ? ep + (s0 - x) * sizeof(uint32_t) : ep;
     }
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
-    const unsigned long long r0 = LAYOUT == 0 ? s0 + (unsigned long long)threadIdx.x * kPerThread
-                                              : s0 + threadIdx.x;
+    // edge offsets relative to the tile start s0
+    const uint32_t r0 = LAYOUT == 0 ? threadIdx.x * kPerThread : threadIdx.x;
+    const uint32_t rend = (uint32_t)(e0 - s0);
     uint32_t lo = 0, hi = cnt;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -792,8 +805,8 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
     }
 #pragma unroll
     for (uint32_t j = 0; j < kPerThread; ++j) {
-      const unsigned long long r = LAYOUT == 0 ? r0 + j : r0 + (unsigned long long)j * kBlock;
-      if (r < e0) {
+      const uint32_t r = LAYOUT == 0 ? r0 + j : r0 + j * kBlock;
+      if (r < rend) {
         if (LAYOUT == 0) {
           while (lo + 1 < cnt && s_gx[lo + 1] <= r) ++lo;
         } else {
