@@ -576,9 +576,10 @@ typedef struct scaph_ms_params {
   uint32_t push_mdiv;         /* ... and m_active * push_mdiv < m */
   uint32_t pull_block;        /* threads per pull block: 256, 512 or 1024 */
   uint32_t pull_grid;         /* cap on pull blocks (0: one per block of vertices, at most 16384) */
-  uint32_t pad;
+  uint32_t num_hubs;          /* entries of hub_list (<= SCAPH_MS_MAX_HUBS; 0: none) */
   unsigned long long m;       /* edges of the graph */
 } scaph_ms_params;
+#define SCAPH_MS_MAX_HUBS 262144u
 typedef struct scaph_ms_state {
   uint64_t* seen;             /* n x 4 words */
   uint64_t* front;            /* max_levels x n x 4 words */
@@ -586,6 +587,9 @@ typedef struct scaph_ms_state {
   scaph_ms_counters* ctr;     /* max_levels + 1 records */
   uint32_t* queue;            /* n entries */
   uint64_t* heavy;            /* heavy_cap chunk items */
+  uint64_t* hub_front;        /* SCAPH_MS_MAX_HUBS x 4 words: the hubs' frontier words of the
+                                 level being pulled (hub i = hub_list[i]; set where hub_bits is) */
+  uint32_t* hub_bits;         /* SCAPH_MS_MAX_HUBS bits: hub i in that level's frontier */
   uint32_t n, max_levels;
   unsigned long long act_words; /* ceil(n / 64) */
   unsigned long long heavy_cap;
@@ -600,7 +604,11 @@ int scaph_ms_state_carve(void* base, uint32_t n, unsigned long long m, uint32_t 
 int scaph_ms_init(const scaph_ms_state* st, const uint32_t* sources, uint32_t count,
                   const uint32_t* degree, void* stream);
 /* One level (level -> level+1): push (out_graph) or pull (in_graph with
- * hub-tagged in-lists, as scaph_bfs_pull) chosen on the device. */
+ * hub-tagged in-lists, as scaph_bfs_pull) chosen on the device.  A pull
+ * level first gathers the frontier words of the prm->num_hubs hubs into
+ * st->hub_front (their membership bits into st->hub_bits, staged in LDS),
+ * so a hub-tagged in-edge is one LDS bit test and one load from a 32-byte
+ * x num_hubs array instead of three dependent loads. */
 int scaph_ms_level(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
                    const uint32_t* hub_list, const uint32_t* degree, const scaph_ms_state* st,
                    uint32_t level, const scaph_ms_params* prm, void* stream);

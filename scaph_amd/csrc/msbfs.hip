@@ -277,9 +277,50 @@ k_ms_finalize(scaph_ms_state s, uint32_t L, scaph_ms_params prm, const uint32_t*
   flush_counts(&s.ctr[L + 1], bits, nact, mact);
 }
 
-__device__ __forceinline__ uint32_t resolve(uint32_t e, const uint32_t* hub_list) {
-  return (e & kHubTag) ? hub_list[e & ~kHubTag] : e;
+// The frontier words of in-list entry e at the level being pulled, or
+// nullptr when e is not in that frontier: a hub (kHubTag | hub index) is
+// tested in the LDS copy of the hub bits and read from the hubs' compact
+// copy of their words, any other vertex through the level's bitmap.
+__device__ __forceinline__ const uint64_t* ms_front_of(uint32_t e, const uint32_t* s_hub,
+                                                       const uint64_t* hub_front,
+                                                       const uint64_t* act, const uint64_t* fr) {
+  if (e & kHubTag) {
+    const uint32_t h = e & ~kHubTag;
+    return ((s_hub[h >> 5] >> (h & 31)) & 1u) ? hub_front + (size_t)h * 4 : nullptr;
+  }
+  return ((act[e >> 6] >> (e & 63)) & 1ull) ? fr + (size_t)e * 4 : nullptr;
 }
+
+// Pull levels only: the frontier words of the hubs (hub i = hub_list[i])
+// copied to hub_front[i] where hub i is in the level's frontier, and those
+// memberships as bits (hub_bits, whole words; the pull stages them in LDS).
+__global__ void __launch_bounds__(kBlock)
+k_ms_hub_gather(scaph_ms_state s, uint32_t L, scaph_ms_params prm, const uint32_t* hub_list) {
+  const scaph_ms_counters* c = &s.ctr[L];
+  if (c->n_active == 0 || use_push(c, s.n, prm)) return;
+  const uint32_t nh = prm.num_hubs;
+  const uint64_t* fr = front_of(s, L);
+  const uint64_t* act = act_of(s, L);
+  const uint32_t lane = lane_id();
+  // base is uniform per block, and a multiple of 256 below nh <= the hub
+  // capacity (a multiple of 256), so every word written is in range
+  for (uint32_t base = blockIdx.x * kBlock; base < nh; base += gridDim.x * kBlock) {
+    const uint32_t i = base + threadIdx.x;
+    bool a = false;
+    if (i < nh) {
+      const uint32_t u = hub_list[i];
+      a = (act[u >> 6] >> (u & 63)) & 1ull;
+      if (a) st4(s.hub_front + (size_t)i * 4, ld4(fr + (size_t)u * 4));
+    }
+    const uint64_t b = __ballot(a);
+    const uint32_t w = (i - lane) / 32;
+    if (lane == 0) s.hub_bits[w] = (uint32_t)b;
+    if (lane == 32) s.hub_bits[w + 1] = (uint32_t)(b >> 32);
+  }
+}
+
+// LDS bytes of the hub bits, behind the page table.
+inline size_t ms_hub_lds_bytes(uint32_t num_hubs) { return ((size_t)(num_hubs + 127) / 128) * 16; }
 
 __device__ __forceinline__ uint64_t wave_or64(uint64_t x) {
 #pragma unroll
@@ -288,11 +329,15 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t x) {
 }
 
 __global__ void __launch_bounds__(1024)
-k_ms_pull(scaph_graph_desc g, const uint32_t* hub_list, scaph_ms_state s, uint32_t L,
-          scaph_ms_params prm, const uint32_t* degree) {
+k_ms_pull(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm,
+          const uint32_t* degree) {
   extern __shared__ __align__(16) unsigned char lds[];
   const scaph_ms_counters* c = &s.ctr[L];
   if (c->n_active == 0 || use_push(c, s.n, prm)) return;
+  const uint32_t np = g.num_pages < SCAPH_LDS_PAGES ? g.num_pages : SCAPH_LDS_PAGES;
+  uint32_t* const s_hub = reinterpret_cast<uint32_t*>(lds + (size_t)np * 16);
+  for (uint32_t i = threadIdx.x; i < (prm.num_hubs + 31) / 32; i += blockDim.x)
+    s_hub[i] = s.hub_bits[i];
   const PageTab t = stage_pages(g, lds);
   __syncthreads();
   const uint64_t* fr = front_of(s, L);
@@ -326,18 +371,16 @@ k_ms_pull(scaph_graph_desc g, const uint32_t* hub_list, scaph_ms_state s, uint32
       // query found" runs once per step
       for (uint32_t i = 0; i < lim; i += kMsProbe) {
         uint32_t u[kMsProbe];
-        bool a[kMsProbe];
+        const uint64_t* fp[kMsProbe];
 #pragma unroll
         for (uint32_t j = 0; j < kMsProbe; ++j) u[j] = i + j < lim ? edges[i + j] : 0u;
 #pragma unroll
-        for (uint32_t j = 0; j < kMsProbe; ++j) u[j] = i + j < lim ? resolve(u[j], hub_list) : 0u;
-#pragma unroll
         for (uint32_t j = 0; j < kMsProbe; ++j)
-          a[j] = i + j < lim && ((act[u[j] >> 6] >> (u[j] & 63)) & 1ull);
+          fp[j] = i + j < lim ? ms_front_of(u[j], s_hub, s.hub_front, act, fr) : nullptr;
 #pragma unroll
         for (uint32_t j = 0; j < kMsProbe; ++j) {
-          if (a[j]) {
-            const W4 f = ld4(fr + (size_t)u[j] * 4);
+          if (fp[j]) {
+            const W4 f = ld4(fp[j]);
 #pragma unroll
             for (int k = 0; k < 4; ++k) acc.w[k] |= f.w[k] & need.w[k];
           }
@@ -367,9 +410,9 @@ k_ms_pull(scaph_graph_desc g, const uint32_t* hub_list, scaph_ms_state s, uint32
       for (uint32_t i0 = kPullSerial; i0 < ll; i0 += 8 * SCAPH_WAVE) {
         const uint32_t i1 = i0 + 8 * SCAPH_WAVE < ll ? i0 + 8 * SCAPH_WAVE : ll;
         for (uint32_t i = i0 + lane_id(); i < i1; i += SCAPH_WAVE) {
-          const uint32_t u = resolve(el[i], hub_list);
-          if ((act[u >> 6] >> (u & 63)) & 1ull) {
-            const W4 f = ld4(fr + (size_t)u * 4);
+          const uint64_t* p = ms_front_of(el[i], s_hub, s.hub_front, act, fr);
+          if (p) {
+            const W4 f = ld4(p);
 #pragma unroll
             for (int k = 0; k < 4; ++k) a.w[k] |= f.w[k] & nl.w[k];
           }
@@ -474,7 +517,8 @@ size_t scaph_ms_state_bytes(uint32_t n, unsigned long long m, uint32_t max_level
   return align256((size_t)n * 32) + align256((size_t)max_levels * n * 32) +
          align256((size_t)max_levels * act_words * 8) +
          align256(((size_t)max_levels + 1) * sizeof(scaph_ms_counters)) +
-         align256((size_t)n * 4 + 4) + align256(heavy_cap * 8);
+         align256((size_t)n * 4 + 4) + align256(heavy_cap * 8) +
+         align256((size_t)SCAPH_MS_MAX_HUBS * 32) + align256(SCAPH_MS_MAX_HUBS / 8);
 }
 
 int scaph_ms_state_carve(void* base, uint32_t n, unsigned long long m, uint32_t max_levels,
@@ -496,6 +540,10 @@ int scaph_ms_state_carve(void* base, uint32_t n, unsigned long long m, uint32_t 
   st->queue = reinterpret_cast<uint32_t*>(p);
   p += align256((size_t)n * 4 + 4);
   st->heavy = reinterpret_cast<uint64_t*>(p);
+  p += align256(st->heavy_cap * 8);
+  st->hub_front = reinterpret_cast<uint64_t*>(p);
+  p += align256((size_t)SCAPH_MS_MAX_HUBS * 32);
+  st->hub_bits = reinterpret_cast<uint32_t*>(p);
   return 0;
 }
 
@@ -527,7 +575,8 @@ int scaph_ms_level(const scaph_graph_desc* out_graph, const scaph_graph_desc* in
                    uint32_t level, const scaph_ms_params* prm, void* stream) {
   if (!out_graph || !in_graph || !degree || !st || !prm || level + 1 >= st->max_levels ||
       prm->num_queries == 0 || prm->num_queries > SCAPH_MS_MAX_QUERIES || prm->push_div == 0 ||
-      prm->push_mdiv == 0)
+      prm->push_mdiv == 0 || prm->num_hubs > SCAPH_MS_MAX_HUBS ||
+      (prm->num_hubs != 0 && !hub_list))
     return SCAPH_STATUS_INVALID;
   const uint32_t pb = prm->pull_block;
   if (pb != 256 && pb != 512 && pb != 1024) return SCAPH_STATUS_INVALID;
@@ -546,11 +595,15 @@ int scaph_ms_level(const scaph_graph_desc* out_graph, const scaph_graph_desc* in
   hipLaunchKernelGGL(k_ms_finalize, dim3(grid_for(s.n, kBlock, 8192)), dim3(kBlock), 0, sm, s,
                      level, p, degree);
   SCAPH_LAUNCH_CHECK();
-  const size_t li = page_lds_bytes(in_graph->num_pages);
+  if (p.num_hubs) {
+    hipLaunchKernelGGL(k_ms_hub_gather, dim3(grid_for(p.num_hubs, kBlock, 1024)), dim3(kBlock), 0,
+                       sm, s, level, p, hub_list);
+    SCAPH_LAUNCH_CHECK();
+  }
+  const size_t li = page_lds_bytes(in_graph->num_pages) + ms_hub_lds_bytes(p.num_hubs);
   unsigned pgrid = grid_for(s.n, pb, 16384);
   if (p.pull_grid && p.pull_grid < pgrid) pgrid = p.pull_grid;
-  hipLaunchKernelGGL(k_ms_pull, dim3(pgrid), dim3(pb), li, sm, *in_graph,
-                     hub_list, s, level, p, degree);
+  hipLaunchKernelGGL(k_ms_pull, dim3(pgrid), dim3(pb), li, sm, *in_graph, s, level, p, degree);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

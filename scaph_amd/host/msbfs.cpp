@@ -75,6 +75,7 @@ MsStats MultiSource::run(const uint32_t* sources, uint32_t count, const MsTune& 
   p.push_mdiv = t.push_mdiv;
   p.pull_block = t.pull_block;
   p.pull_grid = t.pull_grid;
+  p.num_hubs = e_.num_hubs();
   p.m = e_.num_edges();
   void* st = e_.stream();
   std::memcpy(hsrc_.get(), sources, (size_t)count * 4);
