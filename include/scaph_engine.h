@@ -146,6 +146,7 @@ typedef struct scaph_ms_tune {
   uint32_t use_graph;           /* replay hipGraph-captured level segments */
   uint32_t segment;             /* levels per captured segment */
   uint32_t pull_grid;           /* cap on pull blocks (0 = one per block of vertices) */
+  uint32_t persistent;          /* every level in one persistent cooperative launch */
 } scaph_ms_tune;
 typedef struct scaph_ms_stats {
   double ms;                    /* device time: init and every level */

@@ -612,6 +612,15 @@ int scaph_ms_init(const scaph_ms_state* st, const uint32_t* sources, uint32_t co
 int scaph_ms_level(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
                    const uint32_t* hub_list, const uint32_t* degree, const scaph_ms_state* st,
                    uint32_t level, const scaph_ms_params* prm, void* stream);
+/* Every level from first_level on, in one cooperative launch of resident
+ * 256-thread blocks (the same phases as scaph_ms_level, separated by grid
+ * barriers; prm->pull_block and pull_grid are not used).  Stops when a
+ * level's frontier is empty or at level max_levels - 1, whose counters then
+ * show a frontier still active.  No reference counterpart. */
+int scaph_ms_run_persistent(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
+                            const uint32_t* hub_list, const uint32_t* degree,
+                            const scaph_ms_state* st, uint32_t first_level,
+                            const scaph_ms_params* prm, void* stream);
 /* uint32 levels of one query (SCAPH_UNREACHED where not reached) from the
  * first `depth` levels. */
 int scaph_ms_levels_u32(const scaph_ms_state* st, uint32_t depth, uint32_t query, uint32_t* out,

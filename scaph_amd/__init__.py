@@ -197,7 +197,8 @@ class BFS:
         if tune is not None:
             t = _lib.MsTune(tune.get("push_div", 64), tune.get("push_mdiv", 16),
                             tune.get("pull_block", 256), tune.get("use_graph", 0),
-                            tune.get("segment", 8), tune.get("pull_grid", 0))
+                            tune.get("segment", 8), tune.get("pull_grid", 0),
+                            tune.get("persistent", 0))
         e = _err()
         _lib.check(self._lib.scaph_engine_msbfs(self._h, _ptr(src), len(src),
                                                 ctypes.byref(t) if t is not None else None,

@@ -62,7 +62,8 @@ class ExploreRecord(ctypes.Structure):
 class MsTune(ctypes.Structure):
     _fields_ = [("push_div", ctypes.c_uint32), ("push_mdiv", ctypes.c_uint32),
                 ("pull_block", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
-                ("segment", ctypes.c_uint32), ("pull_grid", ctypes.c_uint32)]
+                ("segment", ctypes.c_uint32), ("pull_grid", ctypes.c_uint32),
+                ("persistent", ctypes.c_uint32)]
 
 
 class MsStats(ctypes.Structure):

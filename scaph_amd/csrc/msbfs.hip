@@ -14,6 +14,8 @@
 //        than kPullSerial finish on the whole wave.
 // The direction is a pure function of the level's counters, evaluated by
 // each kernel on the device, so a level is a fixed launch sequence.
+#include <hip/hip_cooperative_groups.h>
+
 #include "common.h"
 
 namespace {
@@ -130,13 +132,15 @@ __global__ void k_ms_sources(scaph_ms_state s, const uint32_t* sources, uint32_t
   }
 }
 
+// Each phase of a level is a device function over blocks bid of nblk, run
+// by its own launch (the per-level sequence) or by every block of the
+// persistent kernel between grid barriers.
+
 // Push preparation: clear level L+1's words and list the frontier.
-__global__ void __launch_bounds__(kBlock)
-k_ms_prep(scaph_ms_state s, uint32_t L, scaph_ms_params prm) {
-  const scaph_ms_counters* c = &s.ctr[L];
-  if (c->n_active == 0 || !use_push(c, s.n, prm)) return;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ms_prep(const scaph_ms_state& s, uint32_t L, uint32_t bid,
+                                        uint32_t nblk) {
+  const size_t stride = (size_t)nblk * blockDim.x;
+  const size_t tid = (size_t)bid * blockDim.x + threadIdx.x;
   ulonglong2* fn = reinterpret_cast<ulonglong2*>(front_of(s, L + 1));
   for (size_t i = tid; i < (size_t)s.n * 2; i += stride) fn[i] = make_ulonglong2(0, 0);
   uint64_t* an = act_of(s, L + 1);
@@ -163,6 +167,13 @@ k_ms_prep(scaph_ms_state s, uint32_t L, scaph_ms_params prm) {
   }
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_ms_prep(scaph_ms_state s, uint32_t L, scaph_ms_params prm) {
+  const scaph_ms_counters* c = &s.ctr[L];
+  if (c->n_active == 0 || !use_push(c, s.n, prm)) return;
+  ms_prep(s, L, blockIdx.x, gridDim.x);
+}
+
 __device__ __forceinline__ void push_edges(const uint32_t* edges, uint32_t begin, uint32_t end,
                                            const W4& f, const uint64_t* seen, uint64_t* fn) {
   for (uint32_t j = begin + lane_id(); j < end; j += SCAPH_WAVE) {
@@ -179,19 +190,15 @@ __device__ __forceinline__ void push_edges(const uint32_t* edges, uint32_t begin
   }
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_ms_push(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  const scaph_ms_counters* c = &s.ctr[L];
-  if (c->n_active == 0 || !use_push(c, s.n, prm)) return;
-  const PageTab t = stage_pages(g, lds);
-  __syncthreads();
-  const unsigned long long q_len = c->q_len;
+__device__ __forceinline__ void ms_push(const PageTab& t, const scaph_graph_desc& g,
+                                        const scaph_ms_state& s, uint32_t L, uint32_t bid,
+                                        uint32_t nblk) {
+  const unsigned long long q_len = s.ctr[L].q_len;
   const uint64_t* fr = front_of(s, L);
   uint64_t* fn = front_of(s, L + 1);
   const uint32_t nw = blockDim.x / SCAPH_WAVE;
-  const unsigned long long waves = (unsigned long long)gridDim.x * nw;
-  for (unsigned long long i = (unsigned long long)blockIdx.x * nw + threadIdx.x / SCAPH_WAVE;
+  const unsigned long long waves = (unsigned long long)nblk * nw;
+  for (unsigned long long i = (unsigned long long)bid * nw + threadIdx.x / SCAPH_WAVE;
        i < q_len; i += waves) {
     const uint32_t u = s.queue[i];
     const W4 f = ld4(fr + (size_t)u * 4);
@@ -213,20 +220,25 @@ k_ms_push(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm)
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_ms_push_heavy(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm) {
+k_ms_push(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm) {
   extern __shared__ __align__(16) unsigned char lds[];
   const scaph_ms_counters* c = &s.ctr[L];
   if (c->n_active == 0 || !use_push(c, s.n, prm)) return;
-  unsigned long long hl = c->heavy_len;
-  if (hl == 0) return;
-  if (hl > s.heavy_cap) hl = s.heavy_cap;
   const PageTab t = stage_pages(g, lds);
   __syncthreads();
+  ms_push(t, g, s, L, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void ms_push_heavy(const PageTab& t, const scaph_graph_desc& g,
+                                              const scaph_ms_state& s, uint32_t L, uint32_t bid,
+                                              uint32_t nblk) {
+  unsigned long long hl = s.ctr[L].heavy_len;
+  if (hl > s.heavy_cap) hl = s.heavy_cap;
   const uint64_t* fr = front_of(s, L);
   uint64_t* fn = front_of(s, L + 1);
   const uint32_t nw = blockDim.x / SCAPH_WAVE;
-  const unsigned long long waves = (unsigned long long)gridDim.x * nw;
-  for (unsigned long long i = (unsigned long long)blockIdx.x * nw + threadIdx.x / SCAPH_WAVE;
+  const unsigned long long waves = (unsigned long long)nblk * nw;
+  for (unsigned long long i = (unsigned long long)bid * nw + threadIdx.x / SCAPH_WAVE;
        i < hl; i += waves) {
     const uint64_t item = s.heavy[i];
     const uint32_t u = (uint32_t)(item >> 32), start = (uint32_t)item;
@@ -237,18 +249,26 @@ k_ms_push_heavy(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_param
   }
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_ms_push_heavy(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const scaph_ms_counters* c = &s.ctr[L];
+  if (c->n_active == 0 || !use_push(c, s.n, prm) || c->heavy_len == 0) return;
+  const PageTab t = stage_pages(g, lds);
+  __syncthreads();
+  ms_push_heavy(t, g, s, L, blockIdx.x, gridDim.x);
+}
+
 // After push: front[L+1] &= ~seen, seen |= front[L+1], membership bits and
 // counters of level L+1.
-__global__ void __launch_bounds__(kBlock)
-k_ms_finalize(scaph_ms_state s, uint32_t L, scaph_ms_params prm, const uint32_t* degree) {
-  const scaph_ms_counters* c = &s.ctr[L];
-  if (c->n_active == 0 || !use_push(c, s.n, prm)) return;
+__device__ __forceinline__ void ms_finalize(const scaph_ms_state& s, uint32_t L,
+                                            const uint32_t* degree, uint32_t bid, uint32_t nblk) {
   uint64_t* fn = front_of(s, L + 1);
   uint64_t* an = act_of(s, L + 1);
   unsigned long long bits = 0, nact = 0, mact = 0;
   const uint32_t nw = blockDim.x / SCAPH_WAVE;
-  const size_t wstride = (size_t)gridDim.x * nw * SCAPH_WAVE;
-  for (size_t base = ((size_t)blockIdx.x * nw + threadIdx.x / SCAPH_WAVE) * SCAPH_WAVE;
+  const size_t wstride = (size_t)nblk * nw * SCAPH_WAVE;
+  for (size_t base = ((size_t)bid * nw + threadIdx.x / SCAPH_WAVE) * SCAPH_WAVE;
        base < s.n; base += wstride) {
     const size_t v = base + lane_id();
     bool hit = false;
@@ -277,6 +297,13 @@ k_ms_finalize(scaph_ms_state s, uint32_t L, scaph_ms_params prm, const uint32_t*
   flush_counts(&s.ctr[L + 1], bits, nact, mact);
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_ms_finalize(scaph_ms_state s, uint32_t L, scaph_ms_params prm, const uint32_t* degree) {
+  const scaph_ms_counters* c = &s.ctr[L];
+  if (c->n_active == 0 || !use_push(c, s.n, prm)) return;
+  ms_finalize(s, L, degree, blockIdx.x, gridDim.x);
+}
+
 // The frontier words of in-list entry e at the level being pulled, or
 // nullptr when e is not in that frontier: a hub (kHubTag | hub index) is
 // tested in the LDS copy of the hub bits and read from the hubs' compact
@@ -294,17 +321,16 @@ __device__ __forceinline__ const uint64_t* ms_front_of(uint32_t e, const uint32_
 // Pull levels only: the frontier words of the hubs (hub i = hub_list[i])
 // copied to hub_front[i] where hub i is in the level's frontier, and those
 // memberships as bits (hub_bits, whole words; the pull stages them in LDS).
-__global__ void __launch_bounds__(kBlock)
-k_ms_hub_gather(scaph_ms_state s, uint32_t L, scaph_ms_params prm, const uint32_t* hub_list) {
-  const scaph_ms_counters* c = &s.ctr[L];
-  if (c->n_active == 0 || use_push(c, s.n, prm)) return;
-  const uint32_t nh = prm.num_hubs;
+// (256-thread blocks.)
+__device__ __forceinline__ void ms_hub_gather(const scaph_ms_state& s, uint32_t L, uint32_t nh,
+                                              const uint32_t* hub_list, uint32_t bid,
+                                              uint32_t nblk) {
   const uint64_t* fr = front_of(s, L);
   const uint64_t* act = act_of(s, L);
   const uint32_t lane = lane_id();
   // base is uniform per block, and a multiple of 256 below nh <= the hub
   // capacity (a multiple of 256), so every word written is in range
-  for (uint32_t base = blockIdx.x * kBlock; base < nh; base += gridDim.x * kBlock) {
+  for (uint32_t base = bid * kBlock; base < nh; base += nblk * kBlock) {
     const uint32_t i = base + threadIdx.x;
     bool a = false;
     if (i < nh) {
@@ -319,6 +345,18 @@ k_ms_hub_gather(scaph_ms_state s, uint32_t L, scaph_ms_params prm, const uint32_
   }
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_ms_hub_gather(scaph_ms_state s, uint32_t L, scaph_ms_params prm, const uint32_t* hub_list) {
+  const scaph_ms_counters* c = &s.ctr[L];
+  if (c->n_active == 0 || use_push(c, s.n, prm)) return;
+  ms_hub_gather(s, L, prm.num_hubs, hub_list, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void stage_hub_bits(uint32_t* s_hub, const uint32_t* hub_bits,
+                                               uint32_t nh) {
+  for (uint32_t i = threadIdx.x; i < (nh + 31) / 32; i += blockDim.x) s_hub[i] = hub_bits[i];
+}
+
 // LDS bytes of the hub bits, behind the page table.
 inline size_t ms_hub_lds_bytes(uint32_t num_hubs) { return ((size_t)(num_hubs + 127) / 128) * 16; }
 
@@ -328,18 +366,10 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t x) {
   return x;
 }
 
-__global__ void __launch_bounds__(1024)
-k_ms_pull(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm,
-          const uint32_t* degree) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  const scaph_ms_counters* c = &s.ctr[L];
-  if (c->n_active == 0 || use_push(c, s.n, prm)) return;
-  const uint32_t np = g.num_pages < SCAPH_LDS_PAGES ? g.num_pages : SCAPH_LDS_PAGES;
-  uint32_t* const s_hub = reinterpret_cast<uint32_t*>(lds + (size_t)np * 16);
-  for (uint32_t i = threadIdx.x; i < (prm.num_hubs + 31) / 32; i += blockDim.x)
-    s_hub[i] = s.hub_bits[i];
-  const PageTab t = stage_pages(g, lds);
-  __syncthreads();
+__device__ __forceinline__ void ms_pull(const PageTab& t, const scaph_graph_desc& g,
+                                        const uint32_t* s_hub, const scaph_ms_state& s,
+                                        uint32_t L, const scaph_ms_params& prm,
+                                        const uint32_t* degree, uint32_t bid, uint32_t nblk) {
   const uint64_t* fr = front_of(s, L);
   const uint64_t* act = act_of(s, L);
   uint64_t* fn = front_of(s, L + 1);
@@ -347,8 +377,8 @@ k_ms_pull(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm,
   const W4 qm = query_mask(prm.num_queries);
   unsigned long long bits = 0, nact = 0, mact = 0;
   const uint32_t nw = blockDim.x / SCAPH_WAVE;
-  const size_t wstride = (size_t)gridDim.x * nw * SCAPH_WAVE;
-  for (size_t base = ((size_t)blockIdx.x * nw + threadIdx.x / SCAPH_WAVE) * SCAPH_WAVE;
+  const size_t wstride = (size_t)nblk * nw * SCAPH_WAVE;
+  for (size_t base = ((size_t)bid * nw + threadIdx.x / SCAPH_WAVE) * SCAPH_WAVE;
        base < s.n; base += wstride) {
     const size_t v = base + lane_id();
     const bool valid = v < s.n;
@@ -448,6 +478,78 @@ k_ms_pull(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm,
     if (lane_id() == 0) an[base >> 6] = b;
   }
   flush_counts(&s.ctr[L + 1], bits, nact, mact);
+}
+
+__global__ void __launch_bounds__(1024)
+k_ms_pull(scaph_graph_desc g, scaph_ms_state s, uint32_t L, scaph_ms_params prm,
+          const uint32_t* degree) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const scaph_ms_counters* c = &s.ctr[L];
+  if (c->n_active == 0 || use_push(c, s.n, prm)) return;
+  const uint32_t np = g.num_pages < SCAPH_LDS_PAGES ? g.num_pages : SCAPH_LDS_PAGES;
+  uint32_t* const s_hub = reinterpret_cast<uint32_t*>(lds + (size_t)np * 16);
+  stage_hub_bits(s_hub, s.hub_bits, prm.num_hubs);
+  const PageTab t = stage_pages(g, lds);
+  __syncthreads();
+  ms_pull(t, g, s_hub, s, L, prm, degree, blockIdx.x, gridDim.x);
+}
+
+// Device bytes of a page table staged in LDS (page_lds_bytes on the host).
+__device__ __forceinline__ size_t page_lds_bytes_dev(uint32_t num_pages) {
+  return (size_t)(num_pages < SCAPH_LDS_PAGES ? num_pages : SCAPH_LDS_PAGES) * 16;
+}
+
+__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every level of one multi-source traversal in one cooperative launch of
+// 256-thread blocks, all resident: the phases of a level (push: prep, push,
+// heavy push, finalize; pull: hub gather, pull) are separated by grid
+// barriers, and each block takes the level's direction and the loop's end
+// from the counters, read at device scope after a barrier, so every block
+// makes the same decisions and reaches the same barriers.  Levels
+// first .. max_levels - 2 at most: a traversal still active after the last
+// level buffer stops there (the host reports it).
+__global__ void __launch_bounds__(kBlock)
+k_ms_persistent(scaph_graph_desc gout, scaph_graph_desc gin, const uint32_t* hub_list,
+                const uint32_t* degree, scaph_ms_state s, scaph_ms_params prm,
+                uint32_t first_level) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const PageTab tout = stage_pages(gout, lds);
+  unsigned char* lin = lds + page_lds_bytes_dev(gout.num_pages);
+  const PageTab tin = stage_pages(gin, lin);
+  uint32_t* const s_hub = reinterpret_cast<uint32_t*>(lin + page_lds_bytes_dev(gin.num_pages));
+  __syncthreads();
+  const uint32_t bid = blockIdx.x, nblk = gridDim.x;
+  for (uint32_t L = first_level; L + 1 < s.max_levels; ++L) {
+    const scaph_ms_counters* c = &s.ctr[L];
+    const unsigned long long na = ld_agent(&c->n_active);
+    if (na == 0) break;
+    const bool push = na * (unsigned long long)prm.push_div < (unsigned long long)s.n &&
+                      ld_agent(&c->m_active) * (unsigned long long)prm.push_mdiv < prm.m;
+    if (push) {
+      ms_prep(s, L, bid, nblk);
+      grid.sync();
+      ms_push(tout, gout, s, L, bid, nblk);
+      grid.sync();
+      if (ld_agent(&c->heavy_len)) {
+        ms_push_heavy(tout, gout, s, L, bid, nblk);
+        grid.sync();
+      }
+      ms_finalize(s, L, degree, bid, nblk);
+    } else {
+      if (prm.num_hubs) {
+        ms_hub_gather(s, L, prm.num_hubs, hub_list, bid, nblk);
+        grid.sync();
+        stage_hub_bits(s_hub, s.hub_bits, prm.num_hubs);
+        __syncthreads();
+      }
+      ms_pull(tin, gin, s_hub, s, L, prm, degree, bid, nblk);
+    }
+    grid.sync();
+  }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -570,13 +672,47 @@ int scaph_ms_init(const scaph_ms_state* st, const uint32_t* sources, uint32_t co
   return 0;
 }
 
+static bool ms_args_ok(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
+                       const uint32_t* hub_list, const uint32_t* degree, const scaph_ms_state* st,
+                       uint32_t level, const scaph_ms_params* prm) {
+  return out_graph && in_graph && degree && st && prm && level + 1 < st->max_levels &&
+         prm->num_queries != 0 && prm->num_queries <= SCAPH_MS_MAX_QUERIES &&
+         prm->push_div != 0 && prm->push_mdiv != 0 && prm->num_hubs <= SCAPH_MS_MAX_HUBS &&
+         (prm->num_hubs == 0 || hub_list);
+}
+
+int scaph_ms_run_persistent(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
+                            const uint32_t* hub_list, const uint32_t* degree,
+                            const scaph_ms_state* st, uint32_t first_level,
+                            const scaph_ms_params* prm, void* stream) {
+  if (!ms_args_ok(out_graph, in_graph, hub_list, degree, st, first_level, prm))
+    return SCAPH_STATUS_INVALID;
+  const size_t lds = page_lds_bytes(out_graph->num_pages) + page_lds_bytes(in_graph->num_pages) +
+                     ms_hub_lds_bytes(prm->num_hubs);
+  int dev = 0, cus = 0, per_cu = 0;
+  SCAPH_TRY(hipGetDevice(&dev));
+  SCAPH_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // every block must be resident for the grid barriers: the grid is what the
+  // occupancy calculation allows, and the cooperative launch refuses more
+  SCAPH_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(k_ms_persistent), (int)kBlock, lds));
+  if (per_cu < 1 || cus < 1) return SCAPH_STATUS_INVALID;
+  scaph_graph_desc go = *out_graph, gi = *in_graph;
+  scaph_ms_state s = *st;
+  scaph_ms_params p = *prm;
+  uint32_t first = first_level;
+  void* args[] = {&go, &gi, const_cast<uint32_t**>(&hub_list), const_cast<uint32_t**>(&degree),
+                  &s, &p, &first};
+  SCAPH_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_ms_persistent),
+                                       dim3((unsigned)(per_cu * cus)), dim3(kBlock), args,
+                                       (unsigned)lds, as_stream(stream)));
+  return 0;
+}
+
 int scaph_ms_level(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
                    const uint32_t* hub_list, const uint32_t* degree, const scaph_ms_state* st,
                    uint32_t level, const scaph_ms_params* prm, void* stream) {
-  if (!out_graph || !in_graph || !degree || !st || !prm || level + 1 >= st->max_levels ||
-      prm->num_queries == 0 || prm->num_queries > SCAPH_MS_MAX_QUERIES || prm->push_div == 0 ||
-      prm->push_mdiv == 0 || prm->num_hubs > SCAPH_MS_MAX_HUBS ||
-      (prm->num_hubs != 0 && !hub_list))
+  if (!ms_args_ok(out_graph, in_graph, hub_list, degree, st, level, prm))
     return SCAPH_STATUS_INVALID;
   const uint32_t pb = prm->pull_block;
   if (pb != 256 && pb != 512 && pb != 1024) return SCAPH_STATUS_INVALID;

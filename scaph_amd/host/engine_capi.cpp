@@ -357,6 +357,7 @@ scaph::MsTune from_c(const scaph_ms_tune& t) {
   o.use_graph = t.use_graph;
   o.segment = t.segment ? t.segment : 8;
   o.pull_grid = t.pull_grid;
+  o.persistent = t.persistent;
   return o;
 }
 scaph_ms_tune to_c(const scaph::MsTune& t) {
@@ -367,6 +368,7 @@ scaph_ms_tune to_c(const scaph::MsTune& t) {
   o.use_graph = t.use_graph;
   o.segment = t.segment;
   o.pull_grid = t.pull_grid;
+  o.persistent = t.persistent;
   return o;
 }
 }  // namespace

@@ -84,7 +84,19 @@ MsStats MultiSource::run(const uint32_t* sources, uint32_t count, const MsTune& 
   const size_t rec = sizeof(scaph_ms_counters);
   Event e0, e1;
   uint32_t depth = 0;
-  if (!t.use_graph) {
+  if (t.persistent) {
+    e0.record(st);
+    H_ERR(scaph_ms_init(&st_, src_.get<uint32_t>(), count, e_.d_degree(), st));
+    H_ERR(scaph_ms_run_persistent(&e_.out_desc(), &e_.in_desc(), e_.d_hub_list(), e_.d_degree(),
+                                  &st_, 0, &p, st));
+    e1.record(st);
+    const uint32_t last = max_levels_ - 1;
+    H_ERR(scaph_copy_d2h_async(hc + last, reinterpret_cast<char*>(st_.ctr) + (size_t)last * rec,
+                               rec, st));
+    e1.sync();
+    if (hc[last].n_active != 0)
+      throw Error(SCAPH_STATUS_INVALID, "multi-source BFS deeper than its level buffers");
+  } else if (!t.use_graph) {
     e0.record(st);
     H_ERR(scaph_ms_init(&st_, src_.get<uint32_t>(), count, e_.d_degree(), st));
     for (uint32_t L = 0;; ++L) {
@@ -122,7 +134,7 @@ MsStats MultiSource::run(const uint32_t* sources, uint32_t count, const MsTune& 
   MsStats out;
   out.ms = Event::elapsed_ms(e0, e1);
   out.queries = count;
-  if (t.use_graph) {
+  if (t.use_graph || t.persistent) {
     depth = 0;
     while (depth < max_levels_ && hc[depth].n_active) ++depth;
   }
@@ -169,6 +181,14 @@ MsTune MultiSource::autotune(const uint32_t* sources, uint32_t count,
           t.pull_grid = pg;
           cand.push_back(t);
         }
+  // every level in one cooperative launch, for each direction threshold
+  for (const auto& th : thr) {
+    MsTune t;
+    t.push_div = th[0];
+    t.push_mdiv = th[1];
+    t.persistent = 1;
+    cand.push_back(t);
+  }
   MsTune best;
   double best_ms = 1e300;
   for (const MsTune& t : cand) {

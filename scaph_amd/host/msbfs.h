@@ -23,9 +23,11 @@ struct MsTune {
   uint32_t use_graph = 0;     // replay captured level segments
   uint32_t segment = 8;       // levels per captured segment
   uint32_t pull_grid = 0;     // cap on pull blocks (0: one per block of vertices)
+  uint32_t persistent = 0;    // all levels in one cooperative launch (scaph_ms_run_persistent)
   bool operator==(const MsTune& o) const {
     return push_div == o.push_div && push_mdiv == o.push_mdiv && pull_block == o.pull_block &&
-           use_graph == o.use_graph && segment == o.segment && pull_grid == o.pull_grid;
+           use_graph == o.use_graph && segment == o.segment && pull_grid == o.pull_grid &&
+           persistent == o.persistent;
   }
 };
 

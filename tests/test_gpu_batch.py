@@ -43,7 +43,8 @@ def _check_all(e, off, ngh, src, label):
     None, PUSH_ALWAYS, PULL_ALWAYS,
     {"pull_block": 1024, "use_graph": 1, "segment": 3},
     {"pull_block": 512, "use_graph": 1, "segment": 8, "push_div": 16, "push_mdiv": 8},
-    {"pull_block": 256, "pull_grid": 8}])
+    {"pull_block": 256, "pull_grid": 8},
+    {"persistent": 1}, {"persistent": 1, **PUSH_ALWAYS}, {"persistent": 1, **PULL_ALWAYS}])
 def test_msbfs_256_queries_vs_oracle(tune):
     off, ngh = O.rmat_csr(14, 16, seed=21)
     src = _sources(off, 256, 5)
@@ -58,6 +59,21 @@ def test_msbfs_256_queries_vs_oracle(tune):
         assert st["pairs"] >= reached
         cnt = e.msbfs_counters()
         assert cnt[0]["bits"] == 256 and cnt[st["depth"]]["n_active"] == 0
+
+
+@pytest.mark.parametrize("flags", [["-pull_hubs=0"], ["-pull_hubs=300"], ["-pull_hubs=229376"]])
+def test_msbfs_hub_counts(flags):
+    """Pull levels with no hubs, a few hubs and every vertex a hub (the
+    hubs' frontier words gathered per level): every query equals the
+    oracle."""
+    off, ngh = O.rmat_csr(14, 16, seed=24)
+    src = _sources(off, 256, 6)
+    with scaph_amd.BFS("-max_memory=65536", *flags) as e:
+        e.load_csr(off, ngh)
+        say("msbfs hub counts", flags)
+        for tune in (None, PULL_ALWAYS):
+            e.msbfs(src, tune)
+            _check_all(e, off, ngh, src, f"{flags} {tune}")
 
 
 @pytest.mark.parametrize("count", [1, 37, 64, 65, 200])
@@ -87,7 +103,8 @@ def test_msbfs_many_pages_and_heavy_out_lists():
             e.load_csr(off, ngh)
             if name == "er":
                 assert e.info()["pages"] > 1024
-            for tune in (PUSH_ALWAYS, PULL_ALWAYS, None):
+            for tune in (PUSH_ALWAYS, PULL_ALWAYS, None, {"persistent": 1, **PUSH_ALWAYS},
+                         {"persistent": 1, **PULL_ALWAYS}):
                 say("msbfs", name, flags, tune)
                 e.msbfs(src, tune)
                 _check_all(e, off, ngh, src, f"{name} {tune}")
@@ -101,13 +118,30 @@ def test_msbfs_edge_cases():
     src = np.array([0, 0, 3, 7, 4, 6], np.uint32)
     with scaph_amd.BFS("-max_memory=65536") as e:
         e.load_csr(off, ngh)
-        for tune in (None, PUSH_ALWAYS, PULL_ALWAYS, {"use_graph": 1, "segment": 2}):
+        for tune in (None, PUSH_ALWAYS, PULL_ALWAYS, {"use_graph": 1, "segment": 2},
+                     {"persistent": 1}, {"persistent": 1, **PULL_ALWAYS}):
             st = e.msbfs(src, tune)
             _check_all(e, off, ngh, src, f"edge cases {tune}")
         with pytest.raises(scaph_amd.ScaphError):
             e.msbfs(np.array([n], np.uint32))
         with pytest.raises(scaph_amd.ScaphError):
             e.msbfs(np.zeros(257, np.uint32))
+
+
+def test_msbfs_deeper_than_level_buffers():
+    """A path longer than the 32 level buffers: every mode reports it (the
+    persistent launch stops at the last buffer), and a shorter path after it
+    runs normally."""
+    n = 40
+    off, ngh = O.csr_from_edges(n, list(range(n - 1)), list(range(1, n)))
+    with scaph_amd.BFS("-max_memory=65536") as e:
+        e.load_csr(off, ngh)
+        for tune in (None, {"use_graph": 1, "segment": 4}, {"persistent": 1}):
+            say("msbfs deep path", tune)
+            with pytest.raises(scaph_amd.ScaphError):
+                e.msbfs(np.array([0], np.uint32), tune)
+            e.msbfs(np.array([20, 35], np.uint32), tune)
+            _check_all(e, off, ngh, np.array([20, 35], np.uint32), f"short path {tune}")
 
 
 def _check_batch_graph(b, g):
