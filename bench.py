@@ -203,6 +203,8 @@ def bench_msbfs(scale, ef, queries, seed):
            "single_source_total_ms": sum(single),
            "speedup_vs_single_source": sum(single) / ms,
            "tune": best, "autotune_candidates": len(log_),
+           "persistent_best_ms": min((ms_ for t_, ms_ in log_ if t_.get("persistent")),
+                                     default=None),
            "levels_equal_single_source": exact,
            "k_ms_pull_roofline": profiled_roofline("msbfs_s24_ef16_256q_k_ms_pull_traffic.json")}
     eng.close()
