@@ -612,6 +612,11 @@ int scaph_ms_init(const scaph_ms_state* st, const uint32_t* sources, uint32_t co
 int scaph_ms_level(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
                    const uint32_t* hub_list, const uint32_t* degree, const scaph_ms_state* st,
                    uint32_t level, const scaph_ms_params* prm, void* stream);
+/* Pull blocks of pull_block threads that stay resident on one CU for an
+ * in-graph of in_pages pages and num_hubs hubs (a pull_grid of this times
+ * the CU count keeps every launched block resident). */
+int scaph_ms_pull_blocks_per_cu(uint32_t pull_block, uint32_t in_pages, uint32_t num_hubs,
+                                uint32_t* out);
 /* Every level from first_level on, in one cooperative launch of resident
  * 256-thread blocks (the same phases as scaph_ms_level, separated by grid
  * barriers; prm->pull_block and pull_grid are not used).  Stops when a

@@ -681,6 +681,18 @@ static bool ms_args_ok(const scaph_graph_desc* out_graph, const scaph_graph_desc
          (prm->num_hubs == 0 || hub_list);
 }
 
+int scaph_ms_pull_blocks_per_cu(uint32_t pull_block, uint32_t in_pages, uint32_t num_hubs,
+                                 uint32_t* out) {
+  if (!out || (pull_block != 256 && pull_block != 512 && pull_block != 1024))
+    return SCAPH_STATUS_INVALID;
+  int nb = 0;
+  SCAPH_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, reinterpret_cast<const void*>(k_ms_pull), (int)pull_block,
+      page_lds_bytes(in_pages) + ms_hub_lds_bytes(num_hubs)));
+  *out = nb > 0 ? (uint32_t)nb : 1u;
+  return 0;
+}
+
 int scaph_ms_run_persistent(const scaph_graph_desc* out_graph, const scaph_graph_desc* in_graph,
                             const uint32_t* hub_list, const uint32_t* degree,
                             const scaph_ms_state* st, uint32_t first_level,

@@ -168,11 +168,13 @@ MsTune MultiSource::autotune(const uint32_t* sources, uint32_t count,
   scaph_device_props props;
   const uint32_t cus =
       scaph_get_device_props(0, &props) == 0 && props.compute_units > 0 ? props.compute_units : 256;
-  for (uint32_t pb : {256u, 512u, 1024u})
+  for (uint32_t pb : {256u, 512u, 1024u}) {
+    // pull grid: one block per vertex block, or as many as stay resident
+    uint32_t per_cu = 1;
+    H_ERR(scaph_ms_pull_blocks_per_cu(pb, e_.in_desc().num_pages, e_.num_hubs(), &per_cu));
     for (const auto& th : thr)
       for (uint32_t g : {0u, 1u})
-        // pull grid: one block per vertex block, or 1024 threads per CU resident
-        for (uint32_t pg : {0u, 4u * cus * 256u / pb}) {
+        for (uint32_t pg : {0u, per_cu * cus}) {
           MsTune t;
           t.pull_block = pb;
           t.push_div = th[0];
@@ -181,6 +183,7 @@ MsTune MultiSource::autotune(const uint32_t* sources, uint32_t count,
           t.pull_grid = pg;
           cand.push_back(t);
         }
+  }
   // every level in one cooperative launch, for each direction threshold
   for (const auto& th : thr) {
     MsTune t;

@@ -21,7 +21,9 @@ then runs one workload, writing every run's per-level records:
   --workload zipf   BFS on the resident power-law digraph (105 M vertices,
                     3.7 B edges, seed 3), push only;
   --workload msbfs  256-query multi-source BFS on R-MAT scale 24, edge
-                    factor 16, seed 3, without graph replay.
+                    factor 16, seed 3: the autotuner's sweep, then four runs
+                    of its best per-level launch sequence without graph
+                    replay.
 `summarize --kernel K` scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
 by what they read for those two kernels, maps K's dispatches of the timed
 runs onto their levels, and writes bytes per launch beside the algorithmic
@@ -36,6 +38,8 @@ import os
 import re
 import statistics
 import sys
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CALIB_WORDS = 256 << 20  # 1 GiB
@@ -72,8 +76,14 @@ def run(args):
         e.gen_rmat_device(24, 16, 0.57, 0.19, 0.19, seed=3)
         info = e.info()
         srcs, _ = bench.pick_sources(e, info["n"], 256, seed=3, min_reach_frac=0.01)
-        tune = {"push_div": 16, "push_mdiv": 8, "pull_block": 256, "use_graph": 0,
-                "segment": 8, "pull_grid": 1024}
+        if args.ms_tune:
+            tune = json.loads(args.ms_tune)
+        else:
+            # the autotuner's best per-level launch sequence (its launches
+            # come first; summarize takes the last dispatches of the kernel)
+            _, _, log_ = e.msbfs_autotune(np.array(srcs, np.uint32))
+            tune = min((t for t in log_ if not t[0].get("persistent")), key=lambda t: t[1])[0]
+            tune["use_graph"] = 0
         runs = []
         for _ in range(4):  # one warm-up, three timed
             st = e.msbfs(srcs, tune)
@@ -101,6 +111,7 @@ def run(args):
     e.close()
     with open(args.records, "w") as f:
         json.dump({"workload": args.workload, "n": info["n"], "m": info["m"], "flags": flags,
+                   "ms_tune": tune if args.workload == "msbfs" else None,
                    "timed_from": 1, "runs": runs}, f)
     print(json.dumps({"runs": len(runs)}), flush=True)
 
@@ -281,6 +292,8 @@ def main():
     r.add_argument("--seed", type=int, default=2)
     r.add_argument("--sources", type=int, default=8)
     r.add_argument("--flag", action="append", default=[])
+    r.add_argument("--ms-tune", default="",
+                   help="msbfs: this tune (JSON) instead of the autotuner's choice")
     s = sub.add_parser("summarize")
     s.add_argument("--dir", required=True)
     s.add_argument("--out", required=True)
