@@ -254,6 +254,18 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
                    const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
                    uint32_t cur_level, unsigned long long max_edges, void* workspace,
                    size_t workspace_bytes, const scaph_pull_tuning* tune, void* stream);
+/* scaph_bfs_pull for a level with few unvisited vertices left: each wave
+ * gathers the unvisited vertices of its 1024-vertex windows into LDS and
+ * probes them 64 at a time, instead of probing window by window with one
+ * or two lanes busy.  Same arguments, results, counters and workspace as
+ * scaph_bfs_pull; faster when unvisited vertices are a small share of the
+ * graph, slower otherwise.  No reference counterpart. */
+int scaph_bfs_pull_sparse(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,
+                          const uint32_t* hub_list, uint32_t num_hubs, const uint32_t* degree,
+                          uint32_t* level, uint32_t* visited, const uint32_t* front_in,
+                          uint32_t* front_out, scaph_level_counters* counters, uint32_t cur_level,
+                          unsigned long long max_edges, void* workspace, size_t workspace_bytes,
+                          const scaph_pull_tuning* tune, void* stream);
 
 /* Frontier format conversions (no reference counterpart). */
 int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint32_t* bitmap,

@@ -990,6 +990,130 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// What one pull batch needs besides its vertices.
+struct PullCtx {
+  scaph_graph_desc g;
+  PageTab s;
+  const uint32_t* s_hub;
+  const uint32_t* front_in;
+  const uint4* head;
+  uint32_t head_outdeg, head_indexed;
+  PullWs hw;
+  scaph_level_counters* counters;
+};
+
+// One pull batch, one vertex per lane (act): does v have an in-neighbour in
+// the frontier?  hd is v's head record when the context has heads; hint is
+// a page at or before v's.  outdeg gets v's out-degree from the head record
+// (0xFFFFFFFF when it has none); acc_s counts the in-edges scanned.  Called
+// by the whole wave (the long lists are probed cooperatively).
+__device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t v, const uint4& hd,
+                                           uint32_t hint, uint32_t& outdeg,
+                                           unsigned long long& acc_s) {
+  const scaph_graph_desc& g = pc.g;
+  const PageTab& s = pc.s;
+  const uint32_t* s_hub = pc.s_hub;
+  const uint32_t* front_in = pc.front_in;
+  const uint4* head = pc.head;
+  const uint32_t head_outdeg = pc.head_outdeg, head_indexed = pc.head_indexed;
+  const PullWs& hw = pc.hw;
+  scaph_level_counters* counters = pc.counters;
+  const uint32_t lane = lane_id();
+  bool found = false, longl = false;
+  NodeRef nr;
+  nr.edges = nullptr;
+  nr.len = 0;
+  outdeg = 0xFFFFFFFFu;
+  if (act) {
+    uint32_t k = 0;
+    if (head) {
+      // the first three in-list entries and the in-degree sit in a
+      // 16-byte record read in vertex order: a vertex found there (the
+      // lists start with their hubs) never touches its page
+      // (head_outdeg: {e0, e1, in-degree, out-degree} instead, so a
+      // found vertex also needs no degree[] load)
+      // (head_indexed: {e0, e1, Node idx << 8 | min(in-degree, 255),
+      // out-degree}, so a list past its head needs no Node record)
+      const uint32_t cap = head_outdeg ? 2u : 3u;
+      const uint32_t len = head_indexed ? (hd.z & 0xFFu) : head_outdeg ? hd.z : hd.w;
+      if (head_outdeg) outdeg = hd.w;
+      nr.len = len;
+      const uint32_t l0 = len < cap ? len : cap;
+      const bool h0 = l0 > 0 && in_front(hd.x, s_hub, front_in);
+      const bool h1 = l0 > 1 && in_front(hd.y, s_hub, front_in);
+      const bool h2 = l0 > 2 && in_front(hd.z, s_hub, front_in);
+      k = l0;
+      found = h0 | h1 | h2;
+      if (!found && len > cap) {
+        if (head_indexed && len < 255u)
+          nr.edges = page_edges_hint(s, g, v, hint) + (hd.z >> 8);
+        else
+          nr = load_node_hint(s, g, v, hint);
+      }
+    } else {
+      nr = load_node_hint(s, g, v, hint);
+    }
+    const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
+    for (; !found && k < lim; k += kPullProbe) {
+      uint32_t u[kPullProbe];
+#pragma unroll
+      for (uint32_t j = 0; j < kPullProbe; ++j) u[j] = (k + j < lim) ? nr.edges[k + j] : 0u;
+      bool h = false;
+#pragma unroll
+      for (uint32_t j = 0; j < kPullProbe; ++j) h |= (k + j < lim) && in_front(u[j], s_hub, front_in);
+      if (h) {
+        found = true;
+        k += kPullProbe;
+        break;
+      }
+    }
+    acc_s += k < lim ? k : lim;
+    longl = !found && nr.len > kPullSerial;
+  }
+  // lists past the serial probes: the whole wave probes 64 at a time up
+  // to kHeavy entries; what is left after that is deferred
+  uint64_t ml = __ballot(longl);
+  while (ml) {
+    const int j = __ffsll((unsigned long long)ml) - 1;
+    ml &= ml - 1;
+    const uint64_t ep = wave_bcast64(reinterpret_cast<uint64_t>(nr.edges), j);
+    const uint32_t len = wave_bcast(nr.len, j);
+    const uint32_t stop = len < kHeavy ? len : kHeavy;
+    const uint32_t* e = reinterpret_cast<const uint32_t*>(ep);
+    bool hitj = false;
+    for (uint32_t k = kPullSerial; k < stop; k += SCAPH_WAVE) {
+      const uint32_t kk = k + lane;
+      const bool hit = kk < stop && in_front(e[kk], s_hub, front_in);
+      const uint64_t mh = __ballot(hit);
+      if (lane == 0) acc_s += min(stop - k, (uint32_t)SCAPH_WAVE);
+      if (mh) {
+        hitj = true;
+        break;
+      }
+    }
+    if (hitj) {
+      if ((int)lane == j) found = true;
+    } else if (len > kHeavy && (int)lane == j) {
+      // grid-wide follow-up: one work item per kHeavyChunk in-edges
+      const uint32_t rest = len - kHeavy;
+      const uint32_t nch = (rest + kHeavyChunk - 1) / kHeavyChunk;
+      const unsigned long long at = atomicAdd(hw.count, (unsigned long long)nch);
+      if (at + nch <= hw.cap) {
+        for (uint32_t c = 0; c < nch; ++c) {
+          HeavyChunk h;
+          h.v = v;
+          h.len = min(kHeavyChunk, rest - c * kHeavyChunk);
+          h.ptr = reinterpret_cast<uint64_t>(e + kHeavy + (uint64_t)c * kHeavyChunk);
+          hw.chunks[at + c] = h;
+        }
+      } else {
+        counters->reserved = 1;  // overflow: the host reports an error
+      }
+    }
+  }
+  return found;
+}
+
 // WAVES waves per block share one copy of the hub bits; the window queue
 // holds 10-bit window offsets as u16, so 8-wave blocks fit three to a CU.
 // MINW > 1 asks the compiler for registers that let MINW waves share a SIMD.
@@ -1011,6 +1135,7 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
   stage_hubs(s_hub, hw.hub_front, num_hubs);
   const PageTab s = stage_pages(g, dyn_lds);
   __syncthreads();
+  const PullCtx pc{g, s, s_hub, front_in, head, head_outdeg, head_indexed, hw, counters};
   const uint32_t n = g.num_vertices;
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x / SCAPH_WAVE;
@@ -1079,100 +1204,11 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
       const uint32_t v = (uint32_t)(w0 * 32) + loc;
       const uint4 hd_cur = hd_next;
       if (PREFETCH && head && i + SCAPH_WAVE < tot) hd_next = head[hidx(q[i + SCAPH_WAVE])];
-      bool found = false, longl = false;
-      NodeRef nr;
-      nr.edges = nullptr;
-      nr.len = 0;
-      uint32_t outdeg = 0xFFFFFFFFu;
-      if (act) {
-        ++acc_x;
-        uint32_t k = 0;
-        if (head) {
-          // the first three in-list entries and the in-degree sit in a
-          // 16-byte record read in vertex order: a vertex found there (the
-          // lists start with their hubs) never touches its page
-          // (head_outdeg: {e0, e1, in-degree, out-degree} instead, so a
-          // found vertex also needs no degree[] load)
-          // (head_indexed: {e0, e1, Node idx << 8 | min(in-degree, 255),
-          // out-degree}, so a list past its head needs no Node record)
-          const uint4 hd = PREFETCH ? hd_cur : head[hidx(loc)];
-          const uint32_t cap = head_outdeg ? 2u : 3u;
-          const uint32_t len = head_indexed ? (hd.z & 0xFFu) : head_outdeg ? hd.z : hd.w;
-          if (head_outdeg) outdeg = hd.w;
-          nr.len = len;
-          const uint32_t l0 = len < cap ? len : cap;
-          const bool h0 = l0 > 0 && in_front(hd.x, s_hub, front_in);
-          const bool h1 = l0 > 1 && in_front(hd.y, s_hub, front_in);
-          const bool h2 = l0 > 2 && in_front(hd.z, s_hub, front_in);
-          k = l0;
-          found = h0 | h1 | h2;
-          if (!found && len > cap) {
-            if (head_indexed && len < 255u)
-              nr.edges = page_edges_hint(s, g, v, hint) + (hd.z >> 8);
-            else
-              nr = load_node_hint(s, g, v, hint);
-          }
-        } else {
-          nr = load_node_hint(s, g, v, hint);
-        }
-        const uint32_t lim = nr.len < kPullSerial ? nr.len : kPullSerial;
-        for (; !found && k < lim; k += kPullProbe) {
-          uint32_t u[kPullProbe];
-#pragma unroll
-          for (uint32_t j = 0; j < kPullProbe; ++j) u[j] = (k + j < lim) ? nr.edges[k + j] : 0u;
-          bool h = false;
-#pragma unroll
-          for (uint32_t j = 0; j < kPullProbe; ++j) h |= (k + j < lim) && in_front(u[j], s_hub, front_in);
-          if (h) {
-            found = true;
-            k += kPullProbe;
-            break;
-          }
-        }
-        acc_s += k < lim ? k : lim;
-        longl = !found && nr.len > kPullSerial;
-      }
-      // lists past the serial probes: the whole wave probes 64 at a time up
-      // to kHeavy entries; what is left after that is deferred
-      uint64_t ml = __ballot(longl);
-      while (ml) {
-        const int j = __ffsll((unsigned long long)ml) - 1;
-        ml &= ml - 1;
-        const uint64_t ep = wave_bcast64(reinterpret_cast<uint64_t>(nr.edges), j);
-        const uint32_t len = wave_bcast(nr.len, j);
-        const uint32_t stop = len < kHeavy ? len : kHeavy;
-        const uint32_t* e = reinterpret_cast<const uint32_t*>(ep);
-        bool hitj = false;
-        for (uint32_t k = kPullSerial; k < stop; k += SCAPH_WAVE) {
-          const uint32_t kk = k + lane;
-          const bool hit = kk < stop && in_front(e[kk], s_hub, front_in);
-          const uint64_t mh = __ballot(hit);
-          if (lane == 0) acc_s += min(stop - k, (uint32_t)SCAPH_WAVE);
-          if (mh) {
-            hitj = true;
-            break;
-          }
-        }
-        if (hitj) {
-          if ((int)lane == j) found = true;
-        } else if (len > kHeavy && (int)lane == j) {
-          // grid-wide follow-up: one work item per kHeavyChunk in-edges
-          const uint32_t rest = len - kHeavy;
-          const uint32_t nch = (rest + kHeavyChunk - 1) / kHeavyChunk;
-          const unsigned long long at = atomicAdd(hw.count, (unsigned long long)nch);
-          if (at + nch <= hw.cap) {
-            for (uint32_t c = 0; c < nch; ++c) {
-              HeavyChunk h;
-              h.v = v;
-              h.len = min(kHeavyChunk, rest - c * kHeavyChunk);
-              h.ptr = reinterpret_cast<uint64_t>(e + kHeavy + (uint64_t)c * kHeavyChunk);
-              hw.chunks[at + c] = h;
-            }
-          } else {
-            counters->reserved = 1;  // overflow: the host reports an error
-          }
-        }
-      }
+      uint4 hd = hd_cur;
+      if (!PREFETCH && head && act) hd = head[hidx(loc)];
+      if (act) ++acc_x;
+      uint32_t outdeg;
+      const bool found = pull_batch(pc, act, v, hd, hint, outdeg, acc_s);
       if (found) {
         level[v] = next_level;
         acc_n += 1;
@@ -1210,6 +1246,138 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
     if (a) atomicAdd(&counters->n_next, a);
     if (b) atomicAdd(&counters->m_next, b);
     if (c) atomicAdd(&counters->scanned, c);
+    if (x) atomicAdd(&counters->examined, x);
+  }
+}
+
+// Sparse pull levels (few unvisited vertices left).  k_pull gives a wave
+// one 1024-vertex window at a time; with one or two unvisited vertices per
+// window its lanes idle and the wave runs those vertices' dependent chains
+// (head record, list, probe) one window after the other.  Here a wave
+// gathers the unvisited vertices of its windows, with their head-record
+// index, into an LDS buffer and probes them 64 at a time once the buffer
+// fills or its windows end.  Found vertices are claimed with atomics on
+// visited and on the next frontier, which the launch before clears.
+constexpr uint32_t kSparseBuf = 256;  // vertices a wave holds before it probes them
+
+template <int WAVES>
+__global__ void __launch_bounds__(WAVES * SCAPH_WAVE)
+k_pull_sparse(scaph_graph_desc g, const uint4* head, const uint32_t* no_in,
+              const uint32_t* live_base, uint32_t head_outdeg, uint32_t head_indexed,
+              const uint32_t* degree, uint32_t* level, uint32_t* visited,
+              const uint32_t* front_in, uint32_t* front_out, scaph_level_counters* counters,
+              uint32_t next_level, PullWs hw, uint32_t num_hubs) {
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  __shared__ uint32_t s_v[WAVES][kSparseBuf];
+  __shared__ uint32_t s_h[WAVES][kSparseBuf];
+  __shared__ unsigned long long red[4][WAVES];
+  uint32_t* const s_hub = hub_lds(dyn_lds, g.num_pages);
+  stage_hubs(s_hub, hw.hub_front, num_hubs);
+  const PageTab s = stage_pages(g, dyn_lds);
+  __syncthreads();
+  const PullCtx pc{g, s, s_hub, front_in, head, head_outdeg, head_indexed, hw, counters};
+  const uint32_t n = g.num_vertices;
+  const uint32_t lane = lane_id();
+  const uint32_t wid = threadIdx.x / SCAPH_WAVE;
+  uint32_t* const sv = s_v[wid];
+  uint32_t* const sh = s_h[wid];
+  const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
+  const uint64_t windows = (words + kPullWinWords - 1) / kPullWinWords;
+  const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+  unsigned long long acc_n = 0, acc_m = 0, acc_s = 0, acc_x = 0;
+  uint32_t held = 0;  // wave-uniform
+  auto drain = [&]() {
+    wave_lds_sync();
+    for (uint32_t i0 = 0; i0 < held; i0 += SCAPH_WAVE) {
+      const uint32_t i = i0 + lane;
+      const bool act = i < held;
+      const uint32_t v = act ? sv[i] : 0u;
+      uint4 hd = make_uint4(0u, 0u, 0u, 0u);
+      if (head && act) hd = head[sh[i]];
+      const uint32_t hint = act ? find_page(s, g, v) : 0u;
+      if (act) ++acc_x;
+      uint32_t outdeg;
+      const bool found = pull_batch(pc, act, v, hd, hint, outdeg, acc_s);
+      if (found) {
+        const uint32_t bit = 1u << (v & 31);
+        level[v] = next_level;
+        acc_n += 1;
+        acc_m += outdeg != 0xFFFFFFFFu ? outdeg : degree[v];
+        atomicOr(&visited[v >> 5], bit);
+        atomicOr(&front_out[v >> 5], bit);
+      }
+    }
+    wave_lds_sync();
+    held = 0;
+  };
+  const uint64_t first = (uint64_t)blockIdx.x * WAVES + wid;
+  // a window's visited word, in-edge word and live base are requested one
+  // window ahead, so the walk waits on none of them
+  uint32_t vw_next = ~0u, nw_next = ~0u, base_next = 0u;
+  auto request = [&](uint64_t win) {
+    const uint64_t w = win * kPullWinWords + lane;
+    const bool own = lane < kPullWinWords && w < words;
+    vw_next = own ? visited[w] : ~0u;
+    if (live_base) {
+      nw_next = own ? no_in[w] : ~0u;
+      base_next = live_base[win];
+    }
+  };
+  if (first < windows) request(first);
+  for (uint64_t win = first; win < windows; win += nwaves) {
+    const uint64_t w = win * kPullWinWords + lane;
+    const uint32_t vw = vw_next, nw = nw_next, wbase = base_next;
+    if (win + nwaves < windows) request(win + nwaves);
+    const uint32_t unv = ~vw;
+    const uint32_t c = __popc(unv);
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t tot = wave_bcast(incl, SCAPH_WAVE - 1);
+    if (tot == 0) continue;
+    // head index = live rank, as k_pull's hidx computes it
+    uint32_t lw = 0, lb = 0;
+    if (live_base) {
+      lw = ~nw;
+      const uint32_t lc = __popc(lw);
+      lb = wave_incl_scan(lc) - lc;
+    }
+    // the window's vertices in order, kSparseBuf at a time
+    for (uint32_t lo = 0; lo < tot; lo += kSparseBuf) {
+      const uint32_t take = tot - lo < kSparseBuf ? tot - lo : kSparseBuf;
+      if (held + take > kSparseBuf) drain();
+      uint32_t o = incl - c;
+      for (uint32_t m = unv; m; m &= m - 1, ++o) {
+        if (o < lo || o >= lo + take) continue;
+        const uint32_t b = __ffs(m) - 1;
+        sv[held + o - lo] = (uint32_t)(w * 32 + b);
+        sh[held + o - lo] =
+            live_base ? wbase + lb + __popc(lw & ((1u << b) - 1u)) : (uint32_t)(w * 32 + b);
+      }
+      held += take;
+    }
+  }
+  if (held) drain();
+  const unsigned long long sn = wave_sum(acc_n);
+  const unsigned long long sm = wave_sum(acc_m);
+  const unsigned long long ss = wave_sum(acc_s);
+  const unsigned long long sx = wave_sum(acc_x);
+  if (lane == 0) {
+    red[0][wid] = sn;
+    red[1][wid] = sm;
+    red[2][wid] = ss;
+    red[3][wid] = sx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b2 = 0, c2 = 0, x = 0;
+    for (uint32_t i = 0; i < WAVES; ++i) {
+      a += red[0][i];
+      b2 += red[1][i];
+      c2 += red[2][i];
+      x += red[3][i];
+    }
+    if (a) atomicAdd(&counters->n_next, a);
+    if (b2) atomicAdd(&counters->m_next, b2);
+    if (c2) atomicAdd(&counters->scanned, c2);
     if (x) atomicAdd(&counters->examined, x);
   }
 }
@@ -1856,6 +2024,50 @@ int scaph_bfs_push_runs(const scaph_page_run* runs, uint32_t num_runs, uint32_t 
 
 size_t scaph_pull_workspace_bytes(unsigned long long max_edges) {
   return carve_pull_ws(nullptr, max_edges).bytes;
+}
+
+int scaph_bfs_pull_sparse(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,
+                          const uint32_t* hub_list, uint32_t num_hubs, const uint32_t* degree,
+                          uint32_t* level, uint32_t* visited, const uint32_t* front_in,
+                          uint32_t* front_out, scaph_level_counters* counters, uint32_t cur_level,
+                          unsigned long long max_edges, void* workspace, size_t workspace_bytes,
+                          const scaph_pull_tuning* tune, void* stream) {
+  if (!in_graph || !degree || !level || !visited || !front_in || !front_out || !counters ||
+      !workspace || num_hubs > kMaxHubs || (num_hubs && !hub_list))
+    return SCAPH_STATUS_INVALID;
+  PullWs hw = carve_pull_ws(workspace, max_edges);
+  if (hw.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
+  const uint4* head = heads ? reinterpret_cast<const uint4*>(heads->head) : nullptr;
+  const uint32_t* no_in = head ? heads->no_in_edges : nullptr;
+  const uint32_t* live_base = no_in ? heads->live_base : nullptr;
+  if (head && heads->live_base && !no_in) return SCAPH_STATUS_INVALID;
+  const uint32_t indexed_heads = tune ? tune->head_indexed : 0u;
+  const uint32_t outdeg_heads = tune ? (tune->head_outdeg | indexed_heads) : 0u;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, hw.count, 1u);
+  SCAPH_LAUNCH_CHECK();
+  if (num_hubs) {
+    hipLaunchKernelGGL(k_hub_gather, dim3((num_hubs + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       hub_list, num_hubs, front_in, hw.hub_front);
+    SCAPH_LAUNCH_CHECK();
+  }
+  const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
+  SCAPH_TRY((hipError_t)scaph_fill_u32(front_out, 0u, words, stream));
+  const size_t lds = page_lds_bytes(in_graph->num_pages) + hub_lds_bytes(num_hubs);
+  const unsigned threads = 8 * SCAPH_WAVE;
+  const unsigned grid =
+      grid_for(words, (size_t)kPullWinWords * 8,
+               resident_blocks(reinterpret_cast<const void*>(k_pull_sparse<8>), threads, lds) *
+                   cu_count());
+  hipLaunchKernelGGL(k_pull_sparse<8>, dim3(grid), dim3(threads), lds, st, *in_graph, head, no_in,
+                     live_base, outdeg_heads, indexed_heads, degree, level, visited, front_in,
+                     front_out, counters, cur_level + 1, hw, num_hubs);
+  SCAPH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_pull_heavy, dim3(2048), dim3(kBlock), hub_lds_bytes(num_hubs), st,
+                     hw.chunks, hw.count, degree, level, visited, front_in, front_out, counters,
+                     cur_level + 1, hw.hub_front, num_hubs);
+  SCAPH_LAUNCH_CHECK();
+  return 0;
 }
 
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,

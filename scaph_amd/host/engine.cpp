@@ -64,6 +64,8 @@ Config& global_flags() {
           "waves per pull block, 4 or 8 (0 = default); the hub bits are staged once per block");
     r.add("pull_hubs", FlagType::kInt32, &c.pull_hubs,
           "highest-out-degree vertices whose frontier bits the pull keeps in LDS (<= 262144)");
+    r.add("pull_sparse", FlagType::kInt32, &c.pull_sparse,
+          "pull over a list of the unvisited vertices when fewer than n / this are left (0 = never)");
     r.add("pull_simd", FlagType::kInt32, &c.pull_simd,
           "8: pull kernel registers capped so 8 waves share a SIMD (0 = compiler's choice)");
     r.add("learned_fused", FlagType::kBool, &c.learned_fused,

@@ -51,6 +51,8 @@ struct Config {
   int32_t pull_waves = 0;        // waves per pull block, 4 or 8 (0: the kernel default, 8)
   int32_t pull_simd = 0;         // 8: pull registers capped for 8 waves per SIMD (0: compiler)
   int32_t pull_hubs = 131072;    // in-list sources tagged as hubs, frontier bits in LDS
+  int32_t pull_sparse = 128;     // pull over a list of the unvisited vertices when fewer than
+                                 // n / this are left (known after a pull level; 0 = never)
   bool pull_head_outdeg = true;  // head records {e0, e1, in-degree, out-degree}
   bool pull_head_live = true;    // head records only for vertices with in-edges (live rank)
   bool pull_prefetch = true;     // request the next batch's head records early
@@ -266,7 +268,9 @@ class Engine {
     bool snap_ok = false;  // vsnap_ holds visited from before the push that made the queue
   };
   void ensure_forms(Front& f, bool queue, bool bitmap, uint64_t n_f);
-  void launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m_f);
+  // sparse: a pull level over a list of the few unvisited vertices left
+  void launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m_f,
+                   bool sparse = false);
   struct DeviceModel;
   std::unique_ptr<DeviceModel> model_;
   void ensure_model_buffers();

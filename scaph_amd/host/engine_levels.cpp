@@ -55,7 +55,7 @@ void Engine::ensure_forms(Front& f, bool queue, bool bitmap, uint64_t n_f) {
 }
 
 // Launches one level in direction d (counters must be cleared first).
-void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m_f) {
+void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m_f, bool sparse) {
   void* st = main_->get();
   auto* cnt = counters_.get<scaph_level_counters>();
   uint32_t* level = level_.get<uint32_t>();
@@ -78,10 +78,17 @@ void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m
     scaph_pull_heads heads{in_head_.get<uint32_t>(),
                            live_base_.get() ? static_visited_.get<uint32_t>() : nullptr,
                            live_base_.get<uint32_t>()};
-    H_ERR(scaph_bfs_pull(&g_in_, in_head_.get() ? &heads : nullptr, hub_list_.get<uint32_t>(),
-                         num_hubs_, deg, level, visited,
-                         front_[f.fa].get<uint32_t>(), front_[f.fa ^ 1].get<uint32_t>(), cnt, cur,
-                         m_, pws_.get(), pws_bytes_, &pull_tune_, st));
+    if (sparse) {
+      H_ERR(scaph_bfs_pull_sparse(&g_in_, in_head_.get() ? &heads : nullptr,
+                                  hub_list_.get<uint32_t>(), num_hubs_, deg, level, visited,
+                                  front_[f.fa].get<uint32_t>(), front_[f.fa ^ 1].get<uint32_t>(),
+                                  cnt, cur, m_, pws_.get(), pws_bytes_, &pull_tune_, st));
+    } else {
+      H_ERR(scaph_bfs_pull(&g_in_, in_head_.get() ? &heads : nullptr, hub_list_.get<uint32_t>(),
+                           num_hubs_, deg, level, visited,
+                           front_[f.fa].get<uint32_t>(), front_[f.fa ^ 1].get<uint32_t>(), cnt,
+                           cur, m_, pws_.get(), pws_bytes_, &pull_tune_, st));
+    }
     f.fa ^= 1;
     f.front_ok = true;
     f.queue_ok = false;
@@ -200,7 +207,13 @@ RunStats Engine::bfs_resident(uint32_t src) {
       while (ev_.size() < eb + 3) ev_.emplace_back(new Event());
       ev_[eb]->record(st);
     }
-    launch_step(d, f, cur, n_f, m_f);
+    // after a pull level the unvisited vertices left are known exactly: it
+    // examined every one and found n_next of them
+    const LevelRecord* prev = rs.levels.empty() ? nullptr : &rs.levels.back();
+    const bool sparse = d == Dir::kPull && prev && prev->dir == (int)Dir::kPull &&
+                        cfg_.pull_sparse > 0 &&
+                        (prev->examined - prev->n_next) * (uint64_t)cfg_.pull_sparse < n_;
+    launch_step(d, f, cur, n_f, m_f, sparse);
     if (timed) {
       ev_[eb + 1]->record(st);
       lev_ev.push_back(eb);

@@ -377,7 +377,10 @@ def test_pull_heavy_in_lists():
                                    ["-pull_simd=8"], ["-pull_simd=8", "-pull_prefetch=false"],
                                    ["-pull_simd=8", "-pull_waves=4"], ["-pull_hubs=0"],
                                    ["-pull_hubs=1000"], ["-pull_hubs=229376", "-pull_waves=4"],
-                                   ["-pull_hubs=262144"],
+                                   ["-pull_hubs=262144"], ["-pull_sparse=0"], ["-pull_sparse=1"],
+                                   ["-pull_sparse=1", "-pull_head_live=false"],
+                                   ["-pull_sparse=1", "-pull_head=false"],
+                                   ["-pull_sparse=1", "-pull_head_index=false", "-pull_hubs=0"],
                                    ["-push_small=1", "-push_scan=1"],
                                    ["-push_scan=1000000000"], ["-push_small=1", "-push_xcd=1"],
                                    ["-push_small=1", "-push_xcd=1", "-push_rows=false",
