@@ -224,16 +224,16 @@ class BFS:
         src = np.ascontiguousarray(sources, dtype=np.uint32)
         best = _lib.MsTune()
         bms = ctypes.c_double()
-        logt = (_lib.MsTune * 64)()
-        logm = (ctypes.c_double * 64)()
+        logt = (_lib.MsTune * 128)()
+        logm = (ctypes.c_double * 128)()
         cnt = ctypes.c_uint32()
         e = _err()
         _lib.check(self._lib.scaph_engine_msbfs_autotune(self._h, _ptr(src), len(src),
                                                          ctypes.byref(best), ctypes.byref(bms),
-                                                         logt, logm, 64, ctypes.byref(cnt), e,
+                                                         logt, logm, 128, ctypes.byref(cnt), e,
                                                          1024), e.value.decode())
         as_d = lambda t: {k: getattr(t, k) for k, _ in _lib.MsTune._fields_}  # noqa: E731
-        return as_d(best), bms.value, [(as_d(logt[i]), logm[i]) for i in range(min(cnt.value, 64))]
+        return as_d(best), bms.value, [(as_d(logt[i]), logm[i]) for i in range(min(cnt.value, 128))]
 
     def levels(self):
         n = self.info()["n"]

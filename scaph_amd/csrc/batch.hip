@@ -102,14 +102,23 @@ k_batch_bfs(const uint64_t* offsets, const uint32_t* ngh, const scaph_batch_entr
               if (c_pre[mid] <= ef) lo = mid; else hi = mid;
             }
           }
+          // the run's neighbour loads go out together, then its claims
+          uint32_t vr[kRun];
+#pragma unroll
+          for (uint32_t j = 0; j < kRun; ++j) {
+            const uint32_t e = ef + j;
+            vr[j] = 0;
+            if (e < T) {
+              while (lo + 1 < rows && c_pre[lo + 1] <= e) ++lo;
+              vr[j] = (uint32_t)(ngh[c_off[lo] + (e - c_pre[lo])] - vb);
+            }
+          }
 #pragma unroll
           for (uint32_t j = 0; j < kRun; ++j) {
             const uint32_t e = ef + j;
             bool claimed = false;
-            uint32_t v = 0;
+            const uint32_t v = vr[j];
             if (e < T) {
-              while (lo + 1 < rows && c_pre[lo + 1] <= e) ++lo;
-              v = (uint32_t)(ngh[c_off[lo] + (e - c_pre[lo])] - vb);
               const uint32_t bit = 1u << (v & 31);
               if (!(vis[v >> 5] & bit)) claimed = !(atomicOr(&vis[v >> 5], bit) & bit);
             }

@@ -825,9 +825,14 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
             if (!(old & bit)) {
               level[v] = next_level;
               acc_n += 1;
-              acc_m += degree[v];
-              if (MODE == OUT_BITMAP) atomicOr(&front_out[w], bit);
-              else s_claim[atomicAdd(&s_cnt, 1u)] = v;
+              if (MODE == OUT_BITMAP) {
+                acc_m += degree[v];
+                atomicOr(&front_out[w], bit);
+              } else {
+                // the out-degree is read when the tile's claims are written
+                // out, off the claim path
+                s_claim[atomicAdd(&s_cnt, 1u)] = v;
+              }
             }
           }
         }
@@ -839,7 +844,11 @@ k_expand(const uint64_t* gx, const uint64_t* gep, const uint32_t* tstart,
       const uint32_t c = s_cnt;
       if (threadIdx.x == 0 && c) s_base = atomicAdd(&counters->q_len, (unsigned long long)c);
       __syncthreads();
-      for (uint32_t k = threadIdx.x; k < c; k += kBlock) q_out[s_base + k] = s_claim[k];
+      for (uint32_t k = threadIdx.x; k < c; k += kBlock) {
+        const uint32_t v = s_claim[k];
+        q_out[s_base + k] = v;
+        acc_m += degree[v];
+      }
       __syncthreads();
     }
   }
