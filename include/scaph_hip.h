@@ -247,6 +247,8 @@ typedef struct scaph_pull_tuning {
   uint32_t head_indexed; /* heads from scaph_list_heads_indexed (implies head_outdeg) */
   uint32_t block_waves; /* waves per pull block: 4 or 8 (0: 8); the grid is what stays resident */
   uint32_t waves_per_simd; /* 8: registers capped so 8 waves share a SIMD (0: compiler's choice) */
+  uint32_t two_phase;   /* with head records: a window's vertices probed by their heads first,
+                           the undecided ones by their lists after (batches of like chains) */
 } scaph_pull_tuning;
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,
                    const uint32_t* hub_list, uint32_t num_hubs,

@@ -1016,9 +1016,13 @@ struct PullCtx {
 // a page at or before v's.  outdeg gets v's out-degree from the head record
 // (0xFFFFFFFF when it has none); acc_s counts the in-edges scanned.  Called
 // by the whole wave (the long lists are probed cooperatively).
+// PHASE 0 probes the head and then the list; with head records, PHASE 1
+// probes only the head (more: the list must still be read) and PHASE 2
+// only the list past the head, for vertices PHASE 1 left undecided.
+template <int PHASE>
 __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t v, const uint4& hd,
                                            uint32_t hint, uint32_t& outdeg,
-                                           unsigned long long& acc_s) {
+                                           unsigned long long& acc_s, bool& more) {
   const scaph_graph_desc& g = pc.g;
   const PageTab& s = pc.s;
   const uint32_t* s_hub = pc.s_hub;
@@ -1033,8 +1037,9 @@ __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t
   nr.edges = nullptr;
   nr.len = 0;
   outdeg = 0xFFFFFFFFu;
+  more = false;
   if (act) {
-    uint32_t k = 0;
+    uint32_t k = 0, k_head = 0;
     if (head) {
       // the first three in-list entries and the in-degree sit in a
       // 16-byte record read in vertex order: a vertex found there (the
@@ -1048,11 +1053,19 @@ __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t
       if (head_outdeg) outdeg = hd.w;
       nr.len = len;
       const uint32_t l0 = len < cap ? len : cap;
-      const bool h0 = l0 > 0 && in_front(hd.x, s_hub, front_in);
-      const bool h1 = l0 > 1 && in_front(hd.y, s_hub, front_in);
-      const bool h2 = l0 > 2 && in_front(hd.z, s_hub, front_in);
       k = l0;
-      found = h0 | h1 | h2;
+      if (PHASE != 2) {
+        const bool h0 = l0 > 0 && in_front(hd.x, s_hub, front_in);
+        const bool h1 = l0 > 1 && in_front(hd.y, s_hub, front_in);
+        const bool h2 = l0 > 2 && in_front(hd.z, s_hub, front_in);
+        found = h0 | h1 | h2;
+      }
+      if (PHASE == 1) {
+        more = !found && len > cap;
+        acc_s += l0 < kPullSerial ? l0 : kPullSerial;
+        return found;
+      }
+      k_head = l0 < kPullSerial ? l0 : kPullSerial;  // counted by PHASE 1
       if (!found && len > cap) {
         if (head_indexed && len < 255u)
           nr.edges = page_edges_hint(s, g, v, hint) + (hd.z >> 8);
@@ -1076,9 +1089,10 @@ __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t
         break;
       }
     }
-    acc_s += k < lim ? k : lim;
+    acc_s += (k < lim ? k : lim) - (PHASE == 2 ? (k_head < lim ? k_head : lim) : 0u);
     longl = !found && nr.len > kPullSerial;
   }
+  if (PHASE == 1) return found;
   // lists past the serial probes: the whole wave probes 64 at a time up
   // to kHeavy entries; what is left after that is deferred
   uint64_t ml = __ballot(longl);
@@ -1126,7 +1140,10 @@ __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t
 // WAVES waves per block share one copy of the hub bits; the window queue
 // holds 10-bit window offsets as u16, so 8-wave blocks fit three to a CU.
 // MINW > 1 asks the compiler for registers that let MINW waves share a SIMD.
-template <int PREFETCH, int WAVES, int MINW>
+// TWO_PHASE (head records required): a window's vertices are first probed by
+// their head records alone, then the undecided ones by their lists, so that
+// a batch's lanes wait for chains of like length.
+template <int PREFETCH, int WAVES, int MINW, int TWO_PHASE>
 __global__ void __launch_bounds__(WAVES * SCAPH_WAVE, MINW)
 k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint32_t* live_base,
        uint32_t head_outdeg, uint32_t head_indexed, const uint32_t* degree, uint32_t* level,
@@ -1206,6 +1223,7 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
     // is probed (one extra load in flight per lane)
     uint4 hd_next = make_uint4(0u, 0u, 0u, 0u);
     if (PREFETCH && head && lane < tot) hd_next = head[hidx(q[lane])];
+    uint32_t nq2 = 0;  // two phases: vertices moved to the list phase (wave-uniform)
     for (uint32_t i0 = 0; i0 < tot; i0 += SCAPH_WAVE) {
       const uint32_t i = i0 + lane;
       const bool act = i < tot;
@@ -1217,12 +1235,44 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
       if (!PREFETCH && head && act) hd = head[hidx(loc)];
       if (act) ++acc_x;
       uint32_t outdeg;
-      const bool found = pull_batch(pc, act, v, hd, hint, outdeg, acc_s);
+      bool more;
+      const bool found = TWO_PHASE ? pull_batch<1>(pc, act, v, hd, hint, outdeg, acc_s, more)
+                                   : pull_batch<0>(pc, act, v, hd, hint, outdeg, acc_s, more);
       if (found) {
         level[v] = next_level;
         acc_n += 1;
         acc_m += outdeg != 0xFFFFFFFFu ? outdeg : degree[v];
         atomicOr(&fw[loc >> 5], 1u << (loc & 31));
+      }
+      if (TWO_PHASE) {
+        // the vertices whose list must still be read move to the front of
+        // the window queue (below every index this or a later batch reads)
+        const uint64_t mm = __ballot(more);
+        if (more) q[nq2 + lanes_below(mm)] = (uint16_t)loc;
+        nq2 += (uint32_t)__popcll(mm);
+      }
+    }
+    if (TWO_PHASE && nq2) {
+      // their lists, in batches of vertices that all need one
+      wave_lds_sync();
+      uint4 hd2_next = make_uint4(0u, 0u, 0u, 0u);
+      if (lane < nq2) hd2_next = head[hidx(q[lane])];
+      for (uint32_t i0 = 0; i0 < nq2; i0 += SCAPH_WAVE) {
+        const uint32_t i = i0 + lane;
+        const bool act = i < nq2;
+        const uint32_t loc = act ? q[i] : 0u;
+        const uint32_t v = (uint32_t)(w0 * 32) + loc;
+        const uint4 hd = hd2_next;
+        if (i + SCAPH_WAVE < nq2) hd2_next = head[hidx(q[i + SCAPH_WAVE])];
+        uint32_t outdeg;
+        bool more;
+        const bool found = pull_batch<2>(pc, act, v, hd, hint, outdeg, acc_s, more);
+        if (found) {
+          level[v] = next_level;
+          acc_n += 1;
+          acc_m += outdeg != 0xFFFFFFFFu ? outdeg : degree[v];
+          atomicOr(&fw[loc >> 5], 1u << (loc & 31));
+        }
       }
     }
     wave_lds_sync();
@@ -1306,7 +1356,8 @@ k_pull_sparse(scaph_graph_desc g, const uint4* head, const uint32_t* no_in,
       const uint32_t hint = act ? find_page(s, g, v) : 0u;
       if (act) ++acc_x;
       uint32_t outdeg;
-      const bool found = pull_batch(pc, act, v, hd, hint, outdeg, acc_s);
+      bool more;
+      const bool found = pull_batch<0>(pc, act, v, hd, hint, outdeg, acc_s, more);
       if (found) {
         const uint32_t bit = 1u << (v & 31);
         level[v] = next_level;
@@ -2111,11 +2162,19 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
   const bool prefetch = !tune || !tune->no_prefetch;
   const uint32_t simd = tune ? tune->waves_per_simd : 0u;
   if (simd != 0 && simd != 8) return SCAPH_STATUS_INVALID;
-  const int variant = (prefetch ? 1 : 0) | (waves == 8 ? 2 : 0) | (simd == 8 ? 4 : 0);
-  static const void* const kerns[8] = {
-      (const void*)k_pull<0, 4, 1>, (const void*)k_pull<1, 4, 1>, (const void*)k_pull<0, 8, 1>,
-      (const void*)k_pull<1, 8, 1>, (const void*)k_pull<0, 4, 8>, (const void*)k_pull<1, 4, 8>,
-      (const void*)k_pull<0, 8, 8>, (const void*)k_pull<1, 8, 8>};
+  // two phases need head records
+  const bool two = tune && tune->two_phase && head;
+  const int variant =
+      (prefetch ? 1 : 0) | (waves == 8 ? 2 : 0) | (simd == 8 ? 4 : 0) | (two ? 8 : 0);
+  static const void* const kerns[16] = {
+      (const void*)k_pull<0, 4, 1, 0>, (const void*)k_pull<1, 4, 1, 0>,
+      (const void*)k_pull<0, 8, 1, 0>, (const void*)k_pull<1, 8, 1, 0>,
+      (const void*)k_pull<0, 4, 8, 0>, (const void*)k_pull<1, 4, 8, 0>,
+      (const void*)k_pull<0, 8, 8, 0>, (const void*)k_pull<1, 8, 8, 0>,
+      (const void*)k_pull<0, 4, 1, 1>, (const void*)k_pull<1, 4, 1, 1>,
+      (const void*)k_pull<0, 8, 1, 1>, (const void*)k_pull<1, 8, 1, 1>,
+      (const void*)k_pull<0, 4, 8, 1>, (const void*)k_pull<1, 4, 8, 1>,
+      (const void*)k_pull<0, 8, 8, 1>, (const void*)k_pull<1, 8, 8, 1>};
   const void* kern = kerns[variant];
   // persistent-style: as many blocks as stay resident on every CU, each
   // walking its share of the window groups (a launch larger than what is
@@ -2125,19 +2184,27 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
   unsigned grid = grid_for(words, (size_t)kPullWinWords * waves,
                            resident_blocks(kern, threads, lds) * cu_count());
   if (tune && tune->grid_blocks) grid = tune->grid_blocks < grid ? tune->grid_blocks : grid;
-#define SCAPH_PULL_LAUNCH(P, W, M)                                                        \
-  hipLaunchKernelGGL((k_pull<P, W, M>), dim3(grid), dim3(threads), lds, st, *in_graph, head, \
-                     no_in, live_base, outdeg_heads, indexed_heads, degree, level, visited,    \
+#define SCAPH_PULL_LAUNCH(P, W, M, T)                                                        \
+  hipLaunchKernelGGL((k_pull<P, W, M, T>), dim3(grid), dim3(threads), lds, st, *in_graph, head, \
+                     no_in, live_base, outdeg_heads, indexed_heads, degree, level, visited,       \
                      front_in, front_out, counters, cur_level + 1, hw, num_hubs)
   switch (variant) {
-    case 0: SCAPH_PULL_LAUNCH(0, 4, 1); break;
-    case 1: SCAPH_PULL_LAUNCH(1, 4, 1); break;
-    case 2: SCAPH_PULL_LAUNCH(0, 8, 1); break;
-    case 3: SCAPH_PULL_LAUNCH(1, 8, 1); break;
-    case 4: SCAPH_PULL_LAUNCH(0, 4, 8); break;
-    case 5: SCAPH_PULL_LAUNCH(1, 4, 8); break;
-    case 6: SCAPH_PULL_LAUNCH(0, 8, 8); break;
-    default: SCAPH_PULL_LAUNCH(1, 8, 8); break;
+    case 0: SCAPH_PULL_LAUNCH(0, 4, 1, 0); break;
+    case 1: SCAPH_PULL_LAUNCH(1, 4, 1, 0); break;
+    case 2: SCAPH_PULL_LAUNCH(0, 8, 1, 0); break;
+    case 3: SCAPH_PULL_LAUNCH(1, 8, 1, 0); break;
+    case 4: SCAPH_PULL_LAUNCH(0, 4, 8, 0); break;
+    case 5: SCAPH_PULL_LAUNCH(1, 4, 8, 0); break;
+    case 6: SCAPH_PULL_LAUNCH(0, 8, 8, 0); break;
+    case 7: SCAPH_PULL_LAUNCH(1, 8, 8, 0); break;
+    case 8: SCAPH_PULL_LAUNCH(0, 4, 1, 1); break;
+    case 9: SCAPH_PULL_LAUNCH(1, 4, 1, 1); break;
+    case 10: SCAPH_PULL_LAUNCH(0, 8, 1, 1); break;
+    case 11: SCAPH_PULL_LAUNCH(1, 8, 1, 1); break;
+    case 12: SCAPH_PULL_LAUNCH(0, 4, 8, 1); break;
+    case 13: SCAPH_PULL_LAUNCH(1, 4, 8, 1); break;
+    case 14: SCAPH_PULL_LAUNCH(0, 8, 8, 1); break;
+    default: SCAPH_PULL_LAUNCH(1, 8, 8, 1); break;
   }
 #undef SCAPH_PULL_LAUNCH
   SCAPH_LAUNCH_CHECK();

@@ -66,6 +66,9 @@ Config& global_flags() {
           "highest-out-degree vertices whose frontier bits the pull keeps in LDS (<= 262144)");
     r.add("pull_sparse", FlagType::kInt32, &c.pull_sparse,
           "pull over a list of the unvisited vertices when fewer than n / this are left (0 = never)");
+    r.add("pull_phases", FlagType::kInt32, &c.pull_phases,
+          "2: a pull window's vertices are probed by their head records first and the undecided "
+          "ones by their lists after (1: both per vertex)");
     r.add("pull_simd", FlagType::kInt32, &c.pull_simd,
           "8: pull kernel registers capped so 8 waves share a SIMD (0 = compiler's choice)");
     r.add("learned_fused", FlagType::kBool, &c.learned_fused,
@@ -186,6 +189,7 @@ Engine::Engine(const Config& cfg)
   pull_tune_.no_prefetch = cfg.pull_prefetch ? 0u : 1u;
   pull_tune_.block_waves = (uint32_t)std::max(0, cfg.pull_waves);
   pull_tune_.waves_per_simd = (uint32_t)std::max(0, cfg.pull_simd);
+  pull_tune_.two_phase = cfg.pull_phases == 2 ? 1u : 0u;
   push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
   push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);

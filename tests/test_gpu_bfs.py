@@ -378,6 +378,11 @@ def test_pull_heavy_in_lists():
                                    ["-pull_simd=8", "-pull_waves=4"], ["-pull_hubs=0"],
                                    ["-pull_hubs=1000"], ["-pull_hubs=229376", "-pull_waves=4"],
                                    ["-pull_hubs=262144"], ["-pull_sparse=0"], ["-pull_sparse=1"],
+                                   ["-pull_phases=1"], ["-pull_phases=2", "-pull_prefetch=false"],
+                                   ["-pull_phases=2", "-pull_head_live=false", "-pull_waves=4"],
+                                   ["-pull_phases=2", "-pull_head_index=false", "-pull_hubs=0"],
+                                   ["-pull_phases=2", "-pull_head=false"],
+                                   ["-pull_phases=2", "-pull_head_outdeg=false"],
                                    ["-pull_sparse=1", "-pull_head_live=false"],
                                    ["-pull_sparse=1", "-pull_head=false"],
                                    ["-pull_sparse=1", "-pull_head_index=false", "-pull_hubs=0"],
