@@ -407,11 +407,13 @@ def main():
     frac = [b / (r["ms"] / 1e3) / HBM_PEAK for b, r in zip(balg, runs)]
     ref_gteps = [info["m"] / (r["ms"] / 1e3) / 1e9 for r in runs]
 
-    # dominant kernel: pull or push expand, by summed kernel time
-    agg = {0: [0.0, 0, 0.0], 1: [0.0, 0, 0.0]}  # dir -> [ms, launches, alg bytes]
+    # dominant kernel by summed kernel time: the push expand (0), the pull
+    # over windows (1, k_pull) or the pull over a list of the few unvisited
+    # vertices left (2, k_pull_sparse)
+    agg = {0: [0.0, 0, 0.0], 1: [0.0, 0, 0.0], 2: [0.0, 0, 0.0]}  # -> [ms, launches, alg bytes]
     for r in runs:
         for rec in r["records"]:
-            a = agg[rec["dir"]]
+            a = agg[2 if rec["dir"] == 1 and rec.get("sparse") else rec["dir"]]
             a[0] += rec["kernel_ms"]
             a[1] += 1
             a[2] += alg_bytes_step(rec, n)
@@ -440,9 +442,13 @@ def main():
                 "frac": achieved / 8000.0, "traffic": traffic, "traffic_profile": traffic_src,
                 "traffic_achieved": t_gbps,
                 "traffic_frac": t_gbps / 8000.0 if t_gbps else None,
-                "kernel": "k_pull" if dom == 1 else "push (k_prep_queue+k_scan_blocks+k_mark_tiles+k_expand)",
+                "kernel": {0: "push (k_prep_queue+k_scan_blocks+k_mark_tiles+k_expand)",
+                           1: "k_pull", 2: "k_pull_sparse"}[dom],
                 "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_bytes, "launches": launches,
-                "share_of_kernel_time": ms_tot / max(agg[0][0] + agg[1][0], 1e-9)}
+                "share_of_kernel_time": ms_tot / max(sum(a[0] for a in agg.values()), 1e-9),
+                "pull_sparse": {"avg_launch_ms": agg[2][0] / max(agg[2][1], 1),
+                                "launches": agg[2][1],
+                                "alg_bytes_per_launch": agg[2][2] / max(agg[2][1], 1)}}
 
     # the learned schedule (committed model) on the same sources, paired
     # with the fixed schedule source by source (fixed, then learned, on the

@@ -56,6 +56,8 @@ typedef struct scaph_level_record {
   uint64_t bytes_h2d;
   float h2d_ms;
   float logit; /* learned schedule: the model's pull logit behind this level's direction */
+  uint32_t sparse; /* 1: a pull over the list of the few unvisited vertices left */
+  uint32_t pad;
 } scaph_level_record;
 
 /* App(argc, argv): flags in gflags syntax (argv[0] is the program name). */

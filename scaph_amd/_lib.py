@@ -109,7 +109,7 @@ class LevelRecord(ctypes.Structure):
                 ("pages_full", ctypes.c_uint32), ("pages_compacted", ctypes.c_uint32),
                 ("pages_prefetched", ctypes.c_uint32), ("pages_direct", ctypes.c_uint32),
                 ("bytes_h2d", ctypes.c_uint64), ("h2d_ms", ctypes.c_float),
-                ("logit", ctypes.c_float)]
+                ("logit", ctypes.c_float), ("sparse", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 _hip = None

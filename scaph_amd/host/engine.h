@@ -110,6 +110,7 @@ struct LevelRecord {
   uint64_t bytes_h2d = 0;
   float h2d_ms = 0;
   float logit = 0;  // learned schedule: the model's pull logit for this level (0 otherwise)
+  bool sparse = false;  // a pull over the list of the unvisited vertices (-pull_sparse)
 };
 
 struct RunStats {

@@ -323,6 +323,8 @@ int scaph_engine_level_records(scaph_engine* e, scaph_level_record* out, uint32_
     out[i].bytes_h2d = lv[i].bytes_h2d;
     out[i].h2d_ms = lv[i].h2d_ms;
     out[i].logit = lv[i].logit;
+    out[i].sparse = lv[i].sparse ? 1u : 0u;
+    out[i].pad = 0;
   }
   if (count) *count = (uint32_t)lv.size();
   return 0;

@@ -259,6 +259,7 @@ RunStats Engine::bfs_resident(uint32_t src) {
     rec.m_next = hc->m_next;
     rec.scanned = hc->scanned;
     rec.examined = hc->examined;
+    rec.sparse = sparse;
     if (hc->reserved) throw Error(SCAPH_STATUS_INVALID, "pull work list overflow");
     if (learned) {
       logit = hout[0];

@@ -167,7 +167,7 @@ def _launches(rec, kernel):
         return out
     for r in timed:
         for lv in r["levels"]:
-            if kernel == "k_pull" and lv["dir"] == 1:
+            if kernel == "k_pull" and lv["dir"] == 1 and not lv.get("sparse"):
                 out.append((lv, None))
             elif (kernel == "k_expand" and lv["dir"] == 0 and
                   (lv["m_f"] > 8192 or lv["n_f"] > 65536)):
