@@ -249,6 +249,12 @@ typedef struct scaph_pull_tuning {
   uint32_t waves_per_simd; /* 8: registers capped so 8 waves share a SIMD (0: compiler's choice) */
   uint32_t two_phase;   /* with head records: a window's vertices probed by their heads first,
                            the undecided ones by their lists after (batches of like chains) */
+  uint32_t summary_shift; /* 0, or 5..31: scaph_bfs_pull first builds a frontier summary (one
+                             bit per 2^shift vertices, set when any of them is in front_in;
+                             ceil(n / 2^shift) <= 2^22) and tests it before each non-hub
+                             in-neighbour's frontier word.  Pays when the frontier is sparse:
+                             most probes then end at the summary.  scaph_bfs_pull_sparse
+                             ignores it. */
 } scaph_pull_tuning;
 int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* heads,
                    const uint32_t* hub_list, uint32_t num_hubs,

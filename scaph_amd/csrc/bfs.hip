@@ -32,6 +32,7 @@ constexpr uint32_t kPullSerial = 8;                // per-lane scan before wave 
 constexpr uint32_t kHeavy = 512;                   // longer in-lists go to k_pull_heavy
 constexpr uint32_t kHeavyChunk = 4096;             // in-edges per heavy work item
 constexpr uint32_t kMaxHubs = 262144;              // hub frontier bits kept in LDS (<= 32 KB)
+constexpr uint32_t kSummaryMaxBits = 1u << 22;     // frontier summary bits (512 KB)
 constexpr uint32_t kHubTag = 0x80000000u;          // in-list entry = kHubTag | hub index
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kBitsNodes = SCAPH_RUN_SEGMENT;  // Nodes per run segment (32 per thread)
@@ -893,21 +894,50 @@ struct HeavyChunk {
 struct PullWs {
   unsigned long long* count;  // heavy chunks appended
   uint32_t* hub_front;        // frontier bits of the hubs (kMaxHubs bits)
+  uint32_t* summary;          // frontier summary (kSummaryMaxBits bits)
   HeavyChunk* chunks;
   unsigned long long cap;
   size_t bytes;
+  uint32_t summary_shift;     // 0: no summary; else one bit per 2^shift vertices
 };
 
 PullWs carve_pull_ws(void* base, unsigned long long m) {
   PullWs w;
   w.cap = m / kHeavyChunk + m / kHeavy + 64;  // >= sum over lists of ceil((len-kHeavy)/chunk)
   const size_t hub_bytes = kMaxHubs / 8;
-  w.bytes = 256 + hub_bytes + w.cap * sizeof(HeavyChunk);
+  const size_t summary_bytes = kSummaryMaxBits / 8;
+  w.bytes = 256 + hub_bytes + summary_bytes + w.cap * sizeof(HeavyChunk);
   char* p = static_cast<char*>(base);
   w.count = reinterpret_cast<unsigned long long*>(p);
   w.hub_front = p ? reinterpret_cast<uint32_t*>(p + 256) : nullptr;
-  w.chunks = p ? reinterpret_cast<HeavyChunk*>(p + 256 + hub_bytes) : nullptr;
+  w.summary = p ? reinterpret_cast<uint32_t*>(p + 256 + hub_bytes) : nullptr;
+  w.chunks = p ? reinterpret_cast<HeavyChunk*>(p + 256 + hub_bytes + summary_bytes) : nullptr;
+  w.summary_shift = 0;
   return w;
+}
+
+// Frontier summary: bit i is set when any of vertices [i << shift,
+// (i + 1) << shift) is in the frontier (shift >= 5, so a bit covers whole
+// frontier words).  Words up to (nbits + 31) / 32 are written.
+__global__ void __launch_bounds__(kBlock)
+k_front_summary(const uint32_t* front, uint64_t words, uint32_t* summary, uint32_t nbits,
+                uint32_t shift) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool b = false;
+  if (i < nbits) {
+    const uint64_t w0 = (uint64_t)i << (shift - 5);
+    const uint64_t w1e = w0 + (1ull << (shift - 5));
+    const uint64_t w1 = w1e < words ? w1e : words;
+    uint32_t acc = 0;
+    for (uint64_t w = w0; w < w1; ++w) acc |= front[w];
+    b = acc != 0;
+  }
+  const uint64_t m = __ballot(b);
+  const uint32_t lane = lane_id();
+  const uint32_t base = (blockIdx.x * kBlock + threadIdx.x - lane) / 32;
+  const uint32_t nw = (nbits + 31) / 32;
+  if (lane == 0 && base < nw) summary[base] = (uint32_t)m;
+  if (lane == 32 && base + 1 < nw) summary[base + 1] = (uint32_t)(m >> 32);
 }
 
 // Frontier bits of the hub list, packed (hub i -> bit i).
@@ -1011,6 +1041,18 @@ struct PullCtx {
   scaph_level_counters* counters;
 };
 
+// in_front, with the frontier summary (when the context has one) tested
+// before a non-hub entry's frontier word: a clear summary bit answers from a
+// small, cache-resident array instead of a random line of the bitmap.
+__device__ __forceinline__ bool probe(const PullCtx& pc, uint32_t u) {
+  if (u & kHubTag) {
+    const uint32_t h = u & ~kHubTag;
+    return (pc.s_hub[h >> 5] >> (h & 31)) & 1u;
+  }
+  if (pc.hw.summary_shift && !bit_test(pc.hw.summary, u >> pc.hw.summary_shift)) return false;
+  return bit_test(pc.front_in, u);
+}
+
 // One pull batch, one vertex per lane (act): does v have an in-neighbour in
 // the frontier?  hd is v's head record when the context has heads; hint is
 // a page at or before v's.  outdeg gets v's out-degree from the head record
@@ -1025,8 +1067,6 @@ __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t
                                            unsigned long long& acc_s, bool& more) {
   const scaph_graph_desc& g = pc.g;
   const PageTab& s = pc.s;
-  const uint32_t* s_hub = pc.s_hub;
-  const uint32_t* front_in = pc.front_in;
   const uint4* head = pc.head;
   const uint32_t head_outdeg = pc.head_outdeg, head_indexed = pc.head_indexed;
   const PullWs& hw = pc.hw;
@@ -1055,9 +1095,9 @@ __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t
       const uint32_t l0 = len < cap ? len : cap;
       k = l0;
       if (PHASE != 2) {
-        const bool h0 = l0 > 0 && in_front(hd.x, s_hub, front_in);
-        const bool h1 = l0 > 1 && in_front(hd.y, s_hub, front_in);
-        const bool h2 = l0 > 2 && in_front(hd.z, s_hub, front_in);
+        const bool h0 = l0 > 0 && probe(pc, hd.x);
+        const bool h1 = l0 > 1 && probe(pc, hd.y);
+        const bool h2 = l0 > 2 && probe(pc, hd.z);
         found = h0 | h1 | h2;
       }
       if (PHASE == 1) {
@@ -1082,7 +1122,7 @@ __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t
       for (uint32_t j = 0; j < kPullProbe; ++j) u[j] = (k + j < lim) ? nr.edges[k + j] : 0u;
       bool h = false;
 #pragma unroll
-      for (uint32_t j = 0; j < kPullProbe; ++j) h |= (k + j < lim) && in_front(u[j], s_hub, front_in);
+      for (uint32_t j = 0; j < kPullProbe; ++j) h |= (k + j < lim) && probe(pc, u[j]);
       if (h) {
         found = true;
         k += kPullProbe;
@@ -1106,7 +1146,7 @@ __device__ __forceinline__ bool pull_batch(const PullCtx& pc, bool act, uint32_t
     bool hitj = false;
     for (uint32_t k = kPullSerial; k < stop; k += SCAPH_WAVE) {
       const uint32_t kk = k + lane;
-      const bool hit = kk < stop && in_front(e[kk], s_hub, front_in);
+      const bool hit = kk < stop && probe(pc, e[kk]);
       const uint64_t mh = __ballot(hit);
       if (lane == 0) acc_s += min(stop - k, (uint32_t)SCAPH_WAVE);
       if (mh) {
@@ -2141,6 +2181,14 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
     return SCAPH_STATUS_INVALID;
   PullWs hw = carve_pull_ws(workspace, max_edges);
   if (hw.bytes > workspace_bytes) return SCAPH_STATUS_INVALID;
+  const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
+  const uint32_t shift = tune ? tune->summary_shift : 0u;
+  uint64_t summary_bits = 0;
+  if (shift) {
+    if (shift < 5 || shift > 31) return SCAPH_STATUS_INVALID;
+    summary_bits = ((uint64_t)in_graph->num_vertices + (1ull << shift) - 1) >> shift;
+    if (summary_bits > kSummaryMaxBits) return SCAPH_STATUS_INVALID;
+  }
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, hw.count, 1u);
   SCAPH_LAUNCH_CHECK();
@@ -2149,7 +2197,13 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
                        hub_list, num_hubs, front_in, hw.hub_front);
     SCAPH_LAUNCH_CHECK();
   }
-  const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
+  if (summary_bits) {
+    hipLaunchKernelGGL(k_front_summary, dim3((uint32_t)((summary_bits + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, st, front_in, words, hw.summary, (uint32_t)summary_bits,
+                       shift);
+    SCAPH_LAUNCH_CHECK();
+    hw.summary_shift = shift;
+  }
   const uint32_t waves = tune && tune->block_waves ? tune->block_waves : kPullWaves;
   if (waves != 4 && waves != 8) return SCAPH_STATUS_INVALID;
   const uint32_t indexed_heads = tune ? tune->head_indexed : 0u;

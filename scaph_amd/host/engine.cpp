@@ -66,6 +66,12 @@ Config& global_flags() {
           "highest-out-degree vertices whose frontier bits the pull keeps in LDS (<= 262144)");
     r.add("pull_sparse", FlagType::kInt32, &c.pull_sparse,
           "pull over a list of the unvisited vertices when fewer than n / this are left (0 = never)");
+    r.add("pull_summary", FlagType::kInt32, &c.pull_summary,
+          "pull: test a frontier summary of one bit per 2^this vertices before each non-hub "
+          "in-neighbour's frontier word (0 = never, else >= 5; raised until n / 2^this <= 2^22)");
+    r.add("pull_summary_div", FlagType::kInt32, &c.pull_summary_div,
+          "pull: use the summary on levels whose frontier has fewer than n / 2^pull_summary / "
+          "this vertices (0 = on every pull level)");
     r.add("pull_phases", FlagType::kInt32, &c.pull_phases,
           "2: a pull window's vertices are probed by their head records first and the undecided "
           "ones by their lists after (1: both per vertex)");

@@ -54,6 +54,10 @@ struct Config {
   int32_t pull_hubs = 131072;    // in-list sources tagged as hubs, frontier bits in LDS
   int32_t pull_sparse = 128;     // pull over a list of the unvisited vertices when fewer than
                                  // n / this are left (known after a pull level; 0 = never)
+  int32_t pull_summary = 5;      // pull: frontier summary of one bit per 2^this vertices tested
+                                 // before each non-hub probe (0 = never, else >= 5) ...
+  int32_t pull_summary_div = 4;  // ... on levels whose frontier has fewer than n / 2^shift / this
+                                 // vertices (0: on every pull level)
   bool pull_head_outdeg = true;  // head records {e0, e1, in-degree, out-degree}
   bool pull_head_live = true;    // head records only for vertices with in-edges (live rank)
   bool pull_prefetch = true;     // request the next batch's head records early

@@ -84,10 +84,18 @@ void Engine::launch_step(Dir d, Front& f, uint32_t cur, uint64_t n_f, uint64_t m
                                   front_[f.fa].get<uint32_t>(), front_[f.fa ^ 1].get<uint32_t>(),
                                   cnt, cur, m_, pws_.get(), pws_bytes_, &pull_tune_, st));
     } else {
+      // a sparse frontier: most non-hub probes end at the summary's clear bits
+      scaph_pull_tuning tune = pull_tune_;
+      if (cfg_.pull_summary > 0 && cfg_.pull_summary_div >= 0) {
+        uint32_t sh = (uint32_t)std::max(5, cfg_.pull_summary);
+        while (sh < 31 && ((n_ + (1ull << sh) - 1) >> sh) > (1ull << 22)) ++sh;
+        if (cfg_.pull_summary_div == 0 || n_f * (uint64_t)cfg_.pull_summary_div < (n_ >> sh))
+          tune.summary_shift = sh;
+      }
       H_ERR(scaph_bfs_pull(&g_in_, in_head_.get() ? &heads : nullptr, hub_list_.get<uint32_t>(),
                            num_hubs_, deg, level, visited,
                            front_[f.fa].get<uint32_t>(), front_[f.fa ^ 1].get<uint32_t>(), cnt,
-                           cur, m_, pws_.get(), pws_bytes_, &pull_tune_, st));
+                           cur, m_, pws_.get(), pws_bytes_, &tune, st));
     }
     f.fa ^= 1;
     f.front_ok = true;

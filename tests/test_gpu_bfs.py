@@ -386,6 +386,10 @@ def test_pull_heavy_in_lists():
                                    ["-pull_sparse=1", "-pull_head_live=false"],
                                    ["-pull_sparse=1", "-pull_head=false"],
                                    ["-pull_sparse=1", "-pull_head_index=false", "-pull_hubs=0"],
+                                   ["-pull_summary=6", "-pull_hubs=1000"],
+                                   ["-pull_summary=5", "-pull_summary_div=0", "-pull_hubs=1000"],
+                                   ["-pull_summary=9", "-pull_summary_div=0", "-pull_phases=1",
+                                    "-pull_hubs=0"],
                                    ["-push_small=1", "-push_scan=1"],
                                    ["-push_scan=1000000000"], ["-push_small=1", "-push_xcd=1"],
                                    ["-push_small=1", "-push_xcd=1", "-push_rows=false",
@@ -403,6 +407,40 @@ def test_launch_shape_variants(flags):
                 e.set_schedule(sched)
                 e.run(src)
                 np.testing.assert_array_equal(e.levels(), want, err_msg=f"{flags} {src} {sched}")
+
+
+@pytest.mark.parametrize("flags", [["-pull_summary=5"], ["-pull_summary=6"],
+                                   ["-pull_summary=7", "-pull_head=false"],
+                                   ["-pull_summary=10", "-pull_hubs=0"],
+                                   ["-pull_summary=6", "-pull_head_live=false", "-pull_waves=4"]])
+def test_pull_frontier_summary(flags):
+    """The pull with the frontier summary (one bit per 2^shift vertices)
+    tested before each non-hub probe, forced on every pull level, on vertex
+    counts that are not a multiple of the summary's span: levels equal the
+    oracle's.  Few hubs (graphs this small would otherwise make every source
+    a hub, and the summary guards non-hub probes only)."""
+    rng = np.random.default_rng(11)
+    graphs = [O.rmat_csr(15, 16, seed=5)]
+    for n in (100003, 70001):
+        m = 12 * n
+        src = rng.integers(0, n, m)
+        # a skewed in-side: low ids collect many in-edges, so the in-lists
+        # mix hubs and long non-hub runs
+        dst = (rng.pareto(1.2, m) * 50).astype(np.int64) % n
+        graphs.append(O.csr_from_edges(n, src, dst))
+    hubs = [] if any(f.startswith("-pull_hubs") for f in flags) else ["-pull_hubs=500"]
+    with engine(*flags, *hubs, "-pull_summary_div=0") as e:
+        for off, ngh in graphs:
+            e.load_csr(off, ngh)
+            deg = np.diff(off.astype(np.int64))
+            for src in (int(np.argmax(deg)), int(np.nonzero(deg)[0][5])):
+                want = O.bfs(off, ngh, src)
+                for sched in ("pull", "fixed"):
+                    say("summary", flags, len(off) - 1, src, sched)
+                    e.set_schedule(sched)
+                    e.run(src)
+                    np.testing.assert_array_equal(e.levels(), want,
+                                                  err_msg=f"{flags} {src} {sched}")
 
 
 def test_xcd_round_robin_dispatch():
