@@ -918,11 +918,11 @@ PullWs carve_pull_ws(void* base, unsigned long long m) {
 
 // Frontier summary: bit i is set when any of vertices [i << shift,
 // (i + 1) << shift) is in the frontier (shift >= 5, so a bit covers whole
-// frontier words).  Words up to (nbits + 31) / 32 are written.
-__global__ void __launch_bounds__(kBlock)
-k_front_summary(const uint32_t* front, uint64_t words, uint32_t* summary, uint32_t nbits,
-                uint32_t shift) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+// frontier words).  Words up to (nbits + 31) / 32 are written; blk is the
+// block's index among ceil(nbits / kBlock).
+__device__ __forceinline__ void front_summary(uint32_t blk, const uint32_t* front, uint64_t words,
+                                              uint32_t* summary, uint32_t nbits, uint32_t shift) {
+  const uint32_t i = blk * kBlock + threadIdx.x;
   bool b = false;
   if (i < nbits) {
     const uint64_t w0 = (uint64_t)i << (shift - 5);
@@ -934,23 +934,38 @@ k_front_summary(const uint32_t* front, uint64_t words, uint32_t* summary, uint32
   }
   const uint64_t m = __ballot(b);
   const uint32_t lane = lane_id();
-  const uint32_t base = (blockIdx.x * kBlock + threadIdx.x - lane) / 32;
+  const uint32_t base = (blk * kBlock + threadIdx.x - lane) / 32;
   const uint32_t nw = (nbits + 31) / 32;
   if (lane == 0 && base < nw) summary[base] = (uint32_t)m;
   if (lane == 32 && base + 1 < nw) summary[base + 1] = (uint32_t)(m >> 32);
 }
 
-// Frontier bits of the hub list, packed (hub i -> bit i).
-__global__ void __launch_bounds__(kBlock)
-k_hub_gather(const uint32_t* hub_list, uint32_t num_hubs, const uint32_t* front,
-             uint32_t* hub_front) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+// Frontier bits of the hub list, packed (hub i -> bit i); blk is the
+// block's index among ceil(num_hubs / kBlock).
+__device__ __forceinline__ void hub_gather(uint32_t blk, const uint32_t* hub_list,
+                                           uint32_t num_hubs, const uint32_t* front,
+                                           uint32_t* hub_front) {
+  const uint32_t i = blk * kBlock + threadIdx.x;
   const bool b = i < num_hubs && bit_test(front, hub_list[i]);
   const uint64_t m = __ballot(b);
   const uint32_t lane = lane_id();
-  const uint32_t base = (blockIdx.x * kBlock + threadIdx.x - lane) / 32;
+  const uint32_t base = (blk * kBlock + threadIdx.x - lane) / 32;
   if (lane == 0 && base < kMaxHubs / 32) hub_front[base] = (uint32_t)m;
   if (lane == 32 && base + 1 < kMaxHubs / 32) hub_front[base + 1] = (uint32_t)(m >> 32);
+}
+
+// What a pull level needs before its expand, in one launch: the heavy-chunk
+// counter zeroed, the hubs' frontier bits gathered (blocks [0, hub_blocks))
+// and the frontier summary built (the nbits > 0 blocks after them).
+__global__ void __launch_bounds__(kBlock)
+k_pull_prologue(unsigned long long* count, const uint32_t* hub_list, uint32_t num_hubs,
+                uint32_t hub_blocks, const uint32_t* front, uint64_t words, uint32_t* hub_front,
+                uint32_t* summary, uint32_t nbits, uint32_t shift) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0ull;
+  if (blockIdx.x < hub_blocks)
+    hub_gather(blockIdx.x, hub_list, num_hubs, front, hub_front);
+  else if (nbits)
+    front_summary(blockIdx.x - hub_blocks, front, words, summary, nbits, shift);
 }
 
 // The hub bits sit in dynamic LDS behind the page table; a launch passes
@@ -1211,23 +1226,44 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
   unsigned long long acc_n = 0, acc_m = 0, acc_s = 0, acc_x = 0;
   uint16_t* q = s_q[wid];
   uint32_t* fw = s_fw[wid];
-  // visited word of this lane in the wave's window of iteration bi
+  // visited word of this lane in the wave's window of iteration bi, and
+  // (live-rank heads) its no-in-edge word and the window's first rank
   auto visited_word = [&](uint64_t bi) -> uint32_t {
     const uint64_t ww = bi * per_block + (uint64_t)wid * kPullWinWords + lane;
     return (lane < kPullWinWords && ww < words) ? visited[ww] : ~0u;
   };
-  uint32_t vw_next = PREFETCH && blockIdx.x < iters ? visited_word(blockIdx.x) : 0u;
+  auto no_in_word = [&](uint64_t bi) -> uint32_t {
+    const uint64_t ww = bi * per_block + (uint64_t)wid * kPullWinWords + lane;
+    return (live_base && lane < kPullWinWords && ww < words) ? no_in[ww] : ~0u;
+  };
+  auto window_base = [&](uint64_t bi) -> uint32_t {
+    const uint64_t ww = bi * per_block + (uint64_t)wid * kPullWinWords;
+    return (live_base && ww < words) ? live_base[ww / kPullWinWords] : 0u;
+  };
+  const bool first = blockIdx.x < iters;
+  uint32_t vw_next = PREFETCH && first ? visited_word(blockIdx.x) : 0u;
+  uint32_t nw_next = PREFETCH && first ? no_in_word(blockIdx.x) : 0u;
+  uint32_t wb_next = PREFETCH && first ? window_base(blockIdx.x) : 0u;
   for (uint64_t bi = blockIdx.x; bi < iters; bi += gridDim.x) {
     const uint64_t w0 = bi * per_block + (uint64_t)wid * kPullWinWords;
     const uint64_t w = w0 + lane;
     const bool own = lane < kPullWinWords && w < words;
-    uint32_t vw;
+    uint32_t vw, nw, wb;
     if (PREFETCH) {
-      // this window's word was requested one window earlier; request the next
+      // this window's words were requested one window earlier; request the
+      // next window's
       vw = vw_next;
-      if (bi + gridDim.x < iters) vw_next = visited_word(bi + gridDim.x);
+      nw = nw_next;
+      wb = wb_next;
+      if (bi + gridDim.x < iters) {
+        vw_next = visited_word(bi + gridDim.x);
+        nw_next = no_in_word(bi + gridDim.x);
+        wb_next = window_base(bi + gridDim.x);
+      }
     } else {
       vw = own ? visited[w] : ~0u;
+      nw = no_in_word(bi);
+      wb = window_base(bi);
     }
     uint32_t unv = ~vw;
     const uint32_t c = __popc(unv);
@@ -1242,14 +1278,14 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
     if (lane < kPullWinWords) fw[lane] = 0u;
     uint32_t wbase = 0;
     if (live_base && tot) {
-      const uint32_t lw = own ? ~no_in[w] : 0u;
+      const uint32_t lw = own ? ~nw : 0u;
       const uint32_t lc = __popc(lw);
       const uint32_t li = wave_incl_scan(lc);
       if (lane < kPullWinWords) {
         s_lw[wid][lane] = lw;
         s_lb[wid][lane] = li - lc;
       }
-      wbase = live_base[w0 / kPullWinWords];
+      wbase = wb;
     }
     wave_lds_sync();
     // index of the head record of window vertex loc
@@ -1495,9 +1531,12 @@ k_pull_heavy(const HeavyChunk* chunks, const unsigned long long* count, const ui
   __shared__ HeavyChunk s_c;
   __shared__ int s_skip;
   uint32_t* const s_hub = hub_lds(dyn_lds, 0);
+  const unsigned long long total = *count;
+  // most levels defer nothing: leave before staging the hub bits (a
+  // block-uniform exit)
+  if (blockIdx.x >= total) return;
   stage_hubs(s_hub, hub_front, num_hubs);
   __syncthreads();
-  const unsigned long long total = *count;
   unsigned long long scanned = 0;
   for (unsigned long long e = blockIdx.x; e < total; e += gridDim.x) {
     if (threadIdx.x == 0) {
@@ -2144,14 +2183,14 @@ int scaph_bfs_pull_sparse(const scaph_graph_desc* in_graph, const scaph_pull_hea
   const uint32_t indexed_heads = tune ? tune->head_indexed : 0u;
   const uint32_t outdeg_heads = tune ? (tune->head_outdeg | indexed_heads) : 0u;
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, hw.count, 1u);
-  SCAPH_LAUNCH_CHECK();
-  if (num_hubs) {
-    hipLaunchKernelGGL(k_hub_gather, dim3((num_hubs + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                       hub_list, num_hubs, front_in, hw.hub_front);
+  const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
+  {
+    const uint32_t hub_blocks = (num_hubs + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_pull_prologue, dim3(hub_blocks ? hub_blocks : 1u), dim3(kBlock), 0, st,
+                       hw.count, hub_list, num_hubs, hub_blocks, front_in, words, hw.hub_front,
+                       nullptr, 0u, 5u);
     SCAPH_LAUNCH_CHECK();
   }
-  const uint64_t words = (((uint64_t)in_graph->num_vertices + 63) / 64) * 2;
   SCAPH_TRY((hipError_t)scaph_fill_u32(front_out, 0u, words, stream));
   const size_t lds = page_lds_bytes(in_graph->num_pages) + hub_lds_bytes(num_hubs);
   const unsigned threads = 8 * SCAPH_WAVE;
@@ -2190,19 +2229,14 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
     if (summary_bits > kSummaryMaxBits) return SCAPH_STATUS_INVALID;
   }
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, hw.count, 1u);
-  SCAPH_LAUNCH_CHECK();
-  if (num_hubs) {
-    hipLaunchKernelGGL(k_hub_gather, dim3((num_hubs + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                       hub_list, num_hubs, front_in, hw.hub_front);
+  {
+    const uint32_t hub_blocks = (num_hubs + kBlock - 1) / kBlock;
+    const uint32_t blocks = hub_blocks + (uint32_t)((summary_bits + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_pull_prologue, dim3(blocks ? blocks : 1u), dim3(kBlock), 0, st, hw.count,
+                       hub_list, num_hubs, hub_blocks, front_in, words, hw.hub_front, hw.summary,
+                       (uint32_t)summary_bits, shift ? shift : 5u);
     SCAPH_LAUNCH_CHECK();
-  }
-  if (summary_bits) {
-    hipLaunchKernelGGL(k_front_summary, dim3((uint32_t)((summary_bits + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, st, front_in, words, hw.summary, (uint32_t)summary_bits,
-                       shift);
-    SCAPH_LAUNCH_CHECK();
-    hw.summary_shift = shift;
+    if (summary_bits) hw.summary_shift = shift;
   }
   const uint32_t waves = tune && tune->block_waves ? tune->block_waves : kPullWaves;
   if (waves != 4 && waves != 8) return SCAPH_STATUS_INVALID;
