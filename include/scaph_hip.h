@@ -248,7 +248,9 @@ typedef struct scaph_pull_tuning {
   uint32_t block_waves; /* waves per pull block: 4 or 8 (0: 8); the grid is what stays resident */
   uint32_t waves_per_simd; /* 8: registers capped so 8 waves share a SIMD (0: compiler's choice) */
   uint32_t two_phase;   /* with head records: a window's vertices probed by their heads first,
-                           the undecided ones by their lists after (batches of like chains) */
+                           the undecided ones by their lists after (batches of like chains);
+                           2: the head probes pipelined, a batch's words requested while the
+                           batch before it is answered (0, 1 or 2) */
   uint32_t summary_shift; /* 0, or 5..31: scaph_bfs_pull first builds a frontier summary (one
                              bit per 2^shift vertices, set when any of them is in front_in;
                              ceil(n / 2^shift) <= 2^22) and tests it before each non-hub

@@ -1068,6 +1068,68 @@ __device__ __forceinline__ bool probe(const PullCtx& pc, uint32_t u) {
   return bit_test(pc.front_in, u);
 }
 
+// probe() in two steps, so that a batch's loads can be requested one batch
+// before its answers are needed: probe_word requests the word that decides
+// entry u (none for a hub: its bit is in LDS), probe_answer reads it.
+// With a summary the word is the summary's; a set bit then still needs the
+// frontier word, read at once.
+__device__ __forceinline__ uint32_t probe_word(const PullCtx& pc, uint32_t u, bool need) {
+  if (!need || (u & kHubTag)) return 0u;
+  if (pc.hw.summary_shift) return pc.hw.summary[(u >> pc.hw.summary_shift) >> 5];
+  return pc.front_in[u >> 5];
+}
+
+__device__ __forceinline__ bool probe_answer(const PullCtx& pc, uint32_t u, bool need,
+                                             uint32_t w) {
+  if (!need) return false;
+  if (u & kHubTag) {
+    const uint32_t h = u & ~kHubTag;
+    return (pc.s_hub[h >> 5] >> (h & 31)) & 1u;
+  }
+  if (pc.hw.summary_shift) {
+    if (!((w >> ((u >> pc.hw.summary_shift) & 31)) & 1u)) return false;
+    return bit_test(pc.front_in, u);
+  }
+  return (w >> (u & 31)) & 1u;
+}
+
+// The words of one head record's entries (head_probe_words) and the batch's
+// answer from them (head_probe_answer): PHASE 1 of pull_batch split in two.
+struct HeadWords {
+  uint32_t w0, w1, w2;
+};
+
+__device__ __forceinline__ uint32_t head_len(const PullCtx& pc, const uint4& hd) {
+  return pc.head_indexed ? (hd.z & 0xFFu) : pc.head_outdeg ? hd.z : hd.w;
+}
+
+__device__ __forceinline__ HeadWords head_probe_words(const PullCtx& pc, bool act,
+                                                      const uint4& hd) {
+  const uint32_t cap = pc.head_outdeg ? 2u : 3u;
+  const uint32_t len = head_len(pc, hd);
+  const uint32_t l0 = act ? (len < cap ? len : cap) : 0u;
+  HeadWords w;
+  w.w0 = probe_word(pc, hd.x, l0 > 0);
+  w.w1 = probe_word(pc, hd.y, l0 > 1);
+  w.w2 = probe_word(pc, hd.z, l0 > 2);
+  return w;
+}
+
+__device__ __forceinline__ bool head_probe_answer(const PullCtx& pc, bool act, const uint4& hd,
+                                                  const HeadWords& w, uint32_t& outdeg,
+                                                  unsigned long long& acc_s, bool& more) {
+  const uint32_t cap = pc.head_outdeg ? 2u : 3u;
+  const uint32_t len = head_len(pc, hd);
+  const uint32_t l0 = act ? (len < cap ? len : cap) : 0u;
+  outdeg = pc.head_outdeg ? hd.w : 0xFFFFFFFFu;
+  const bool found = probe_answer(pc, hd.x, l0 > 0, w.w0) |
+                     probe_answer(pc, hd.y, l0 > 1, w.w1) |
+                     probe_answer(pc, hd.z, l0 > 2, w.w2);
+  more = act && !found && len > cap;
+  acc_s += l0 < kPullSerial ? l0 : kPullSerial;
+  return found;
+}
+
 // One pull batch, one vertex per lane (act): does v have an in-neighbour in
 // the frontier?  hd is v's head record when the context has heads; hint is
 // a page at or before v's.  outdeg gets v's out-degree from the head record
@@ -1297,10 +1359,44 @@ k_pull(scaph_graph_desc g, const uint4* head, const uint32_t* no_in, const uint3
     const uint32_t hint = tot ? find_page(s, g, (uint32_t)(w0 * 32)) : 0u;
     // the head records of the next batch are requested before this batch
     // is probed (one extra load in flight per lane)
-    uint4 hd_next = make_uint4(0u, 0u, 0u, 0u);
-    if (PREFETCH && head && lane < tot) hd_next = head[hidx(q[lane])];
     uint32_t nq2 = 0;  // two phases: vertices moved to the list phase (wave-uniform)
-    for (uint32_t i0 = 0; i0 < tot; i0 += SCAPH_WAVE) {
+    if (TWO_PHASE == 2) {
+      // head phase, pipelined: while batch b is answered, the words that
+      // answer batch b + 1 and the head records of batch b + 2 are in flight
+      auto head_of = [&](uint32_t i) -> uint4 {
+        return i < tot ? head[hidx(q[i])] : make_uint4(0u, 0u, 0u, 0u);
+      };
+      uint4 hd_c = head_of(lane);
+      uint4 hd_n = head_of(SCAPH_WAVE + lane);
+      HeadWords pw_c = head_probe_words(pc, lane < tot, hd_c);
+      for (uint32_t i0 = 0; i0 < tot; i0 += SCAPH_WAVE) {
+        const uint32_t i = i0 + lane;
+        const bool act = i < tot;
+        const uint32_t loc = act ? q[i] : 0u;
+        const uint32_t v = (uint32_t)(w0 * 32) + loc;
+        const uint4 hd_n2 = head_of(i + 2 * SCAPH_WAVE);
+        const HeadWords pw_n = head_probe_words(pc, i + SCAPH_WAVE < tot, hd_n);
+        if (act) ++acc_x;
+        uint32_t outdeg;
+        bool more;
+        const bool found = head_probe_answer(pc, act, hd_c, pw_c, outdeg, acc_s, more);
+        if (found) {
+          level[v] = next_level;
+          acc_n += 1;
+          acc_m += outdeg != 0xFFFFFFFFu ? outdeg : degree[v];
+          atomicOr(&fw[loc >> 5], 1u << (loc & 31));
+        }
+        const uint64_t mm = __ballot(more);
+        if (more) q[nq2 + lanes_below(mm)] = (uint16_t)loc;
+        nq2 += (uint32_t)__popcll(mm);
+        hd_c = hd_n;
+        hd_n = hd_n2;
+        pw_c = pw_n;
+      }
+    }
+    uint4 hd_next = make_uint4(0u, 0u, 0u, 0u);
+    if (TWO_PHASE != 2 && PREFETCH && head && lane < tot) hd_next = head[hidx(q[lane])];
+    for (uint32_t i0 = 0; TWO_PHASE != 2 && i0 < tot; i0 += SCAPH_WAVE) {
       const uint32_t i = i0 + lane;
       const bool act = i < tot;
       const uint32_t loc = act ? q[i] : 0u;
@@ -2251,10 +2347,11 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
   const uint32_t simd = tune ? tune->waves_per_simd : 0u;
   if (simd != 0 && simd != 8) return SCAPH_STATUS_INVALID;
   // two phases need head records
-  const bool two = tune && tune->two_phase && head;
+  const uint32_t two = tune && head ? tune->two_phase : 0u;
+  if (two > 2) return SCAPH_STATUS_INVALID;
   const int variant =
-      (prefetch ? 1 : 0) | (waves == 8 ? 2 : 0) | (simd == 8 ? 4 : 0) | (two ? 8 : 0);
-  static const void* const kerns[16] = {
+      (prefetch ? 1 : 0) | (waves == 8 ? 2 : 0) | (simd == 8 ? 4 : 0) | (int)(two * 8);
+  static const void* const kerns[24] = {
       (const void*)k_pull<0, 4, 1, 0>, (const void*)k_pull<1, 4, 1, 0>,
       (const void*)k_pull<0, 8, 1, 0>, (const void*)k_pull<1, 8, 1, 0>,
       (const void*)k_pull<0, 4, 8, 0>, (const void*)k_pull<1, 4, 8, 0>,
@@ -2262,7 +2359,11 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
       (const void*)k_pull<0, 4, 1, 1>, (const void*)k_pull<1, 4, 1, 1>,
       (const void*)k_pull<0, 8, 1, 1>, (const void*)k_pull<1, 8, 1, 1>,
       (const void*)k_pull<0, 4, 8, 1>, (const void*)k_pull<1, 4, 8, 1>,
-      (const void*)k_pull<0, 8, 8, 1>, (const void*)k_pull<1, 8, 8, 1>};
+      (const void*)k_pull<0, 8, 8, 1>, (const void*)k_pull<1, 8, 8, 1>,
+      (const void*)k_pull<0, 4, 1, 2>, (const void*)k_pull<1, 4, 1, 2>,
+      (const void*)k_pull<0, 8, 1, 2>, (const void*)k_pull<1, 8, 1, 2>,
+      (const void*)k_pull<0, 4, 8, 2>, (const void*)k_pull<1, 4, 8, 2>,
+      (const void*)k_pull<0, 8, 8, 2>, (const void*)k_pull<1, 8, 8, 2>};
   const void* kern = kerns[variant];
   // persistent-style: as many blocks as stay resident on every CU, each
   // walking its share of the window groups (a launch larger than what is
@@ -2292,6 +2393,14 @@ int scaph_bfs_pull(const scaph_graph_desc* in_graph, const scaph_pull_heads* hea
     case 12: SCAPH_PULL_LAUNCH(0, 4, 8, 1); break;
     case 13: SCAPH_PULL_LAUNCH(1, 4, 8, 1); break;
     case 14: SCAPH_PULL_LAUNCH(0, 8, 8, 1); break;
+    case 16: SCAPH_PULL_LAUNCH(0, 4, 1, 2); break;
+    case 17: SCAPH_PULL_LAUNCH(1, 4, 1, 2); break;
+    case 18: SCAPH_PULL_LAUNCH(0, 8, 1, 2); break;
+    case 19: SCAPH_PULL_LAUNCH(1, 8, 1, 2); break;
+    case 20: SCAPH_PULL_LAUNCH(0, 4, 8, 2); break;
+    case 21: SCAPH_PULL_LAUNCH(1, 4, 8, 2); break;
+    case 22: SCAPH_PULL_LAUNCH(0, 8, 8, 2); break;
+    case 23: SCAPH_PULL_LAUNCH(1, 8, 8, 2); break;
     default: SCAPH_PULL_LAUNCH(1, 8, 8, 1); break;
   }
 #undef SCAPH_PULL_LAUNCH

@@ -74,7 +74,8 @@ Config& global_flags() {
           "this vertices (0 = on every pull level)");
     r.add("pull_phases", FlagType::kInt32, &c.pull_phases,
           "2: a pull window's vertices are probed by their head records first and the undecided "
-          "ones by their lists after (1: both per vertex)");
+          "ones by their lists after (1: both per vertex; 3: as 2, the head probes pipelined "
+          "one batch ahead)");
     r.add("pull_simd", FlagType::kInt32, &c.pull_simd,
           "8: pull kernel registers capped so 8 waves share a SIMD (0 = compiler's choice)");
     r.add("learned_fused", FlagType::kBool, &c.learned_fused,
@@ -195,7 +196,7 @@ Engine::Engine(const Config& cfg)
   pull_tune_.no_prefetch = cfg.pull_prefetch ? 0u : 1u;
   pull_tune_.block_waves = (uint32_t)std::max(0, cfg.pull_waves);
   pull_tune_.waves_per_simd = (uint32_t)std::max(0, cfg.pull_simd);
-  pull_tune_.two_phase = cfg.pull_phases == 2 ? 1u : 0u;
+  pull_tune_.two_phase = cfg.pull_phases == 3 ? 2u : cfg.pull_phases == 2 ? 1u : 0u;
   push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
   push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);

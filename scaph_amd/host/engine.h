@@ -50,7 +50,8 @@ struct Config {
   int32_t pull_grid = 0;         // cap on pull blocks (0: as many as stay resident)
   int32_t pull_waves = 0;        // waves per pull block, 4 or 8 (0: the kernel default, 8)
   int32_t pull_simd = 0;         // 8: pull registers capped for 8 waves per SIMD (0: compiler)
-  int32_t pull_phases = 2;       // 2: a window's heads are probed before its lists (1: together)
+  int32_t pull_phases = 3;       // 2: a window's heads are probed before its lists (1: together;
+                                 // 3: as 2, the head probes pipelined one batch ahead)
   int32_t pull_hubs = 131072;    // in-list sources tagged as hubs, frontier bits in LDS
   int32_t pull_sparse = 128;     // pull over a list of the unvisited vertices when fewer than
                                  // n / this are left (known after a pull level; 0 = never)

@@ -383,6 +383,11 @@ def test_pull_heavy_in_lists():
                                    ["-pull_phases=2", "-pull_head_index=false", "-pull_hubs=0"],
                                    ["-pull_phases=2", "-pull_head=false"],
                                    ["-pull_phases=2", "-pull_head_outdeg=false"],
+                                   ["-pull_phases=3"], ["-pull_phases=3", "-pull_prefetch=false"],
+                                   ["-pull_phases=3", "-pull_head_live=false", "-pull_waves=4"],
+                                   ["-pull_phases=3", "-pull_head_index=false", "-pull_hubs=0"],
+                                   ["-pull_phases=3", "-pull_head_outdeg=false"],
+                                   ["-pull_phases=3", "-pull_simd=8", "-pull_hubs=1000"],
                                    ["-pull_sparse=1", "-pull_head_live=false"],
                                    ["-pull_sparse=1", "-pull_head=false"],
                                    ["-pull_sparse=1", "-pull_head_index=false", "-pull_hubs=0"],
@@ -412,7 +417,9 @@ def test_launch_shape_variants(flags):
 @pytest.mark.parametrize("flags", [["-pull_summary=5"], ["-pull_summary=6"],
                                    ["-pull_summary=7", "-pull_head=false"],
                                    ["-pull_summary=10", "-pull_hubs=0"],
-                                   ["-pull_summary=6", "-pull_head_live=false", "-pull_waves=4"]])
+                                   ["-pull_summary=6", "-pull_head_live=false", "-pull_waves=4"],
+                                   ["-pull_summary=5", "-pull_phases=3"],
+                                   ["-pull_summary=8", "-pull_phases=3", "-pull_head_outdeg=false"]])
 def test_pull_frontier_summary(flags):
     """The pull with the frontier summary (one bit per 2^shift vertices)
     tested before each non-hub probe, forced on every pull level, on vertex
