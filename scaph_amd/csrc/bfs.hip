@@ -1693,46 +1693,50 @@ k_queue_to_bitmap(const uint32_t* q, uint32_t nq, uint32_t* bitmap, uint32_t n) 
   }
 }
 
-// Frontier bitmap -> vertex list.  Each block owns a contiguous range of
-// words: it counts the set bits of its range, takes its slice of the list
-// with a single atomic, then writes the vertices in order.
+// Frontier bitmap -> vertex list.  Each block owns kBlock * kQueueWords
+// consecutive words, column j being words b0 + j * kBlock + threadIdx.x:
+// all of a thread's words are requested at once and kept in registers.  The
+// block counts their set bits, takes its slice of the list with a single
+// atomic, then writes the vertices in order, column by column.
+constexpr uint32_t kQueueWords = 8;
+
 __global__ void __launch_bounds__(kBlock)
 k_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* q,
                   scaph_level_counters* counters) {
   __shared__ uint32_t wtot[kBlock / SCAPH_WAVE];
   __shared__ unsigned long long s_base;
   const uint64_t words = ((uint64_t)n + 31) / 32;
-  uint64_t per = (words + gridDim.x - 1) / gridDim.x;
-  per = (per + kBlock - 1) / kBlock * kBlock;
-  const uint64_t b0 = (uint64_t)blockIdx.x * per;
-  const uint64_t b1 = min(b0 + per, words);
-  auto word_bits = [&](uint64_t w) -> uint32_t {
-    uint32_t bits = bitmap[w];
-    const uint64_t base = w * 32;
-    if (base + 32 > n) {
-      const uint32_t valid = (uint32_t)(n - base);
-      bits &= valid >= 32 ? ~0u : ((1u << valid) - 1u);
-    }
-    return bits;
-  };
+  const uint64_t b0 = (uint64_t)blockIdx.x * kBlock * kQueueWords;
+  uint32_t bits[kQueueWords];
   uint32_t cnt = 0;
-  for (uint64_t w = b0 + threadIdx.x; w < b1; w += kBlock) cnt += __popc(word_bits(w));
+#pragma unroll
+  for (uint32_t j = 0; j < kQueueWords; ++j) {
+    const uint64_t w = b0 + (uint64_t)j * kBlock + threadIdx.x;
+    uint32_t x = w < words ? bitmap[w] : 0u;
+    if (w < words && w * 32 + 32 > n) {
+      const uint32_t valid = (uint32_t)(n - w * 32);
+      x &= valid >= 32 ? ~0u : ((1u << valid) - 1u);
+    }
+    bits[j] = x;
+    cnt += __popc(x);
+  }
   uint32_t total;
   block256_incl_scan(cnt, wtot, &total);
   if (threadIdx.x == 0) s_base = total ? atomicAdd(&counters->q_len, (unsigned long long)total) : 0;
   __syncthreads();
   if (total == 0) return;
   unsigned long long run = s_base;
-  for (uint64_t c0 = b0; c0 < b1; c0 += kBlock) {
-    const uint64_t w = c0 + threadIdx.x;
-    uint32_t bits = w < b1 ? word_bits(w) : 0u;
-    const uint32_t c = __popc(bits);
+#pragma unroll
+  for (uint32_t j = 0; j < kQueueWords; ++j) {
+    const uint64_t w = b0 + (uint64_t)j * kBlock + threadIdx.x;
+    uint32_t x = bits[j];
+    const uint32_t c = __popc(x);
     uint32_t ctot;
     const uint32_t incl = block256_incl_scan(c, wtot, &ctot);
     unsigned long long o = run + incl - c;
-    while (bits) {
-      const uint32_t b = __ffs(bits) - 1;
-      bits &= bits - 1;
+    while (x) {
+      const uint32_t b = __ffs(x) - 1;
+      x &= x - 1;
       q[o++] = (uint32_t)(w * 32 + b);
     }
     run += ctot;
@@ -2444,8 +2448,11 @@ int scaph_frontier_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t*
                                    scaph_level_counters* counters, void* stream) {
   if (!bitmap || !queue || !counters) return SCAPH_STATUS_INVALID;
   const size_t words = ((size_t)n + 31) / 32;
-  hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid_for(words, kBlock, 1024)), dim3(kBlock), 0,
-                     as_stream(stream), bitmap, n, queue, counters);
+  const size_t per_block = (size_t)kBlock * kQueueWords;
+  const uint32_t grid = (uint32_t)((words + per_block - 1) / per_block);
+  if (grid == 0) return 0;
+  hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid), dim3(kBlock), 0, as_stream(stream), bitmap, n,
+                     queue, counters);
   SCAPH_LAUNCH_CHECK();
   return 0;
 }

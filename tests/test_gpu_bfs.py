@@ -282,6 +282,51 @@ def test_page_counts_kernel_vs_oracle():
         hip.scaph_dev_free(p)
 
 
+def test_frontier_bitmap_to_queue_ragged():
+    """scaph_frontier_bitmap_to_queue on vertex counts around its 2048-word
+    block span and the word edge (bits past n in the last word are set and
+    must be ignored): the list holds exactly the frontier vertices, each
+    once, in increasing order within a block."""
+    hip = _lib.hip()
+    rng = np.random.default_rng(21)
+    for n, dens in ((1, 1.0), (31, 0.5), (33, 0.5), (65536, 0.001), (65537 + 31, 0.3),
+                    (2048 * 32 * 3 + 5, 0.02), (1000003, 0.6), (4194321, 0.0)):
+        say("bitmap_to_queue", n, dens)
+        words = ((n + 63) // 64) * 2
+        bits = np.zeros(words * 32, np.uint8)
+        bits[:n] = rng.random(n) < dens
+        bits[n:] = 1  # garbage past n
+        bitmap = np.packbits(bits, bitorder="little").view(np.uint32)
+        want = np.nonzero(bits[:n])[0].astype(np.uint32)
+        d_bm, d_q, d_c = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(hip.scaph_dev_alloc(ctypes.byref(d_bm), bitmap.nbytes))
+        _lib.check(hip.scaph_dev_alloc(ctypes.byref(d_q), max(4 * n, 8)))
+        cnt = _lib.LevelCounters()
+        _lib.check(hip.scaph_dev_alloc(ctypes.byref(d_c), ctypes.sizeof(cnt)))
+        try:
+            _lib.check(hip.scaph_copy_h2d_async(d_bm, bitmap.ctypes.data, bitmap.nbytes, None))
+            _lib.check(hip.scaph_copy_h2d_async(d_c, ctypes.addressof(cnt), ctypes.sizeof(cnt),
+                                                None))
+            _lib.check(hip.scaph_frontier_bitmap_to_queue(d_bm, n, d_q, d_c, None))
+            _lib.check(hip.scaph_device_sync())
+            _lib.check(hip.scaph_copy_d2h_async(ctypes.addressof(cnt), d_c, ctypes.sizeof(cnt),
+                                                None))
+            got = np.zeros(len(want), np.uint32)
+            if len(want):
+                _lib.check(hip.scaph_copy_d2h_async(got.ctypes.data, d_q, got.nbytes, None))
+            _lib.check(hip.scaph_device_sync())
+        finally:
+            for p_ in (d_bm, d_q, d_c):
+                hip.scaph_dev_free(p_)
+        assert cnt.q_len == len(want)
+        np.testing.assert_array_equal(np.sort(got), want)
+        # each block's slice is in increasing order
+        blk = got // (2048 * 32)
+        for b in np.unique(blk):
+            seg = got[blk == b]
+            assert np.all(np.diff(seg.astype(np.int64)) > 0)
+
+
 @pytest.mark.parametrize("page_words,max_mib,threshold,multi,streams", [
     (65536, 1, 3, 1, 5), (65536, 1, 16, 1, 2), (16384, 1, 0, 1, 5), (16384, 1, 8, 2, 3),
     (8192, 1, 12, 1, 1)])
