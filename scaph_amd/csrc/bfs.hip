@@ -1035,7 +1035,7 @@ __device__ __forceinline__ const uint32_t* page_edges_hint(const PageTab& t,
 
 constexpr uint32_t kPullWinWords = 32;
 constexpr uint32_t kPullWin = kPullWinWords * 32;
-constexpr uint32_t kPullProbe = 2;
+constexpr uint32_t kPullProbe = 4;
 constexpr uint32_t kPullWaves = 8;  // waves per pull block by default
 
 __device__ __forceinline__ void wave_lds_sync() {
