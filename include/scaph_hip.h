@@ -286,6 +286,9 @@ int scaph_frontier_queue_to_bitmap(const uint32_t* queue, uint32_t n_queue, uint
  * frontier vertex. */
 int scaph_frontier_from_visited(const uint32_t* visited, const uint32_t* snapshot,
                                 uint32_t* bitmap, uint32_t n, void* stream);
+/* bitmap (16-byte aligned) -> the list of its set bits below n, appended
+ * at counters->q_len (which it advances); in vertex order within each run
+ * of 262 144 vertices. */
 int scaph_frontier_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* queue,
                                    scaph_level_counters* counters, void* stream);
 

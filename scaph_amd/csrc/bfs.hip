@@ -1693,12 +1693,13 @@ k_queue_to_bitmap(const uint32_t* q, uint32_t nq, uint32_t* bitmap, uint32_t n) 
   }
 }
 
-// Frontier bitmap -> vertex list.  Each block owns kBlock * kQueueWords
-// consecutive words, column j being words b0 + j * kBlock + threadIdx.x:
-// all of a thread's words are requested at once and kept in registers.  The
-// block counts their set bits, takes its slice of the list with a single
-// atomic, then writes the vertices in order, column by column.
-constexpr uint32_t kQueueWords = 8;
+// Frontier bitmap -> vertex list.  Each thread owns kQueueWords consecutive
+// words (its block kBlock * kQueueWords), requested at once as 16-byte loads
+// and kept in registers.  One block scan gives every thread its place; the
+// block takes its slice of the list with a single atomic (few blocks, so
+// the shared counter is not a queue of atomics) and each thread writes its
+// vertices in order.
+constexpr uint32_t kQueueWords = 32;
 
 __global__ void __launch_bounds__(kBlock)
 k_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* q,
@@ -1706,40 +1707,46 @@ k_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* q,
   __shared__ uint32_t wtot[kBlock / SCAPH_WAVE];
   __shared__ unsigned long long s_base;
   const uint64_t words = ((uint64_t)n + 31) / 32;
-  const uint64_t b0 = (uint64_t)blockIdx.x * kBlock * kQueueWords;
+  const uint64_t w0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kQueueWords;
   uint32_t bits[kQueueWords];
+  if (w0 + kQueueWords <= words) {
+    const uint4* p4 = reinterpret_cast<const uint4*>(bitmap + w0);
+#pragma unroll
+    for (uint32_t j = 0; j < kQueueWords / 4; ++j) {
+      const uint4 t = p4[j];
+      bits[4 * j] = t.x;
+      bits[4 * j + 1] = t.y;
+      bits[4 * j + 2] = t.z;
+      bits[4 * j + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < kQueueWords; ++j) bits[j] = w0 + j < words ? bitmap[w0 + j] : 0u;
+  }
   uint32_t cnt = 0;
 #pragma unroll
   for (uint32_t j = 0; j < kQueueWords; ++j) {
-    const uint64_t w = b0 + (uint64_t)j * kBlock + threadIdx.x;
-    uint32_t x = w < words ? bitmap[w] : 0u;
+    const uint64_t w = w0 + j;
     if (w < words && w * 32 + 32 > n) {
       const uint32_t valid = (uint32_t)(n - w * 32);
-      x &= valid >= 32 ? ~0u : ((1u << valid) - 1u);
+      bits[j] &= valid >= 32 ? ~0u : ((1u << valid) - 1u);
     }
-    bits[j] = x;
-    cnt += __popc(x);
+    cnt += __popc(bits[j]);
   }
   uint32_t total;
-  block256_incl_scan(cnt, wtot, &total);
+  const uint32_t incl = block256_incl_scan(cnt, wtot, &total);
   if (threadIdx.x == 0) s_base = total ? atomicAdd(&counters->q_len, (unsigned long long)total) : 0;
   __syncthreads();
   if (total == 0) return;
-  unsigned long long run = s_base;
+  unsigned long long o = s_base + incl - cnt;
 #pragma unroll
   for (uint32_t j = 0; j < kQueueWords; ++j) {
-    const uint64_t w = b0 + (uint64_t)j * kBlock + threadIdx.x;
     uint32_t x = bits[j];
-    const uint32_t c = __popc(x);
-    uint32_t ctot;
-    const uint32_t incl = block256_incl_scan(c, wtot, &ctot);
-    unsigned long long o = run + incl - c;
     while (x) {
       const uint32_t b = __ffs(x) - 1;
       x &= x - 1;
-      q[o++] = (uint32_t)(w * 32 + b);
+      q[o++] = (uint32_t)((w0 + j) * 32 + b);
     }
-    run += ctot;
   }
 }
 
@@ -2446,7 +2453,8 @@ int scaph_frontier_from_visited(const uint32_t* visited, const uint32_t* snapsho
 
 int scaph_frontier_bitmap_to_queue(const uint32_t* bitmap, uint32_t n, uint32_t* queue,
                                    scaph_level_counters* counters, void* stream) {
-  if (!bitmap || !queue || !counters) return SCAPH_STATUS_INVALID;
+  if (!bitmap || !queue || !counters || (reinterpret_cast<uintptr_t>(bitmap) & 15u))
+    return SCAPH_STATUS_INVALID;
   const size_t words = ((size_t)n + 31) / 32;
   const size_t per_block = (size_t)kBlock * kQueueWords;
   const uint32_t grid = (uint32_t)((words + per_block - 1) / per_block);

@@ -283,14 +283,16 @@ def test_page_counts_kernel_vs_oracle():
 
 
 def test_frontier_bitmap_to_queue_ragged():
-    """scaph_frontier_bitmap_to_queue on vertex counts around its 2048-word
-    block span and the word edge (bits past n in the last word are set and
-    must be ignored): the list holds exactly the frontier vertices, each
-    once, in increasing order within a block."""
+    """scaph_frontier_bitmap_to_queue on vertex counts around its block span
+    (8192 words), its 32-word thread runs and the word edge (bits past n in
+    the last word are set and must be ignored): the list holds exactly the
+    frontier vertices, each once, in increasing order within a block."""
     hip = _lib.hip()
     rng = np.random.default_rng(21)
+    span = 8192 * 32
     for n, dens in ((1, 1.0), (31, 0.5), (33, 0.5), (65536, 0.001), (65537 + 31, 0.3),
-                    (2048 * 32 * 3 + 5, 0.02), (1000003, 0.6), (4194321, 0.0)):
+                    (span * 3 + 5, 0.02), (span * 2 + 32 * 5 + 7, 0.4), (1000003, 0.6),
+                    (4194321, 0.0)):
         say("bitmap_to_queue", n, dens)
         words = ((n + 63) // 64) * 2
         bits = np.zeros(words * 32, np.uint8)
@@ -321,7 +323,7 @@ def test_frontier_bitmap_to_queue_ragged():
         assert cnt.q_len == len(want)
         np.testing.assert_array_equal(np.sort(got), want)
         # each block's slice is in increasing order
-        blk = got // (2048 * 32)
+        blk = got // span
         for b in np.unique(blk):
             seg = got[blk == b]
             assert np.all(np.diff(seg.astype(np.int64)) > 0)
