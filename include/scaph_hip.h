@@ -177,7 +177,10 @@ typedef struct scaph_push_tuning {
   uint32_t xcd_slices;  /* levels with at least 1024 x this many out-edges expand XCD-sliced:
                            tiles inside one out-list go to the XCD owning their targets'
                            eighth of the vertex ids (0: off) */
-  uint32_t reserved[3];
+  uint32_t coop_edges;  /* frontiers of at most 256 vertices with at most this many out-edges
+                           run in one launch with their edges spread over the grid (0: 2^20;
+                           1: effectively never) */
+  uint32_t reserved[2];
 } scaph_push_tuning;
 /* Top-down (push) expand of one level from a frontier queue — the resident
  * counterpart of gpu_push_csr (src/gpu_engine.cuh:41-169) with the BFS

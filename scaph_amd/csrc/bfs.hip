@@ -238,6 +238,101 @@ k_push_small(scaph_graph_desc g, const uint32_t* q_in, uint32_t n_in, const uint
   }
 }
 
+// A frontier of at most kBlock vertices with its out-lists spread over the
+// whole grid (a hub's edges no longer fall to one wave): every block reads
+// the frontier's Nodes and the prefix of their out-degrees into LDS, and a
+// thread takes kCoopEdges consecutive edges of the concatenated lists per
+// pass (owner by binary search in the prefix, then advanced), all its
+// target loads issued before its claims.  Claims as in k_expand, gathered
+// per block in LDS and appended to the list with one atomic per pass.
+constexpr uint32_t kCoopEdges = 8;
+constexpr unsigned long long kCoopDefault = 1ull << 20;  // out-edge cap of the path
+
+__global__ void __launch_bounds__(kBlock)
+k_push_coop(scaph_graph_desc g, const uint32_t* q_in, uint32_t n_in, const uint32_t* degree,
+            uint32_t* level, uint32_t* visited, uint32_t* q_out, scaph_level_counters* counters,
+            uint32_t next_level) {
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  __shared__ const uint32_t* s_ep[kBlock];
+  __shared__ unsigned long long s_off[kBlock + 1];
+  __shared__ uint32_t s_claim[kBlock * kCoopEdges];
+  __shared__ unsigned long long wtot[kBlock / SCAPH_WAVE];
+  __shared__ uint32_t s_nc;
+  __shared__ unsigned long long s_base;
+  const PageTab t = stage_pages(g, dyn_lds);
+  __syncthreads();
+  const uint32_t n = g.num_vertices;
+  unsigned long long len = 0;
+  const uint32_t* ep = nullptr;
+  if (threadIdx.x < n_in) {
+    const uint32_t u = q_in[threadIdx.x];
+    if (u < n) {
+      const NodeRef nr = load_node(t, g, u);
+      len = nr.len;
+      ep = nr.edges;
+    }
+  }
+  unsigned long long total;
+  const unsigned long long incl = block256_incl_scan(len, wtot, &total);
+  s_off[threadIdx.x] = incl - len;  // entries past n_in: len 0, offset total
+  s_ep[threadIdx.x] = ep;
+  if (threadIdx.x == 0) s_off[kBlock] = total;
+  unsigned long long acc_s = 0, acc_m = 0, acc_n = 0;
+  const uint64_t pass = (uint64_t)gridDim.x * kBlock * kCoopEdges;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * kBlock * kCoopEdges; c0 < total; c0 += pass) {
+    if (threadIdx.x == 0) s_nc = 0;
+    __syncthreads();  // also orders the prefix writes before the first pass's reads
+    const uint64_t e0 = c0 + (uint64_t)threadIdx.x * kCoopEdges;
+    uint32_t vv[kCoopEdges];
+    if (e0 < total) {
+      // owner of e0: the first list whose end lies past it
+      uint32_t lo = 1, hi = kBlock;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] > e0) hi = mid;
+        else lo = mid + 1;
+      }
+      uint32_t own = lo - 1;
+#pragma unroll
+      for (uint32_t j = 0; j < kCoopEdges; ++j) {
+        const uint64_t e = e0 + j;
+        vv[j] = 0xFFFFFFFFu;
+        if (e < total) {
+          while (s_off[own + 1] <= e) ++own;
+          vv[j] = s_ep[own][e - s_off[own]];
+        }
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < kCoopEdges; ++j) {
+        if (e0 + j >= total) continue;
+        ++acc_s;
+        const uint32_t v = vv[j];
+        if (v >= n) continue;
+        const uint32_t w = v >> 5, bit = 1u << (v & 31);
+        if (visited[w] & bit) continue;
+        if (atomicOr(&visited[w], bit) & bit) continue;
+        level[v] = next_level;
+        s_claim[atomicAdd(&s_nc, 1u)] = v;
+        acc_n += 1;
+        acc_m += degree[v];
+      }
+    }
+    __syncthreads();
+    const uint32_t nc = s_nc;
+    if (threadIdx.x == 0) s_base = nc ? atomicAdd(&counters->q_len, (unsigned long long)nc) : 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nc; i += kBlock) q_out[s_base + i] = s_claim[i];
+  }
+  acc_s = wave_sum(acc_s);
+  acc_m = wave_sum(acc_m);
+  acc_n = wave_sum(acc_n);
+  if (lane_id() == 0) {
+    if (acc_n) atomicAdd(&counters->n_next, acc_n);
+    if (acc_m) atomicAdd(&counters->m_next, acc_m);
+    if (acc_s) atomicAdd(&counters->scanned, acc_s);
+  }
+}
+
 // Frontier bits of vertices [v, v + c), c in 1..32: bit k is vertex v + k.
 __device__ __forceinline__ uint32_t bits_at(const uint32_t* bm, uint32_t v, uint32_t c) {
   const uint32_t w = v >> 5, sh = v & 31;
@@ -2103,6 +2198,19 @@ int push_queue(const scaph_graph_desc* out_graph, const uint32_t* degree, uint32
     return SCAPH_STATUS_INVALID;
   if (n_in == 0) return 0;
   if (!q_in) return SCAPH_STATUS_INVALID;
+  const unsigned long long coop = tune && tune->coop_edges ? tune->coop_edges : kCoopDefault;
+  if (n_in <= kBlock && m_in <= coop) {
+    // at least one block (m_in only sizes the grid; the blocks stride over
+    // the frontier's true edge count)
+    const unsigned long long per = (unsigned long long)kBlock * kCoopEdges;
+    const unsigned long long nb = (m_in + per - 1) / per;
+    const unsigned grid = nb == 0 ? 1u : (unsigned)nb;
+    hipLaunchKernelGGL(k_push_coop, dim3(grid), dim3(kBlock), page_lds_bytes(out_graph->num_pages),
+                       st, *out_graph, q_in, n_in, degree, level, visited, q_out, counters,
+                       cur_level + 1);
+    SCAPH_LAUNCH_CHECK();
+    return 0;
+  }
   const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
   if (m_in <= small && n_in <= 65536) {
     const unsigned grid = grid_for(n_in, kBlock / SCAPH_WAVE, 64);

@@ -97,6 +97,9 @@ Config& global_flags() {
     r.add("push_rows", FlagType::kBool, &c.push_rows, "push expand reads coalesced edge rows");
     r.add("push_small", FlagType::kInt32, &c.push_small,
           "out-edges below which a push level runs as one launch (0 = 8192, 1 = never)");
+    r.add("push_coop", FlagType::kInt32, &c.push_coop,
+          "frontiers of at most 256 vertices with at most this many out-edges run in one launch, "
+          "their edges spread over the grid (0 = 2^20, 1 = never)");
     r.add("push_scan", FlagType::kInt32, &c.push_scan,
           "prep blocks scanned by one block before the scan takes three launches (0 = 1024)");
     r.add("push_xcd", FlagType::kInt32, &c.push_xcd,
@@ -200,6 +203,7 @@ Engine::Engine(const Config& cfg)
   push_tune_.grid_blocks = (uint32_t)std::max(0, cfg.push_grid);
   push_tune_.edge_rows = cfg.push_rows ? 1u : 0u;
   push_tune_.small_edges = (uint32_t)std::max(0, cfg.push_small);
+  push_tune_.coop_edges = (uint32_t)std::max(0, cfg.push_coop);
   push_tune_.scan_single = (uint32_t)std::max(0, cfg.push_scan);
   push_tune_.xcd_slices = (uint32_t)std::max(0, cfg.push_xcd);
 }

@@ -71,6 +71,8 @@ struct Config {
   int32_t push_grid = 0;         // cap on push expand blocks (0: one per tile, at most 8192)
   bool push_rows = true;         // push expand reads coalesced 256-edge rows
   int32_t push_small = 0;        // out-edges below which a push level is one launch (0: 8192)
+  int32_t push_coop = 0;         // frontiers of <= 256 vertices with at most this many out-edges:
+                                 // one launch, edges spread over the grid (0: 2^20; 1: never)
   int32_t push_scan = 0;         // prep blocks scanned by one block; more: 3 launches (0: 1024)
   int32_t push_xcd = 65536;      // out-edges/1024 from which a push level expands XCD-sliced
   bool page_hint = true;         // paged: size page pushes by their page counts, read full

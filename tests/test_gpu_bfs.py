@@ -442,6 +442,9 @@ def test_pull_heavy_in_lists():
                                    ["-pull_summary=5", "-pull_summary_div=0", "-pull_hubs=1000"],
                                    ["-pull_summary=9", "-pull_summary_div=0", "-pull_phases=1",
                                     "-pull_hubs=0"],
+                                   ["-push_coop=1"], ["-push_coop=1000000000"],
+                                   ["-push_coop=1", "-push_small=1"],
+                                   ["-push_coop=300", "-push_small=1"],
                                    ["-push_small=1", "-push_scan=1"],
                                    ["-push_scan=1000000000"], ["-push_small=1", "-push_xcd=1"],
                                    ["-push_small=1", "-push_xcd=1", "-push_rows=false",
