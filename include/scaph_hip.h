@@ -178,8 +178,8 @@ typedef struct scaph_push_tuning {
                            tiles inside one out-list go to the XCD owning their targets'
                            eighth of the vertex ids (0: off) */
   uint32_t coop_edges;  /* frontiers of at most 256 vertices with at most this many out-edges
-                           run in one launch with their edges spread over the grid (0: 2^20;
-                           1: effectively never) */
+                           (and more than small_edges, or more than 64 per vertex) run in one
+                           launch with their edges spread over the grid (0: 2^20; 1: never) */
   uint32_t reserved[2];
 } scaph_push_tuning;
 /* Top-down (push) expand of one level from a frontier queue — the resident

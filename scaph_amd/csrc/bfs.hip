@@ -2199,7 +2199,10 @@ int push_queue(const scaph_graph_desc* out_graph, const uint32_t* degree, uint32
   if (n_in == 0) return 0;
   if (!q_in) return SCAPH_STATUS_INVALID;
   const unsigned long long coop = tune && tune->coop_edges ? tune->coop_edges : kCoopDefault;
-  if (n_in <= kBlock && m_in <= coop) {
+  const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
+  // a wave per vertex (k_push_small) is quicker for a few short lists; the
+  // grid-wide spread pays once lists average more than a wave's width
+  if (n_in <= kBlock && m_in <= coop && (m_in > small || m_in > (unsigned long long)SCAPH_WAVE * n_in)) {
     // at least one block (m_in only sizes the grid; the blocks stride over
     // the frontier's true edge count)
     const unsigned long long per = (unsigned long long)kBlock * kCoopEdges;
@@ -2211,7 +2214,6 @@ int push_queue(const scaph_graph_desc* out_graph, const uint32_t* degree, uint32
     SCAPH_LAUNCH_CHECK();
     return 0;
   }
-  const unsigned long long small = tune && tune->small_edges ? tune->small_edges : kSmallEdges;
   if (m_in <= small && n_in <= 65536) {
     const unsigned grid = grid_for(n_in, kBlock / SCAPH_WAVE, 64);
     hipLaunchKernelGGL(k_push_small, dim3(grid), dim3(kBlock), page_lds_bytes(out_graph->num_pages),
