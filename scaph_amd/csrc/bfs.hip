@@ -1608,7 +1608,9 @@ k_pull_sparse(scaph_graph_desc g, const uint4* head, const uint32_t* no_in,
   uint32_t* const sv = s_v[wid];
   uint32_t* const sh = s_h[wid];
   const uint64_t words = (((uint64_t)n + 63) / 64) * 2;
-  const uint64_t windows = (words + kPullWinWords - 1) / kPullWinWords;
+  // the walk takes two 1024-vertex windows per step, one word per lane
+  constexpr uint32_t kSpan = 2 * kPullWinWords;
+  const uint64_t windows = (words + kSpan - 1) / kSpan;
   const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
   unsigned long long acc_n = 0, acc_m = 0, acc_s = 0, acc_x = 0;
   uint32_t held = 0;  // wave-uniform
@@ -1642,17 +1644,17 @@ k_pull_sparse(scaph_graph_desc g, const uint4* head, const uint32_t* no_in,
   // window ahead, so the walk waits on none of them
   uint32_t vw_next = ~0u, nw_next = ~0u, base_next = 0u;
   auto request = [&](uint64_t win) {
-    const uint64_t w = win * kPullWinWords + lane;
-    const bool own = lane < kPullWinWords && w < words;
+    const uint64_t w = win * kSpan + lane;
+    const bool own = w < words;
     vw_next = own ? visited[w] : ~0u;
     if (live_base) {
       nw_next = own ? no_in[w] : ~0u;
-      base_next = live_base[win];
+      base_next = own ? live_base[w / kPullWinWords] : 0u;  // this lane's window
     }
   };
   if (first < windows) request(first);
   for (uint64_t win = first; win < windows; win += nwaves) {
-    const uint64_t w = win * kPullWinWords + lane;
+    const uint64_t w = win * kSpan + lane;
     const uint32_t vw = vw_next, nw = nw_next, wbase = base_next;
     if (win + nwaves < windows) request(win + nwaves);
     const uint32_t unv = ~vw;
@@ -1660,12 +1662,15 @@ k_pull_sparse(scaph_graph_desc g, const uint4* head, const uint32_t* no_in,
     const uint32_t incl = wave_incl_scan(c);
     const uint32_t tot = wave_bcast(incl, SCAPH_WAVE - 1);
     if (tot == 0) continue;
-    // head index = live rank, as k_pull's hidx computes it
+    // head index = live rank, as k_pull's hidx computes it: the rank
+    // restarts at each window's first word (lane 32 starts the second)
     uint32_t lw = 0, lb = 0;
     if (live_base) {
       lw = ~nw;
       const uint32_t lc = __popc(lw);
-      lb = wave_incl_scan(lc) - lc;
+      const uint32_t li = wave_incl_scan(lc);
+      const uint32_t first_half = wave_bcast(li, kPullWinWords - 1);
+      lb = li - lc - (lane >= kPullWinWords ? first_half : 0u);
     }
     // the window's vertices in order, kSparseBuf at a time
     for (uint32_t lo = 0; lo < tot; lo += kSparseBuf) {
