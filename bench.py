@@ -10,8 +10,8 @@ Prints ONE JSON line:
   roofline     the dominant kernel (by device time): algorithmic bytes per
                launch / its average launch time (HIP events around it)
   bfs_hbm_frac B_alg / t_BFS / 8 TB/s with B_alg = 4 m_c + 12 n_c + 4 n
-  cpu_baseline the oracle's single-core BFS on this host: config 1 in full
-               and the first --cpu-sources sources of this graph
+  cpu_baseline the oracle's single-core BFS on this host: R-MAT s20 ef16
+               seed 1 in full and the first --cpu-sources sources of this graph
 """
 import argparse
 import ctypes
@@ -141,8 +141,8 @@ def _timed_oracle_bfs(off, ngh, src, runs):
 
 
 def cpu_baseline(eng, srcs, runs):
-    """The oracle's single-core BFS timed on this host:
-    config 1 (R-MAT s20 ef16 seed 1, highest out-degree source) in full, and
+    """The oracle's single-core BFS timed on this host: R-MAT s20 ef16
+    seed 1 from its highest out-degree source in full (median of 5), and
     the first len(srcs) sources of this graph, each the median of `runs`
     runs.  `value` is the GTEPS of this graph's sample; the levels of every
     timed source are compared with the GPU's."""
@@ -165,11 +165,11 @@ def cpu_baseline(eng, srcs, runs):
     t_tot = sum(x["seconds"] for x in per)
     return {"value": m_tot / t_tot / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
             "sample": f"oracle BFS on 1 pinned core: the first {len(srcs)} of the 64 sources of "
-                      f"this graph, median of {runs} run(s) each ({t_tot:.1f} s); config 1 "
-                      f"(R-MAT s20 ef16 seed 1, source {c1_src}) in full, median of 5; levels "
+                      f"this graph, median of {runs} run(s) each ({t_tot:.1f} s); R-MAT s20 "
+                      f"ef16 seed 1 (source {c1_src}) in full, median of 5; levels "
                       f"bit-exact vs GPU: {parity}",
             "per_source": per,
-            "config1": {"src": c1_src, "seconds": c1_t, "m_c": c1_m,
+            "rmat_s20_ef16_seed1": {"src": c1_src, "seconds": c1_t, "m_c": c1_m,
                         "gteps": c1_m / c1_t / 1e9, "runs": 5},
             "host": host_cpu()}, parity
 
@@ -338,9 +338,9 @@ def main():
     ap.add_argument("--learned-check", type=int, default=8,
                     help="sources whose levels are compared between the schedules")
     ap.add_argument("--cpu-sources", type=int, default=1,
-                    help="sources of this graph the CPU baseline times (BASELINE.md: 4)")
+                    help="sources of this graph the CPU baseline times (the full protocol: 4)")
     ap.add_argument("--cpu-runs", type=int, default=1,
-                    help="runs per CPU-baseline source, median taken (BASELINE.md: 5)")
+                    help="runs per CPU-baseline source, median taken (the full protocol: 5)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
     args = ap.parse_args()

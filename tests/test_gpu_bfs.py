@@ -166,7 +166,7 @@ def test_indexed_head_records_builder():
             hip.scaph_dev_free(p)
 
 
-def test_config1_s20_ef16_seed1_from_max_degree():
+def test_rmat_s20_ef16_seed1_from_max_degree():
     """Config 1: R-MAT s20 ef16 (a,b,c)=(.57,.19,.19) seed 1, source = max out-degree."""
     off, ngh = O.rmat_csr(20, 16, seed=1)
     deg = np.diff(off.astype(np.int64))
@@ -176,14 +176,14 @@ def test_config1_s20_ef16_seed1_from_max_degree():
     with engine() as e:
         e.load_csr(off, ngh)
         for sched in SCHEDULES:
-            say("config1", sched)
+            say("rmat s20 ef16 seed 1", sched)
             e.set_schedule(sched)
             st = e.run(src)
             got = e.levels()
             np.testing.assert_array_equal(got, want)
             np.testing.assert_array_equal(np.where(got <= 62, got, 63), want63)
             assert e.validate(src) == (0, 0)
-            say("config1", sched, "ms", round(st["ms"], 3), "depth", st["depth"])
+            say("rmat s20 ef16 seed 1", sched, "ms", round(st["ms"], 3), "depth", st["depth"])
 
 
 def test_many_pages_global_page_lookup():
